@@ -1,0 +1,452 @@
+"""ctypes binding of libArrowMetalC.so (the MI355X build of the ArrowMetal C ABI).
+
+It mirrors the reference's Python package (python/arrowmetal/__init__.py: the ctypes loader at
+:26-78, MetalArray at :251-475, GroupByKeys / group_by at :2344-2510, batch at :787-797) for the
+argsort / take / group-by path, so that tests read like the reference's own. Arrays cross the C Data
+Interface: pyarrow arrays through ``_export_to_c`` / ``_import_from_c`` as the reference does, NumPy
+arrays through hand-built ArrowArray structs (no pyarrow needed).
+
+The library is looked up in ``$ARROWMETAL_LIB`` (an explicit path; a wrong value is an error, never a
+fall-through), then next to this file. There is no fallback implementation: without the HIP library,
+or without a ROCm device, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_LIB_NAME = "libArrowMetalC.so"
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def library_path() -> str:
+    env = os.environ.get("ARROWMETAL_LIB")
+    if env:
+        if not os.path.exists(env):
+            raise OSError(f"arrowmetal: $ARROWMETAL_LIB is set to {env}, which does not exist")
+        return env
+    p = os.path.join(_HERE, _LIB_NAME)
+    if not os.path.exists(p):
+        raise OSError(f"arrowmetal: {p} not found; build it with `python arrowmetal_amd/build.py`")
+    return p
+
+
+_P = ctypes.c_void_p
+_lib = None
+
+
+class ArrowMetalError(RuntimeError):
+    pass
+
+
+class _ArrowSchema(ctypes.Structure):
+    pass
+
+
+_ArrowSchema._fields_ = [("format", ctypes.c_char_p), ("name", ctypes.c_char_p), ("metadata", ctypes.c_char_p),
+                         ("flags", ctypes.c_int64), ("n_children", ctypes.c_int64),
+                         ("children", ctypes.POINTER(ctypes.POINTER(_ArrowSchema))),
+                         ("dictionary", ctypes.POINTER(_ArrowSchema)),
+                         ("release", ctypes.CFUNCTYPE(None, ctypes.POINTER(_ArrowSchema))),
+                         ("private_data", ctypes.c_void_p)]
+
+
+class _ArrowArray(ctypes.Structure):
+    pass
+
+
+_ArrowArray._fields_ = [("length", ctypes.c_int64), ("null_count", ctypes.c_int64), ("offset", ctypes.c_int64),
+                        ("n_buffers", ctypes.c_int64), ("n_children", ctypes.c_int64),
+                        ("buffers", ctypes.POINTER(ctypes.c_void_p)),
+                        ("children", ctypes.POINTER(ctypes.POINTER(_ArrowArray))),
+                        ("dictionary", ctypes.POINTER(_ArrowArray)),
+                        ("release", ctypes.CFUNCTYPE(None, ctypes.POINTER(_ArrowArray))),
+                        ("private_data", ctypes.c_void_p)]
+
+
+class _ArrowDeviceArray(ctypes.Structure):
+    _fields_ = [("array", _ArrowArray), ("device_id", ctypes.c_int64), ("device_type", ctypes.c_int32),
+                ("reserved", ctypes.c_int64 * 3), ("sync_event", ctypes.c_void_p)]
+
+
+# Exported symbols of include/arrowmetal.h and their ctypes signatures.
+_SIGNATURES = {
+    "am_version": ([], ctypes.c_char_p),
+    "am_device_name": ([], ctypes.c_char_p),
+    "am_last_error": ([], ctypes.c_char_p),
+    "am_import": ([_P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_import_device": ([_P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_export": ([_P, _P, _P], ctypes.c_int),
+    "am_export_device": ([_P, _P, _P], ctypes.c_int),
+    "am_release": ([_P], None),
+    "am_length": ([_P], ctypes.c_int64),
+    "am_null_count": ([_P], ctypes.c_int64),
+    "am_format": ([_P], ctypes.c_char_p),
+    "am_compute_format": ([_P], ctypes.c_char_p),
+    "am_take": ([_P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_argsort": ([_P, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_argsort_ex": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_argsort_ex2": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_group_by": ([_P, ctypes.c_int64, ctypes.c_int, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_batch_begin": ([], ctypes.c_int),
+    "am_batch_end": ([], ctypes.c_int),
+    "am_group_by_keys": ([ctypes.POINTER(_P), ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_group_by_group_count": ([_P], ctypes.c_int64),
+    "am_group_by_keys_result": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_group_by_ids": ([_P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_group_by_release": ([_P], None),
+    "am_group_agg_ex": ([_P, _P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_unique_rows": ([ctypes.POINTER(_P), ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
+    "am_prof_reset": ([], ctypes.c_int),
+    "am_prof_report": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int64),
+}
+
+
+def lib():
+    """Loads libArrowMetalC.so (once) and declares the signatures of every entry point."""
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(library_path())
+        for name, (args, res) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def version():
+    return lib().am_version().decode()
+
+
+def device_name():
+    return lib().am_device_name().decode()
+
+
+def _check(rc):
+    if rc == 2:
+        raise ArrowMetalError("bad argument (NULL handle)")
+    if rc != 0:
+        raise ArrowMetalError((lib().am_last_error() or b"unknown error").decode())
+
+
+def _call(fn, *args):
+    out = _P()
+    _check(fn(*args, ctypes.byref(out)))
+    return MetalArray(out)
+
+
+_NP_FORMAT = {np.dtype(np.int8): "c", np.dtype(np.uint8): "C", np.dtype(np.int16): "s",
+              np.dtype(np.uint16): "S", np.dtype(np.int32): "i", np.dtype(np.uint32): "I",
+              np.dtype(np.int64): "l", np.dtype(np.uint64): "L", np.dtype(np.float32): "f",
+              np.dtype(np.float64): "g"}
+_FORMAT_NP = {v: k for k, v in _NP_FORMAT.items()}
+
+
+def _np_of_format(fmt: str):
+    if fmt in _FORMAT_NP:
+        return _FORMAT_NP[fmt]
+    if fmt in ("tdD", "tts", "ttm", "tiM"):
+        return np.dtype(np.int32)
+    if fmt.startswith("t"):
+        return np.dtype(np.int64)
+    raise ArrowMetalError(f"no NumPy type for Arrow format {fmt!r}")
+
+
+# Producer-side release callbacks for arrays built from NumPy. The library moves the struct (it sets
+# the caller's release to NULL) and calls this on its own copy once it has copied the buffers.
+_keep = {}
+
+
+@ctypes.CFUNCTYPE(None, ctypes.POINTER(_ArrowArray))
+def _release_array(p):
+    _keep.pop(p.contents.private_data, None)
+    p.contents.release = ctypes.CFUNCTYPE(None, ctypes.POINTER(_ArrowArray))()
+
+
+def _c_array(values: np.ndarray, valid=None, offset: int = 0, length=None):
+    """A C Data Interface array over NumPy buffers (bit-packed validity from a bool mask or bytes)."""
+    values = np.ascontiguousarray(values)
+    n = len(values) - offset if length is None else length
+    bitmap = None
+    nulls = 0
+    if valid is not None:
+        valid = np.asarray(valid)
+        if valid.dtype == bool:
+            nulls = int(len(valid) - np.count_nonzero(valid[offset:offset + n])) if n else 0
+            bitmap = np.packbits(valid, bitorder="little")
+        else:
+            bitmap = np.ascontiguousarray(valid, dtype=np.uint8)
+            nulls = -1
+    bufs = (ctypes.c_void_p * 2)(None if bitmap is None else bitmap.ctypes.data, values.ctypes.data)
+    arr = _ArrowArray()
+    arr.length = n
+    arr.null_count = nulls
+    arr.offset = offset
+    arr.n_buffers = 2
+    arr.n_children = 0
+    arr.buffers = ctypes.cast(bufs, ctypes.POINTER(ctypes.c_void_p))
+    token = id(arr)
+    arr.private_data = token
+    arr.release = _release_array
+    _keep[token] = (values, bitmap, bufs)
+    return arr
+
+
+class MetalArray:
+    """An Arrow array resident in HBM. Immutable; every operation returns a new array."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.am_release(h)
+            self._h = None
+
+    # ---- interop
+    @classmethod
+    def from_numpy(cls, values, valid=None, fmt=None, offset=0, length=None):
+        """Imports a NumPy array (copied into HBM once). `valid` is a bool mask (True = valid) or an
+        Arrow bitmap; `fmt` overrides the Arrow format (temporal types)."""
+        values = np.ascontiguousarray(values)
+        fmt = fmt or _NP_FORMAT[values.dtype]
+        schema = _ArrowSchema()
+        schema.format = fmt.encode()
+        schema.name = b""
+        schema.flags = 2
+        arr = _c_array(values, valid, offset, length)
+        out = _P()
+        rc = lib().am_import(ctypes.addressof(schema), ctypes.addressof(arr), ctypes.byref(out))
+        if arr.release:           # not moved (an argument error): release it ourselves
+            arr.release(ctypes.byref(arr))
+        _check(rc)
+        return cls(out)
+
+    @classmethod
+    def from_arrow(cls, obj):
+        """Imports anything pyarrow can export through the C Data Interface."""
+        import pyarrow as pa
+        if isinstance(obj, pa.ChunkedArray):
+            obj = obj.combine_chunks()
+        if not isinstance(obj, pa.Array):
+            obj = pa.array(obj)
+        schema = _ArrowSchema()
+        array = _ArrowArray()
+        obj._export_to_c(ctypes.addressof(array), ctypes.addressof(schema))
+        try:
+            return _call(lib().am_import, ctypes.addressof(schema), ctypes.addressof(array))
+        finally:
+            if schema.release:
+                schema.release(ctypes.byref(schema))
+            if array.release:
+                array.release(ctypes.byref(array))
+
+    def to_arrow(self):
+        import pyarrow as pa
+        schema = _ArrowSchema()
+        array = _ArrowArray()
+        _check(lib().am_export(self._h, ctypes.addressof(schema), ctypes.addressof(array)))
+        return pa.Array._import_from_c(ctypes.addressof(array), ctypes.addressof(schema))
+
+    def to_numpy(self):
+        """(values, valid) as NumPy arrays; `valid` is a bool mask, or None when there are no nulls."""
+        schema = _ArrowSchema()
+        array = _ArrowArray()
+        _check(lib().am_export(self._h, ctypes.addressof(schema), ctypes.addressof(array)))
+        try:
+            fmt = schema.format.decode()
+            dt = _np_of_format(fmt)
+            n = array.length
+            vals = np.empty(n, dtype=dt)
+            if n:
+                ctypes.memmove(vals.ctypes.data, array.buffers[1] + array.offset * dt.itemsize, n * dt.itemsize)
+            valid = None
+            if array.buffers[0] and array.null_count != 0 and n:
+                nbytes = (array.offset + n + 7) // 8
+                raw = np.empty(nbytes, dtype=np.uint8)
+                ctypes.memmove(raw.ctypes.data, array.buffers[0], nbytes)
+                valid = np.unpackbits(raw, bitorder="little")[array.offset:array.offset + n].astype(bool)
+            return vals, valid
+        finally:
+            array.release(ctypes.byref(array))
+            schema.release(ctypes.byref(schema))
+
+    def device_buffers(self):
+        """(values_ptr, validity_ptr, offset, length, null_count) of the HBM buffers (am_export_device)."""
+        schema = _ArrowSchema()
+        darr = _ArrowDeviceArray()
+        _check(lib().am_export_device(self._h, ctypes.addressof(schema), ctypes.addressof(darr)))
+        a = darr.array
+        out = (a.buffers[1], a.buffers[0], a.offset, a.length, a.null_count, darr.device_type)
+        a.release(ctypes.byref(a))
+        schema.release(ctypes.byref(schema))
+        return out
+
+    # ---- properties
+    def __len__(self):
+        return lib().am_length(self._h)
+
+    @property
+    def null_count(self):
+        return lib().am_null_count(self._h)
+
+    @property
+    def format(self):
+        return lib().am_format(self._h).decode()
+
+    # ---- selection and sorting
+    def take(self, indices):
+        if not isinstance(indices, MetalArray):
+            indices = MetalArray.from_numpy(np.asarray(indices))
+        return _call(lib().am_take, self._h, indices._h)
+
+    def argsort(self, descending=False, null_placement="at_end", float_order="ieee"):
+        """uint32 indices that sort the array stably (Arrow `array_sort_indices` with the reference's
+        null placement and float orders)."""
+        if null_placement == "at_end" and float_order == "ieee":
+            return _call(lib().am_argsort, self._h, 1 if descending else 0)
+        return _call(lib().am_argsort_ex2, self._h, 1 if descending else 0,
+                     NULL_PLACEMENTS.index(null_placement), _float_order(float_order))
+
+    def group_by(self, key_count):
+        """Dense-key group-by (`am_group_by`): this array holds keys in [0, key_count)."""
+        return GroupBy(self, key_count)
+
+
+NULL_PLACEMENTS = ("at_end", "at_start")
+FLOAT_ORDERS = ("ieee", "total", "nan_largest")
+
+
+def _float_order(v):
+    if isinstance(v, int):
+        return v
+    return FLOAT_ORDERS.index(v)
+
+
+_AGG = {"sum": 0, "count": 1, "min": 2, "max": 3, "mean": 4, "count_values": 5}
+
+
+class GroupBy:
+    """am_group_by over dense keys in [0, key_count)."""
+
+    def __init__(self, keys, key_count):
+        self.keys, self.key_count = keys, key_count
+
+    def _agg(self, name, values):
+        return _call(lib().am_group_by, self.keys._h, self.key_count, _AGG[name],
+                     values._h if values is not None else None)
+
+    def count(self): return self._agg("count", None)
+    def sum(self, values): return self._agg("sum", values)
+    def min(self, values): return self._agg("min", values)
+    def max(self, values): return self._agg("max", values)
+    def mean(self, values): return self._agg("mean", values)
+    def count_values(self, values): return self._agg("count_values", values)
+
+
+_GROUP_AGG = {"sum": 0, "count_all": 1, "count": 2, "mean": 3, "min": 4, "max": 5}
+
+
+class GroupByKeys:
+    """Arrow `hash_*` aggregation over arbitrary key columns (am_group_by_keys + am_group_agg_ex).
+    Groups come out ascending by key, the null group last."""
+
+    def __init__(self, columns):
+        cols = [c if isinstance(c, MetalArray) else MetalArray.from_numpy(np.asarray(c)) for c in columns]
+        if not cols:
+            raise ArrowMetalError("group_by needs at least one key column")
+        self._columns = cols
+        handles = (_P * len(cols))(*[c._h for c in cols])
+        out = _P()
+        _check(lib().am_group_by_keys(handles, len(cols), ctypes.byref(out)))
+        self._h = out
+        self.group_count = lib().am_group_by_group_count(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.am_group_by_release(h)
+            self._h = None
+
+    def __len__(self):
+        return self.group_count
+
+    def key_arrays(self):
+        out = []
+        for i in range(len(self._columns)):
+            h = _P()
+            _check(lib().am_group_by_keys_result(self._h, i, ctypes.byref(h)))
+            out.append(MetalArray(h))
+        return out
+
+    def keys(self):
+        return [k.to_numpy() for k in self.key_arrays()]
+
+    def ids(self):
+        return _call(lib().am_group_by_ids, self._h)
+
+    def _agg(self, name, values, p1=0.0):
+        column = None if values is None else (values if isinstance(values, MetalArray)
+                                              else MetalArray.from_numpy(np.asarray(values)))
+        out = _P()
+        _check(lib().am_group_agg_ex(self._h, None if column is None else column._h, _GROUP_AGG[name],
+                                     float(p1), ctypes.byref(out)))
+        return MetalArray(out)
+
+    def sum(self, values): return self._agg("sum", values)
+    def count(self, values): return self._agg("count", values)
+    def count_all(self): return self._agg("count_all", None)
+    def mean(self, values): return self._agg("mean", values)
+    def min(self, values): return self._agg("min", values)
+    def max(self, values): return self._agg("max", values)
+
+
+def group_by(keys):
+    if not isinstance(keys, (list, tuple)):
+        keys = [keys]
+    return GroupByKeys(keys)
+
+
+def unique_rows(columns):
+    """Keep-first unique rows over key columns (the plan engine's `unique`), as uint32 row numbers."""
+    handles = (_P * len(columns))(*[c._h for c in columns])
+    out = _P()
+    _check(lib().am_unique_rows(handles, len(columns), ctypes.byref(out)))
+    return MetalArray(out)
+
+
+class batch:
+    """Context manager: calls issued inside queue on the thread's stream and complete at exit."""
+
+    def __enter__(self):
+        _check(lib().am_batch_begin())
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        _check(lib().am_batch_end())
+        return False
+
+
+def prof_enable(on=True):
+    lib().am_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    _check(lib().am_prof_reset())
+
+
+def prof_report():
+    """{kernel name: (launches, total_ms)} measured with HIP events on the launching stream."""
+    need = lib().am_prof_report(None, 0)
+    buf = ctypes.create_string_buffer(int(max(need, 1)))
+    lib().am_prof_report(buf, need)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, count, ms = line.split()
+        out[name] = (int(count), float(ms))
+    return out

@@ -1,0 +1,185 @@
+// Internal declarations of libArrowMetalC (MI355X / gfx950 build).
+//
+// Layout of the library:
+//   runtime.cpp   device context, per-thread HIP streams, the caching device allocator, the am_array
+//                 handle, Arrow C Data / C Device Data import and export, errors, batching, profiling
+//   sort.hip      the stable LSD radix argsort (onesweep: one histogram sweep, then one scatter
+//                 kernel per 8-bit digit with a decoupled look-back) and its key maps
+//   take.hip      the gather behind am_take
+//   groupby.hip   the hash group-by: key table build, ascending relabel, grouped aggregates
+//   capi.cpp      the extern "C" am_* entry points of include/arrowmetal.h
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/arrow_abi.h"
+
+namespace am {
+
+// ---------------------------------------------------------------- errors
+
+// An operation error: the entry point returns 1 and the message goes to am_last_error().
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+[[noreturn]] void fail(const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define AM_HIP(x) ::am::hip_check((x), #x)
+
+void set_last_error(const std::string& msg);
+
+// ---------------------------------------------------------------- element types
+
+// Type codes shared by every kernel family (and by the CPU oracle's tests).
+enum TypeCode : int {
+  T_I8 = 0, T_I16 = 1, T_I32 = 2, T_I64 = 3,
+  T_U8 = 4, T_U16 = 5, T_U32 = 6, T_U64 = 7,
+  T_F32 = 8, T_F64 = 9,
+  T_BOOL = 10,
+  T_UNSUPPORTED = -1
+};
+__host__ __device__ inline int type_width(int t) {
+  switch (t) {
+    case T_I8: case T_U8: return 1;
+    case T_I16: case T_U16: return 2;
+    case T_I32: case T_U32: case T_F32: return 4;
+    case T_I64: case T_U64: case T_F64: return 8;
+    default: return 0;
+  }
+}
+__host__ __device__ inline bool type_is_float(int t) { return t == T_F32 || t == T_F64; }
+__host__ __device__ inline bool type_is_signed(int t) { return t >= T_I8 && t <= T_I64; }
+__host__ __device__ inline bool type_is_int(int t) { return t >= T_I8 && t <= T_U64; }
+// The storage type of an Arrow format string: primitives, and temporal types as their integers.
+int type_of_format(const std::string& fmt);
+
+// ---------------------------------------------------------------- device memory
+
+// One device allocation, returned to the caching pool (or to its producer) when the last reference
+// goes. `bytes` is the usable size.
+struct DevBlock {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  std::function<void(void*)> on_free;   // empty: the block belongs to the pool
+  ~DevBlock();
+};
+using DevBuf = std::shared_ptr<DevBlock>;
+
+// Device memory from the library's caching pool (size classes of 2^k and 1.5 * 2^k above 4 KiB).
+// The pool never hands a block back to the driver while the process runs, so steady-state calls
+// allocate nothing from HIP.
+DevBuf device_alloc(size_t bytes);
+// A view of memory owned elsewhere (zero-copy device import); `release` runs when the view dies.
+DevBuf device_wrap(void* ptr, size_t bytes, std::function<void(void*)> release);
+// Bytes the pool holds (in use + cached), for admission control and tests.
+size_t pool_bytes_in_use();
+size_t pool_bytes_cached();
+void pool_trim();
+
+// ---------------------------------------------------------------- streams and devices
+
+struct DeviceInfo {
+  int id = 0;
+  int cu_count = 256;
+  size_t lds_per_block = 65536;
+  std::string name;
+};
+const DeviceInfo& device();
+
+// The calling thread's stream (created on first use, default priority, non-blocking).
+hipStream_t stream();
+// Every public call is synchronous unless the thread is inside am_batch_begin / am_batch_end:
+// `sync()` waits for the thread's stream unless a batch is open.
+void sync();
+bool in_batch();
+// Keeps `b` alive until the thread's open batch ends (no-op outside a batch).
+void retain(const DevBuf& b);
+// Runs `f` after the stream has drained at the next sync point (deferred GPU-side error flags).
+void after_sync(std::function<void()> f);
+
+// Kernel-time profiling of the library's own launches (extension API am_prof_*): when enabled,
+// each named launch is bracketed with HIP events on the launching stream.
+struct ProfScope {
+  ProfScope(const char* name);
+  ~ProfScope();
+  const char* name_;
+  hipEvent_t start_ = nullptr;
+};
+
+// ---------------------------------------------------------------- arrays
+
+struct Array {
+  std::atomic<int> refs{1};
+  std::string format;      // Arrow C data format string, as imported
+  int type = T_UNSUPPORTED;
+  int64_t length = 0;
+  int64_t offset = 0;      // Arrow offset, in elements, carried rather than applied
+  int64_t null_count = 0;
+  DevBuf values;           // device memory of element 0 (before the offset)
+  DevBuf validity;         // device bitmap, bit (offset + i) for element i; null when no nulls
+  // Cached host copy of the exported buffers (so repeated am_export calls copy once).
+  std::shared_ptr<std::vector<uint8_t>> host_values, host_validity;
+
+  int width() const { return type_width(type); }
+  const void* data() const {   // element `offset`
+    return values ? static_cast<const char*>(values->ptr) + offset * width() : nullptr;
+  }
+  void* mutable_data() { return values ? static_cast<char*>(values->ptr) + offset * width() : nullptr; }
+  const uint8_t* valid_bits() const { return validity ? static_cast<const uint8_t*>(validity->ptr) : nullptr; }
+  bool has_nulls() const { return validity && null_count != 0; }
+};
+
+Array* new_array(const std::string& format, int type, int64_t length);   // refs = 1
+void retain_array(Array* a);
+void release_array(Array* a);
+// Counts the unset bits of the array's validity, on the device.
+int64_t count_nulls(const Array& a);
+
+// ---------------------------------------------------------------- limits
+
+// Row numbers are uint32 ("I"): every index-returning call refuses more than 2^32 rows
+// (Sources/ArrowMetal/Kernels/Dispatch.swift:186-203).
+void check_length(int64_t n);
+void check_index_rows(int64_t rows, const char* op);
+
+// ---------------------------------------------------------------- operations
+
+// sort.hip
+Array* argsort(const Array& a, bool descending, int null_placement, int float_order);
+// Device-level argsort on raw buffers, shared by the group-by relabel and the batched path.
+// `vals` points at element 0 of `n` elements of `type`; validity bit (vbit_off + i) or null.
+void argsort_device(const void* vals, int type, const uint8_t* validity, int64_t vbit_off, int64_t n,
+                    bool descending, int null_placement, int float_order, uint32_t* out_idx,
+                    hipStream_t s);
+
+// take.hip
+Array* take(const Array& a, const Array& indices);
+
+// groupby.hip
+struct GroupBy;
+GroupBy* group_by_keys(Array* const* columns, int64_t count);
+int64_t group_by_group_count(const GroupBy* g);
+Array* group_by_keys_result(GroupBy* g, int64_t i);
+Array* group_by_ids(GroupBy* g);
+Array* group_agg(GroupBy* g, const Array* values, int op, double p1);
+void group_by_release(GroupBy* g);
+Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values);
+Array* unique_rows(Array* const* columns, int64_t count);
+
+// Launch-shape parameters the autotuner chooses (runtime.cpp keeps the table).
+struct SortTune {
+  int items = 16;        // keys per thread per scatter tile (tile = 256 * items)
+};
+SortTune sort_tune(int64_t n, int key_bytes);
+
+}  // namespace am

@@ -1,0 +1,207 @@
+// The extern "C" entry points of include/arrowmetal.h.
+//
+// Each one mirrors the reference's @_cdecl function (Sources/ArrowMetalC/ArrowMetalC.swift,
+// ArrowMetalC_Options.swift, ArrowMetalC_GroupByExtra.swift): a NULL handle or out-pointer returns 2
+// with no message; an operation error returns 1 and leaves its message in the thread-local
+// am_last_error(); success returns 0 and writes a new reference to `*out`.
+#include <cstring>
+
+#include "../../include/arrowmetal.h"
+#include "am_internal.hpp"
+
+namespace am {
+const char* last_error_cstr();
+Array* import_host(const ArrowSchema* schema, ArrowArray* array);
+Array* import_device(const ArrowSchema* schema, ArrowDeviceArray* array);
+void export_host(Array* a, ArrowSchema* schema, ArrowArray* array);
+void export_device(Array* a, ArrowSchema* schema, ArrowDeviceArray* array);
+void batch_begin();
+void batch_end();
+void prof_enable(bool on);
+void prof_reset();
+std::string prof_report();
+}  // namespace am
+
+using am::Array;
+
+namespace {
+
+inline Array* A(am_array* p) { return reinterpret_cast<Array*>(p); }
+inline am_array* H(Array* p) { return reinterpret_cast<am_array*>(p); }
+inline am::GroupBy* G(am_groupby* p) { return reinterpret_cast<am::GroupBy*>(p); }
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    am::set_last_error(e.what());
+    return 1;
+  } catch (...) {
+    am::set_last_error("unknown error");
+    return 1;
+  }
+}
+
+thread_local std::string t_text;
+
+}  // namespace
+
+extern "C" {
+
+const char* am_version(void) { return "0.1.0-mi355x"; }
+
+const char* am_device_name(void) {
+  try {
+    t_text = am::device().name;
+  } catch (const std::exception& e) {
+    t_text = std::string("no ROCm device: ") + e.what();
+  }
+  return t_text.c_str();
+}
+
+const char* am_last_error(void) { return am::last_error_cstr(); }
+
+int am_import(const struct ArrowSchema* schema, struct ArrowArray* array, am_array** out) {
+  if (!schema || !array || !out) return 2;
+  return guarded([&] { *out = H(am::import_host(schema, array)); });
+}
+
+int am_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array, am_array** out) {
+  if (!schema || !array || !out) return 2;
+  return guarded([&] { *out = H(am::import_device(schema, array)); });
+}
+
+int am_export(am_array* a, struct ArrowSchema* schema, struct ArrowArray* array) {
+  if (!a || !schema || !array) return 2;
+  return guarded([&] { am::export_host(A(a), schema, array); });
+}
+
+int am_export_device(am_array* a, struct ArrowSchema* schema, struct ArrowDeviceArray* array) {
+  if (!a || !schema || !array) return 2;
+  return guarded([&] { am::export_device(A(a), schema, array); });
+}
+
+void am_release(am_array* a) {
+  if (a) am::release_array(A(a));
+}
+
+int64_t am_length(am_array* a) { return a ? A(a)->length : -1; }
+
+int64_t am_null_count(am_array* a) {
+  if (!a) return -1;
+  Array* x = A(a);
+  if (x->null_count < 0) {
+    try {
+      am::sync();
+      if (x->null_count < 0) x->null_count = am::count_nulls(*x);
+    } catch (const std::exception& e) {
+      am::set_last_error(e.what());
+      return -1;
+    }
+  }
+  return x->null_count;
+}
+
+const char* am_format(am_array* a) { return a ? A(a)->format.c_str() : nullptr; }
+const char* am_compute_format(am_array* a) { return a ? A(a)->format.c_str() : nullptr; }
+
+int am_take(am_array* a, am_array* indices, am_array** out) {
+  if (!a || !indices || !out) return 2;
+  return guarded([&] { *out = H(am::take(*A(a), *A(indices))); });
+}
+
+int am_argsort(am_array* a, int descending, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::argsort(*A(a), descending != 0, 0, 0)); });
+}
+
+int am_argsort_ex(am_array* a, int descending, int null_placement, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::argsort(*A(a), descending != 0, null_placement == 1 ? 1 : 0, 0)); });
+}
+
+int am_argsort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] {
+    if (float_order < 0 || float_order > 2) am::fail("unknown float_order " + std::to_string(float_order));
+    *out = H(am::argsort(*A(a), descending != 0, null_placement == 1 ? 1 : 0, float_order));
+  });
+}
+
+int am_group_by(am_array* keys, int64_t key_count, int agg, am_array* values, am_array** out) {
+  if (!keys || !out) return 2;
+  return guarded([&] { *out = H(am::group_by_dense(*A(keys), key_count, agg, values ? A(values) : nullptr)); });
+}
+
+int am_batch_begin(void) {
+  return guarded([] { am::batch_begin(); });
+}
+
+int am_batch_end(void) {
+  return guarded([] { am::batch_end(); });
+}
+
+int am_group_by_keys(am_array** columns, int64_t count, am_groupby** out) {
+  if (!columns || !out || count <= 0) return 2;
+  for (int64_t i = 0; i < count; i++)
+    if (!columns[i]) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_groupby*>(am::group_by_keys(reinterpret_cast<Array* const*>(columns), count));
+  });
+}
+
+int64_t am_group_by_group_count(am_groupby* gb) { return gb ? am::group_by_group_count(G(gb)) : -1; }
+
+int am_group_by_keys_result(am_groupby* gb, int64_t i, am_array** out) {
+  if (!gb || !out) return 2;
+  return guarded([&] { *out = H(am::group_by_keys_result(G(gb), i)); });
+}
+
+int am_group_by_ids(am_groupby* gb, am_array** out) {
+  if (!gb || !out) return 2;
+  return guarded([&] { *out = H(am::group_by_ids(G(gb))); });
+}
+
+void am_group_by_release(am_groupby* gb) {
+  if (gb) am::group_by_release(G(gb));
+}
+
+int am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_array** out) {
+  if (!gb || !out) return 2;
+  return guarded([&] { *out = H(am::group_agg(G(gb), values ? A(values) : nullptr, op, p1)); });
+}
+
+int am_unique_rows(am_array** columns, int64_t count, am_array** out) {
+  if (!columns || !out || count <= 0) return 2;
+  for (int64_t i = 0; i < count; i++)
+    if (!columns[i]) return 2;
+  return guarded([&] { *out = H(am::unique_rows(reinterpret_cast<Array* const*>(columns), count)); });
+}
+
+int am_prof_enable(int on) {
+  am::prof_enable(on != 0);
+  return 0;
+}
+
+int am_prof_reset(void) {
+  return guarded([] { am::prof_reset(); });
+}
+
+int64_t am_prof_report(char* buf, int64_t cap) {
+  std::string r;
+  try {
+    r = am::prof_report();
+  } catch (...) {
+    return -1;
+  }
+  if (buf && cap > 0) {
+    size_t k = std::min<size_t>(size_t(cap - 1), r.size());
+    std::memcpy(buf, r.data(), k);
+    buf[k] = 0;
+  }
+  return int64_t(r.size() + 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,1239 @@
+// Hash group-by for CDNA4 (gfx950).
+//
+// Semantics follow the reference's GroupByKeys / GroupBy / GroupSumExact / GroupByExtrema
+// (Sources/ArrowMetal/Kernels/GroupByKeys.swift:49-58, 80-149, 387-413; HashTable.swift:290-302;
+// GroupBy.swift:46-182; GroupSumExactSource.swift:3-32; GroupByExtremaSource.swift:3-55):
+//   * one group per distinct key; -0.0 == 0.0 and every NaN is one key; a null key is its own group;
+//   * group order ascending by key with the null group last (lexicographic for several columns);
+//   * the representative row of a group is its lowest row;
+//   * integer sums wrap in 64 bits; float64 sums and means are correctly rounded;
+//   * min / max compare on an order-preserving key (float64: the raw sign flip, so -0.0 < +0.0) and
+//     skip NaN; a group with no value to answer with is null.
+//
+// How it runs here (this build's own design; CDNA4 has native 64-bit global and LDS atomics, so none
+// of the reference's 32-bit carry work-arounds are needed):
+//   gb_build     one read of the key column into an open-addressing table of 16-byte slots
+//                (64-bit key, lowest row) with linear probing and a 64-bit CAS insert; every row
+//                records its slot. The key 0x8000000000000000 is the empty marker and has a slot of
+//                its own past the end of the table.
+//   gb_compact   the occupied slots, in any order
+//   (argsort)    the distinct keys' order keys, sorted by the radix argsort of sort.hip
+//   gb_relabel   slot -> ascending group id, and the representative rows
+//   gb_ids       each row's slot -> its group id, in place
+//   gb_agg<P>    one pass over (ids, values) for a family of aggregates: a private LDS table per
+//                workgroup when the groups fit, otherwise an LDS cache of the groups a workgroup sees
+//                (claimed by CAS) in front of global 64-bit atomics; flushed once per workgroup.
+#include "am_internal.hpp"
+
+#include <algorithm>
+#include <map>
+
+namespace am {
+
+namespace {
+
+constexpr unsigned long long EMPTY = 0x8000000000000000ull;
+constexpr uint32_t NULL_SLOT = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool bit_get(const uint8_t* v, int64_t i) { return (v[i >> 3] >> (i & 7)) & 1; }
+
+// fmix64, the MurmurHash3 finaliser (a published mixing function), after a golden-ratio xor.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+  k ^= 0x9E3779B97F4A7C15ull;
+  k ^= k >> 33;
+  k *= 0xFF51AFD7ED558CCDull;
+  k ^= k >> 33;
+  k *= 0xC4CEB9FE1A85EC53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+// The 64-bit grouping key of a non-null element: integers widen (sign-extended when signed), floats
+// are normalised so that bit equality is Arrow value equality (HashTable.swift:290-302).
+__device__ __forceinline__ unsigned long long group_key(const void* vals, int64_t i, int type) {
+  switch (type) {
+    case T_I64: case T_U64: return static_cast<const unsigned long long*>(vals)[i];
+    case T_I32: return (unsigned long long)(long long)static_cast<const int32_t*>(vals)[i];
+    case T_I16: return (unsigned long long)(long long)static_cast<const int16_t*>(vals)[i];
+    case T_I8: return (unsigned long long)(long long)static_cast<const int8_t*>(vals)[i];
+    case T_U32: return static_cast<const uint32_t*>(vals)[i];
+    case T_U16: return static_cast<const uint16_t*>(vals)[i];
+    case T_U8: return static_cast<const uint8_t*>(vals)[i];
+    case T_F64: {
+      unsigned long long b = static_cast<const unsigned long long*>(vals)[i];
+      if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
+      if ((b & 0x7FFFFFFFFFFFFFFFull) == 0ull) return 0ull;
+      return b;
+    }
+    default: {   // T_F32
+      uint32_t b = static_cast<const uint32_t*>(vals)[i];
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000u;
+      if ((b & 0x7FFFFFFFu) == 0u) return 0u;
+      return b;
+    }
+  }
+}
+
+// Ascending order of the groups: signed, unsigned or float value order (the one NaN after +inf).
+__device__ __forceinline__ unsigned long long order_key(unsigned long long gk, int type) {
+  if (type == T_F64) return (gk & 0x8000000000000000ull) ? ~gk : (gk | 0x8000000000000000ull);
+  if (type == T_F32) {
+    uint32_t x = uint32_t(gk);
+    return (x & 0x80000000u) ? uint32_t(~x) : (x | 0x80000000u);
+  }
+  if (type == T_I8 || type == T_I16 || type == T_I32 || type == T_I64) return gk ^ 0x8000000000000000ull;
+  return gk;
+}
+
+struct Slot {
+  unsigned long long key;
+  uint32_t first;   // lowest row of the slot's key
+  uint32_t pad;
+};
+
+struct BuildStatus {
+  uint32_t overflow;
+  uint32_t null_first;
+  uint32_t distinct;
+  uint32_t special_used;   // the EMPTY key has been seen: slot C holds it
+  unsigned long long nulls;
+};
+
+__global__ void gb_table_init(Slot* table, size_t slots, BuildStatus* st) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < slots; i += stride) {
+    table[i].key = EMPTY;
+    table[i].first = 0xFFFFFFFFu;
+    table[i].pad = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->overflow = 0;
+    st->null_first = 0xFFFFFFFFu;
+    st->distinct = 0;
+    st->special_used = 0;
+    st->nulls = 0;
+  }
+}
+
+__device__ __forceinline__ void lower_first(uint32_t* p, uint32_t row) {
+  if (row < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, row);
+}
+
+// Inserts every row's key; writes the slot of each row (NULL_SLOT for a null key).
+__global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
+                                                uint32_t n, Slot* table, uint32_t cap_mask, uint32_t budget,
+                                                uint32_t* slot_of, BuildStatus* st) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t special = cap_mask + 1u;
+  unsigned long long nulls = 0;
+  for (; i < n; i += stride) {
+    if (valid && !bit_get(valid, voff + int64_t(i))) {
+      slot_of[i] = NULL_SLOT;
+      nulls++;
+      lower_first(&st->null_first, i);
+      continue;
+    }
+    unsigned long long k = group_key(vals, i, type);
+    uint32_t s;
+    if (k == EMPTY) {
+      s = special;
+      if (!__hip_atomic_load(&st->special_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicExch(&st->special_used, 1u);
+    } else {
+      s = uint32_t(mix64(k)) & cap_mask;
+      uint32_t probes = 0;
+      while (true) {
+        unsigned long long cur = __hip_atomic_load(&table[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == k) break;
+        if (cur == EMPTY) {
+          unsigned long long prev = atomicCAS(&table[s].key, EMPTY, k);
+          if (prev == EMPTY || prev == k) break;
+        }
+        if (++probes >= budget) {
+          atomicOr(&st->overflow, 1u);
+          s = NULL_SLOT;
+          break;
+        }
+        s = (s + 1u) & cap_mask;
+      }
+      if (s == NULL_SLOT) {
+        slot_of[i] = NULL_SLOT;
+        continue;
+      }
+    }
+    slot_of[i] = s;
+    lower_first(&table[s].first, i);
+  }
+  for (int off = 32; off > 0; off >>= 1) nulls += __shfl_down(nulls, off, 64);
+  if ((threadIdx.x & 63) == 0 && nulls) atomicAdd(&st->nulls, nulls);
+}
+
+// Occupied slots -> (order key, slot). The special slot C is occupied when the EMPTY key was seen.
+__global__ __launch_bounds__(256) void gb_compact(const Slot* table, uint32_t cap, int type, BuildStatus* st,
+                                                  unsigned long long* okeys, uint32_t* slots) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool occ = false;
+  unsigned long long k = 0;
+  if (s < cap) {
+    k = table[s].key;
+    occ = k != EMPTY;
+  } else if (s == cap) {
+    occ = st->special_used != 0;
+    k = EMPTY;
+  }
+  unsigned long long bal = __ballot(occ);
+  uint32_t base = 0;
+  if (lane == 0 && bal) base = atomicAdd(&st->distinct, uint32_t(__popcll(bal)));
+  base = __shfl(base, 0, 64);
+  if (occ) {
+    uint32_t at = base + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
+    okeys[at] = order_key(k, type);
+    slots[at] = s;
+  }
+}
+
+__global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t D, const Slot* table,
+                           uint32_t* gid_of_slot, uint32_t* rep, const BuildStatus* st, int has_null) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < D) {
+    uint32_t s = slots[perm[r]];
+    gid_of_slot[s] = r;
+    rep[r] = table[s].first;
+  } else if (r == D && has_null) {
+    rep[D] = st->null_first;
+  }
+}
+
+__global__ void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot, uint32_t null_gid) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t s = slot_of[i];
+    slot_of[i] = s == NULL_SLOT ? null_gid : gid_of_slot[s];
+  }
+}
+
+__global__ void gb_combine(const int32_t* a, const int32_t* b, uint32_t n, long long kb, long long* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = (long long)a[i] * kb + (long long)b[i];
+}
+
+int grid_for(int64_t n, int per_cu = 8) {
+  int64_t b = (n + 255) / 256;
+  return int(std::max<int64_t>(1, std::min<int64_t>(b, int64_t(device().cu_count) * per_cu)));
+}
+
+// ================================================================ aggregation framework
+
+enum WordOp : int { W_ADD = 0, W_MIN = 1, W_MAX = 2, W_OR = 3, W_ADD128 = 4 /* lo */, W_HI = 5 /* its hi */ };
+
+// Where a row's group id comes from: the int32 ids of a key mapping (always in [0, K)), or a dense
+// key column of int32 / int64 / uint32 whose null and out-of-range keys are skipped (am_group_by).
+struct IdSource {
+  const void* ids;
+  int type;               // T_I32, T_I64 or T_U32
+  const uint8_t* valid;   // key validity, dense form only
+  int64_t voff;
+};
+
+__device__ __forceinline__ long long load_gid(const IdSource& s, int64_t i, long long K) {
+  long long g;
+  if (s.type == T_I32) g = static_cast<const int32_t*>(s.ids)[i];
+  else if (s.type == T_U32) g = static_cast<const uint32_t*>(s.ids)[i];
+  else g = static_cast<const long long*>(s.ids)[i];
+  if (s.valid && !bit_get(s.valid, s.voff + i)) return -1;
+  return (g < 0 || g >= K) ? -1 : g;
+}
+
+struct ValSource {
+  const void* vals;
+  int type;
+  const uint8_t* valid;
+  int64_t voff;
+};
+
+__device__ __forceinline__ unsigned long long raw64(const ValSource& v, int64_t i) {
+  switch (v.type) {
+    case T_I64: case T_U64: case T_F64: return static_cast<const unsigned long long*>(v.vals)[i];
+    case T_I32: return (unsigned long long)(long long)static_cast<const int32_t*>(v.vals)[i];
+    case T_I16: return (unsigned long long)(long long)static_cast<const int16_t*>(v.vals)[i];
+    case T_I8: return (unsigned long long)(long long)static_cast<const int8_t*>(v.vals)[i];
+    case T_U32: case T_F32: return static_cast<const uint32_t*>(v.vals)[i];
+    case T_U16: return static_cast<const uint16_t*>(v.vals)[i];
+    default: return static_cast<const uint8_t*>(v.vals)[i];
+  }
+}
+
+__device__ __forceinline__ bool val_valid(const ValSource& v, int64_t i) {
+  return !v.valid || bit_get(v.valid, v.voff + i);
+}
+
+// Aggregate families. Each has W 64-bit words per group with an op per word, an identity per word,
+// and a per-row contribution (false when the row contributes nothing).
+struct AggArgs {
+  IdSource ids;
+  ValSource v;
+  int64_t n;
+  long long K;
+  int extra;                     // family-specific (float64 sums: the guard bits G)
+  const unsigned long long* aux; // family-specific (float64 sums: the pass-A table)
+};
+
+// count_all (values ignored) and count (non-null values).
+template <bool VALID_ONLY>
+struct CountP {
+  static constexpr int W = 1;
+  __host__ __device__ static constexpr int op(int) { return W_ADD; }
+  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[1]) {
+    if (VALID_ONLY && !val_valid(a.v, i)) return false;
+    c[0] = 1;
+    return true;
+  }
+};
+
+// Integer sum (wrapping) with the count of non-null values (null result where it is zero).
+struct SumCountP {
+  static constexpr int W = 2;
+  __host__ __device__ static constexpr int op(int) { return W_ADD; }
+  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
+    if (!val_valid(a.v, i)) return false;
+    c[0] = raw64(a.v, i);
+    c[1] = 1;
+    return true;
+  }
+};
+
+// min and max of an order-preserving unsigned key; NaN skipped. A group has a value exactly when
+// its min key <= its max key, which the identities (all ones, zero) never satisfy.
+struct MinMaxP {
+  static constexpr int W = 2;
+  __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MIN : W_MAX; }
+  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
+    if (!val_valid(a.v, i)) return false;
+    unsigned long long b = raw64(a.v, i), key;
+    switch (a.v.type) {
+      case T_F64:
+        if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return false;
+        key = (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+        break;
+      case T_F32: {
+        uint32_t x = uint32_t(b);
+        if ((x & 0x7FFFFFFFu) > 0x7F800000u) return false;
+        key = (x & 0x80000000u) ? uint32_t(~x) : (x | 0x80000000u);
+        break;
+      }
+      case T_I8: case T_I16: case T_I32: case T_I64: key = b ^ 0x8000000000000000ull; break;
+      default: key = b; break;
+    }
+    c[0] = key;
+    c[1] = key;
+    return true;
+  }
+};
+
+// Float64 exact sum, pass A: the largest biased exponent of the finite non-zero values, the special
+// values seen (bit 0 +inf, bit 1 -inf, bit 2 any NaN, bit 3 a +0.0), the count of non-null values and
+// the lowest row holding a NaN.
+struct F64AP {
+  static constexpr int W = 4;
+  __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MAX : (w == 1 ? W_OR : (w == 2 ? W_ADD : W_MIN)); }
+  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[4]) {
+    if (!val_valid(a.v, i)) return false;
+    unsigned long long v = raw64(a.v, i);
+    uint32_t e = uint32_t((v >> 52) & 0x7FF);
+    unsigned long long mag = v & 0x7FFFFFFFFFFFFFFFull;
+    c[0] = 0;
+    c[1] = 0;
+    c[2] = 1;
+    c[3] = ~0ull;
+    if (e == 0x7FF) {
+      if (mag == 0x7FF0000000000000ull) c[1] = (v >> 63) ? 2u : 1u;
+      else { c[1] = 4u; c[3] = (unsigned long long)i; }
+    } else if (mag == 0) {
+      if (v == 0) c[1] = 8u;
+    } else {
+      c[0] = e ? e : 1u;
+    }
+    return true;
+  }
+};
+
+constexpr uint32_t FX_SPECIAL = 7u;   // +inf, -inf or NaN: the result is not a rounded sum
+
+__device__ __forceinline__ uint32_t fx_window(uint32_t emax, uint32_t G) { return emax > G + 1u ? emax - G : 1u; }
+
+// The fixed-point integer of a finite non-zero value in a window whose unit has biased exponent U:
+// (lo, hi) magnitude; `dropped` when bits fell below the window.
+__device__ __forceinline__ void fx_fixed(unsigned long long v, uint32_t U, unsigned long long& lo,
+                                         unsigned long long& hi, bool& dropped) {
+  uint32_t e = uint32_t((v >> 52) & 0x7FF);
+  unsigned long long m = (v & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+  int sh = int(e ? e : 1u) - int(U);
+  dropped = false;
+  if (sh >= 64) { lo = 0; hi = m << (sh - 64); }
+  else if (sh > 0) { lo = m << sh; hi = m >> (64 - sh); }
+  else if (sh == 0) { lo = m; hi = 0; }
+  else {
+    int r = -sh;
+    hi = 0;
+    if (r >= 64) { lo = 0; dropped = true; }
+    else { lo = m >> r; dropped = (m & ((1ull << r) - 1ull)) != 0ull; }
+  }
+}
+
+// Float64 exact sum, pass B: the two's-complement 128-bit fixed-point sum in the group's window, and
+// the count of values truncated by the window. aux = the pass-A table (4 words per group).
+struct F64BP {
+  static constexpr int W = 3;
+  __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_ADD128 : (w == 1 ? W_HI : W_ADD); }
+  __device__ static bool contrib(const AggArgs& a, int64_t i, long long g, unsigned long long (&c)[3]) {
+    if (!val_valid(a.v, i)) return false;
+    unsigned long long v = raw64(a.v, i);
+    uint32_t e = uint32_t((v >> 52) & 0x7FF);
+    if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) return false;
+    const unsigned long long* st = a.aux + size_t(g) * 4;
+    if (st[1] & FX_SPECIAL) return false;
+    uint32_t U = fx_window(uint32_t(st[0]), uint32_t(a.extra));
+    unsigned long long lo, hi;
+    bool dropped;
+    fx_fixed(v, U, lo, hi, dropped);
+    if (v >> 63) {   // two's complement of the magnitude
+      lo = ~lo + 1ull;
+      hi = ~hi + (lo == 0 ? 1ull : 0ull);
+    }
+    c[0] = lo;
+    c[1] = hi;
+    c[2] = dropped ? 1u : 0u;
+    return (c[0] | c[1] | c[2]) != 0;
+  }
+};
+
+__device__ __forceinline__ unsigned long long identity(int op) { return op == W_MIN ? ~0ull : 0ull; }
+
+// One word update, LDS or global (the generic atomics pick the address space). MIN / MAX / OR read
+// first and skip the atomic when it cannot change anything, which keeps hot groups read-mostly.
+template <int OP>
+__device__ __forceinline__ void word_update(unsigned long long* p, unsigned long long x) {
+  if (OP == W_ADD) {
+    if (x) atomicAdd(p, x);
+  } else if (OP == W_MIN) {
+    if (x < *reinterpret_cast<volatile unsigned long long*>(p)) atomicMin(p, x);
+  } else if (OP == W_MAX) {
+    if (x > *reinterpret_cast<volatile unsigned long long*>(p)) atomicMax(p, x);
+  } else if (OP == W_OR) {
+    if (x & ~*reinterpret_cast<volatile unsigned long long*>(p)) atomicOr(p, x);
+  }
+}
+
+// 128-bit two's-complement add at p[0] (lo), p[1] (hi): exact modulo 2^128 under any interleaving.
+__device__ __forceinline__ void add128(unsigned long long* p, unsigned long long lo, unsigned long long hi) {
+  unsigned long long carry = 0;
+  if (lo) {
+    unsigned long long old = atomicAdd(&p[0], lo);
+    carry = (old + lo < old) ? 1ull : 0ull;
+  }
+  unsigned long long h = hi + carry;
+  if (h) atomicAdd(&p[1], h);
+}
+
+template <class P>
+__device__ __forceinline__ void apply(unsigned long long* acc, const unsigned long long (&c)[P::W]) {
+#pragma unroll
+  for (int w = 0; w < P::W; w++) {
+    const int op = P::op(w);
+    if (op == W_ADD) word_update<W_ADD>(&acc[w], c[w]);
+    else if (op == W_MIN) word_update<W_MIN>(&acc[w], c[w]);
+    else if (op == W_MAX) word_update<W_MAX>(&acc[w], c[w]);
+    else if (op == W_OR) word_update<W_OR>(&acc[w], c[w]);
+    else if (op == W_ADD128) add128(&acc[w], c[w], c[w + 1 < P::W ? w + 1 : w]);
+    // W_HI: added together with its W_ADD128 word
+  }
+}
+
+// Merges one LDS accumulator into the global table (skipping identities).
+template <class P>
+__device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsigned long long* lacc) {
+  unsigned long long c[P::W];
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < P::W; w++) {
+    c[w] = lacc[w];
+    any |= c[w] != identity(P::op(w));
+  }
+  if (any) apply<P>(gacc, c);
+}
+
+// Mode 0: every group has a private LDS accumulator (K <= cap). Mode 1: an LDS cache of `cap`
+// groups, claimed by CAS on a key word, in front of the global table; a row whose group cannot be
+// cached within 8 probes goes to the global table directly.
+template <class P>
+__global__ __launch_bounds__(256) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rows_per_block,
+                                              unsigned long long* gtab) {
+  extern __shared__ unsigned long long lds[];
+  unsigned long long* lacc = lds;                                     // [cap][W]
+  unsigned int* lkey = reinterpret_cast<unsigned int*>(lds + size_t(cap) * P::W);   // [cap] (mode 1)
+  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+#pragma unroll
+    for (int w = 0; w < P::W; w++) lacc[size_t(s) * P::W + w] = identity(P::op(w));
+    if (mode == 1) lkey[s] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  const int64_t begin = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t end = std::min<int64_t>(a.n, begin + rows_per_block);
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    long long g = load_gid(a.ids, i, a.K);
+    if (g < 0) continue;
+    unsigned long long c[P::W];
+    if (!P::contrib(a, i, g, c)) continue;
+    if (mode == 0) {
+      apply<P>(&lacc[size_t(g) * P::W], c);
+      continue;
+    }
+    uint32_t h = uint32_t(g) * 2654435761u;
+    uint32_t s = h & (cap - 1u);
+    bool done = false;
+    for (int probe = 0; probe < 8; probe++) {
+      unsigned int cur = lkey[s];
+      if (cur == 0xFFFFFFFFu) {
+        unsigned int prev = atomicCAS(&lkey[s], 0xFFFFFFFFu, uint32_t(g));
+        cur = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
+      }
+      if (cur == uint32_t(g)) {
+        apply<P>(&lacc[size_t(s) * P::W], c);
+        done = true;
+        break;
+      }
+      s = (s + 1u) & (cap - 1u);
+    }
+    if (!done) apply<P>(&gtab[size_t(g) * P::W], c);
+  }
+  __syncthreads();
+  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+    long long g;
+    if (mode == 0) g = s < uint32_t(a.K) ? (long long)s : -1;
+    else g = lkey[s] == 0xFFFFFFFFu ? -1 : (long long)lkey[s];
+    if (g >= 0) flush_one<P>(&gtab[size_t(g) * P::W], &lacc[size_t(s) * P::W]);
+  }
+}
+
+template <class P>
+__global__ void gb_agg_init(unsigned long long* gtab, long long K) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < K * P::W; i += stride) gtab[i] = identity(P::op(int(i % P::W)));
+}
+
+// Runs family P over the rows into a fresh global table of K x W words.
+template <class P>
+DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
+  DevBuf gtab = device_alloc(size_t(std::max<long long>(a.K, 1)) * P::W * 8);
+  auto* g = static_cast<unsigned long long*>(gtab->ptr);
+  hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(a.K * P::W, 4)), dim3(256), 0, s, g, a.K);
+  AM_HIP(hipGetLastError());
+  if (a.n == 0) return gtab;
+  // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
+  const size_t budget = 48 * 1024;
+  const uint32_t per_slot = P::W * 8;
+  uint32_t dense_cap = uint32_t(budget / per_slot);
+  int mode;
+  uint32_t cap;
+  if (a.K <= (long long)dense_cap) {
+    mode = 0;
+    cap = uint32_t(a.K);
+  } else {
+    mode = 1;
+    cap = 1;
+    while (cap * 2 * (per_slot + 4) <= budget) cap *= 2;
+  }
+  size_t shmem = size_t(cap) * per_slot + (mode == 1 ? size_t(cap) * 4 : 0);
+  int blocks = int(std::min<int64_t>((a.n + 4095) / 4096, int64_t(device().cu_count) * 4));
+  blocks = std::max(blocks, 1);
+  int64_t rpb = (a.n + blocks - 1) / blocks;
+  ProfScope ps(name);
+  hipLaunchKernelGGL(gb_agg<P>, dim3(blocks), dim3(256), shmem, s, a, mode, cap, rpb, g);
+  AM_HIP(hipGetLastError());
+  return gtab;
+}
+
+// ---------------------------------------------------------------- finalize kernels
+
+// Validity bytes -> Arrow bitmap (one ballot per 64 groups) and the null count.
+__global__ void pack_validity(const uint8_t* vb, long long K, unsigned long long* bits, unsigned long long* nulls) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i - lane < K; i += stride) {
+    bool v = i < K && vb[i];
+    unsigned long long bal = __ballot(v);
+    if (lane == 0) {
+      long long rows = std::min<long long>(64, K - (i - lane));
+      unsigned long long mask = rows >= 64 ? ~0ull : ((1ull << rows) - 1ull);
+      bits[(i - lane) >> 6] = bal & mask;
+      unsigned long long z = (unsigned long long)rows - __popcll(bal & mask);
+      if (z) atomicAdd(nulls, z);
+    }
+  }
+}
+
+// kind: 0 sum (int64 / uint64 bits), 1 integer mean (signed), 2 integer mean (unsigned),
+// 3 count, 4 min, 5 max (elements written at `width` bytes, from the unkeyed order key).
+__global__ void gb_finalize_simple(const unsigned long long* gtab, int W, long long K, int kind, int vtype,
+                                   void* out, uint8_t* vb) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const unsigned long long* t = gtab + k * W;
+  if (kind == 3) {
+    static_cast<unsigned long long*>(out)[k] = t[0];
+    vb[k] = 1;
+    return;
+  }
+  if (kind <= 2) {
+    unsigned long long s = t[0], c = t[1];
+    vb[k] = c != 0;
+    if (kind == 0) {
+      static_cast<unsigned long long*>(out)[k] = c ? s : 0ull;
+    } else {
+      double m = 0.0;
+      if (c) m = (kind == 1 ? double((long long)s) : double(s)) / double(c);
+      static_cast<double*>(out)[k] = m;
+    }
+    return;
+  }
+  unsigned long long mn = t[0], mx = t[1];
+  bool has = mn <= mx;
+  vb[k] = has;
+  unsigned long long key = kind == 4 ? mn : mx, v = 0;
+  if (has) {
+    switch (vtype) {
+      case T_F64: v = (key & 0x8000000000000000ull) ? (key & 0x7FFFFFFFFFFFFFFFull) : ~key; break;
+      case T_F32: {
+        uint32_t u = uint32_t(key);
+        v = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : uint32_t(~u);
+        break;
+      }
+      case T_I8: case T_I16: case T_I32: case T_I64: v = key ^ 0x8000000000000000ull; break;
+      default: v = key; break;
+    }
+  }
+  switch (type_width(vtype)) {
+    case 1: static_cast<uint8_t*>(out)[k] = uint8_t(v); break;
+    case 2: static_cast<uint16_t*>(out)[k] = uint16_t(v); break;
+    case 4: static_cast<uint32_t*>(out)[k] = uint32_t(v); break;
+    default: static_cast<unsigned long long*>(out)[k] = v; break;
+  }
+}
+
+// ---------------------------------------------------------------- exact float64 rounding
+
+// Round-to-nearest-even of (w[0..nw) + sticky) * 2^(lsb - 1074), w != 0, as binary64 bits.
+__device__ unsigned long long round_words(const unsigned long long* w, int nw, int lsb, bool sticky, bool neg) {
+  int top = -1;
+  for (int j = nw - 1; j >= 0; j--)
+    if (w[j]) { top = j * 64 + 63 - __clzll(w[j]); break; }
+  unsigned long long sgn = neg ? 0x8000000000000000ull : 0ull;
+  int P = top + lsb;
+  if (P >= 2098) return sgn | 0x7FF0000000000000ull;
+  int s = P >= 52 ? top - 52 : -lsb;
+  unsigned long long r;
+  if (s <= 0) {
+    r = w[0] << (-s);
+  } else {
+    int j = s >> 6, b = s & 63;
+    r = j < nw ? (w[j] >> b) : 0ull;
+    if (b && j + 1 < nw) r |= w[j + 1] << (64 - b);
+    int rbpos = s - 1;
+    bool rb = ((w[rbpos >> 6] >> (rbpos & 63)) & 1ull) != 0;
+    bool st = sticky;
+    for (int t = 0; t < (rbpos >> 6) && !st; t++) st = w[t] != 0;
+    int bb = rbpos & 63;
+    if (!st && bb) st = (w[rbpos >> 6] & ((1ull << bb) - 1ull)) != 0;
+    if (rb && (st || (r & 1ull))) r++;
+  }
+  unsigned long long bits = P >= 52 ? r + ((unsigned long long)(P - 52) << 52) : r;
+  if (bits >= 0x7FF0000000000000ull) bits = 0x7FF0000000000000ull;
+  return sgn | bits;
+}
+
+// The sum (cnt == 0) or the mean of the signed 128-bit W * 2^(lsb - 1074); zero gives +0.0.
+__device__ unsigned long long fx_value(unsigned long long lo, unsigned long long hi, int lsb, unsigned long long cnt) {
+  bool neg = (hi >> 63) != 0;
+  if (neg) {
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0 ? 1ull : 0ull);
+  }
+  if (lo == 0 && hi == 0) return 0ull;
+  if (cnt == 0) {
+    unsigned long long w[2] = {lo, hi};
+    return round_words(w, 2, lsb, false, neg);
+  }
+  // (W << 128) / cnt by long division over 32-bit digits; the remainder is sticky.
+  // cnt counts rows, so it is below 2^32 and every partial dividend fits 64 bits.
+  uint32_t d[8] = {0, 0, 0, 0, uint32_t(lo), uint32_t(lo >> 32), uint32_t(hi), uint32_t(hi >> 32)};
+  uint32_t q[8];
+  unsigned long long rem = 0;
+  for (int j = 7; j >= 0; j--) {
+    unsigned long long cur = (rem << 32) | d[j];
+    q[j] = uint32_t(cur / cnt);
+    rem = cur % cnt;
+  }
+  unsigned long long qw[4];
+  for (int j = 0; j < 4; j++) qw[j] = (unsigned long long)q[2 * j] | ((unsigned long long)q[2 * j + 1] << 32);
+  return round_words(qw, 4, lsb - 128, rem != 0, neg);
+}
+
+// One thread per group: the sum and the mean, their validity, and whether the window cannot settle
+// the rounding (then the group is summed again with the full-width accumulator below).
+__global__ void gb_f64_finalize(const unsigned long long* ta, const unsigned long long* tb, long long K, uint32_t G,
+                                const unsigned long long* vals, unsigned long long* out_sum,
+                                unsigned long long* out_mean, uint8_t* vb, uint32_t* redo, uint32_t* redo_count) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const unsigned long long* a = ta + k * 4;
+  const unsigned long long* b = tb + k * 3;
+  unsigned long long cnt = a[2];
+  uint32_t emax = uint32_t(a[0]);
+  unsigned long long flags = a[1];
+  redo[k] = 0xFFFFFFFFu;
+  if (cnt == 0) { out_sum[k] = 0; out_mean[k] = 0; vb[k] = 0; return; }
+  vb[k] = 1;
+  if (flags & 4u) {                     // any NaN: the first NaN in row order, quieted
+    unsigned long long r = vals[a[3]] | (1ull << 51);
+    out_sum[k] = r; out_mean[k] = r;
+    return;
+  }
+  if (flags & 3u) {
+    unsigned long long r = (flags & 3u) == 3u ? 0x7FF8000000000000ull
+                         : ((flags & 1u) ? 0x7FF0000000000000ull : 0xFFF0000000000000ull);
+    out_sum[k] = r; out_mean[k] = r;
+    return;
+  }
+  if (emax == 0) {                      // every value is a zero
+    unsigned long long z = (flags & 8u) ? 0ull : 0x8000000000000000ull;
+    out_sum[k] = z; out_mean[k] = z;
+    return;
+  }
+  unsigned long long lo = b[0], hi = b[1], drops = b[2];
+  int lsb = int(fx_window(emax, G)) - 1;
+  if (drops == 0) {
+    out_sum[k] = fx_value(lo, hi, lsb, 0);
+    out_mean[k] = fx_value(lo, hi, lsb, cnt);
+    return;
+  }
+  // The exact sum lies strictly between W - drops and W + drops units.
+  unsigned long long dlo = lo - drops, dhi = hi - (lo < drops ? 1ull : 0ull);
+  unsigned long long ulo = lo + drops, uhi = hi + (ulo < lo ? 1ull : 0ull);
+  unsigned long long s0 = fx_value(dlo, dhi, lsb, 0), s1 = fx_value(ulo, uhi, lsb, 0);
+  unsigned long long m0 = fx_value(dlo, dhi, lsb, cnt), m1 = fx_value(ulo, uhi, lsb, cnt);
+  bool z0 = dlo == 0 && dhi == 0, z1 = ulo == 0 && uhi == 0;
+  if (s0 == s1 && m0 == m1 && !z0 && !z1) { out_sum[k] = s0; out_mean[k] = m0; return; }
+  redo[k] = atomicAdd(redo_count, 1u);
+}
+
+// Full-width exact accumulation for the groups the window could not settle: 72 signed 64-bit digits
+// per group, digit j holding the sum of the value bits at [32 j, 32 j + 32) (bit 0 weighs 2^-1074).
+// Each value touches three digits and no digit can overflow for fewer than 2^31 rows.
+constexpr int XD = 72;
+
+__global__ void gb_f64_redo_rows(AggArgs a, const uint32_t* redo, long long* digits) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (; i < a.n; i += stride) {
+    long long g = load_gid(a.ids, i, a.K);
+    if (g < 0) continue;
+    uint32_t slot = redo[g];
+    if (slot == 0xFFFFFFFFu || !val_valid(a.v, i)) continue;
+    unsigned long long v = raw64(a.v, i);
+    uint32_t e = uint32_t((v >> 52) & 0x7FF);
+    if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) continue;
+    unsigned long long m = (v & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
+    uint32_t off = (e ? e : 1u) - 1u;
+    uint32_t d0 = off >> 5, sh = off & 31u;
+    unsigned long long lo64 = m << sh, hi64 = sh ? (m >> (64 - sh)) : 0ull;
+    long long p0 = (long long)(lo64 & 0xFFFFFFFFull), p1 = (long long)(lo64 >> 32), p2 = (long long)hi64;
+    if (v >> 63) { p0 = -p0; p1 = -p1; p2 = -p2; }
+    long long* D = digits + size_t(slot) * XD + d0;
+    if (p0) atomicAdd(reinterpret_cast<unsigned long long*>(&D[0]), (unsigned long long)p0);
+    if (p1) atomicAdd(reinterpret_cast<unsigned long long*>(&D[1]), (unsigned long long)p1);
+    if (p2) atomicAdd(reinterpret_cast<unsigned long long*>(&D[2]), (unsigned long long)p2);
+  }
+}
+
+__global__ void gb_f64_redo_finalize(const uint32_t* redo, long long K, const long long* digits,
+                                     const unsigned long long* ta, unsigned long long* out_sum,
+                                     unsigned long long* out_mean) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K || redo[k] == 0xFFFFFFFFu) return;
+  const long long* D = digits + size_t(redo[k]) * XD;
+  // Normalise the carry-save digits into a two's-complement number of 36 64-bit words.
+  constexpr int NW = XD / 2;
+  unsigned long long w[NW];
+  long long carry = 0;
+  for (int j = 0; j < XD; j++) {
+    // value of digit j plus the carry, split into the low 32 bits and an arithmetic carry
+    long long x = D[j] + carry;            // |x| < 2^63 for fewer than 2^31 rows
+    uint32_t low = uint32_t(x);
+    carry = (x - (long long)low) >> 32;   // exact: x - low is a multiple of 2^32
+    if (j & 1) w[j >> 1] |= (unsigned long long)low << 32;
+    else w[j >> 1] = low;
+  }
+  bool neg = carry < 0;
+  if (neg) {
+    unsigned long long c = 1;
+    for (int j = 0; j < NW; j++) {
+      unsigned long long x = ~w[j] + c;
+      c = (c && x == 0) ? 1ull : 0ull;
+      w[j] = x;
+    }
+  }
+  bool zero = true;
+  for (int j = 0; j < NW; j++) zero = zero && w[j] == 0;
+  if (zero) { out_sum[k] = 0; out_mean[k] = 0; return; }
+  out_sum[k] = round_words(w, NW, 0, false, neg);
+  // Mean: (W << 128) / cnt over 32-bit digits, highest first.
+  unsigned long long cnt = ta[k * 4 + 2];
+  unsigned long long q[NW + 2];
+  for (int j = 0; j < NW + 2; j++) q[j] = 0;
+  unsigned long long rem = 0;
+  for (int j = 2 * (NW + 2) - 1; j >= 0; j--) {
+    uint32_t dj = (j < 4) ? 0u : uint32_t(w[(j - 4) >> 1] >> (((j - 4) & 1) * 32));
+    unsigned long long cur = (rem << 32) | dj;
+    unsigned long long qq = cur / cnt;
+    rem = cur % cnt;
+    q[j >> 1] |= (qq & 0xFFFFFFFFull) << ((j & 1) * 32);
+  }
+  out_mean[k] = round_words(q, NW + 2, -128, rem != 0, neg);
+}
+
+}  // namespace
+
+// ================================================================ host: key mapping
+
+struct GroupBy {
+  std::vector<Array*> columns;
+  int64_t n = 0;
+  int64_t K = 0;
+  DevBuf ids;     // int32 [n]
+  DevBuf rep;     // uint32 [K]
+  std::vector<Array*> key_results;
+  // A min and a max of one column share a pass; a sum and a mean of one float64 column share theirs.
+  const Array* mm_col = nullptr;
+  DevBuf mm_tab;
+  const Array* fx_col = nullptr;
+  DevBuf fx_sum, fx_mean, fx_valid;
+  ~GroupBy() {
+    for (Array* a : columns) release_array(a);
+    for (Array* a : key_results) release_array(a);
+  }
+};
+
+namespace {
+
+// Maps one key column (values, type, validity) to dense ascending ids. Returns K and fills `ids`
+// (int32 [n]) and `rep` (uint32 [K]).
+int64_t map_column(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, DevBuf& ids_out,
+                   DevBuf& rep_out) {
+  hipStream_t s = stream();
+  uint32_t un = uint32_t(n);
+  DevBuf ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
+  DevBuf stb = device_alloc(sizeof(BuildStatus));
+  auto* st = static_cast<BuildStatus*>(stb->ptr);
+  // Capacity: two slots per row up to 2^21 slots; beyond that, grow on overflow.
+  uint64_t want = std::max<uint64_t>(1024, 2 * uint64_t(n));
+  uint64_t cap = 1024;
+  while (cap < want && cap < (1ull << 21)) cap <<= 1;
+  BuildStatus host{};
+  DevBuf table;
+  for (int attempt = 0;; attempt++) {
+    bool last_try = cap >= want;
+    table = device_alloc(size_t(cap + 1) * sizeof(Slot));
+    auto* T = static_cast<Slot*>(table->ptr);
+    hipLaunchKernelGGL(gb_table_init, dim3(grid_for(int64_t(cap + 1), 4)), dim3(256), 0, s, T, size_t(cap + 1), st);
+    AM_HIP(hipGetLastError());
+    {
+      ProfScope ps("gb_build");
+      hipLaunchKernelGGL(gb_build, dim3(grid_for(n)), dim3(256), 0, s, vals, type, valid, voff, un, T,
+                         uint32_t(cap - 1), last_try ? uint32_t(cap) : 128u, static_cast<uint32_t*>(ids->ptr), st);
+      AM_HIP(hipGetLastError());
+    }
+    AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    if (!host.overflow) break;
+    if (attempt > 8) fail("group-by: the key table could not be sized");
+    cap = std::min<uint64_t>(cap * 8, [&] { uint64_t c = 1024; while (c < want) c <<= 1; return c; }());
+  }
+  auto* T = static_cast<Slot*>(table->ptr);
+  // Compact the occupied slots and sort their keys.
+  DevBuf okeys = device_alloc(size_t(cap + 1) * 8), slots = device_alloc(size_t(cap + 1) * 4);
+  {
+    ProfScope ps("gb_compact");
+    hipLaunchKernelGGL(gb_compact, dim3(uint32_t((cap + 1 + 255) / 256)), dim3(256), 0, s, T, uint32_t(cap), type, st,
+                       static_cast<unsigned long long*>(okeys->ptr), static_cast<uint32_t*>(slots->ptr));
+    AM_HIP(hipGetLastError());
+  }
+  AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  uint32_t D = host.distinct;
+  bool has_null = host.nulls > 0;
+  int64_t K = int64_t(D) + (has_null ? 1 : 0);
+  DevBuf perm = device_alloc(size_t(std::max<uint32_t>(D, 1)) * 4);
+  if (D > 0)
+    argsort_device(okeys->ptr, T_U64, nullptr, 0, D, false, 0, 1, static_cast<uint32_t*>(perm->ptr), s);
+  DevBuf gid_of_slot = device_alloc(size_t(cap + 1) * 4);
+  DevBuf rep = device_alloc(size_t(std::max<int64_t>(K, 1)) * 4);
+  {
+    ProfScope ps("gb_relabel");
+    hipLaunchKernelGGL(gb_relabel, dim3(uint32_t((D + 1 + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(perm->ptr), static_cast<const uint32_t*>(slots->ptr), D, T,
+                       static_cast<uint32_t*>(gid_of_slot->ptr), static_cast<uint32_t*>(rep->ptr), st,
+                       has_null ? 1 : 0);
+    AM_HIP(hipGetLastError());
+  }
+  if (n > 0) {
+    ProfScope ps("gb_ids");
+    hipLaunchKernelGGL(gb_ids, dim3(grid_for(n)), dim3(256), 0, s, static_cast<uint32_t*>(ids->ptr), un,
+                       static_cast<const uint32_t*>(gid_of_slot->ptr), D);
+    AM_HIP(hipGetLastError());
+  }
+  AM_HIP(hipStreamSynchronize(s));
+  ids_out = ids;
+  rep_out = rep;
+  return K;
+}
+
+}  // namespace
+
+GroupBy* group_by_keys(Array* const* columns, int64_t count) {
+  if (count < 1) fail("group-by needs at least one key column");
+  int64_t n = columns[0]->length;
+  for (int64_t c = 1; c < count; c++)
+    if (columns[c]->length != n)
+      fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(columns[c]->length));
+  check_length(n);
+  for (int64_t c = 0; c < count; c++) {
+    int t = columns[c]->type;
+    if (t == T_UNSUPPORTED || t == T_BOOL)
+      fail("group-by over a " + columns[c]->format + " key column is not supported by this build");
+  }
+  auto* g = new GroupBy();
+  try {
+    for (int64_t c = 0; c < count; c++) {
+      retain_array(columns[c]);
+      g->columns.push_back(columns[c]);
+    }
+    g->n = n;
+    if (n == 0) {
+      g->K = 0;
+      g->ids = device_alloc(4);
+      g->rep = device_alloc(4);
+      return g;
+    }
+    hipStream_t s = stream();
+    const Array* a0 = columns[0];
+    int64_t K = map_column(a0->data(), a0->type, a0->has_nulls() ? a0->valid_bits() : nullptr, a0->offset, n, g->ids,
+                           g->rep);
+    // Further columns fold pairwise: ids a (K groups) and b (K2) become the key a * K2 + b, which is
+    // injective and ascending in (a, b), and that key is mapped again (GroupByKeys.swift:103-112).
+    for (int64_t c = 1; c < count; c++) {
+      const Array* ac = columns[c];
+      DevBuf ids2, rep2;
+      int64_t K2 = map_column(ac->data(), ac->type, ac->has_nulls() ? ac->valid_bits() : nullptr, ac->offset, n,
+                              ids2, rep2);
+      if (K2 > 0 && K > INT64_MAX / K2)
+        fail("multi-column group-by overflowed the 64-bit composite key (" + std::to_string(K) + " x " +
+             std::to_string(K2) + " distinct)");
+      DevBuf comp = device_alloc(size_t(n) * 8);
+      hipLaunchKernelGGL(gb_combine, dim3(grid_for(n)), dim3(256), 0, s, static_cast<const int32_t*>(g->ids->ptr),
+                         static_cast<const int32_t*>(ids2->ptr), uint32_t(n), (long long)K2,
+                         static_cast<long long*>(comp->ptr));
+      AM_HIP(hipGetLastError());
+      K = map_column(comp->ptr, T_I64, nullptr, 0, n, g->ids, g->rep);
+    }
+    if (K > int64_t(INT32_MAX)) fail("group-by: more than 2^31 groups");
+    g->K = K;
+    sync();
+  } catch (...) {
+    delete g;
+    throw;
+  }
+  return g;
+}
+
+int64_t group_by_group_count(const GroupBy* g) { return g->K; }
+
+Array* group_by_ids(GroupBy* g) {
+  Array* out = new_array("i", T_I32, g->n);
+  out->values = g->ids;
+  out->null_count = 0;
+  return out;
+}
+
+Array* group_by_keys_result(GroupBy* g, int64_t i) {
+  if (i < 0 || i >= int64_t(g->columns.size()))
+    fail("key column " + std::to_string(i) + " of " + std::to_string(g->columns.size()));
+  if (g->key_results.empty()) g->key_results.assign(g->columns.size(), nullptr);
+  if (!g->key_results[size_t(i)]) {
+    Array* rep = new_array("I", T_U32, g->K);
+    rep->values = g->rep;
+    rep->null_count = 0;
+    Array* keys = nullptr;
+    try {
+      keys = take(*g->columns[size_t(i)], *rep);
+    } catch (...) {
+      release_array(rep);
+      throw;
+    }
+    release_array(rep);
+    g->key_results[size_t(i)] = keys;
+  }
+  Array* r = g->key_results[size_t(i)];
+  retain_array(r);
+  return r;
+}
+
+void group_by_release(GroupBy* g) { delete g; }
+
+// ================================================================ host: aggregates
+
+namespace {
+
+Array* finalize_simple(const DevBuf& gtab, int W, long long K, int kind, int vtype, const std::string& fmt,
+                       int otype) {
+  hipStream_t s = stream();
+  Array* out = new_array(fmt, otype, K);
+  try {
+    out->values = device_alloc(size_t(std::max<long long>(K, 1)) * std::max(type_width(otype), 1));
+    DevBuf vb = device_alloc(size_t(std::max<long long>(K, 1)));
+    if (K > 0) {
+      hipLaunchKernelGGL(gb_finalize_simple, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                         static_cast<const unsigned long long*>(gtab->ptr), W, K, kind, vtype, out->values->ptr,
+                         static_cast<uint8_t*>(vb->ptr));
+      AM_HIP(hipGetLastError());
+    }
+    if (kind != 3 && K > 0) {
+      out->validity = device_alloc(size_t((K + 63) / 64) * 8);
+      DevBuf nz = device_alloc(8);
+      hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, static_cast<unsigned long long*>(nz->ptr), 1);
+      hipLaunchKernelGGL(pack_validity, dim3(grid_for(K, 4)), dim3(256), 0, s, static_cast<const uint8_t*>(vb->ptr), K,
+                         static_cast<unsigned long long*>(out->validity->ptr), static_cast<unsigned long long*>(nz->ptr));
+      AM_HIP(hipGetLastError());
+      unsigned long long nulls = 0;
+      AM_HIP(hipMemcpyAsync(&nulls, nz->ptr, 8, hipMemcpyDeviceToHost, s));
+      AM_HIP(hipStreamSynchronize(s));
+      out->null_count = int64_t(nulls);
+      if (nulls == 0) out->validity.reset();
+    } else {
+      out->null_count = 0;
+      sync();
+    }
+  } catch (...) {
+    release_array(out);
+    throw;
+  }
+  return out;
+}
+
+AggArgs make_args(const IdSource& ids, const Array* v, int64_t n, long long K) {
+  AggArgs a{};
+  a.ids = ids;
+  if (v) {
+    a.v.vals = v->data();
+    a.v.type = v->type;
+    a.v.valid = v->has_nulls() ? v->valid_bits() : nullptr;
+    a.v.voff = v->offset;
+  }
+  a.n = n;
+  a.K = K;
+  return a;
+}
+
+int guard_bits(int64_t rows) {
+  int bits = rows <= 0 ? 1 : 64 - __builtin_clzll(uint64_t(rows));
+  return 74 - bits;
+}
+
+// Correctly rounded float64 sum and mean per group, with their validity bytes.
+void exact_f64(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
+  hipStream_t s = stream();
+  long long K = base.K;
+  AggArgs a = base;
+  a.extra = guard_bits(base.n);
+  DevBuf ta = run_agg<F64AP>(a, s, "gb_f64_a");
+  a.aux = static_cast<const unsigned long long*>(ta->ptr);
+  DevBuf tb = run_agg<F64BP>(a, s, "gb_f64_b");
+  size_t k1 = size_t(std::max<long long>(K, 1));
+  sum = device_alloc(k1 * 8);
+  mean = device_alloc(k1 * 8);
+  vb = device_alloc(k1);
+  DevBuf redo = device_alloc(k1 * 4), rc = device_alloc(8);
+  hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, static_cast<unsigned long long*>(rc->ptr), 1);
+  if (K > 0) {
+    ProfScope ps("gb_f64_finalize");
+    hipLaunchKernelGGL(gb_f64_finalize, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const unsigned long long*>(ta->ptr), static_cast<const unsigned long long*>(tb->ptr),
+                       K, uint32_t(a.extra), static_cast<const unsigned long long*>(base.v.vals),
+                       static_cast<unsigned long long*>(sum->ptr), static_cast<unsigned long long*>(mean->ptr),
+                       static_cast<uint8_t*>(vb->ptr), static_cast<uint32_t*>(redo->ptr),
+                       static_cast<uint32_t*>(rc->ptr));
+    AM_HIP(hipGetLastError());
+  }
+  uint32_t nredo = 0;
+  AM_HIP(hipMemcpyAsync(&nredo, rc->ptr, 4, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  if (nredo > 0) {
+    if (base.n >= (int64_t(1) << 31)) fail("float64 group sum: more than 2^31 rows in an unsettled group");
+    DevBuf digits = device_alloc(size_t(nredo) * XD * 8);
+    hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(grid_for(int64_t(nredo) * XD, 4)), dim3(256), 0, s,
+                       static_cast<unsigned long long*>(digits->ptr), (long long)nredo * XD);
+    ProfScope ps("gb_f64_redo");
+    hipLaunchKernelGGL(gb_f64_redo_rows, dim3(grid_for(base.n)), dim3(256), 0, s, a,
+                       static_cast<const uint32_t*>(redo->ptr), static_cast<long long*>(digits->ptr));
+    hipLaunchKernelGGL(gb_f64_redo_finalize, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(redo->ptr), K, static_cast<const long long*>(digits->ptr),
+                       static_cast<const unsigned long long*>(ta->ptr), static_cast<unsigned long long*>(sum->ptr),
+                       static_cast<unsigned long long*>(mean->ptr));
+    AM_HIP(hipGetLastError());
+    AM_HIP(hipStreamSynchronize(s));
+  }
+}
+
+Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::string& fmt, int type) {
+  hipStream_t s = stream();
+  Array* out = new_array(fmt, type, K);
+  out->values = vals;
+  out->null_count = 0;
+  if (K > 0) {
+    try {
+      out->validity = device_alloc(size_t((K + 63) / 64) * 8);
+      DevBuf nz = device_alloc(8);
+      hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, static_cast<unsigned long long*>(nz->ptr), 1);
+      hipLaunchKernelGGL(pack_validity, dim3(grid_for(K, 4)), dim3(256), 0, s, static_cast<const uint8_t*>(vb->ptr), K,
+                         static_cast<unsigned long long*>(out->validity->ptr), static_cast<unsigned long long*>(nz->ptr));
+      AM_HIP(hipGetLastError());
+      unsigned long long nulls = 0;
+      AM_HIP(hipMemcpyAsync(&nulls, nz->ptr, 8, hipMemcpyDeviceToHost, s));
+      AM_HIP(hipStreamSynchronize(s));
+      out->null_count = int64_t(nulls);
+      if (nulls == 0) out->validity.reset();
+    } catch (...) {
+      release_array(out);
+      throw;
+    }
+  }
+  return out;
+}
+
+// The shared op table of am_group_agg_ex and am_group_by over an id source.
+Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, int op, GroupBy* cache) {
+  hipStream_t s = stream();
+  if (op == 1) {
+    AggArgs a = make_args(ids, nullptr, n, K);
+    DevBuf t = run_agg<CountP<false>>(a, s, "gb_count");
+    return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
+  }
+  if (!v) fail("am_group_agg_ex op " + std::to_string(op) + " needs a value column");
+  if (v->length != n) fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(v->length));
+  if (v->type == T_UNSUPPORTED || v->type == T_BOOL)
+    fail("grouped aggregate over a " + v->format + " column is not supported by this build");
+  AggArgs a = make_args(ids, v, n, K);
+  if (op == 2) {
+    DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
+    return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
+  }
+  if (op == 0 || op == 3) {
+    if (v->type == T_F64) {
+      DevBuf sum, mean, vb;
+      if (cache && cache->fx_col == v) {
+        sum = cache->fx_sum; mean = cache->fx_mean; vb = cache->fx_valid;
+      } else {
+        exact_f64(a, sum, mean, vb);
+        if (cache) { cache->fx_col = v; cache->fx_sum = sum; cache->fx_mean = mean; cache->fx_valid = vb; }
+      }
+      return bits_array(op == 0 ? sum : mean, vb, K, "g", T_F64);
+    }
+    if (v->type == T_F32) fail("float32 grouped sum / mean is not supported by this build");
+    DevBuf t = run_agg<SumCountP>(a, s, "gb_sum");
+    if (op == 0) {
+      bool u64 = v->type == T_U64;
+      return finalize_simple(t, 2, K, 0, v->type, u64 ? "L" : "l", u64 ? T_U64 : T_I64);
+    }
+    return finalize_simple(t, 2, K, type_is_signed(v->type) ? 1 : 2, v->type, "g", T_F64);
+  }
+  if (op == 4 || op == 5) {
+    DevBuf t;
+    if (cache && cache->mm_col == v) {
+      t = cache->mm_tab;
+    } else {
+      t = run_agg<MinMaxP>(a, s, "gb_minmax");
+      if (cache) { cache->mm_col = v; cache->mm_tab = t; }
+    }
+    return finalize_simple(t, 2, K, op, v->type, v->format, v->type);
+  }
+  fail("am_group_agg_ex op " + std::to_string(op) + " is not supported by this build");
+}
+
+}  // namespace
+
+Array* group_agg(GroupBy* g, const Array* values, int op, double /*p1*/) {
+  IdSource ids{g->ids->ptr, T_I32, nullptr, 0};
+  if (op != 1 && values && values->length != g->n)
+    fail("length mismatch: " + std::to_string(g->n) + " vs " + std::to_string(values->length));
+  // GroupBy refuses a zero key count, so an empty mapping aggregates over one empty group and the
+  // result is trimmed to zero rows (GroupByKeys.swift:59-76, 159-165).
+  long long K = std::max<int64_t>(g->K, 1);
+  Array* r = aggregate(ids, g->n, K, values, op, g);
+  if (g->K == 0) { r->length = 0; r->null_count = 0; r->validity.reset(); }
+  return r;
+}
+
+Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values) {
+  if (key_count <= 0 || key_count > int64_t(UINT32_MAX) / 4) fail("keyCount out of range");
+  if (keys.type != T_I32 && keys.type != T_I64 && keys.type != T_U32)
+    fail("am_group_by: keys must be int32, int64 or uint32, got " + keys.format);
+  check_length(keys.length);
+  IdSource ids{keys.data(), keys.type, keys.has_nulls() ? keys.valid_bits() : nullptr, keys.offset};
+  // agg: 0 sum, 1 count rows, 2 min, 3 max, 4 mean, 5 count values (ArrowMetalC.swift:416-469).
+  static const int op_of[6] = {0, 1, 4, 5, 3, 2};
+  if (agg < 0 || agg > 5) fail("am_group_by: unknown agg " + std::to_string(agg));
+  int op = op_of[agg];
+  if ((agg == 2 || agg == 3) && values && values->width() > 4)
+    fail("group-by min/max needs a 32-bit or narrower type (64-bit atomics are not available); cast first");
+  if (agg == 1) values = nullptr;
+  return aggregate(ids, keys.length, key_count, values, op, nullptr);
+}
+
+Array* unique_rows(Array* const* columns, int64_t count) {
+  GroupBy* g = group_by_keys(columns, count);
+  Array* out = nullptr;
+  try {
+    // The representative rows sorted ascending: keep-first order (Executor.swift:134-148).
+    out = new_array("I", T_U32, g->K);
+    out->values = device_alloc(size_t(std::max<int64_t>(g->K, 1)) * 4);
+    out->null_count = 0;
+    if (g->K > 0)
+      argsort_device(g->rep->ptr, T_U32, nullptr, 0, g->K, false, 0, 0, static_cast<uint32_t*>(out->values->ptr),
+                     stream());
+    // argsort gives the permutation; the rows themselves are rep[perm]: gather them.
+    if (g->K > 0) {
+      Array* perm = out;
+      Array* rep = new_array("I", T_U32, g->K);
+      rep->values = g->rep;
+      out = take(*rep, *perm);
+      release_array(rep);
+      release_array(perm);
+    } else {
+      sync();
+    }
+  } catch (...) {
+    if (out) release_array(out);
+    group_by_release(g);
+    throw;
+  }
+  group_by_release(g);
+  return out;
+}
+
+}  // namespace am

@@ -1,0 +1,599 @@
+// Runtime of libArrowMetalC: device, streams, caching allocator, arrays, Arrow C Data import/export,
+// errors, batching and kernel-time profiling.
+//
+// The reference's runtime (Sources/ArrowMetal/MetalContext.swift, MetalArrowBuffer.swift,
+// CInterop.swift) wraps Apple unified memory, where an imported array is visible to the GPU without a
+// copy. Here host memory and HBM are separate: am_import copies a CPU array into HBM once, and
+// am_import_device takes ROCm device buffers without a copy. Work runs on one HIP stream per calling
+// thread; every call is synchronous unless the thread has a batch open (MetalContext.swift:245-312).
+#include "am_internal.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <unordered_map>
+
+namespace am {
+
+// ---------------------------------------------------------------- errors
+
+namespace {
+thread_local std::string t_last_error;
+}
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+const char* last_error_cstr() { return t_last_error.c_str(); }
+
+[[noreturn]] void fail(const std::string& msg) { throw Error(msg); }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") in " + what);
+  }
+}
+
+// ---------------------------------------------------------------- types
+
+int type_of_format(const std::string& f) {
+  if (f == "c") return T_I8;
+  if (f == "C") return T_U8;
+  if (f == "s") return T_I16;
+  if (f == "S") return T_U16;
+  if (f == "i") return T_I32;
+  if (f == "I") return T_U32;
+  if (f == "l") return T_I64;
+  if (f == "L") return T_U64;
+  if (f == "f") return T_F32;
+  if (f == "g") return T_F64;
+  // Temporal types compute on their storage integers (Sources/ArrowMetal/Kernels/MultiSort.swift:76-113).
+  if (f == "tdD" || f == "tts" || f == "ttm" || f == "tiM") return T_I32;
+  if (f == "tdm" || f == "ttu" || f == "ttn") return T_I64;
+  if (f.size() >= 4 && f[0] == 't' && f[1] == 's' && f[3] == ':') return T_I64;   // timestamps
+  if (f.size() == 3 && f[0] == 't' && f[1] == 'D') return T_I64;                  // durations
+  return T_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------- device
+
+namespace {
+std::once_flag g_dev_once;
+DeviceInfo g_dev;
+}
+
+const DeviceInfo& device() {
+  std::call_once(g_dev_once, [] {
+    int id = 0;
+    AM_HIP(hipGetDevice(&id));
+    hipDeviceProp_t p;
+    AM_HIP(hipGetDeviceProperties(&p, id));
+    g_dev.id = id;
+    g_dev.cu_count = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    g_dev.lds_per_block = p.sharedMemPerBlock;
+    g_dev.name = std::string(p.name) + " (" + p.gcnArchName + ", " + std::to_string(g_dev.cu_count) + " CUs)";
+  });
+  return g_dev;
+}
+
+// ---------------------------------------------------------------- caching allocator
+
+namespace {
+
+struct Pool {
+  std::mutex mu;
+  std::map<size_t, std::vector<void*>> free_lists;   // size class -> cached blocks
+  size_t in_use = 0, cached = 0;
+};
+Pool& pool() {
+  static Pool* p = new Pool();   // never destroyed: blocks may be freed during process exit
+  return *p;
+}
+
+size_t size_class(size_t bytes) {
+  if (bytes <= 4096) return (bytes + 255) & ~size_t(255);
+  size_t p = 4096;
+  while (p < bytes) {
+    if (p + p / 2 >= bytes) return p + p / 2;
+    p <<= 1;
+  }
+  return p;
+}
+
+void pool_free(void* ptr, size_t cls) {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  P.free_lists[cls].push_back(ptr);
+  P.in_use -= cls;
+  P.cached += cls;
+}
+
+}  // namespace
+
+DevBlock::~DevBlock() {
+  if (!ptr) return;
+  if (on_free) on_free(ptr);
+  else pool_free(ptr, bytes);
+}
+
+DevBuf device_alloc(size_t bytes) {
+  size_t cls = size_class(bytes ? bytes : 1);
+  Pool& P = pool();
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    auto it = P.free_lists.find(cls);
+    if (it != P.free_lists.end() && !it->second.empty()) {
+      ptr = it->second.back();
+      it->second.pop_back();
+      P.cached -= cls;
+      P.in_use += cls;
+    }
+  }
+  if (!ptr) {
+    device();
+    hipError_t e = hipMalloc(&ptr, cls);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      pool_trim();
+      e = hipMalloc(&ptr, cls);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail("out of device memory allocating " + std::to_string(bytes) + " bytes");
+      }
+    }
+    std::lock_guard<std::mutex> g(P.mu);
+    P.in_use += cls;
+  }
+  auto b = std::make_shared<DevBlock>();
+  b->ptr = ptr;
+  b->bytes = cls;
+  retain(b);   // inside a batch, scratch outlives the queued kernels that use it
+  return b;
+}
+
+DevBuf device_wrap(void* ptr, size_t bytes, std::function<void(void*)> release) {
+  auto b = std::make_shared<DevBlock>();
+  b->ptr = ptr;
+  b->bytes = bytes;
+  b->on_free = release ? std::move(release) : [](void*) {};
+  return b;
+}
+
+size_t pool_bytes_in_use() {
+  std::lock_guard<std::mutex> g(pool().mu);
+  return pool().in_use;
+}
+size_t pool_bytes_cached() {
+  std::lock_guard<std::mutex> g(pool().mu);
+  return pool().cached;
+}
+void pool_trim() {
+  Pool& P = pool();
+  std::map<size_t, std::vector<void*>> lists;
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    lists.swap(P.free_lists);
+    P.cached = 0;
+  }
+  for (auto& kv : lists)
+    for (void* p : kv.second) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------- per-thread stream and batch
+
+namespace {
+
+struct ThreadState {
+  hipStream_t s = nullptr;
+  int batch_depth = 0;
+  std::vector<DevBuf> retained;
+  std::vector<std::function<void()>> deferred;
+  ~ThreadState() {
+    // Streams are left to the driver at process exit; a live thread's stream is reused.
+    if (s && batch_depth == 0) (void)hipStreamDestroy(s);
+  }
+};
+thread_local ThreadState t_state;
+
+// Profiling state.
+struct ProfEntry { int64_t count = 0; double ms = 0; };
+std::atomic<bool> g_prof_on{false};
+std::mutex g_prof_mu;
+std::map<std::string, ProfEntry> g_prof;
+struct PendingTiming { const char* name; hipEvent_t a, b; };
+thread_local std::vector<PendingTiming> t_pending;
+
+void drain_prof() {
+  for (auto& p : t_pending) {
+    float ms = 0;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      std::lock_guard<std::mutex> g(g_prof_mu);
+      auto& e = g_prof[p.name];
+      e.count++;
+      e.ms += ms;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  t_pending.clear();
+}
+
+}  // namespace
+
+hipStream_t stream() {
+  if (!t_state.s) {
+    device();
+    AM_HIP(hipStreamCreateWithFlags(&t_state.s, hipStreamNonBlocking));
+  }
+  return t_state.s;
+}
+
+bool in_batch() { return t_state.batch_depth > 0; }
+
+void retain(const DevBuf& b) {
+  if (t_state.batch_depth > 0 && b) t_state.retained.push_back(b);
+}
+
+void after_sync(std::function<void()> f) { t_state.deferred.push_back(std::move(f)); }
+
+static void run_deferred() {
+  auto fs = std::move(t_state.deferred);
+  t_state.deferred.clear();
+  std::string first_error;
+  for (auto& f : fs) {
+    try {
+      f();
+    } catch (const Error& e) {
+      if (first_error.empty()) first_error = e.what();
+    }
+  }
+  if (!first_error.empty()) fail(first_error);
+}
+
+void sync() {
+  if (t_state.batch_depth > 0) return;
+  AM_HIP(hipStreamSynchronize(stream()));
+  if (!t_pending.empty()) drain_prof();
+  run_deferred();
+}
+
+void batch_begin() { t_state.batch_depth++; }
+
+void batch_end() {
+  if (t_state.batch_depth == 0) fail("am_batch_end without am_batch_begin");
+  if (--t_state.batch_depth > 0) return;
+  hipError_t e = hipStreamSynchronize(stream());
+  t_state.retained.clear();
+  if (!t_pending.empty()) drain_prof();
+  AM_HIP(e);
+  run_deferred();
+}
+
+ProfScope::ProfScope(const char* name) : name_(name) {
+  if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  if (hipEventCreate(&start_) != hipSuccess) { start_ = nullptr; return; }
+  (void)hipEventRecord(start_, stream());
+}
+ProfScope::~ProfScope() {
+  if (!start_) return;
+  hipEvent_t end = nullptr;
+  if (hipEventCreate(&end) != hipSuccess) { (void)hipEventDestroy(start_); return; }
+  (void)hipEventRecord(end, stream());
+  t_pending.push_back({name_, start_, end});
+}
+
+void prof_enable(bool on) { g_prof_on.store(on); }
+void prof_reset() {
+  drain_prof();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof.clear();
+}
+std::string prof_report() {
+  drain_prof();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  std::string out;
+  char line[256];
+  for (auto& kv : g_prof) {
+    snprintf(line, sizeof line, "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.count, kv.second.ms);
+    out += line;
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------- arrays
+
+Array* new_array(const std::string& format, int type, int64_t length) {
+  Array* a = new Array();
+  a->format = format;
+  a->type = type;
+  a->length = length;
+  return a;
+}
+void retain_array(Array* a) { a->refs.fetch_add(1, std::memory_order_relaxed); }
+void release_array(Array* a) {
+  if (a && a->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete a;
+}
+
+void check_length(int64_t n) {
+  if (n > int64_t(UINT32_MAX)) fail("arrays above 2^32 elements are not supported yet");
+}
+void check_index_rows(int64_t rows, const char* op) {
+  int64_t last = rows - 1;
+  if (last > int64_t(UINT32_MAX))
+    fail(std::string(op) + ": row " + std::to_string(last) +
+         " does not fit the UInt32 index type (row numbers go up to " + std::to_string(UINT32_MAX) + ")");
+}
+
+namespace {
+
+__global__ void popcount_kernel(const uint8_t* bits, int64_t bit_off, int64_t n, unsigned long long* zeros) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  unsigned long long z = 0;
+  for (; i < n; i += stride) {
+    int64_t b = bit_off + i;
+    z += ((bits[b >> 3] >> (b & 7)) & 1) ? 0 : 1;
+  }
+  // Wave reduction, then one atomic per wave.
+  for (int off = 32; off > 0; off >>= 1) z += __shfl_down(z, off, 64);
+  if ((threadIdx.x & 63) == 0 && z) atomicAdd(zeros, z);
+}
+
+}  // namespace
+
+int64_t count_nulls(const Array& a) {
+  if (!a.validity || a.length == 0) return 0;
+  auto cnt = device_alloc(8);
+  hipStream_t s = stream();
+  AM_HIP(hipMemsetAsync(cnt->ptr, 0, 8, s));
+  int blocks = int(std::min<int64_t>((a.length + 255) / 256, 4096));
+  {
+    ProfScope ps("count_nulls");
+    hipLaunchKernelGGL(popcount_kernel, dim3(blocks), dim3(256), 0, s, a.valid_bits(), a.offset, a.length,
+                       static_cast<unsigned long long*>(cnt->ptr));
+    AM_HIP(hipGetLastError());
+  }
+  unsigned long long z = 0;
+  AM_HIP(hipMemcpyAsync(&z, cnt->ptr, 8, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  return int64_t(z);
+}
+
+// ---------------------------------------------------------------- import / export
+
+namespace {
+
+// Copies bits [off, off + n) of `src` into a new LSB-first bitmap starting at bit 0.
+std::vector<uint8_t> shift_bitmap(const uint8_t* src, int64_t off, int64_t n, int64_t* zeros) {
+  std::vector<uint8_t> out(size_t((n + 7) / 8), 0);
+  int64_t z = 0;
+  if (off % 8 == 0) {
+    std::memcpy(out.data(), src + off / 8, out.size());
+    if (n % 8) out.back() &= uint8_t((1u << (n % 8)) - 1);
+    for (uint8_t b : out) z += __builtin_popcount(b);
+    z = n - z;
+  } else {
+    for (int64_t i = 0; i < n; i++) {
+      int64_t b = off + i;
+      int v = (src[b >> 3] >> (b & 7)) & 1;
+      out[size_t(i >> 3)] |= uint8_t(v << (i & 7));
+      z += !v;
+    }
+  }
+  if (zeros) *zeros = z;
+  return out;
+}
+
+struct ImportedArray {
+  ArrowArray arr;   // the producer's array, moved here
+  ~ImportedArray() {
+    if (arr.release) arr.release(&arr);
+  }
+};
+
+}  // namespace
+
+Array* import_host(const ArrowSchema* schema, ArrowArray* array) {
+  if (!schema || !array || !schema->format) fail("am_import: NULL schema or array");
+  if (!array->release) fail("am_import: the array has already been released");
+  std::string fmt = schema->format;
+  int t = type_of_format(fmt);
+  if (t == T_UNSUPPORTED) fail("am_import: Arrow format \"" + fmt + "\" is not supported by this build");
+  if (array->n_buffers != 2) fail("am_import: a primitive array needs 2 buffers");
+  // Move the producer's struct: its release is ours to call from here on.
+  auto holder = std::make_shared<ImportedArray>();
+  holder->arr = *array;
+  array->release = nullptr;
+  ArrowArray& A = holder->arr;
+  int64_t n = A.length, off = A.offset;
+  check_length(n);
+  int w = type_width(t);
+  Array* out = new_array(fmt, t, n);
+  try {
+    hipStream_t s = stream();
+    out->values = device_alloc(size_t(std::max<int64_t>(n, 1)) * w);
+    const uint8_t* vals = static_cast<const uint8_t*>(A.buffers[1]);
+    if (n > 0) {
+      if (!vals) fail("am_import: NULL values buffer");
+      AM_HIP(hipMemcpyAsync(out->values->ptr, vals + off * w, size_t(n) * w, hipMemcpyHostToDevice, s));
+    }
+    const uint8_t* bits = static_cast<const uint8_t*>(A.buffers[0]);
+    if (bits && A.null_count != 0 && n > 0) {
+      int64_t zeros = 0;
+      auto bm = shift_bitmap(bits, off, n, &zeros);
+      out->null_count = zeros;
+      if (zeros > 0) {
+        out->validity = device_alloc(bm.size());
+        AM_HIP(hipMemcpyAsync(out->validity->ptr, bm.data(), bm.size(), hipMemcpyHostToDevice, s));
+      }
+      AM_HIP(hipStreamSynchronize(s));   // bm is a host temporary
+    } else {
+      out->null_count = 0;
+      AM_HIP(hipStreamSynchronize(s));
+    }
+  } catch (...) {
+    release_array(out);
+    throw;
+  }
+  return out;   // holder goes out of scope: the producer's release runs now, the data is copied
+}
+
+Array* import_device(const ArrowSchema* schema, ArrowDeviceArray* darr) {
+  if (!schema || !darr || !schema->format) fail("am_import_device: NULL schema or array");
+  if (darr->device_type == ARROW_DEVICE_CPU || darr->device_type == ARROW_DEVICE_ROCM_HOST)
+    return import_host(schema, &darr->array);
+  if (darr->device_type != ARROW_DEVICE_ROCM)
+    fail("am_import_device: device type " + std::to_string(darr->device_type) +
+         " is not ROCm (10) or CPU (1)");
+  if (darr->device_id != device().id)
+    fail("am_import_device: device " + std::to_string(darr->device_id) + " is not the current device " +
+         std::to_string(device().id));
+  ArrowArray* array = &darr->array;
+  if (!array->release) fail("am_import_device: the array has already been released");
+  std::string fmt = schema->format;
+  int t = type_of_format(fmt);
+  if (t == T_UNSUPPORTED) fail("am_import_device: Arrow format \"" + fmt + "\" is not supported by this build");
+  if (array->n_buffers != 2) fail("am_import_device: a primitive array needs 2 buffers");
+  if (darr->sync_event) AM_HIP(hipStreamWaitEvent(stream(), *static_cast<hipEvent_t*>(darr->sync_event), 0));
+  auto holder = std::make_shared<ImportedArray>();
+  holder->arr = *array;
+  array->release = nullptr;
+  ArrowArray& A = holder->arr;
+  check_length(A.length);
+  int w = type_width(t);
+  Array* out = new_array(fmt, t, A.length);
+  out->offset = A.offset;
+  void* vals = const_cast<void*>(A.buffers[1]);
+  out->values = device_wrap(vals, size_t(A.offset + A.length) * w, [holder](void*) {});
+  void* bits = const_cast<void*>(A.buffers[0]);
+  if (bits && A.null_count != 0 && A.length > 0) {
+    out->validity = device_wrap(bits, size_t((A.offset + A.length + 7) / 8), [holder](void*) {});
+    try {
+      out->null_count = A.null_count >= 0 ? A.null_count : count_nulls(*out);
+    } catch (...) {
+      release_array(out);
+      throw;
+    }
+    if (out->null_count == 0) out->validity.reset();
+  } else {
+    out->null_count = 0;
+  }
+  return out;
+}
+
+namespace {
+
+struct ExportHolder {
+  std::string format;
+  std::shared_ptr<std::vector<uint8_t>> values, validity;
+  const void* buffers[2] = {nullptr, nullptr};
+  Array* keep = nullptr;   // device export: the array whose buffers are handed out
+  ~ExportHolder() {
+    if (keep) release_array(keep);
+  }
+};
+
+void release_export_array(ArrowArray* a) {
+  if (!a || !a->release) return;
+  delete static_cast<ExportHolder*>(a->private_data);
+  a->release = nullptr;
+}
+void release_export_schema(ArrowSchema* s) {
+  if (!s || !s->release) return;
+  delete static_cast<std::string*>(s->private_data);
+  s->release = nullptr;
+}
+
+void fill_schema(const Array& a, ArrowSchema* schema) {
+  auto* f = new std::string(a.format);
+  std::memset(schema, 0, sizeof *schema);
+  schema->format = f->c_str();
+  schema->name = "";
+  schema->metadata = nullptr;
+  schema->flags = ARROW_FLAG_NULLABLE;
+  schema->n_children = 0;
+  schema->children = nullptr;
+  schema->dictionary = nullptr;
+  schema->release = release_export_schema;
+  schema->private_data = f;
+}
+
+}  // namespace
+
+void export_host(Array* a, ArrowSchema* schema, ArrowArray* array) {
+  sync();   // a batch open on this thread completes before its results are read
+  if (!a->host_values) {
+    int w = a->width();
+    auto hv = std::make_shared<std::vector<uint8_t>>(size_t(std::max<int64_t>(a->length, 1)) * w);
+    hipStream_t s = stream();
+    if (a->length > 0) AM_HIP(hipMemcpyAsync(hv->data(), a->data(), size_t(a->length) * w, hipMemcpyDeviceToHost, s));
+    std::shared_ptr<std::vector<uint8_t>> hb;
+    if (a->has_nulls() && a->length > 0) {
+      int64_t first_byte = a->offset / 8;
+      int64_t nbytes = (a->offset + a->length + 7) / 8 - first_byte;
+      std::vector<uint8_t> raw(static_cast<size_t>(nbytes));
+      AM_HIP(hipMemcpyAsync(raw.data(), a->valid_bits() + first_byte, size_t(nbytes), hipMemcpyDeviceToHost, s));
+      AM_HIP(hipStreamSynchronize(s));
+      hb = std::make_shared<std::vector<uint8_t>>(shift_bitmap(raw.data(), a->offset % 8, a->length, nullptr));
+    }
+    AM_HIP(hipStreamSynchronize(s));
+    a->host_values = hv;
+    a->host_validity = hb;
+  }
+  fill_schema(*a, schema);
+  auto* h = new ExportHolder();
+  h->format = a->format;
+  h->values = a->host_values;
+  h->validity = a->host_validity;
+  h->buffers[0] = h->validity ? h->validity->data() : nullptr;
+  h->buffers[1] = h->values->data();
+  std::memset(array, 0, sizeof *array);
+  array->length = a->length;
+  array->null_count = h->validity ? a->null_count : 0;
+  array->offset = 0;
+  array->n_buffers = 2;
+  array->n_children = 0;
+  array->buffers = h->buffers;
+  array->children = nullptr;
+  array->dictionary = nullptr;
+  array->release = release_export_array;
+  array->private_data = h;
+}
+
+void export_device(Array* a, ArrowSchema* schema, ArrowDeviceArray* darr) {
+  sync();
+  fill_schema(*a, schema);
+  auto* h = new ExportHolder();
+  h->format = a->format;
+  retain_array(a);
+  h->keep = a;
+  h->buffers[0] = a->has_nulls() ? a->validity->ptr : nullptr;
+  h->buffers[1] = a->values ? a->values->ptr : nullptr;
+  std::memset(darr, 0, sizeof *darr);
+  ArrowArray* array = &darr->array;
+  array->length = a->length;
+  array->null_count = a->has_nulls() ? a->null_count : 0;
+  array->offset = a->offset;
+  array->n_buffers = 2;
+  array->buffers = h->buffers;
+  array->release = release_export_array;
+  array->private_data = h;
+  darr->device_id = device().id;
+  darr->device_type = ARROW_DEVICE_ROCM;
+  darr->sync_event = nullptr;
+}
+
+// ---------------------------------------------------------------- tuning table
+
+SortTune sort_tune(int64_t n, int key_bytes) {
+  // Tiles of 256 threads x `items` keys. Small inputs take short tiles so that the grid still covers
+  // every CU; large ones take long tiles, which amortise the per-tile look-back over more keys.
+  SortTune t;
+  int64_t cus = device().cu_count;
+  if (n <= cus * 256 * 4) t.items = 4;
+  else if (n <= cus * 256 * 8 * 2) t.items = 8;
+  else t.items = key_bytes == 8 ? 16 : 16;
+  return t;
+}
+
+}  // namespace am

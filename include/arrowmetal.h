@@ -1,0 +1,119 @@
+/* ArrowMetal C ABI, MI355X (gfx950) build: the argsort / take / group-by subset.
+ *
+ * The declarations keep the reference's signatures, argument meanings, return codes and ownership
+ * rules, so that its callers (the DuckDB extension, the Spark FFM downcalls, the Polars plugin, the
+ * Python/Go/Node/R/Rust bindings) load libArrowMetalC.so in place of libArrowMetalC.dylib unchanged.
+ * Each declaration cites the reference declaration it replaces (include/arrowmetal.h:LINE in the
+ * reference checkout) and the Swift entry point that implemented it there.
+ *
+ * Conventions (reference include/arrowmetal.h:1-3, Sources/ArrowMetalC/ArrowMetalC.swift:9-34):
+ *   return 0 on success; 1 on an operation error, with the message in am_last_error() (thread-local,
+ *   valid until the next call on the thread); 2 for a NULL or bad handle argument (no message).
+ *   Every am_array* / am_groupby* written to an out-parameter is a new reference the caller releases.
+ *   Calls are synchronous: results are complete on return, except between am_batch_begin and
+ *   am_batch_end on the calling thread, where the GPU work is ordered on the thread's stream and the
+ *   next sync point (am_batch_end, am_export, a count read back) completes it.
+ *
+ * Device arrays are HIP device memory on the current ROCm device. am_export_device reports
+ * ARROW_DEVICE_ROCM (10) and a NULL sync_event (the data is ready when the call returns).
+ */
+#ifndef ARROWMETAL_H
+#define ARROWMETAL_H
+#include <stdint.h>
+#include "arrow_abi.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference include/arrowmetal.h:12 */
+typedef struct am_array am_array;
+/* reference include/arrowmetal.h:900 */
+typedef struct am_groupby am_groupby;
+
+/* reference include/arrowmetal.h:15-17 (ArrowMetalC.swift am_version / am_device_name / am_last_error) */
+const char* am_version(void);
+const char* am_device_name(void);
+const char* am_last_error(void);
+
+/* reference include/arrowmetal.h:20-21 (CInterop.swift:119-139): the producer's array is moved (its
+ * release is set to NULL). A CPU array is copied into HBM once; am_import_device with
+ * device_type ARROW_DEVICE_ROCM keeps the producer's device buffers (zero copy) and releases the
+ * producer when the last handle dies. ARROW_DEVICE_CPU goes through am_import. */
+int  am_import(const struct ArrowSchema* schema, struct ArrowArray* array, am_array** out);
+int  am_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array, am_array** out);
+/* reference include/arrowmetal.h:48-58 (CInterop.swift:338-412): am_export fills host buffers
+ * (copied from HBM once per handle); am_export_device hands out the device buffers themselves. */
+int  am_export(am_array* a, struct ArrowSchema* schema, struct ArrowArray* array);
+int  am_export_device(am_array* a, struct ArrowSchema* schema, struct ArrowDeviceArray* array);
+void am_release(am_array* a);
+int64_t     am_length(am_array* a);
+int64_t     am_null_count(am_array* a);
+const char* am_format(am_array* a);
+const char* am_compute_format(am_array* a);
+
+/* reference include/arrowmetal.h:93 (ArrowMetalC.swift:372-383, Take.swift:41-80). indices: int32,
+ * int64 or uint32; a null index gives a null; an index outside [0, length) is the error
+ * "take: index out of range (array length N)". */
+int  am_take(am_array* a, am_array* indices, am_array** out);
+
+/* reference include/arrowmetal.h:108 (ArrowMetalC.swift:394-399): stable uint32 ("I") indices,
+ * ascending or descending, nulls last, float order ieee. */
+int  am_argsort(am_array* a, int descending, am_array** out);
+
+/* reference include/arrowmetal.h:113 (ArrowMetalC.swift:416-469): dense keys int32/int64/uint32 in
+ * [0, key_count); null and out-of-range keys are skipped. agg: 0 sum, 1 count(rows), 2 min, 3 max,
+ * 4 mean, 5 count(values). min/max take 32-bit-or-narrower values. */
+int  am_group_by(am_array* keys, int64_t key_count, int agg, am_array* values, am_array** out);
+
+/* reference include/arrowmetal.h:184-185 (MetalContext.swift:245-312): per-thread batching. */
+int  am_batch_begin(void);
+int  am_batch_end(void);
+
+/* reference include/arrowmetal.h:911-918 (ArrowMetalC_GroupByExtra.swift:42-89). Group order is
+ * ascending by key (nulls last) for numeric columns, lexicographic in column order for several
+ * columns; ids are int32 and never null; am_group_by_group_count returns -1 for a NULL handle. */
+int  am_group_by_keys(am_array** columns, int64_t count, am_groupby** out);
+int64_t am_group_by_group_count(am_groupby* gb);
+int  am_group_by_keys_result(am_groupby* gb, int64_t i, am_array** out);
+int  am_group_by_ids(am_groupby* gb, am_array** out);
+void am_group_by_release(am_groupby* gb);
+
+/* reference include/arrowmetal.h:920-954 (ArrowMetalC_GroupByExtra.swift:92-181). Ops taken here:
+ *   0 hash_sum       integers wrap in 64 bits (int64 out; uint64 out for uint64 values);
+ *                    float64 is correctly rounded (float64 out)
+ *   1 hash_count_all rows per group (values may be NULL)
+ *   2 hash_count     non-null values per group
+ *   3 hash_mean      float64 out; correctly rounded for float64 values
+ *   4 hash_min       values' type; float64 compares -0.0 < +0.0 and skips NaN
+ *   5 hash_max
+ * A group with no value to answer with is null. Other ops return 1 with
+ * "am_group_agg_ex op N is not supported by this build". */
+int  am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_array** out);
+
+/* reference include/arrowmetal.h:1331, 1348 (ArrowMetalC_Options.swift:110-119, 166-175).
+ * null_placement 1 at_start, any other value at_end (ArrowMetalC_Options.swift:50); float_order 0
+ * ieee, 1 total, 2 nan_largest, any other value the error "unknown float_order N" (:52-59). */
+int  am_argsort_ex(am_array* a, int descending, int null_placement, am_array** out);
+int  am_argsort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Extensions of this build (no counterpart in the reference header). They serve the batched and
+ * served forms of the same kernels, the keep-first unique of the plan engine's "unique" operator
+ * (reference Sources/ArrowMetal/Engine/Executor.swift:134-148), and measurement.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Keep-first unique rows over `count` key columns: the lowest row of every distinct key combination,
+ * ascending, as uint32 row numbers (take them with am_take). */
+int  am_unique_rows(am_array** columns, int64_t count, am_array** out);
+
+/* Kernel timing of the library's own launches, measured with HIP events on the stream each kernel
+ * is launched on. am_prof_report writes "name count total_ms\n" lines into buf (NUL-terminated) and
+ * returns the number of bytes needed. */
+int  am_prof_enable(int on);
+int  am_prof_reset(void);
+int64_t am_prof_report(char* buf, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARROWMETAL_H */

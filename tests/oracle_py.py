@@ -1,0 +1,146 @@
+"""ctypes wrapper over the CPU oracle (oracle/liboracle.so). Test infrastructure only.
+
+Only tests, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg import this module, and
+only to check (or time) against the HIP library; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+# Element type codes of oracle/oracle.c.
+I8, I16, I32, I64, U8, U16, U32, U64, F32, F64 = range(10)
+NP_TYPE = {I8: np.int8, I16: np.int16, I32: np.int32, I64: np.int64, U8: np.uint8, U16: np.uint16,
+           U32: np.uint32, U64: np.uint64, F32: np.float32, F64: np.float64}
+TYPE_OF = {np.dtype(v): k for k, v in NP_TYPE.items()}
+
+_lib = None
+
+
+def lib():
+    """Loads the oracle, building it with its Makefile when it is missing (gcc only)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        c_int = ctypes.c_int
+        L.or_splitmix64_fill.argtypes = [ctypes.c_uint64, P, i64]
+        L.or_argsort.argtypes = [P, c_int, P, i64, c_int, c_int, c_int, P]
+        L.or_take.argtypes = [P, c_int, P, i64, P, c_int, P, i64, P, P]
+        L.or_group_keys.argtypes = [c_int, P, P, P, i64, P, P]
+        L.or_group_keys.restype = i64
+        L.or_group_agg.argtypes = [P, i64, i64, P, c_int, P, c_int, P, P]
+        L.or_group_by_dense.argtypes = [P, c_int, P, i64, i64, P, c_int, P, c_int, P, P]
+        L.or_unique_rows.argtypes = [c_int, P, P, P, i64, P]
+        L.or_unique_rows.restype = i64
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pack_validity(valid_bool):
+    """Arrow LSB-first bitmap from a boolean mask (None stays None)."""
+    if valid_bool is None:
+        return None
+    return np.packbits(np.asarray(valid_bool, dtype=bool), bitorder="little")
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().or_splitmix64_fill(ctypes.c_uint64(seed & (2**64 - 1)), _ptr(out), n)
+    return out
+
+
+def argsort(values: np.ndarray, valid=None, descending=False, null_placement=0, float_order=0):
+    values = np.ascontiguousarray(values)
+    bm = pack_validity(valid)
+    out = np.empty(len(values), dtype=np.uint32)
+    lib().or_argsort(_ptr(values), TYPE_OF[values.dtype], _ptr(bm), len(values), int(descending),
+                     int(null_placement), int(float_order), _ptr(out))
+    return out
+
+
+def take(values: np.ndarray, indices: np.ndarray, valid=None, idx_valid=None):
+    """Returns (out, out_valid) or raises IndexError like the reference's range check."""
+    values = np.ascontiguousarray(values)
+    indices = np.ascontiguousarray(indices)
+    bm, ibm = pack_validity(valid), pack_validity(idx_valid)
+    out = np.empty(len(indices), dtype=values.dtype)
+    ov = np.empty(len(indices), dtype=np.uint8)
+    rc = lib().or_take(_ptr(values), values.dtype.itemsize, _ptr(bm), len(values), _ptr(indices),
+                       TYPE_OF[indices.dtype], _ptr(ibm), len(indices), _ptr(out), _ptr(ov))
+    if rc:
+        raise IndexError(f"take: index out of range (array length {len(values)})")
+    return out, ov.astype(bool)
+
+
+def group_keys(columns, valids=None):
+    """Dense ids (ascending key order, null group last) and representative rows for key columns."""
+    columns = [np.ascontiguousarray(c) for c in columns]
+    n = len(columns[0])
+    valids = valids or [None] * len(columns)
+    bms = [pack_validity(v) for v in valids]
+    types = np.array([TYPE_OF[c.dtype] for c in columns], dtype=np.int32)
+    vptrs = (ctypes.c_void_p * len(columns))(*[c.ctypes.data for c in columns])
+    bptrs = (ctypes.c_void_p * len(columns))(*[0 if b is None else b.ctypes.data for b in bms])
+    ids = np.empty(n, dtype=np.int32)
+    rep = np.empty(max(n, 1), dtype=np.uint32)
+    K = lib().or_group_keys(len(columns), _ptr(types), ctypes.cast(vptrs, ctypes.c_void_p),
+                            ctypes.cast(bptrs, ctypes.c_void_p), n, _ptr(ids), _ptr(rep))
+    return ids, int(K), rep[:K].copy()
+
+
+def group_agg(ids: np.ndarray, K: int, values, op: int, valid=None):
+    """One grouped aggregate over dense ids. Returns (bits uint64[K], valid bool[K])."""
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    t = TYPE_OF[values.dtype] if values is not None else I64
+    if values is not None:
+        values = np.ascontiguousarray(values)
+    bm = pack_validity(valid)
+    out = np.zeros(max(K, 1), dtype=np.uint64)
+    ov = np.zeros(max(K, 1), dtype=np.uint8)
+    rc = lib().or_group_agg(_ptr(ids), len(ids), K, _ptr(values), t, _ptr(bm), op, _ptr(out), _ptr(ov))
+    if rc:
+        raise ValueError(f"oracle: op {op} is not defined for this type")
+    return out[:K], ov[:K].astype(bool)
+
+
+def group_by_dense(keys: np.ndarray, K: int, values, agg: int, key_valid=None, valid=None):
+    keys = np.ascontiguousarray(keys)
+    t = TYPE_OF[values.dtype] if values is not None else I64
+    if values is not None:
+        values = np.ascontiguousarray(values)
+    kbm, bm = pack_validity(key_valid), pack_validity(valid)
+    out = np.zeros(max(K, 1), dtype=np.uint64)
+    ov = np.zeros(max(K, 1), dtype=np.uint8)
+    rc = lib().or_group_by_dense(_ptr(keys), TYPE_OF[keys.dtype], _ptr(kbm), len(keys), K, _ptr(values), t,
+                                 _ptr(bm), agg, _ptr(out), _ptr(ov))
+    if rc:
+        raise ValueError(f"oracle: agg {agg} is not defined for this type")
+    return out[:K], ov[:K].astype(bool)
+
+
+def unique_rows(columns, valids=None):
+    columns = [np.ascontiguousarray(c) for c in columns]
+    n = len(columns[0])
+    valids = valids or [None] * len(columns)
+    bms = [pack_validity(v) for v in valids]
+    types = np.array([TYPE_OF[c.dtype] for c in columns], dtype=np.int32)
+    vptrs = (ctypes.c_void_p * len(columns))(*[c.ctypes.data for c in columns])
+    bptrs = (ctypes.c_void_p * len(columns))(*[0 if b is None else b.ctypes.data for b in bms])
+    out = np.empty(max(n, 1), dtype=np.uint32)
+    m = lib().or_unique_rows(len(columns), _ptr(types), ctypes.cast(vptrs, ctypes.c_void_p),
+                             ctypes.cast(bptrs, ctypes.c_void_p), n, _ptr(out))
+    return out[:m].copy()

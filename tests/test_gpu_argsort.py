@@ -1,0 +1,205 @@
+"""GPU parity of the radix argsort (am_argsort / am_argsort_ex / am_argsort_ex2) and am_take against the
+CPU oracle, bit-exact, on the sizes and key shapes of the reference's own suites
+(Tests/ArrowMetalTests/RadixSortScaleTests.swift:28-201, SortOptionsTests.swift, SortTests.swift)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import arrowmetal_amd as am
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# RadixSortScaleTests.swift:28-29: lane, chunk, block and pass-skip boundaries.
+SIZES = [0, 1, 2, 31, 32, 33, 63, 64, 65, 255, 256, 257, 511, 512, 513, 4095, 4096, 4097, 65536,
+         (1 << 18) - 1, 1 << 18, (1 << 18) + 1]
+
+
+def gpu_argsort(vals, valid=None, desc=False, place=0, order=0):
+    a = am.MetalArray.from_numpy(vals, valid)
+    r = a.argsort(bool(desc), am.NULL_PLACEMENTS[place], am.FLOAT_ORDERS[order])
+    assert r.format == "I"
+    out, v = r.to_numpy()
+    assert v is None
+    return out
+
+
+def check(vals, valid=None, desc=False, place=0, order=0, label=""):
+    got = gpu_argsort(vals, valid, desc, place, order)
+    want = O.argsort(vals, valid, desc, place, order)
+    if not np.array_equal(got, want):
+        bad = int(np.nonzero(got != want)[0][0])
+        raise AssertionError(f"{label} n={len(vals)} desc={desc} place={place} order={order}: "
+                             f"first difference at {bad}: got {got[bad]} want {want[bad]}")
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_int64_full_range_sizes(n):
+    vals = O.splitmix64(0xA11CE501 + n, n).view(np.int64)
+    for desc in (False, True):
+        check(vals, None, desc, label="i64")
+
+
+@pytest.mark.parametrize("n", [1, 33, 4097, 70001, (1 << 18) + 1])
+def test_float64_orders_and_placements(n):
+    rng = np.random.default_rng(n)
+    f = rng.uniform(-1e3, 1e3, n).round(0)
+    sp = np.array([np.nan, -np.nan, np.inf, -np.inf, -0.0, 0.0, 5e-324, -5e-324], dtype=np.float64)
+    m = rng.random(n) < 0.1
+    f[m] = sp[rng.integers(0, len(sp), int(m.sum()))]
+    b = f.view(np.uint64)
+    b[rng.random(n) < 0.01] = np.uint64(0x7FF0000000000001)     # signalling NaN payload
+    valid = rng.random(n) >= 0.05
+    for desc in (0, 1):
+        for place in (0, 1):
+            for order in (0, 1, 2):
+                check(f, valid, desc, place, order, "f64")
+
+
+def test_reference_vectors():
+    with open(os.path.join(GOLDEN, "reference_vectors.json")) as fh:
+        v = json.load(fh)
+    s = v["sort_specials"]
+    vals = np.array(s["values"], dtype=np.uint64).view(np.float64)
+    valid = np.array(s["valid"], dtype=bool)
+    assert gpu_argsort(vals, valid).tolist() == s["ascending"]
+    assert gpu_argsort(vals, valid, desc=True).tolist() == s["descending"]
+    t = v["sort_total_specials"]
+    vals = np.array(t["values"], dtype=np.uint64).view(np.float64)
+    valid = np.array(t["valid"], dtype=bool)
+    assert gpu_argsort(vals, valid, desc=True).tolist() == t["desc_ieee"]
+    assert gpu_argsort(vals, valid, desc=True, order=1).tolist() == t["desc_total"]
+
+
+def test_pyarrow_fixtures():
+    fx = np.load(os.path.join(GOLDEN, "sort_fixtures.npz"))
+    for n in [0, 1, 2, 33, 257, 4097, 20001]:
+        f = fx[f"f64_{n}_bits"].view(np.float64)
+        valid = fx[f"f64_{n}_valid"]
+        ints = fx[f"i64_{n}_vals"]
+        for desc in (0, 1):
+            for pl, place in ((0, "at_end"), (1, "at_start")):
+                np.testing.assert_array_equal(gpu_argsort(f, valid, desc, pl, 0), fx[f"f64_{n}_ieee_{desc}_{place}"])
+                np.testing.assert_array_equal(gpu_argsort(f, valid, desc, pl, 1), fx[f"f64_{n}_total_{desc}_{place}"])
+                np.testing.assert_array_equal(gpu_argsort(ints, valid, desc, pl), fx[f"i64_{n}_{desc}_{place}"])
+
+
+@pytest.mark.parametrize("n", [4097, (1 << 18) + 1, 1 << 20])
+def test_key_shapes(n):
+    rng = np.random.default_rng(7)
+    # All keys equal (every pass skipped; the single pass is a stable compaction).
+    check(np.full(n, 42, dtype=np.int64), None, label="all-equal")
+    check(np.full(n, 42, dtype=np.int64), rng.random(n) > 0.5, True, 1, label="all-equal-nulls")
+    # Constant high digits, then constant low digits (pass skipping both ways).
+    check(rng.integers(0, 256, n).astype(np.int64), None, label="low-byte-only")
+    check((rng.integers(0, 256, n).astype(np.int64) << 40) | 0x55, None, True, label="one-mid-byte")
+    # Monotone input, both directions.
+    check(np.arange(n, dtype=np.int64), None, label="ascending-input")
+    check(np.arange(n, 0, -1, dtype=np.int64), None, label="descending-input")
+    # Nulls at 1/1000, 1/4, 1/2, and all null.
+    vals = rng.integers(-1000, 1000, n).astype(np.int64)
+    for frac in (0.001, 0.25, 0.5, 1.0):
+        valid = rng.random(n) >= frac
+        for place in (0, 1):
+            check(vals, valid, False, place, label=f"nulls {frac}")
+            check(vals, valid, True, place, label=f"nulls {frac}")
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.uint32, np.float32, np.int8, np.int16, np.uint8, np.uint16, np.uint64])
+def test_other_key_widths(dtype):
+    rng = np.random.default_rng(3)
+    n = 70001
+    if np.dtype(dtype).kind == "f":
+        vals = rng.uniform(-100, 100, n).astype(dtype)
+        vals[::37] = np.nan
+        vals[::41] = -0.0
+    else:
+        info = np.iinfo(dtype)
+        vals = rng.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+    valid = rng.random(n) > 0.1
+    for desc in (0, 1):
+        for place in (0, 1):
+            for order in ((0, 1, 2) if np.dtype(dtype).kind == "f" else (0,)):
+                check(vals, valid, desc, place, order, str(np.dtype(dtype)))
+
+
+def test_offset_slice_import():
+    rng = np.random.default_rng(1)
+    vals = rng.integers(-50, 50, 10000).astype(np.int64)
+    valid = rng.random(10000) > 0.2
+    for off in (0, 3, 64, 1001):
+        a = am.MetalArray.from_numpy(vals, valid, offset=off)
+        got, v = a.argsort(True, "at_start").to_numpy()
+        want = O.argsort(vals[off:], valid[off:], True, 1)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_repeat_determinism():
+    vals = O.splitmix64(99, 300001).view(np.int64) >> 40      # many duplicate keys
+    first = gpu_argsort(vals)
+    for _ in range(3):
+        np.testing.assert_array_equal(gpu_argsort(vals), first)
+
+
+def test_bad_float_order_is_an_error():
+    a = am.MetalArray.from_numpy(np.arange(4, dtype=np.float64))
+    with pytest.raises(am.ArrowMetalError, match="unknown float_order 7"):
+        a.argsort(False, "at_end", 7)
+
+
+def test_batch_matches_unbatched():
+    vals = O.splitmix64(5, 100000).view(np.int64)
+    a = am.MetalArray.from_numpy(vals)
+    want = a.argsort().to_numpy()[0]
+    with am.batch():
+        r1 = a.argsort()
+        r2 = a.argsort(True)
+    np.testing.assert_array_equal(r1.to_numpy()[0], want)
+    np.testing.assert_array_equal(r2.to_numpy()[0], O.argsort(vals, None, True))
+
+
+# ---------------------------------------------------------------- take
+
+
+def test_take_matches_oracle():
+    rng = np.random.default_rng(4)
+    n = 100003
+    vals = rng.uniform(-1e6, 1e6, n)
+    valid = rng.random(n) > 0.1
+    a = am.MetalArray.from_numpy(vals, valid)
+    for it in (np.int32, np.int64, np.uint32):
+        idx = rng.integers(0, n, 77777).astype(it)
+        ivalid = rng.random(77777) > 0.05
+        got, gv = a.take(am.MetalArray.from_numpy(idx, ivalid)).to_numpy()
+        want, wv = O.take(vals, idx, valid, ivalid)
+        np.testing.assert_array_equal(gv, wv)
+        np.testing.assert_array_equal(got.view(np.uint64)[wv], want.view(np.uint64)[wv])
+    got, gv = a.take(np.arange(n, dtype=np.uint32)[::-1].copy()).to_numpy()
+    np.testing.assert_array_equal(got.view(np.uint64)[gv], vals[::-1].view(np.uint64)[gv])
+
+
+def test_take_out_of_range():
+    a = am.MetalArray.from_numpy(np.arange(10, dtype=np.int64))
+    with pytest.raises(am.ArrowMetalError, match=r"take: index out of range \(array length 10\)"):
+        a.take(np.array([1, 10], dtype=np.int64))
+    with pytest.raises(am.ArrowMetalError, match="out of range"):
+        a.take(np.array([-1], dtype=np.int32))
+
+
+def test_order_by_gather_config_shape():
+    """ORDER BY: argsort descending nulls last, then gather a float64 payload (20 B/row)."""
+    n = 1_000_003
+    keys = (O.splitmix64(0xB0B, n) >> np.uint64(1)).view(np.int64) - (1 << 62)
+    valid = (O.splitmix64(0xC0C, n) >> np.uint64(11)).astype(np.float64) / 2.0**53 < 0.98
+    payload = (O.splitmix64(0xD0D, n) >> np.uint64(11)).astype(np.float64) / 2.0**53 * 2e6 - 1e6
+    k = am.MetalArray.from_numpy(keys, valid)
+    p = am.MetalArray.from_numpy(payload)
+    idx = k.argsort(True, "at_end")
+    got = p.take(idx).to_numpy()[0]
+    want_idx = O.argsort(keys, valid, True, 0)
+    np.testing.assert_array_equal(idx.to_numpy()[0], want_idx)
+    np.testing.assert_array_equal(got, payload[want_idx])

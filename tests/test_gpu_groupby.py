@@ -1,0 +1,267 @@
+"""GPU parity of the hash group-by (am_group_by_keys / am_group_agg_ex / am_group_by) against the CPU
+oracle: bit-exact group order, ids, representative keys, integer aggregates, float64 min / max, and
+correctly rounded float64 sums and means (0 ulp). Cases follow the reference's suites
+(Tests/ArrowMetalTests/GroupByTests.swift, GroupByKeysTests.swift, GroupSumExactTests.swift,
+HashTableTests.swift)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import arrowmetal_amd as am
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def vectors():
+    with open(os.path.join(GOLDEN, "reference_vectors.json")) as fh:
+        return json.load(fh)
+
+
+def runs(pairs):
+    out = []
+    for v, n in pairs:
+        out += [v] * n
+    return out
+
+
+def agg_bits(arr):
+    """(bits uint64, valid bool) of an aggregate result, widened like the oracle's."""
+    vals, valid = arr.to_numpy()
+    if valid is None:
+        valid = np.ones(len(vals), dtype=bool)
+    if vals.dtype.kind == "i":
+        bits = vals.astype(np.int64).view(np.uint64)
+    elif vals.dtype == np.float32:
+        bits = vals.view(np.uint32).astype(np.uint64)
+    elif vals.dtype.kind == "f":
+        bits = vals.view(np.uint64)
+    else:
+        bits = vals.astype(np.uint64)
+    return bits, valid
+
+
+def assert_agg(got, want, label):
+    gb, gv = agg_bits(got)
+    wb, wv = want
+    np.testing.assert_array_equal(gv, wv, err_msg=f"{label}: validity")
+    if not np.array_equal(gb[wv], wb[wv]):
+        k = int(np.nonzero(gb[wv] != wb[wv])[0][0])
+        raise AssertionError(f"{label}: group {k}: got {gb[wv][k]:#x} want {wb[wv][k]:#x}")
+
+
+def check_grouping(key_cols, key_valids, values=None, vvalid=None, label=""):
+    mcols = [am.MetalArray.from_numpy(k, v) for k, v in zip(key_cols, key_valids)]
+    g = am.group_by(mcols)
+    ids_w, K, rep = O.group_keys(key_cols, key_valids)
+    assert g.group_count == K, label
+    ids, iv = g.ids().to_numpy()
+    assert iv is None
+    np.testing.assert_array_equal(ids, ids_w, err_msg=f"{label}: ids")
+    for c, (kc, kv) in enumerate(zip(key_cols, key_valids)):
+        got, gv = g.key_arrays()[c].to_numpy()
+        want_valid = np.ones(K, dtype=bool) if kv is None else kv[rep]
+        if gv is None:
+            gv = np.ones(K, dtype=bool)
+        np.testing.assert_array_equal(gv, want_valid, err_msg=f"{label}: key validity")
+        np.testing.assert_array_equal(got[want_valid].view(np.uint8), kc[rep][want_valid].view(np.uint8),
+                                      err_msg=f"{label}: keys")
+    if values is not None:
+        mv = am.MetalArray.from_numpy(values, vvalid)
+        ops = [1, 2, 4, 5] + ([0, 3] if values.dtype != np.float32 else [])
+        for op in ops:
+            name = {0: "sum", 1: "count_all", 2: "count", 3: "mean", 4: "min", 5: "max"}[op]
+            got = g.count_all() if op == 1 else getattr(g, name)(mv)
+            assert_agg(got, O.group_agg(ids_w, K, values, op, vvalid), f"{label} {name}")
+    return g
+
+
+@pytest.mark.parametrize("n,span", [(0, 1), (1, 1), (33, 7), (4097, 1024), (200003, 100000), (200003, 7)])
+def test_sparse_negative_int_keys(n, span):
+    rng = np.random.default_rng(n + span)
+    keys = rng.integers(-span, span + 1, n).astype(np.int64) * 1_000_003
+    kv = rng.random(n) >= 0.125
+    vals = rng.integers(-1000, 1001, n).astype(np.int64)
+    vv = rng.random(n) >= 0.1
+    check_grouping([keys], [kv], vals, vv, f"n={n} span={span}")
+
+
+def test_pyarrow_fixtures():
+    fx = np.load(os.path.join(GOLDEN, "groupby_fixtures.npz"))
+    for n in (1000, 20000, 30000):
+        keys, kv = fx[f"n{n}_keys"], fx[f"n{n}_kvalid"]
+        vals, vv = fx[f"n{n}_vals"], fx[f"n{n}_vvalid"]
+        g = am.group_by([am.MetalArray.from_numpy(keys, kv)])
+        assert g.group_count == len(fx[f"n{n}_gkeys"])
+        mv = am.MetalArray.from_numpy(vals, vv)
+        for name in ("sum", "min", "max"):
+            got, gv = g._agg(name, mv).to_numpy()
+            gv = np.ones(len(got), bool) if gv is None else gv
+            np.testing.assert_array_equal(gv, fx[f"n{n}_{name}_valid"])
+            np.testing.assert_array_equal(got[gv], fx[f"n{n}_{name}"][gv])
+        np.testing.assert_array_equal(g.count(mv).to_numpy()[0], fx[f"n{n}_count"])
+        np.testing.assert_array_equal(g.count_all().to_numpy()[0], fx[f"n{n}_count_all"])
+
+
+def test_reference_vectors():
+    v = vectors()
+    f = v["group_float_keys"]
+    keys = np.array(f["keys"], dtype=np.uint64).view(np.float64)
+    g = am.group_by([am.MetalArray.from_numpy(keys, np.array(f["keys_valid"], bool))])
+    assert g.group_count == f["group_count"]
+    assert g.sum(np.array(f["values"], dtype=np.int64)).to_numpy()[0].tolist() == f["group_sums"]
+    d = v["group_dense_fast_path"]
+    k = am.MetalArray.from_numpy(np.array(d["keys"], np.int32), np.array(d["keys_valid"], bool))
+    vals = am.MetalArray.from_numpy(np.array(d["values"], np.int64))
+    s, sv = k.group_by(d["key_count"]).sum(vals).to_numpy()
+    assert [int(x) if ok else None for x, ok in zip(s, sv)] == d["dense_sums"]
+    g = am.group_by([k])
+    assert sorted(g.sum(vals).to_numpy()[0].tolist()) == d["mapped_sums_sorted"]
+    a = v["group_all_null"]
+    g = am.group_by([am.MetalArray.from_numpy(np.zeros(3, np.int64), np.array(a["keys_valid"], bool))])
+    assert g.group_count == 1
+    assert g.sum(np.array(a["values"], np.int64)).to_numpy()[0].tolist() == a["sums"]
+    w = v["group_sum_wrap"]
+    keys = am.MetalArray.from_numpy(np.array(runs(w["keys"]), np.int32))
+    for name, dt in (("values", np.int64), ("neg_values", np.int64), ("u64_values", np.uint64)):
+        vals = np.array([x % 2**64 for x in runs(w[name])], dtype=np.uint64).view(dt)
+        got = keys.group_by(2).sum(am.MetalArray.from_numpy(vals)).to_numpy()[0]
+        want = w["sums" if name == "values" else name.replace("values", "sums")]
+        assert got.tolist() == want, name
+    m = v["group_minmax_dense"]
+    k = am.MetalArray.from_numpy(np.array(m["keys"], np.int32), np.array(m["keys_valid"], bool))
+    i32 = am.MetalArray.from_numpy(np.array(m["i32"], np.int32), np.array(m["i32_valid"], bool))
+    gb = k.group_by(m["key_count"])
+    for name, key in (("min", "min"), ("max", "max")):
+        x, xv = getattr(gb, name)(i32).to_numpy()
+        assert [int(a) if ok else None for a, ok in zip(x, xv)] == m[key]
+    assert gb.count_values(i32).to_numpy()[0].tolist() == m["count_values"]
+    assert gb.count().to_numpy()[0].tolist() == m["count_rows"]
+    u8 = am.MetalArray.from_numpy(np.array(m["u8"], np.uint8))
+    x, xv = gb.min(u8).to_numpy()
+    assert [int(a) if ok else None for a, ok in zip(x, xv)] == m["u8_min"]
+    with pytest.raises(am.ArrowMetalError, match="32-bit or narrower"):
+        gb.min(am.MetalArray.from_numpy(np.arange(7, dtype=np.int64)))
+
+
+def test_float_keys_and_values():
+    rng = np.random.default_rng(12)
+    n = 100003
+    keys = rng.integers(-50, 50, n).astype(np.float64) / 4
+    keys[::17] = -0.0
+    keys[::29] = np.nan
+    keys.view(np.uint64)[::31] = np.uint64(0x7FF0000000000001)
+    kv = rng.random(n) > 0.02
+    pick = rng.integers(0, 20, n)
+    vals = rng.uniform(-1e6, 1e6, n)
+    vals[pick == 0] = np.nan
+    vals[pick == 1] = -0.0
+    vals[pick == 2] = 0.0
+    vals[pick == 3] = np.inf
+    vv = rng.random(n) > 0.07
+    check_grouping([keys], [kv], vals, vv, "f64 keys")
+
+
+def test_multi_column_and_unique():
+    rng = np.random.default_rng(8)
+    n = 300007
+    a = rng.integers(0, 1000, n).astype(np.int64)
+    b = rng.integers(0, 100, n).astype(np.int32)
+    av = rng.random(n) > 0.01
+    vals = rng.integers(-100, 101, n).astype(np.int64)
+    check_grouping([a, b], [av, None], vals, None, "two columns")
+    ma, mb = am.MetalArray.from_numpy(a, av), am.MetalArray.from_numpy(b)
+    got = am.unique_rows([ma, mb]).to_numpy()[0]
+    np.testing.assert_array_equal(got, O.unique_rows([a, b], [av, None]))
+
+
+def test_exact_float_sum_edges():
+    groups = vectors()["group_sum_edges"]["groups"]
+    keys, vals = [], []
+    for k, g in enumerate(groups):
+        keys += [k] * len(g) + [k]
+        vals += g + [0]
+    keys = np.array(keys, dtype=np.int64)
+    vals = np.array(vals, dtype=np.uint64).view(np.float64)
+    vv = np.ones(len(vals), dtype=bool)
+    pos = 0
+    for g in groups:                      # a null value in every group
+        pos += len(g)
+        vv[pos] = False
+        pos += 1
+    check_grouping([keys], [None], vals, vv, "edge groups")
+    # The same groups spread over many more groups and reversed (device-table path).
+    k2 = np.concatenate([keys * 700 + r for r in range(3)])[::-1].copy()
+    v2 = np.concatenate([vals] * 3)[::-1].copy()
+    vv2 = np.concatenate([vv] * 3)[::-1].copy()
+    check_grouping([k2], [None], v2, vv2, "edge groups, device table")
+    nanv = vectors()["group_sum_nan_payload"]
+    g = am.group_by([am.MetalArray.from_numpy(np.array(nanv["keys"], np.int64))])
+    got = g.sum(np.array(nanv["values"], dtype=np.uint64).view(np.float64)).to_numpy()[0].view(np.uint64)
+    assert got.tolist() == nanv["sums"]
+
+
+@pytest.mark.parametrize("K,spread", [(1, False), (7, True), (1024, False), (1025, True), (5000, False),
+                                      (100000, True)])
+def test_exact_float_sum_random(K, spread):
+    rng = np.random.default_rng(K)
+    n = 400000
+    keys = rng.integers(0, K, n).astype(np.int64)
+    kv = rng.random(n) > 0.02
+    e = rng.integers(-300, 301, n) if spread else rng.integers(-3, 21, n)
+    vals = rng.uniform(-1, 1, n) * np.exp2(e.astype(np.float64))
+    vv = rng.random(n) > 0.1
+    check_grouping([keys], [kv], vals, vv, f"K={K} spread={spread}")
+
+
+def test_cancellation_heavy():
+    rng = np.random.default_rng(42)
+    for K in (3, 2000):
+        m = 200000
+        k = rng.integers(0, K, m)
+        big = rng.uniform(1e15, 1e17, m)
+        small = rng.uniform(-1, 1, m)
+        keys = np.stack([k, k, k], 1).reshape(-1).astype(np.int64)
+        vals = np.stack([big, -big, small], 1).reshape(-1)
+        check_grouping([keys], [None], vals, None, f"cancel K={K}")
+
+
+def test_zipf_keys_match_oracle():
+    """The 100k-key Zipf(1.1) shape of the group-by workload, at 2M rows."""
+    n, K = 2_000_000, 100_000
+    rng = np.random.default_rng(0x21F)
+    w = 1.0 / np.arange(1, K + 1) ** 1.1
+    cdf = np.cumsum(w) / w.sum()
+    ranks = np.searchsorted(cdf, rng.random(n))
+    table = O.splitmix64(0x5EED, K).view(np.int64)
+    keys = table[ranks]
+    ivals = rng.integers(-10**9, 10**9 + 1, n).astype(np.int64)
+    fvals = rng.uniform(-1e6, 1e6, n)
+    g = check_grouping([keys], [None], ivals, None, "zipf int")
+    ids_w, K2, _ = O.group_keys([keys])
+    mv = am.MetalArray.from_numpy(fvals)
+    for op, name in ((0, "sum"), (4, "min"), (5, "max"), (3, "mean")):
+        assert_agg(getattr(g, name)(mv), O.group_agg(ids_w, K2, fvals, op), f"zipf f64 {name}")
+
+
+def test_dense_group_by_out_of_range_and_nulls():
+    rng = np.random.default_rng(3)
+    n, K = 300000, 1025
+    keys = rng.integers(-5, K + 5, n).astype(np.int32)
+    kv = rng.random(n) > 0.05
+    vals = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    vv = rng.random(n) > 0.1
+    mk = am.MetalArray.from_numpy(keys, kv)
+    mv = am.MetalArray.from_numpy(vals, vv)
+    gb = mk.group_by(K)
+    for agg, name in ((0, "sum"), (1, "count"), (4, "mean"), (5, "count_values")):
+        got = gb.count() if agg == 1 else getattr(gb, name)(mv)
+        assert_agg(got, O.group_by_dense(keys, K, vals if agg != 1 else None, agg, kv, vv), f"dense {name}")
+    for it in (np.int64, np.uint32):
+        keys2 = np.abs(keys).astype(it)
+        gb2 = am.MetalArray.from_numpy(keys2, kv).group_by(K)
+        assert_agg(gb2.sum(mv), O.group_by_dense(keys2, K, vals, 0, kv, vv), f"dense sum {it}")
