@@ -120,6 +120,7 @@ struct ProfScope {
 
 struct Array {
   std::atomic<int> refs{1};
+  uint64_t uid = 0;        // unique per array for the life of the process (cache keys)
   std::string format;      // Arrow C data format string, as imported
   int type = T_UNSUPPORTED;
   int64_t length = 0;

@@ -820,9 +820,10 @@ struct GroupBy {
   DevBuf rep;     // uint32 [K]
   std::vector<Array*> key_results;
   // A min and a max of one column share a pass; a sum and a mean of one float64 column share theirs.
-  const Array* mm_col = nullptr;
+  // Keyed by the column's unique id, never its address, which a later array may reuse.
+  uint64_t mm_col = 0;
   DevBuf mm_tab;
-  const Array* fx_col = nullptr;
+  uint64_t fx_col = 0;
   DevBuf fx_sum, fx_mean, fx_valid;
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
@@ -1146,11 +1147,11 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   if (op == 0 || op == 3) {
     if (v->type == T_F64) {
       DevBuf sum, mean, vb;
-      if (cache && cache->fx_col == v) {
+      if (cache && cache->fx_col == v->uid) {
         sum = cache->fx_sum; mean = cache->fx_mean; vb = cache->fx_valid;
       } else {
         exact_f64(a, sum, mean, vb);
-        if (cache) { cache->fx_col = v; cache->fx_sum = sum; cache->fx_mean = mean; cache->fx_valid = vb; }
+        if (cache) { cache->fx_col = v->uid; cache->fx_sum = sum; cache->fx_mean = mean; cache->fx_valid = vb; }
       }
       return bits_array(op == 0 ? sum : mean, vb, K, "g", T_F64);
     }
@@ -1164,11 +1165,11 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   }
   if (op == 4 || op == 5) {
     DevBuf t;
-    if (cache && cache->mm_col == v) {
+    if (cache && cache->mm_col == v->uid) {
       t = cache->mm_tab;
     } else {
       t = run_agg<MinMaxP>(a, s, "gb_minmax");
-      if (cache) { cache->mm_col = v; cache->mm_tab = t; }
+      if (cache) { cache->mm_col = v->uid; cache->mm_tab = t; }
     }
     return finalize_simple(t, 2, K, op, v->type, v->format, v->type);
   }

@@ -303,7 +303,9 @@ std::string prof_report() {
 // ---------------------------------------------------------------- arrays
 
 Array* new_array(const std::string& format, int type, int64_t length) {
+  static std::atomic<uint64_t> next_uid{1};
   Array* a = new Array();
+  a->uid = next_uid.fetch_add(1, std::memory_order_relaxed);
   a->format = format;
   a->type = type;
   a->length = length;

@@ -14,8 +14,10 @@
 //                  exclusive digit prefix of every pass
 //   sort_first     the first active pass reads the raw column again, maps keys on the fly, and
 //                  scatters values by digit while NaN and null rows go straight to their final slots
-//   sort_mid<p>    one launch per digit; an inactive digit exits at once; the last active pass writes
-//                  only the row numbers, into the output
+//   sort_mid       one launch per further active digit; the last active pass writes only the row
+//                  numbers, into the output. Inputs of 2^18 rows and more read the plan back once and
+//                  launch only the active passes; smaller ones (and graph captures) launch one kernel
+//                  per digit that exits at once when its digit is inactive
 //
 // Each scatter pass is "onesweep": tiles of 256 threads x ITEMS keys take dynamic tile ids, rank
 // their keys inside the tile with wave64 ballots (a 9-bit match per 64-lane row), publish their
@@ -179,6 +181,14 @@ __global__ void sort_init(SortCtl* ctl) {
   for (int i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0u;
   __syncthreads();
   if (threadIdx.x == 0) ctl->key_and = ~0ull;
+}
+
+// Copies the plan (active mask, first and last active pass, value rows) to pinned host memory.
+__global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
+  out[0] = ctl->active_mask;
+  out[1] = ctl->first_active;
+  out[2] = ctl->last_active;
+  out[3] = ctl->m;
 }
 
 // Zeroes the look-back status table (only when it is first allocated or grows).
@@ -450,7 +460,11 @@ __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint
   }
 }
 
-template <typename K, int ITEMS>
+// MODE: PASS_MID and PASS_LAST are launched by a host that read the plan back (only active passes,
+// the last one known); PASS_DEVICE is launched for every digit and decides everything on the GPU.
+constexpr int PASS_MID = 0, PASS_LAST = 1, PASS_DEVICE = 2;
+
+template <typename K, int ITEMS, int MODE>
 __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsigned long long* status,
                                                   uint32_t epoch, K* keys0, uint32_t* rows0, K* keys1,
                                                   uint32_t* rows1, uint32_t* out_idx) {
@@ -476,7 +490,7 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
   const uint32_t tile = s_tile;
   if (tile >= ntiles) return;   // surplus block: the grid is sized for every row, the pass for values
   const uint32_t rank = uint32_t(__builtin_popcount(mask & ((1u << pass) - 1u)));
-  const bool last = uint32_t(pass) == ctl->last_active;
+  const bool last = MODE == PASS_DEVICE ? uint32_t(pass) == ctl->last_active : MODE == PASS_LAST;
   const K* in_keys = (rank & 1u) ? keys0 : keys1;
   const uint32_t* in_rows = (rank & 1u) ? rows0 : rows1;
   K* out_keys = (rank & 1u) ? keys1 : keys0;
@@ -545,6 +559,19 @@ uint32_t reserve_status(size_t entries, hipStream_t s) {
   return e;
 }
 
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Pinned per-thread landing place of the plan readback.
+uint32_t* host_plan_words() {
+  thread_local uint32_t* p = nullptr;
+  if (!p) AM_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocDefault));
+  return p;
+}
+
 template <typename K, int ITEMS>
 void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
               int placement, int forder, uint32_t* out_idx, hipStream_t s) {
@@ -579,6 +606,17 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     hipLaunchKernelGGL(sort_plan<P>, dim3(1), dim3(BLOCK), 0, s, ctl, placement == 1 ? 1 : 0);
     AM_HIP(hipGetLastError());
   }
+  // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
+  // that only active passes are launched, over the value rows only.
+  struct Plan { uint32_t mask, first, last, m; } plan{0xFFu, 0, uint32_t(P - 1), n};
+  const bool analyse = n >= (1u << 18) && !capturing(s);
+  if (analyse) {
+    Plan* hp = reinterpret_cast<Plan*>(host_plan_words());
+    hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp));
+    AM_HIP(hipGetLastError());
+    AM_HIP(hipStreamSynchronize(s));
+    plan = *hp;
+  }
   {
     ProfScope ps("sort_first");
     hipLaunchKernelGGL((sort_first<K, ITEMS>), dim3(tiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
@@ -586,12 +624,34 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
                        out_idx);
     AM_HIP(hipGetLastError());
   }
-  for (int p = 0; p < P; p++) {
-    ProfScope ps("sort_pass");
-    hipLaunchKernelGGL((sort_mid<K, ITEMS>), dim3(tiles), dim3(BLOCK), 0, s, p, ctl, status, epoch + 1u + uint32_t(p),
-                       static_cast<K*>(k0->ptr), static_cast<uint32_t*>(r0->ptr), static_cast<K*>(k1->ptr),
-                       static_cast<uint32_t*>(r1->ptr), out_idx);
-    AM_HIP(hipGetLastError());
+  K* kp0 = static_cast<K*>(k0->ptr);
+  K* kp1 = static_cast<K*>(k1->ptr);
+  uint32_t* rp0 = static_cast<uint32_t*>(r0->ptr);
+  uint32_t* rp1 = static_cast<uint32_t*>(r1->ptr);
+  if (analyse) {
+    // The host read the plan back: launch only the active passes, each over the value rows, with
+    // the last one's kernel known (no key writes).
+    const uint32_t mtiles = std::max<uint32_t>(1, (plan.m + TILE - 1) / TILE);
+    for (int p = 0; p < P; p++) {
+      if (!((plan.mask >> p) & 1u) || uint32_t(p) == plan.first) continue;
+      if (uint32_t(p) == plan.last) {
+        ProfScope ps("sort_scatter_last");
+        hipLaunchKernelGGL((sort_mid<K, ITEMS, PASS_LAST>), dim3(mtiles), dim3(BLOCK), 0, s, p, ctl, status,
+                           epoch + 1u + uint32_t(p), kp0, rp0, kp1, rp1, out_idx);
+      } else {
+        ProfScope ps("sort_scatter");
+        hipLaunchKernelGGL((sort_mid<K, ITEMS, PASS_MID>), dim3(mtiles), dim3(BLOCK), 0, s, p, ctl, status,
+                           epoch + 1u + uint32_t(p), kp0, rp0, kp1, rp1, out_idx);
+      }
+      AM_HIP(hipGetLastError());
+    }
+  } else {
+    for (int p = 0; p < P; p++) {
+      ProfScope ps("sort_pass_dev");
+      hipLaunchKernelGGL((sort_mid<K, ITEMS, PASS_DEVICE>), dim3(tiles), dim3(BLOCK), 0, s, p, ctl, status,
+                         epoch + 1u + uint32_t(p), kp0, rp0, kp1, rp1, out_idx);
+      AM_HIP(hipGetLastError());
+    }
   }
 }
 
