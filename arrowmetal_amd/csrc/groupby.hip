@@ -116,103 +116,195 @@ __global__ void gb_table_init(Slot* table, size_t slots, BuildStatus* st) {
   }
 }
 
+// A slot's lowest row only decreases, so a plain (possibly stale) read is a safe filter: a stale
+// value is never below the true one, and the atomic settles every row that could lower it.
 __device__ __forceinline__ void lower_first(uint32_t* p, uint32_t row) {
-  if (row < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, row);
+  if (row < *p) atomicMin(p, row);
 }
 
-// Inserts every row's key; writes the slot of each row (NULL_SLOT for a null key).
+// Probes (and inserts) key k, starting at its hash slot. Returns its slot, or NULL_SLOT when the probe
+// budget ran out (the table is too full: the caller grows it and builds again).
+__device__ __forceinline__ uint32_t probe_insert(Slot* table, unsigned long long k, uint32_t s, unsigned long long cur,
+                                                 uint32_t cap_mask, uint32_t budget) {
+  uint32_t probes = 0;
+  while (true) {
+    if (cur == k) return s;
+    if (cur == EMPTY) {
+      unsigned long long prev = atomicCAS(&table[s].key, EMPTY, k);
+      if (prev == EMPTY || prev == k) return s;
+    }
+    if (++probes >= budget) return NULL_SLOT;
+    s = (s + 1u) & cap_mask;
+    // A slot's key goes from EMPTY to its key once and never changes again, so a plain read is
+    // enough: a stale EMPTY only sends the row to the CAS, which returns the settled key.
+    cur = table[s].key;
+  }
+}
+
+// Inserts every row's key; writes the slot of each row (NULL_SLOT for a null key). Each thread keeps
+// BUILD_ROWS rows in flight: their keys are loaded, hashed and their first slots read together, so a
+// wave waits on one memory latency per BUILD_ROWS rows rather than per row. TRACK_FIRST also keeps
+// every slot's lowest row (the representative rows of float and multi-column keys).
+constexpr int BUILD_ROWS = 8;
+
+template <bool TRACK_FIRST>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Slot* table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t special = cap_mask + 1u;
   unsigned long long nulls = 0;
-  for (; i < n; i += stride) {
-    if (valid && !bit_get(valid, voff + int64_t(i))) {
-      slot_of[i] = NULL_SLOT;
-      nulls++;
-      lower_first(&st->null_first, i);
-      continue;
-    }
-    unsigned long long k = group_key(vals, i, type);
-    uint32_t s;
-    if (k == EMPTY) {
-      s = special;
-      if (!__hip_atomic_load(&st->special_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicExch(&st->special_used, 1u);
-    } else {
-      s = uint32_t(mix64(k)) & cap_mask;
-      uint32_t probes = 0;
-      while (true) {
-        unsigned long long cur = __hip_atomic_load(&table[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur == k) break;
-        if (cur == EMPTY) {
-          unsigned long long prev = atomicCAS(&table[s].key, EMPTY, k);
-          if (prev == EMPTY || prev == k) break;
+  bool overflow = false;
+  for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
+    unsigned long long k[BUILD_ROWS], cur[BUILD_ROWS];
+    uint32_t s[BUILD_ROWS];
+    int cat[BUILD_ROWS];   // 0 key, 1 null, 2 the EMPTY key, 3 past the end
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {
+      uint64_t i = base + uint64_t(r) * gsize + tid;
+      cat[r] = 3;
+      k[r] = 0;
+      if (i < n) {
+        if (valid && !bit_get(valid, voff + int64_t(i))) cat[r] = 1;
+        else {
+          k[r] = group_key(vals, int64_t(i), type);
+          cat[r] = k[r] == EMPTY ? 2 : 0;
         }
-        if (++probes >= budget) {
-          atomicOr(&st->overflow, 1u);
-          s = NULL_SLOT;
-          break;
-        }
-        s = (s + 1u) & cap_mask;
       }
-      if (s == NULL_SLOT) {
-        slot_of[i] = NULL_SLOT;
-        continue;
-      }
+      s[r] = uint32_t(mix64(k[r])) & cap_mask;
     }
-    slot_of[i] = s;
-    lower_first(&table[s].first, i);
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? table[s[r]].key : 0ull;
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {
+      uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
+      if (cat[r] == 3) continue;
+      uint32_t slot;
+      if (cat[r] == 1) {
+        slot = NULL_SLOT;
+        nulls++;
+        if (TRACK_FIRST) lower_first(&st->null_first, i);
+      } else if (cat[r] == 2) {
+        slot = special;
+        if (!st->special_used) atomicExch(&st->special_used, 1u);
+      } else {
+        slot = probe_insert(table, k[r], s[r], cur[r], cap_mask, budget);
+        if (slot == NULL_SLOT) overflow = true;
+      }
+      slot_of[i] = slot;
+      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&table[slot].first, i);
+    }
   }
+  if (overflow) atomicOr(&st->overflow, 1u);
   for (int off = 32; off > 0; off >>= 1) nulls += __shfl_down(nulls, off, 64);
   if ((threadIdx.x & 63) == 0 && nulls) atomicAdd(&st->nulls, nulls);
 }
 
-// Occupied slots -> (order key, slot). The special slot C is occupied when the EMPTY key was seen.
+// Occupied slots -> (order key, slot), in slot order. One block per contiguous range of slots: a
+// counting sweep, one atomic for the block's base, then the writing sweep. The special slot C is
+// occupied when the EMPTY key was seen.
 __global__ __launch_bounds__(256) void gb_compact(const Slot* table, uint32_t cap, int type, BuildStatus* st,
-                                                  unsigned long long* okeys, uint32_t* slots) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  bool occ = false;
-  unsigned long long k = 0;
-  if (s < cap) {
-    k = table[s].key;
-    occ = k != EMPTY;
-  } else if (s == cap) {
-    occ = st->special_used != 0;
-    k = EMPTY;
+                                                  unsigned long long* okeys, uint32_t* slots, uint32_t per_block) {
+  __shared__ uint32_t wave_cnt[4];
+  __shared__ uint32_t s_base;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t begin = blockIdx.x * per_block;
+  const uint32_t end = min(cap + 1u, begin + per_block);
+  auto occupied = [&](uint32_t x, unsigned long long& key) -> bool {
+    if (x < cap) {
+      key = table[x].key;
+      return key != EMPTY;
+    }
+    key = EMPTY;
+    return x == cap && st->special_used != 0;
+  };
+  uint32_t mine = 0;
+  for (uint32_t x = begin + threadIdx.x; x - threadIdx.x < end; x += 256) {
+    unsigned long long key;
+    bool occ = x < end && occupied(x, key);
+    mine += occ ? 1u : 0u;
   }
-  unsigned long long bal = __ballot(occ);
-  uint32_t base = 0;
-  if (lane == 0 && bal) base = atomicAdd(&st->distinct, uint32_t(__popcll(bal)));
-  base = __shfl(base, 0, 64);
-  if (occ) {
-    uint32_t at = base + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
-    okeys[at] = order_key(k, type);
-    slots[at] = s;
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
+  if (lane == 0) wave_cnt[w] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    s_base = tot ? atomicAdd(&st->distinct, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t run = s_base;
+  for (uint32_t x = begin + threadIdx.x; x - threadIdx.x < end; x += 256) {
+    unsigned long long key = 0;
+    bool occ = x < end && occupied(x, key);
+    unsigned long long bal = __ballot(occ);
+    // The waves of one step write in wave order: each needs the counts of the waves before it.
+    __syncthreads();
+    if (lane == 0) wave_cnt[w] = uint32_t(__popcll(bal));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int v = 0; v < 4; v++) {
+      before += v < w ? wave_cnt[v] : 0u;
+      total += wave_cnt[v];
+    }
+    if (occ) {
+      uint32_t at = run + before + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
+      okeys[at] = order_key(key, type);
+      slots[at] = x;
+    }
+    run += total;
   }
 }
 
 __global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t D, const Slot* table,
-                           uint32_t* gid_of_slot, uint32_t* rep, const BuildStatus* st, int has_null) {
+                           uint32_t* gid_of_slot, uint32_t* rep, const BuildStatus* st, int has_null, int track) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < D) {
     uint32_t s = slots[perm[r]];
     gid_of_slot[s] = r;
-    rep[r] = table[s].first;
-  } else if (r == D && has_null) {
+    if (track) rep[r] = table[s].first;
+  } else if (r == D && has_null && track) {
     rep[D] = st->null_first;
   }
 }
 
-__global__ void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot, uint32_t null_gid) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    uint32_t s = slot_of[i];
-    slot_of[i] = s == NULL_SLOT ? null_gid : gid_of_slot[s];
+__global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot,
+                                              uint32_t null_gid) {
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
+    uint32_t sl[BUILD_ROWS];
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {
+      uint64_t i = base + uint64_t(r) * gsize + tid;
+      sl[r] = i < n ? slot_of[i] : NULL_SLOT;
+    }
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {
+      uint64_t i = base + uint64_t(r) * gsize + tid;
+      if (i < n) slot_of[i] = sl[r] == NULL_SLOT ? null_gid : gid_of_slot[sl[r]];
+    }
+  }
+}
+
+// The group keys of an integer column, straight from the sorted distinct order keys (no
+// representative rows needed: every row of a group holds the same integer). The null group, last,
+// is written as 0 and marked null.
+__global__ void gb_int_keys(const unsigned long long* okeys, const uint32_t* perm, uint32_t D, int type,
+                            void* out, uint8_t* vb, int has_null) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > D || (r == D && !has_null)) return;
+  unsigned long long v = 0;
+  if (r < D) {
+    unsigned long long ok = okeys[perm[r]];
+    v = (type == T_I8 || type == T_I16 || type == T_I32 || type == T_I64) ? ok ^ 0x8000000000000000ull : ok;
+  }
+  vb[r] = r < D;
+  switch (type_width(type)) {
+    case 1: static_cast<uint8_t*>(out)[r] = uint8_t(v); break;
+    case 2: static_cast<uint16_t*>(out)[r] = uint16_t(v); break;
+    case 4: static_cast<uint32_t*>(out)[r] = uint32_t(v); break;
+    default: static_cast<unsigned long long*>(out)[r] = v; break;
   }
 }
 
@@ -287,9 +379,9 @@ struct AggArgs {
 template <bool VALID_ONLY>
 struct CountP {
   static constexpr int W = 1;
+  static constexpr bool SKIP_NULL = VALID_ONLY;
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
   __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[1]) {
-    if (VALID_ONLY && !val_valid(a.v, i)) return false;
     c[0] = 1;
     return true;
   }
@@ -298,9 +390,9 @@ struct CountP {
 // Integer sum (wrapping) with the count of non-null values (null result where it is zero).
 struct SumCountP {
   static constexpr int W = 2;
+  static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
   __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
-    if (!val_valid(a.v, i)) return false;
     c[0] = raw64(a.v, i);
     c[1] = 1;
     return true;
@@ -311,9 +403,9 @@ struct SumCountP {
 // its min key <= its max key, which the identities (all ones, zero) never satisfy.
 struct MinMaxP {
   static constexpr int W = 2;
+  static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MIN : W_MAX; }
   __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
-    if (!val_valid(a.v, i)) return false;
     unsigned long long b = raw64(a.v, i), key;
     switch (a.v.type) {
       case T_F64:
@@ -340,9 +432,9 @@ struct MinMaxP {
 // the lowest row holding a NaN.
 struct F64AP {
   static constexpr int W = 4;
+  static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MAX : (w == 1 ? W_OR : (w == 2 ? W_ADD : W_MIN)); }
   __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[4]) {
-    if (!val_valid(a.v, i)) return false;
     unsigned long long v = raw64(a.v, i);
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     unsigned long long mag = v & 0x7FFFFFFFFFFFFFFFull;
@@ -389,9 +481,9 @@ __device__ __forceinline__ void fx_fixed(unsigned long long v, uint32_t U, unsig
 // the count of values truncated by the window. aux = the pass-A table (4 words per group).
 struct F64BP {
   static constexpr int W = 3;
+  static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_ADD128 : (w == 1 ? W_HI : W_ADD); }
   __device__ static bool contrib(const AggArgs& a, int64_t i, long long g, unsigned long long (&c)[3]) {
-    if (!val_valid(a.v, i)) return false;
     unsigned long long v = raw64(a.v, i);
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) return false;
@@ -414,30 +506,44 @@ struct F64BP {
 
 __device__ __forceinline__ unsigned long long identity(int op) { return op == W_MIN ? ~0ull : 0ull; }
 
-// One word update, LDS or global (the generic atomics pick the address space). MIN / MAX / OR read
-// first and skip the atomic when it cannot change anything, which keeps hot groups read-mostly.
+// The XCD (accelerator complex die) this wave runs on, from the XCC_ID hardware register. Each XCD
+// has its own L2: a table private to one XCD can be updated with workgroup-scope atomics, which the
+// XCD's L2 executes, instead of device-scope ones, which must go to memory.
+constexpr int NXCD = 8;
+__device__ __forceinline__ uint32_t xcd_id() {
+  return uint32_t(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & (NXCD - 1);
+}
+
+// One word update in LDS or in this XCD's private global table; workgroup-scope atomics in both
+// (ds_* in LDS, L2-resident in global memory). MIN / MAX / OR read first and skip the atomic when it
+// cannot change anything, which keeps hot groups read-mostly: a stale read never lets a needed
+// update be skipped, because those words only ever move one way.
 template <int OP>
 __device__ __forceinline__ void word_update(unsigned long long* p, unsigned long long x) {
   if (OP == W_ADD) {
-    if (x) atomicAdd(p, x);
+    if (x) __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MIN) {
-    if (x < *reinterpret_cast<volatile unsigned long long*>(p)) atomicMin(p, x);
+    if (x < *reinterpret_cast<volatile unsigned long long*>(p))
+      __hip_atomic_fetch_min(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MAX) {
-    if (x > *reinterpret_cast<volatile unsigned long long*>(p)) atomicMax(p, x);
+    if (x > *reinterpret_cast<volatile unsigned long long*>(p))
+      __hip_atomic_fetch_max(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_OR) {
-    if (x & ~*reinterpret_cast<volatile unsigned long long*>(p)) atomicOr(p, x);
+    if (x & ~*reinterpret_cast<volatile unsigned long long*>(p))
+      __hip_atomic_fetch_or(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
-// 128-bit two's-complement add at p[0] (lo), p[1] (hi): exact modulo 2^128 under any interleaving.
+// 128-bit two's-complement add at p[0] (lo), p[1] (hi): exact modulo 2^128 under any interleaving,
+// because each carry comes from the lo word's own read-modify-write.
 __device__ __forceinline__ void add128(unsigned long long* p, unsigned long long lo, unsigned long long hi) {
   unsigned long long carry = 0;
   if (lo) {
-    unsigned long long old = atomicAdd(&p[0], lo);
+    unsigned long long old = __hip_atomic_fetch_add(&p[0], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     carry = (old + lo < old) ? 1ull : 0ull;
   }
   unsigned long long h = hi + carry;
-  if (h) atomicAdd(&p[1], h);
+  if (h) __hip_atomic_fetch_add(&p[1], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 template <class P>
@@ -454,7 +560,7 @@ __device__ __forceinline__ void apply(unsigned long long* acc, const unsigned lo
   }
 }
 
-// Merges one LDS accumulator into the global table (skipping identities).
+// Merges one LDS accumulator into this XCD's table (skipping identities).
 template <class P>
 __device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsigned long long* lacc) {
   unsigned long long c[P::W];
@@ -467,15 +573,19 @@ __device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsign
   if (any) apply<P>(gacc, c);
 }
 
+constexpr int AGG_ROWS = 4;   // rows per thread whose loads are issued together
+
 // Mode 0: every group has a private LDS accumulator (K <= cap). Mode 1: an LDS cache of `cap`
-// groups, claimed by CAS on a key word, in front of the global table; a row whose group cannot be
-// cached within 8 probes goes to the global table directly.
+// groups, claimed by CAS on a key word, in front of this XCD's table in global memory (L2); a row
+// whose group cannot be cached within 8 probes goes to that table directly. The tables of the
+// eight XCDs are merged by gb_agg_merge.
 template <class P>
 __global__ __launch_bounds__(256) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rows_per_block,
-                                              unsigned long long* gtab) {
+                                              unsigned long long* gtab8) {
   extern __shared__ unsigned long long lds[];
   unsigned long long* lacc = lds;                                     // [cap][W]
   unsigned int* lkey = reinterpret_cast<unsigned int*>(lds + size_t(cap) * P::W);   // [cap] (mode 1)
+  unsigned long long* gtab = gtab8 + size_t(xcd_id()) * size_t(a.K) * P::W;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
 #pragma unroll
     for (int w = 0; w < P::W; w++) lacc[size_t(s) * P::W + w] = identity(P::op(w));
@@ -484,39 +594,53 @@ __global__ __launch_bounds__(256) void gb_agg(AggArgs a, int mode, uint32_t cap,
   __syncthreads();
   const int64_t begin = int64_t(blockIdx.x) * rows_per_block;
   const int64_t end = std::min<int64_t>(a.n, begin + rows_per_block);
-  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    long long g = load_gid(a.ids, i, a.K);
-    if (g < 0) continue;
-    unsigned long long c[P::W];
-    if (!P::contrib(a, i, g, c)) continue;
-    if (mode == 0) {
-      apply<P>(&lacc[size_t(g) * P::W], c);
-      continue;
-    }
-    uint32_t h = uint32_t(g) * 2654435761u;
-    uint32_t s = h & (cap - 1u);
-    bool done = false;
-    for (int probe = 0; probe < 8; probe++) {
-      unsigned int cur = lkey[s];
-      if (cur == 0xFFFFFFFFu) {
-        unsigned int prev = atomicCAS(&lkey[s], 0xFFFFFFFFu, uint32_t(g));
-        cur = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
+  for (int64_t base = begin; base < end; base += int64_t(blockDim.x) * AGG_ROWS) {
+    long long gs[AGG_ROWS];
+    unsigned long long cs[AGG_ROWS][P::W];
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {
+      int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+      long long g = -1;
+      if (i < end) {
+        g = load_gid(a.ids, i, a.K);
+        if (P::SKIP_NULL && g >= 0 && !val_valid(a.v, i)) g = -1;
+        if (g >= 0 && !P::contrib(a, i, g, cs[r])) g = -1;
       }
-      if (cur == uint32_t(g)) {
-        apply<P>(&lacc[size_t(s) * P::W], c);
-        done = true;
-        break;
-      }
-      s = (s + 1u) & (cap - 1u);
+      gs[r] = g;
     }
-    if (!done) apply<P>(&gtab[size_t(g) * P::W], c);
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {
+      const long long g = gs[r];
+      if (g < 0) continue;
+      if (mode == 0) {
+        apply<P>(&lacc[size_t(g) * P::W], cs[r]);
+        continue;
+      }
+      uint32_t h = uint32_t(g) * 2654435761u;
+      uint32_t sl = (h ^ (h >> 15)) & (cap - 1u);
+      bool done = false;
+      for (int probe = 0; probe < 8; probe++) {
+        unsigned int cur = lkey[sl];
+        if (cur == 0xFFFFFFFFu) {
+          unsigned int prev = atomicCAS(&lkey[sl], 0xFFFFFFFFu, uint32_t(g));
+          cur = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
+        }
+        if (cur == uint32_t(g)) {
+          apply<P>(&lacc[size_t(sl) * P::W], cs[r]);
+          done = true;
+          break;
+        }
+        sl = (sl + 1u) & (cap - 1u);
+      }
+      if (!done) apply<P>(&gtab[size_t(g) * P::W], cs[r]);
+    }
   }
   __syncthreads();
-  for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
+  for (uint32_t sl = threadIdx.x; sl < cap; sl += blockDim.x) {
     long long g;
-    if (mode == 0) g = s < uint32_t(a.K) ? (long long)s : -1;
-    else g = lkey[s] == 0xFFFFFFFFu ? -1 : (long long)lkey[s];
-    if (g >= 0) flush_one<P>(&gtab[size_t(g) * P::W], &lacc[size_t(s) * P::W]);
+    if (mode == 0) g = sl < uint32_t(a.K) ? (long long)sl : -1;
+    else g = lkey[sl] == 0xFFFFFFFFu ? -1 : (long long)lkey[sl];
+    if (g >= 0) flush_one<P>(&gtab[size_t(g) * P::W], &lacc[size_t(sl) * P::W]);
   }
 }
 
@@ -527,14 +651,49 @@ __global__ void gb_agg_init(unsigned long long* gtab, long long K) {
   for (; i < K * P::W; i += stride) gtab[i] = identity(P::op(int(i % P::W)));
 }
 
+// Folds the eight per-XCD tables [8][K][W] into one [K][W] table.
+template <class P>
+__global__ void gb_agg_merge(const unsigned long long* gtab8, long long K, unsigned long long* out) {
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= K) return;
+#pragma unroll
+  for (int w = 0; w < P::W; w++) {
+    const int op = P::op(w);
+    if (op == W_HI) continue;
+    unsigned long long acc = identity(op), hi = 0;
+    for (int x = 0; x < NXCD; x++) {
+      const unsigned long long* t = gtab8 + (size_t(x) * size_t(K) + size_t(g)) * P::W;
+      unsigned long long v = t[w];
+      if (op == W_ADD) acc += v;
+      else if (op == W_MIN) acc = v < acc ? v : acc;
+      else if (op == W_MAX) acc = v > acc ? v : acc;
+      else if (op == W_OR) acc |= v;
+      else {   // W_ADD128: lo in word w, hi in word w + 1
+        unsigned long long lo2 = acc + v;
+        hi += t[w + 1 < P::W ? w + 1 : w] + (lo2 < acc ? 1ull : 0ull);
+        acc = lo2;
+      }
+    }
+    out[size_t(g) * P::W + w] = acc;
+    if (op == W_ADD128 && w + 1 < P::W) out[size_t(g) * P::W + w + 1] = hi;
+  }
+}
+
 // Runs family P over the rows into a fresh global table of K x W words.
 template <class P>
 DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
-  DevBuf gtab = device_alloc(size_t(std::max<long long>(a.K, 1)) * P::W * 8);
-  auto* g = static_cast<unsigned long long*>(gtab->ptr);
-  hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(a.K * P::W, 4)), dim3(256), 0, s, g, a.K);
+  const long long K = std::max<long long>(a.K, 1);
+  DevBuf out = device_alloc(size_t(K) * P::W * 8);
+  auto* o = static_cast<unsigned long long*>(out->ptr);
+  if (a.n == 0) {
+    hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(K * P::W, 4)), dim3(256), 0, s, o, K);
+    AM_HIP(hipGetLastError());
+    return out;
+  }
+  DevBuf tabs = device_alloc(size_t(NXCD) * size_t(K) * P::W * 8);
+  auto* g8 = static_cast<unsigned long long*>(tabs->ptr);
+  hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(NXCD * K * P::W, 4)), dim3(256), 0, s, g8, NXCD * K);
   AM_HIP(hipGetLastError());
-  if (a.n == 0) return gtab;
   // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
   const size_t budget = 48 * 1024;
   const uint32_t per_slot = P::W * 8;
@@ -553,10 +712,14 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
   int blocks = int(std::min<int64_t>((a.n + 4095) / 4096, int64_t(device().cu_count) * 4));
   blocks = std::max(blocks, 1);
   int64_t rpb = (a.n + blocks - 1) / blocks;
-  ProfScope ps(name);
-  hipLaunchKernelGGL(gb_agg<P>, dim3(blocks), dim3(256), shmem, s, a, mode, cap, rpb, g);
+  {
+    ProfScope ps(name);
+    hipLaunchKernelGGL(gb_agg<P>, dim3(blocks), dim3(256), shmem, s, a, mode, cap, rpb, g8);
+    AM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(gb_agg_merge<P>, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s, g8, K, o);
   AM_HIP(hipGetLastError());
-  return gtab;
+  return out;
 }
 
 // ---------------------------------------------------------------- finalize kernels
@@ -817,14 +980,23 @@ struct GroupBy {
   int64_t n = 0;
   int64_t K = 0;
   DevBuf ids;     // int32 [n]
-  DevBuf rep;     // uint32 [K]
+  DevBuf rep;     // uint32 [K], the lowest row of each group (when tracked)
+  bool has_rep = false;
+  // An integer single-column mapping keeps its sorted distinct keys instead of representative rows.
+  DevBuf okeys, perm;
+  uint32_t D = 0;
+  bool has_null = false;
   std::vector<Array*> key_results;
-  // A min and a max of one column share a pass; a sum and a mean of one float64 column share theirs.
-  // Keyed by the column's unique id, never its address, which a later array may reuse.
+  // A min and a max of one column share a pass; a sum and a mean of one float64 column share theirs;
+  // an integer sum also yields the column's count of non-null values. Keyed by the column's unique
+  // id, never its address, which a later array may reuse.
   uint64_t mm_col = 0;
   DevBuf mm_tab;
   uint64_t fx_col = 0;
   DevBuf fx_sum, fx_mean, fx_valid;
+  uint64_t cnt_col = 0;
+  DevBuf cnt_tab;          // K x 1 count of non-null values of column cnt_col
+  DevBuf count_all_tab;    // K x 1 rows per group
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
     for (Array* a : key_results) release_array(a);
@@ -833,13 +1005,20 @@ struct GroupBy {
 
 namespace {
 
-// Maps one key column (values, type, validity) to dense ascending ids. Returns K and fills `ids`
-// (int32 [n]) and `rep` (uint32 [K]).
-int64_t map_column(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, DevBuf& ids_out,
-                   DevBuf& rep_out) {
+struct Mapped {
+  int64_t K = 0;
+  uint32_t D = 0;
+  bool has_null = false;
+  DevBuf ids, rep, okeys, perm;
+};
+
+// Maps one key column (values, type, validity) to dense ascending ids; with `track` it also finds
+// the lowest row of every group.
+Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track) {
   hipStream_t s = stream();
   uint32_t un = uint32_t(n);
-  DevBuf ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
+  Mapped out;
+  out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
   DevBuf stb = device_alloc(sizeof(BuildStatus));
   auto* st = static_cast<BuildStatus*>(stb->ptr);
   // Capacity: two slots per row up to 2^21 slots; beyond that, grow on overflow.
@@ -848,6 +1027,7 @@ int64_t map_column(const void* vals, int type, const uint8_t* valid, int64_t vof
   while (cap < want && cap < (1ull << 21)) cap <<= 1;
   BuildStatus host{};
   DevBuf table;
+  const int build_grid = grid_for((n + BUILD_ROWS - 1) / BUILD_ROWS, 8);
   for (int attempt = 0;; attempt++) {
     bool last_try = cap >= want;
     table = device_alloc(size_t(cap + 1) * sizeof(Slot));
@@ -856,8 +1036,13 @@ int64_t map_column(const void* vals, int type, const uint8_t* valid, int64_t vof
     AM_HIP(hipGetLastError());
     {
       ProfScope ps("gb_build");
-      hipLaunchKernelGGL(gb_build, dim3(grid_for(n)), dim3(256), 0, s, vals, type, valid, voff, un, T,
-                         uint32_t(cap - 1), last_try ? uint32_t(cap) : 128u, static_cast<uint32_t*>(ids->ptr), st);
+      uint32_t budget = last_try ? uint32_t(cap) : 128u;
+      if (track)
+        hipLaunchKernelGGL(gb_build<true>, dim3(build_grid), dim3(256), 0, s, vals, type, valid, voff, un, T,
+                           uint32_t(cap - 1), budget, static_cast<uint32_t*>(out.ids->ptr), st);
+      else
+        hipLaunchKernelGGL(gb_build<false>, dim3(build_grid), dim3(256), 0, s, vals, type, valid, voff, un, T,
+                           uint32_t(cap - 1), budget, static_cast<uint32_t*>(out.ids->ptr), st);
       AM_HIP(hipGetLastError());
     }
     AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
@@ -868,46 +1053,45 @@ int64_t map_column(const void* vals, int type, const uint8_t* valid, int64_t vof
   }
   auto* T = static_cast<Slot*>(table->ptr);
   // Compact the occupied slots and sort their keys.
-  DevBuf okeys = device_alloc(size_t(cap + 1) * 8), slots = device_alloc(size_t(cap + 1) * 4);
+  out.okeys = device_alloc(size_t(cap + 1) * 8);
+  DevBuf slots = device_alloc(size_t(cap + 1) * 4);
   {
     ProfScope ps("gb_compact");
-    hipLaunchKernelGGL(gb_compact, dim3(uint32_t((cap + 1 + 255) / 256)), dim3(256), 0, s, T, uint32_t(cap), type, st,
-                       static_cast<unsigned long long*>(okeys->ptr), static_cast<uint32_t*>(slots->ptr));
+    uint32_t blocks = uint32_t(std::min<uint64_t>((cap + 1 + 4095) / 4096, uint64_t(device().cu_count) * 4));
+    uint32_t per_block = uint32_t((cap + 1 + blocks - 1) / blocks);
+    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, T, uint32_t(cap), type, st,
+                       static_cast<unsigned long long*>(out.okeys->ptr), static_cast<uint32_t*>(slots->ptr), per_block);
     AM_HIP(hipGetLastError());
   }
   AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
   AM_HIP(hipStreamSynchronize(s));
-  uint32_t D = host.distinct;
-  bool has_null = host.nulls > 0;
-  int64_t K = int64_t(D) + (has_null ? 1 : 0);
-  DevBuf perm = device_alloc(size_t(std::max<uint32_t>(D, 1)) * 4);
-  if (D > 0)
-    argsort_device(okeys->ptr, T_U64, nullptr, 0, D, false, 0, 1, static_cast<uint32_t*>(perm->ptr), s);
+  out.D = host.distinct;
+  out.has_null = host.nulls > 0;
+  out.K = int64_t(out.D) + (out.has_null ? 1 : 0);
+  out.perm = device_alloc(size_t(std::max<uint32_t>(out.D, 1)) * 4);
+  if (out.D > 0)
+    argsort_device(out.okeys->ptr, T_U64, nullptr, 0, out.D, false, 0, 1, static_cast<uint32_t*>(out.perm->ptr), s);
   DevBuf gid_of_slot = device_alloc(size_t(cap + 1) * 4);
-  DevBuf rep = device_alloc(size_t(std::max<int64_t>(K, 1)) * 4);
+  out.rep = device_alloc(size_t(std::max<int64_t>(out.K, 1)) * 4);
   {
     ProfScope ps("gb_relabel");
-    hipLaunchKernelGGL(gb_relabel, dim3(uint32_t((D + 1 + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const uint32_t*>(perm->ptr), static_cast<const uint32_t*>(slots->ptr), D, T,
-                       static_cast<uint32_t*>(gid_of_slot->ptr), static_cast<uint32_t*>(rep->ptr), st,
-                       has_null ? 1 : 0);
+    hipLaunchKernelGGL(gb_relabel, dim3(uint32_t((out.D + 1 + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(out.perm->ptr), static_cast<const uint32_t*>(slots->ptr), out.D, T,
+                       static_cast<uint32_t*>(gid_of_slot->ptr), static_cast<uint32_t*>(out.rep->ptr), st,
+                       out.has_null ? 1 : 0, track ? 1 : 0);
     AM_HIP(hipGetLastError());
   }
   if (n > 0) {
     ProfScope ps("gb_ids");
-    hipLaunchKernelGGL(gb_ids, dim3(grid_for(n)), dim3(256), 0, s, static_cast<uint32_t*>(ids->ptr), un,
-                       static_cast<const uint32_t*>(gid_of_slot->ptr), D);
+    hipLaunchKernelGGL(gb_ids, dim3(build_grid), dim3(256), 0, s, static_cast<uint32_t*>(out.ids->ptr), un,
+                       static_cast<const uint32_t*>(gid_of_slot->ptr), out.D);
     AM_HIP(hipGetLastError());
   }
   AM_HIP(hipStreamSynchronize(s));
-  ids_out = ids;
-  rep_out = rep;
-  return K;
+  return out;
 }
 
-}  // namespace
-
-GroupBy* group_by_keys(Array* const* columns, int64_t count) {
+GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep) {
   if (count < 1) fail("group-by needs at least one key column");
   int64_t n = columns[0]->length;
   for (int64_t c = 1; c < count; c++)
@@ -930,28 +1114,49 @@ GroupBy* group_by_keys(Array* const* columns, int64_t count) {
       g->K = 0;
       g->ids = device_alloc(4);
       g->rep = device_alloc(4);
+      g->has_rep = true;
       return g;
     }
     hipStream_t s = stream();
     const Array* a0 = columns[0];
-    int64_t K = map_column(a0->data(), a0->type, a0->has_nulls() ? a0->valid_bits() : nullptr, a0->offset, n, g->ids,
-                           g->rep);
+    // Representative rows are needed for float keys (-0.0 and 0.0, NaN payloads share a group but
+    // not their bits) and for several columns; integer keys are answered from the sorted keys.
+    bool int_single = count == 1 && type_is_int(a0->type) && !need_rep;
+    Mapped m0 = map_column(a0->data(), a0->type, a0->has_nulls() ? a0->valid_bits() : nullptr, a0->offset, n,
+                           count == 1 && !int_single);
+    int64_t K = m0.K;
+    g->ids = m0.ids;
+    if (int_single) {
+      g->okeys = m0.okeys;
+      g->perm = m0.perm;
+      g->D = m0.D;
+      g->has_null = m0.has_null;
+    } else if (count == 1) {
+      g->rep = m0.rep;
+      g->has_rep = true;
+    }
     // Further columns fold pairwise: ids a (K groups) and b (K2) become the key a * K2 + b, which is
     // injective and ascending in (a, b), and that key is mapped again (GroupByKeys.swift:103-112).
     for (int64_t c = 1; c < count; c++) {
       const Array* ac = columns[c];
-      DevBuf ids2, rep2;
-      int64_t K2 = map_column(ac->data(), ac->type, ac->has_nulls() ? ac->valid_bits() : nullptr, ac->offset, n,
-                              ids2, rep2);
+      Mapped mc = map_column(ac->data(), ac->type, ac->has_nulls() ? ac->valid_bits() : nullptr, ac->offset, n, false);
+      int64_t K2 = mc.K;
       if (K2 > 0 && K > INT64_MAX / K2)
         fail("multi-column group-by overflowed the 64-bit composite key (" + std::to_string(K) + " x " +
              std::to_string(K2) + " distinct)");
       DevBuf comp = device_alloc(size_t(n) * 8);
       hipLaunchKernelGGL(gb_combine, dim3(grid_for(n)), dim3(256), 0, s, static_cast<const int32_t*>(g->ids->ptr),
-                         static_cast<const int32_t*>(ids2->ptr), uint32_t(n), (long long)K2,
+                         static_cast<const int32_t*>(mc.ids->ptr), uint32_t(n), (long long)K2,
                          static_cast<long long*>(comp->ptr));
       AM_HIP(hipGetLastError());
-      K = map_column(comp->ptr, T_I64, nullptr, 0, n, g->ids, g->rep);
+      bool last = c == count - 1;
+      Mapped mm = map_column(comp->ptr, T_I64, nullptr, 0, n, last);
+      K = mm.K;
+      g->ids = mm.ids;
+      if (last) {
+        g->rep = mm.rep;
+        g->has_rep = true;
+      }
     }
     if (K > int64_t(INT32_MAX)) fail("group-by: more than 2^31 groups");
     g->K = K;
@@ -962,6 +1167,10 @@ GroupBy* group_by_keys(Array* const* columns, int64_t count) {
   }
   return g;
 }
+
+}  // namespace
+
+GroupBy* group_by_keys(Array* const* columns, int64_t count) { return group_by_keys_impl(columns, count, false); }
 
 int64_t group_by_group_count(const GroupBy* g) { return g->K; }
 
@@ -977,17 +1186,51 @@ Array* group_by_keys_result(GroupBy* g, int64_t i) {
     fail("key column " + std::to_string(i) + " of " + std::to_string(g->columns.size()));
   if (g->key_results.empty()) g->key_results.assign(g->columns.size(), nullptr);
   if (!g->key_results[size_t(i)]) {
-    Array* rep = new_array("I", T_U32, g->K);
-    rep->values = g->rep;
-    rep->null_count = 0;
+    const Array* col = g->columns[size_t(i)];
     Array* keys = nullptr;
-    try {
-      keys = take(*g->columns[size_t(i)], *rep);
-    } catch (...) {
+    if (!g->has_rep) {
+      // Integer keys: the sorted distinct keys themselves, the null group (last) null.
+      hipStream_t s = stream();
+      keys = new_array(col->format, col->type, g->K);
+      try {
+        keys->values = device_alloc(size_t(std::max<int64_t>(g->K, 1)) * col->width());
+        DevBuf vb = device_alloc(size_t(std::max<int64_t>(g->K, 1)));
+        hipLaunchKernelGGL(gb_int_keys, dim3(uint32_t((g->D + 1 + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const unsigned long long*>(g->okeys->ptr),
+                           static_cast<const uint32_t*>(g->perm->ptr), g->D, col->type, keys->values->ptr,
+                           static_cast<uint8_t*>(vb->ptr), g->has_null ? 1 : 0);
+        AM_HIP(hipGetLastError());
+        if (g->has_null) {
+          keys->validity = device_alloc(size_t((g->K + 63) / 64) * 8);
+          DevBuf nz = device_alloc(8);
+          hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s,
+                             static_cast<unsigned long long*>(nz->ptr), 1);
+          hipLaunchKernelGGL(pack_validity, dim3(grid_for(g->K, 4)), dim3(256), 0, s,
+                             static_cast<const uint8_t*>(vb->ptr), (long long)g->K,
+                             static_cast<unsigned long long*>(keys->validity->ptr),
+                             static_cast<unsigned long long*>(nz->ptr));
+          AM_HIP(hipGetLastError());
+          keys->null_count = 1;
+        } else {
+          keys->null_count = 0;
+        }
+        sync();
+      } catch (...) {
+        release_array(keys);
+        throw;
+      }
+    } else {
+      Array* rep = new_array("I", T_U32, g->K);
+      rep->values = g->rep;
+      rep->null_count = 0;
+      try {
+        keys = take(*col, *rep);
+      } catch (...) {
+        release_array(rep);
+        throw;
+      }
       release_array(rep);
-      throw;
     }
-    release_array(rep);
     g->key_results[size_t(i)] = keys;
   }
   Array* r = g->key_results[size_t(i)];
@@ -1207,7 +1450,7 @@ Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array
 }
 
 Array* unique_rows(Array* const* columns, int64_t count) {
-  GroupBy* g = group_by_keys(columns, count);
+  GroupBy* g = group_by_keys_impl(columns, count, true);
   Array* out = nullptr;
   try {
     // The representative rows sorted ascending: keep-first order (Executor.swift:134-148).

@@ -590,11 +590,23 @@ void export_device(Array* a, ArrowSchema* schema, ArrowDeviceArray* darr) {
 SortTune sort_tune(int64_t n, int key_bytes) {
   // Tiles of 256 threads x `items` keys. Small inputs take short tiles so that the grid still covers
   // every CU; large ones take long tiles, which amortise the per-tile look-back over more keys.
+  // ARROWMETAL_SORT_ITEMS (4, 8, 12, 16 or 20) overrides the choice, for tuning sweeps.
   SortTune t;
+  static const int forced = [] {
+    const char* e = getenv("ARROWMETAL_SORT_ITEMS");
+    int v = e ? atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 12 || v == 16 || v == 20) ? v : 0;
+  }();
+  if (forced) {
+    t.items = forced;
+    return t;
+  }
   int64_t cus = device().cu_count;
   if (n <= cus * 256 * 4) t.items = 4;
   else if (n <= cus * 256 * 8 * 2) t.items = 8;
-  else t.items = key_bytes == 8 ? 16 : 16;
+  else if (n <= cus * 256 * 16 * 8) t.items = 16;
+  else t.items = 20;
+  (void)key_bytes;
   return t;
 }
 

@@ -24,23 +24,10 @@
 // per-bin counts and look back over earlier tiles for the exclusive prefix (decoupled look-back with
 // epoch-tagged 64-bit status words, so the status table is never cleared between passes), then stage
 // the tile in LDS in digit order so that the global writes are runs of consecutive addresses.
-#include "am_internal.hpp"
-
-#include <algorithm>
+#include "sort_common.hpp"
 
 namespace am {
 namespace {
-
-constexpr int BLOCK = 256;
-constexpr int WAVES = BLOCK / 64;
-constexpr int RADIX = 256;
-constexpr int BIN_NAN = 256;
-constexpr int BIN_NULL = 257;
-constexpr int BIN_NONE = 258;     // lanes past the end of the tile
-constexpr int NBINS_FIRST = 258;
-constexpr int WC_STRIDE = 260;    // per-wave counter row in LDS
-constexpr int STATUS_STRIDE = 260;
-constexpr uint32_t FLAG_AGG = 1, FLAG_PREFIX = 2;
 
 struct SortCtl {
   uint32_t hist[8][RADIX];          // digit histograms of the value keys, one per pass
@@ -51,127 +38,6 @@ struct SortCtl {
   uint32_t active_mask, first_active, last_active, pad1;
   uint32_t value_base, nan_base, null_base, pad2;
 };
-
-// ---------------------------------------------------------------- key maps
-
-constexpr uint64_t SIGN64 = 0x8000000000000000ull;
-constexpr uint64_t ABS64 = 0x7FFFFFFFFFFFFFFFull;
-constexpr uint64_t INF64 = 0x7FF0000000000000ull;
-
-// Category of row i (0 value, 1 NaN in its own block, 2 null) and, for a value, its ascending
-// order-preserving key with the descending flip applied (SortSource.swift:131-190).
-// float_order: 0 ieee (NaN apart, -0.0 ties +0.0), 1 total (raw sign flip), 2 nan_largest.
-template <typename K>
-__device__ __forceinline__ int classify(const void* vals, const uint8_t* valid, int64_t voff, uint32_t i,
-                                        int type, int forder, int desc, K& key) {
-  if (valid) {
-    int64_t b = voff + int64_t(i);
-    if (!((valid[b >> 3] >> (b & 7)) & 1)) return 2;
-  }
-  uint64_t k;
-  switch (type) {
-    case T_I64: k = uint64_t(static_cast<const int64_t*>(vals)[i]) ^ SIGN64; break;
-    case T_U64: k = static_cast<const uint64_t*>(vals)[i]; break;
-    case T_F64: {
-      uint64_t x = static_cast<const uint64_t*>(vals)[i];
-      if (forder != 1) {
-        if ((x & ABS64) > INF64) {
-          if (forder == 0) return 1;
-          x = 0x7FF0000000000001ull;
-        } else if ((x & ABS64) == 0) {
-          x = 0;
-        }
-      }
-      k = (x & SIGN64) ? ~x : (x | SIGN64);
-      break;
-    }
-    case T_I32: k = uint32_t(static_cast<const int32_t*>(vals)[i]) ^ 0x80000000u; break;
-    case T_I16: k = uint32_t(int32_t(static_cast<const int16_t*>(vals)[i])) ^ 0x80000000u; break;
-    case T_I8: k = uint32_t(int32_t(static_cast<const int8_t*>(vals)[i])) ^ 0x80000000u; break;
-    case T_U32: k = static_cast<const uint32_t*>(vals)[i]; break;
-    case T_U16: k = static_cast<const uint16_t*>(vals)[i]; break;
-    case T_U8: k = static_cast<const uint8_t*>(vals)[i]; break;
-    default: {   // T_F32
-      uint32_t x = static_cast<const uint32_t*>(vals)[i];
-      if (forder != 1) {
-        if ((x & 0x7FFFFFFFu) > 0x7F800000u) {
-          if (forder == 0) return 1;
-          x = 0x7F800001u;
-        } else if ((x & 0x7FFFFFFFu) == 0u) {
-          x = 0u;
-        }
-      }
-      k = (x & 0x80000000u) ? uint32_t(~x) : (x | 0x80000000u);
-      break;
-    }
-  }
-  if (desc) k = ~k;
-  key = K(k);
-  return 0;
-}
-
-// ---------------------------------------------------------------- block helpers
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
-  return x;
-}
-
-// Exclusive scan of one value per thread over the 256-thread block. `tmp` holds WAVES words of LDS.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp, uint32_t& total) {
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t x = wave_incl_scan(v, lane);
-  if (lane == 63) tmp[w] = x;
-  __syncthreads();
-  uint32_t pre = 0, tot = 0;
-#pragma unroll
-  for (int i = 0; i < WAVES; i++) {
-    uint32_t t = tmp[i];
-    pre += (i < w) ? t : 0u;
-    tot += t;
-  }
-  __syncthreads();
-  total = tot;
-  return pre + x - v;
-}
-
-__device__ __forceinline__ unsigned long long status_load(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Ranks a tile held in registers. Element (w, j, lane) of the tile is tile_start + w*64*ITEMS + j*64
-// + lane; its stable rank among the tile's elements of the same bin is
-//   (count in earlier waves) + (count in this wave at earlier items) + (lanes below it in its row).
-// The last two come from here: a 9-bit match over the 64 lanes with ballots, and one LDS counter per
-// (wave, bin) that only its own wave touches. On return wc[w][bin] holds the wave's bin totals.
-template <int ITEMS>
-__device__ __forceinline__ void rank_tile(const uint32_t (&bins)[ITEMS], uint32_t (&lrank)[ITEMS],
-                                          uint32_t* wc_row, int lane) {
-  const unsigned long long lt = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int j = 0; j < ITEMS; j++) {
-    uint32_t b = bins[j];
-    unsigned long long peers = ~0ull;
-#pragma unroll
-    for (int bit = 0; bit < 9; bit++) {
-      unsigned long long bal = __ballot((b >> bit) & 1u);
-      peers &= ((b >> bit) & 1u) ? bal : ~bal;
-    }
-    uint32_t below = uint32_t(__popcll(peers & lt));
-    uint32_t base = wc_row[b];
-    __builtin_amdgcn_wave_barrier();
-    if (below == 0) wc_row[b] = base + uint32_t(__popcll(peers));
-    __builtin_amdgcn_wave_barrier();
-    lrank[j] = base + below;
-  }
-}
 
 // ---------------------------------------------------------------- kernels
 
@@ -191,12 +57,6 @@ __global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
   out[3] = ctl->m;
 }
 
-// Zeroes the look-back status table (only when it is first allocated or grows).
-__global__ void fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
-}
 
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const uint8_t* valid, int64_t voff,
@@ -214,19 +74,33 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
   uint32_t end = min(n, begin + rows_per_block);
   K o = 0, a = K(~K(0));
   uint32_t cm = 0, cq = 0, cz = 0;
-  for (uint32_t i = begin + threadIdx.x; i < end; i += BLOCK) {
-    K k;
-    int c = classify<K>(vals, valid, voff, i, type, forder, desc, k);
-    if (c == 0) {
-      cm++;
-      o |= k;
-      a &= k;
+  // UP_ITEMS rows per thread per step, all loaded before any is counted, so that each wave keeps
+  // several loads in flight.
+  constexpr int UP_ITEMS = 8;
+  for (uint32_t base = begin; base < end; base += BLOCK * UP_ITEMS) {
+    K kk[UP_ITEMS];
+    int cc[UP_ITEMS];
 #pragma unroll
-      for (int p = 0; p < P; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
-    } else if (c == 1) {
-      cq++;
-    } else {
-      cz++;
+    for (int u = 0; u < UP_ITEMS; u++) {
+      uint32_t i = base + uint32_t(u) * BLOCK + threadIdx.x;
+      cc[u] = 3;
+      kk[u] = 0;
+      if (i < end) cc[u] = classify<K>(vals, valid, voff, i, type, forder, desc, kk[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UP_ITEMS; u++) {
+      if (cc[u] == 0) {
+        K k = kk[u];
+        cm++;
+        o |= k;
+        a &= k;
+#pragma unroll
+        for (int p = 0; p < P; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+      } else if (cc[u] == 1) {
+        cq++;
+      } else if (cc[u] == 2) {
+        cz++;
+      }
     }
   }
   // Wave reductions of the counters and the OR / AND, then one LDS atomic per wave.
@@ -295,98 +169,6 @@ __global__ __launch_bounds__(BLOCK) void sort_plan(SortCtl* ctl, int nulls_first
   }
 }
 
-// The shared body of a scatter pass over one tile that is already ranked and looked back.
-// Stages the tile's value elements in LDS in bin order (keys first, then rows, through one buffer)
-// and writes them out as runs: element at staged position `pos` with digit d goes to
-// bin_base[d] + pos - bin_start[d].
-template <typename K, int ITEMS, bool WRITE_KEYS>
-__device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uint32_t (&rows)[ITEMS],
-                                               const uint32_t (&bins)[ITEMS], const uint32_t (&lrank)[ITEMS],
-                                               const uint32_t* wc_row, const uint32_t* bin_start,
-                                               const uint32_t* bin_base, uint8_t* s_digit, K* s_stage,
-                                               uint32_t vcount, K* out_keys, uint32_t* out_rows) {
-  uint32_t pos[ITEMS];
-#pragma unroll
-  for (int j = 0; j < ITEMS; j++) {
-    uint32_t b = bins[j];
-    pos[j] = b < RADIX ? bin_start[b] + wc_row[b] + lrank[j] : 0xFFFFFFFFu;
-    if (b < RADIX) s_digit[pos[j]] = uint8_t(b);
-  }
-  if (WRITE_KEYS) {
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++)
-      if (pos[j] != 0xFFFFFFFFu) s_stage[pos[j]] = keys[j];
-    __syncthreads();
-    for (uint32_t p = threadIdx.x; p < vcount; p += BLOCK) {
-      uint32_t d = s_digit[p];
-      out_keys[bin_base[d] + p - bin_start[d]] = s_stage[p];
-    }
-    __syncthreads();
-  }
-  uint32_t* s_rows = reinterpret_cast<uint32_t*>(s_stage);
-#pragma unroll
-  for (int j = 0; j < ITEMS; j++)
-    if (pos[j] != 0xFFFFFFFFu) s_rows[pos[j]] = rows[j];
-  __syncthreads();
-  for (uint32_t p = threadIdx.x; p < vcount; p += BLOCK) {
-    uint32_t d = s_digit[p];
-    out_rows[bin_base[d] + p - bin_start[d]] = s_rows[p];
-  }
-}
-
-// Per-bin wave prefixes (in place in wc), tile totals, and the bin starts of the staged tile.
-template <int NB>
-__device__ __forceinline__ uint32_t tile_bin_layout(uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
-                                                   uint32_t* bin_start, uint32_t* tmp) {
-  for (int b = threadIdx.x; b < NB; b += BLOCK) {
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; w++) {
-      uint32_t c = wc[w][b];
-      wc[w][b] = run;
-      run += c;
-    }
-    tile_total[b] = run;
-  }
-  __syncthreads();
-  uint32_t vcount;
-  bin_start[threadIdx.x] = block_excl_scan(tile_total[threadIdx.x], tmp, vcount);
-  return vcount;
-}
-
-// Decoupled look-back for bins [0, NB): publish this tile's counts, then accumulate earlier tiles'
-// until one with an inclusive prefix is found. Writes the exclusive prefix into `excl`.
-template <int NB>
-__device__ __forceinline__ void look_back(unsigned long long* status, uint32_t tile, uint32_t epoch,
-                                          const uint32_t* tile_total, uint32_t* excl) {
-  const unsigned long long ep = (unsigned long long)epoch << 34;
-  unsigned long long* mine = status + size_t(tile) * STATUS_STRIDE;
-  for (int b = threadIdx.x; b < NB; b += BLOCK) {
-    uint32_t flag = tile == 0 ? FLAG_PREFIX : FLAG_AGG;
-    status_store(mine + b, ep | ((unsigned long long)flag << 32) | tile_total[b]);
-  }
-  if (tile == 0) {
-    for (int b = threadIdx.x; b < NB; b += BLOCK) excl[b] = 0;
-    return;
-  }
-  for (int b = threadIdx.x; b < NB; b += BLOCK) {
-    uint32_t sum = 0;
-    int64_t t = int64_t(tile) - 1;
-    while (true) {
-      unsigned long long s = status_load(status + size_t(t) * STATUS_STRIDE + b);
-      if (uint32_t(s >> 34) != epoch) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      sum += uint32_t(s);
-      if (((s >> 32) & 3u) == FLAG_PREFIX) break;
-      t--;
-    }
-    excl[b] = sum;
-    status_store(mine + b, ep | ((unsigned long long)FLAG_PREFIX << 32) | (sum + tile_total[b]));
-  }
-}
-
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
                                                     uint32_t n, int type, int forder, int desc,
@@ -432,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint
     }
     bins[j] = b;
   }
-  rank_tile<ITEMS>(bins, lrank, wc[w], lane);
+  rank_tile<ITEMS, 9>(bins, lrank, wc[w], lane);
   __syncthreads();
   uint32_t vcount = tile_bin_layout<NBINS_FIRST>(wc, tile_total, bin_start, tmp);
   look_back<NBINS_FIRST>(status, tile, epoch, tile_total, bin_base);
@@ -513,7 +295,9 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
       bins[j] = BIN_NONE;
     }
   }
-  rank_tile<ITEMS>(bins, lrank, wc[w], lane);
+  // A full tile has no lanes past the end, so its bins need 8 bits; the last tile needs the 9th.
+  if ((tile + 1u) * uint32_t(TILE) <= m) rank_tile<ITEMS, 8>(bins, lrank, wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, lrank, wc[w], lane);
   __syncthreads();
   uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
   look_back<RADIX>(status, tile, epoch, tile_total, bin_base);
@@ -531,46 +315,6 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
 }
 
 // ---------------------------------------------------------------- host side
-
-// Per-thread look-back status table: epoch-tagged, so it is cleared only when it grows.
-struct SortWorkspace {
-  DevBuf status;
-  size_t entries = 0;
-  uint32_t epoch = 1;
-};
-thread_local SortWorkspace t_ws;
-
-uint32_t reserve_status(size_t entries, hipStream_t s) {
-  if (t_ws.entries < entries || t_ws.epoch > (1u << 29)) {
-    size_t want = std::max(entries, t_ws.entries);
-    // Allocated outside the pool's batch retention: the table lives as long as the thread.
-    t_ws.status.reset();
-    void* p = nullptr;
-    AM_HIP(hipMalloc(&p, want * 8));
-    t_ws.status = device_wrap(p, want * 8, [](void* q) { (void)hipFree(q); });
-    t_ws.entries = want;
-    int blocks = int(std::min<size_t>((want + 255) / 256, 4096));
-    hipLaunchKernelGGL(fill_u64, dim3(blocks), dim3(256), 0, s, static_cast<unsigned long long*>(p), want, 0ull);
-    AM_HIP(hipGetLastError());
-    t_ws.epoch = 1;
-  }
-  uint32_t e = t_ws.epoch;
-  t_ws.epoch += 16;
-  return e;
-}
-
-bool capturing(hipStream_t s) {
-  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
-  return st != hipStreamCaptureStatusNone;
-}
-
-// Pinned per-thread landing place of the plan readback.
-uint32_t* host_plan_words() {
-  thread_local uint32_t* p = nullptr;
-  if (!p) AM_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocDefault));
-  return p;
-}
 
 template <typename K, int ITEMS>
 void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
@@ -592,9 +336,9 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     AM_HIP(hipGetLastError());
   }
   {
-    uint32_t max_blocks = uint32_t(device().cu_count) * 2u;
-    uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 4095u) / 4096u));
-    uint32_t rpb = ((n + blocks - 1) / blocks + 255u) / 256u * 256u;
+    uint32_t max_blocks = uint32_t(device().cu_count) * 4u;
+    uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 8191u) / 8192u));
+    uint32_t rpb = ((n + blocks - 1) / blocks + 2047u) / 2048u * 2048u;
     blocks = (n + rpb - 1) / rpb;
     ProfScope ps("sort_upsweep");
     hipLaunchKernelGGL(sort_upsweep<K>, dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
@@ -662,6 +406,7 @@ void dispatch_items(int items, const void* vals, int type, const uint8_t* valid,
     case 4: run_sort<K, 4>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
     case 8: run_sort<K, 8>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
     case 12: run_sort<K, 12>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    case 20: run_sort<K, 20>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
     default: run_sort<K, 16>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
   }
 }
