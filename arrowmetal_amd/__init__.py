@@ -99,6 +99,15 @@ _SIGNATURES = {
     "am_group_by_release": ([_P], None),
     "am_group_agg_ex": ([_P, _P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_P)], ctypes.c_int),
     "am_unique_rows": ([ctypes.POINTER(_P), ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_unique_ex": ([_P, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_slice": ([_P, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_batched_create": ([ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                           ctypes.POINTER(_P)], ctypes.c_int),
+    "am_batched_tune": ([_P, _P, _P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "am_batched_run": ([_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "am_batched_result": ([_P, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_batched_config": ([_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
+    "am_batched_release": ([_P], None),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
     "am_prof_report": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int64),
@@ -317,6 +326,17 @@ class MetalArray:
         """Dense-key group-by (`am_group_by`): this array holds keys in [0, key_count)."""
         return GroupBy(self, key_count)
 
+    def slice(self, offset, length):
+        """Zero-copy window of `length` elements from `offset` (`am_slice`)."""
+        return _call(lib().am_slice, self._h, int(offset), int(length))
+
+    def unique(self, order="first_appearance"):
+        """Arrow `unique` (`am_unique_ex`): "first_appearance" keeps the null once, at the first null
+        row; "sorted" is ascending with nulls dropped."""
+        if order not in ("first_appearance", "sorted"):
+            raise ValueError(f"unknown value order {order!r}")
+        return _call(lib().am_unique_ex, self._h, 1 if order == "sorted" else 0)
+
 
 NULL_PLACEMENTS = ("at_end", "at_start")
 FLOAT_ORDERS = ("ieee", "total", "nan_largest")
@@ -349,7 +369,7 @@ class GroupBy:
     def count_values(self, values): return self._agg("count_values", values)
 
 
-_GROUP_AGG = {"sum": 0, "count_all": 1, "count": 2, "mean": 3, "min": 4, "max": 5}
+_GROUP_AGG = {"sum": 0, "count_all": 1, "count": 2, "mean": 3, "min": 4, "max": 5, "min_max": 6}
 
 
 class GroupByKeys:
@@ -404,6 +424,7 @@ class GroupByKeys:
     def mean(self, values): return self._agg("mean", values)
     def min(self, values): return self._agg("min", values)
     def max(self, values): return self._agg("max", values)
+    def min_max(self, values): return self._agg("min_max", values)
 
 
 def group_by(keys):
@@ -418,6 +439,54 @@ def unique_rows(columns):
     out = _P()
     _check(lib().am_unique_rows(handles, len(columns), ctypes.byref(out)))
     return MetalArray(out)
+
+
+class Batched:
+    """A plan over record batches stored back to back (`am_batched_*`): batch b is rows
+    [offsets[b], offsets[b+1]) of every column. One run sorts every batch and groups every batch in a
+    fixed kernel sequence, captured in a hipGraph and replayed."""
+
+    RESULTS = {"indices": 0, "keys": 1, "sums": 2, "counts": 3, "group_offsets": 4}
+
+    def __init__(self, offsets, hash_capacity=2048, tile_items=16):
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._h = None
+        h = _P()
+        _check(lib().am_batched_create(offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(offs) - 1,
+                                       int(hash_capacity), int(tile_items), ctypes.byref(h)))
+        self._h = h
+        self.nbatches = len(offs) - 1
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.am_batched_release(self._h)
+            self._h = None
+
+    @staticmethod
+    def _handles(sort_keys, group_keys, values):
+        return tuple(None if a is None else a._h for a in (sort_keys, group_keys, values))
+
+    def tune(self, sort_keys=None, group_keys=None, values=None, descending=False, float_order="ieee"):
+        _check(lib().am_batched_tune(self._h, *self._handles(sort_keys, group_keys, values),
+                                     1 if descending else 0, _float_order(float_order)))
+        return self.config
+
+    def run(self, sort_keys=None, group_keys=None, values=None, descending=False, float_order="ieee",
+            use_graph=True):
+        _check(lib().am_batched_run(self._h, *self._handles(sort_keys, group_keys, values),
+                                    1 if descending else 0, _float_order(float_order), 1 if use_graph else 0))
+
+    def result(self, name):
+        out = _P()
+        _check(lib().am_batched_result(self._h, self.RESULTS[name], ctypes.byref(out)))
+        return MetalArray(out)
+
+    @property
+    def config(self):
+        v = (ctypes.c_int64 * 6)()
+        _check(lib().am_batched_config(self._h, v, 6))
+        return {"tile_items": v[0], "hash_capacity": v[1], "strategy": ("lds", "global")[v[2]],
+                "digit_bits": v[3], "regrouped_batches": v[4], "sort_tiles": v[5]}
 
 
 class batch:

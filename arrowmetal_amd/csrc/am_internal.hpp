@@ -46,6 +46,7 @@ enum TypeCode : int {
   T_U8 = 4, T_U16 = 5, T_U32 = 6, T_U64 = 7,
   T_F32 = 8, T_F64 = 9,
   T_BOOL = 10,
+  T_STRUCT = 11,   // a result-only struct of child arrays (hash_min_max)
   T_UNSUPPORTED = -1
 };
 __host__ __device__ inline int type_width(int t) {
@@ -60,6 +61,7 @@ __host__ __device__ inline int type_width(int t) {
 __host__ __device__ inline bool type_is_float(int t) { return t == T_F32 || t == T_F64; }
 __host__ __device__ inline bool type_is_signed(int t) { return t >= T_I8 && t <= T_I64; }
 __host__ __device__ inline bool type_is_int(int t) { return t >= T_I8 && t <= T_U64; }
+__host__ __device__ inline bool type_is_primitive(int t) { return t >= T_I8 && t <= T_F64; }
 // The storage type of an Arrow format string: primitives, and temporal types as their integers.
 int type_of_format(const std::string& fmt);
 
@@ -130,6 +132,10 @@ struct Array {
   DevBuf validity;         // device bitmap, bit (offset + i) for element i; null when no nulls
   // Cached host copy of the exported buffers (so repeated am_export calls copy once).
   std::shared_ptr<std::vector<uint8_t>> host_values, host_validity;
+  // A struct ("+s") holds one reference to each child; it has no values buffer of its own.
+  std::vector<Array*> children;
+  std::vector<std::string> child_names;
+  ~Array();
 
   int width() const { return type_width(type); }
   const void* data() const {   // element `offset`
@@ -176,6 +182,7 @@ Array* group_agg(GroupBy* g, const Array* values, int op, double p1);
 void group_by_release(GroupBy* g);
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values);
 Array* unique_rows(Array* const* columns, int64_t count);
+Array* unique_values(const Array& a, int order);
 
 // Launch-shape parameters the autotuner chooses (runtime.cpp keeps the table).
 struct SortTune {

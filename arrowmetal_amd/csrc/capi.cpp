@@ -20,6 +20,16 @@ void batch_end();
 void prof_enable(bool on);
 void prof_reset();
 std::string prof_report();
+Array* slice(const Array& a, int64_t offset, int64_t length);
+struct Batched;
+Batched* batched_create(const int64_t* offsets, int64_t nbatches, int hash_capacity, int items);
+void batched_release(Batched* b);
+void batched_run(Batched* b, const Array* sort_keys, const Array* group_keys, const Array* values, int desc,
+                 int forder, bool use_graph);
+void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, const Array* values, int desc,
+                  int forder);
+void batched_config(const Batched* b, int64_t* out, int count);
+Array* batched_result(Batched* b, int which);
 }  // namespace am
 
 using am::Array;
@@ -29,6 +39,8 @@ namespace {
 inline Array* A(am_array* p) { return reinterpret_cast<Array*>(p); }
 inline am_array* H(Array* p) { return reinterpret_cast<am_array*>(p); }
 inline am::GroupBy* G(am_groupby* p) { return reinterpret_cast<am::GroupBy*>(p); }
+inline am::Batched* B(am_batched* p) { return reinterpret_cast<am::Batched*>(p); }
+inline const Array* OA(am_array* p) { return p ? reinterpret_cast<Array*>(p) : nullptr; }
 
 template <class F>
 int guarded(F&& f) {
@@ -171,6 +183,55 @@ void am_group_by_release(am_groupby* gb) {
 int am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_array** out) {
   if (!gb || !out) return 2;
   return guarded([&] { *out = H(am::group_agg(G(gb), values ? A(values) : nullptr, op, p1)); });
+}
+
+int am_slice(am_array* a, int64_t offset, int64_t length, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::slice(*A(a), offset, length)); });
+}
+
+int am_batched_create(const int64_t* offsets, int64_t nbatches, int hash_capacity, int tile_items,
+                      am_batched** out) {
+  if (!offsets || !out) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_batched*>(am::batched_create(offsets, nbatches, hash_capacity, tile_items));
+  });
+}
+
+int am_batched_tune(am_batched* b, am_array* sort_keys, am_array* group_keys, am_array* values, int descending,
+                    int float_order) {
+  if (!b) return 2;
+  return guarded([&] { am::batched_tune(B(b), OA(sort_keys), OA(group_keys), OA(values), descending, float_order); });
+}
+
+int am_batched_run(am_batched* b, am_array* sort_keys, am_array* group_keys, am_array* values, int descending,
+                   int float_order, int use_graph) {
+  if (!b) return 2;
+  return guarded([&] {
+    am::batched_run(B(b), OA(sort_keys), OA(group_keys), OA(values), descending, float_order, use_graph != 0);
+  });
+}
+
+int am_batched_config(am_batched* b, int64_t* out, int count) {
+  if (!b || !out) return 2;
+  return guarded([&] { am::batched_config(B(b), out, count); });
+}
+
+int am_batched_result(am_batched* b, int which, am_array** out) {
+  if (!b || !out) return 2;
+  return guarded([&] { *out = H(am::batched_result(B(b), which)); });
+}
+
+void am_batched_release(am_batched* b) {
+  try {
+    am::batched_release(B(b));
+  } catch (...) {
+  }
+}
+
+int am_unique_ex(am_array* a, int order, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::unique_values(*A(a), order == 1 ? 1 : 0)); });
 }
 
 int am_unique_rows(am_array** columns, int64_t count, am_array** out) {

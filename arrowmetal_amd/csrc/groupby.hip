@@ -1100,7 +1100,7 @@ GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep)
   check_length(n);
   for (int64_t c = 0; c < count; c++) {
     int t = columns[c]->type;
-    if (t == T_UNSUPPORTED || t == T_BOOL)
+    if (!type_is_primitive(t))
       fail("group-by over a " + columns[c]->format + " key column is not supported by this build");
   }
   auto* g = new GroupBy();
@@ -1134,6 +1134,7 @@ GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep)
     } else if (count == 1) {
       g->rep = m0.rep;
       g->has_rep = true;
+      g->has_null = m0.has_null;
     }
     // Further columns fold pairwise: ids a (K groups) and b (K2) become the key a * K2 + b, which is
     // injective and ascending in (a, b), and that key is mapped again (GroupByKeys.swift:103-112).
@@ -1380,13 +1381,15 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   }
   if (!v) fail("am_group_agg_ex op " + std::to_string(op) + " needs a value column");
   if (v->length != n) fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(v->length));
-  if (v->type == T_UNSUPPORTED || v->type == T_BOOL)
-    fail("grouped aggregate over a " + v->format + " column is not supported by this build");
-  AggArgs a = make_args(ids, v, n, K);
+  // hash_count needs only the validity, so it takes a column of any type.
   if (op == 2) {
+    AggArgs a = make_args(ids, v, n, K);
     DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
+  if (!type_is_primitive(v->type))
+    fail("grouped aggregate over a " + v->format + " column is not supported by this build");
+  AggArgs a = make_args(ids, v, n, K);
   if (op == 0 || op == 3) {
     if (v->type == T_F64) {
       DevBuf sum, mean, vb;
@@ -1416,6 +1419,22 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     }
     return finalize_simple(t, 2, K, op, v->type, v->format, v->type);
   }
+  if (op == 6) {   // hash_min_max: struct<min, max> over one min/max pass (AggregatesExtra.swift:90-96)
+    Array* lo = aggregate(ids, n, K, v, 4, cache);
+    Array* hi = nullptr;
+    try {
+      hi = aggregate(ids, n, K, v, 5, cache);
+    } catch (...) {
+      release_array(lo);
+      throw;
+    }
+    Array* st = new_array("+s", T_STRUCT, K);
+    st->null_count = 0;
+    st->children = {lo, hi};
+    st->child_names = {"min", "max"};
+    return st;
+  }
+  if (op < 0 || op > 25) fail("unknown am_group_agg_ex op " + std::to_string(op));
   fail("am_group_agg_ex op " + std::to_string(op) + " is not supported by this build");
 }
 
@@ -1429,7 +1448,12 @@ Array* group_agg(GroupBy* g, const Array* values, int op, double /*p1*/) {
   // result is trimmed to zero rows (GroupByKeys.swift:59-76, 159-165).
   long long K = std::max<int64_t>(g->K, 1);
   Array* r = aggregate(ids, g->n, K, values, op, g);
-  if (g->K == 0) { r->length = 0; r->null_count = 0; r->validity.reset(); }
+  if (g->K == 0) {
+    r->length = 0;
+    r->null_count = 0;
+    r->validity.reset();
+    for (Array* c : r->children) { c->length = 0; c->null_count = 0; c->validity.reset(); }
+  }
   return r;
 }
 
@@ -1473,6 +1497,50 @@ Array* unique_rows(Array* const* columns, int64_t count) {
     }
   } catch (...) {
     if (out) release_array(out);
+    group_by_release(g);
+    throw;
+  }
+  group_by_release(g);
+  return out;
+}
+
+}  // namespace am
+
+namespace am {
+
+// Arrow `unique` with the value order spelled out (ArrowMetalC_Options.swift:286-297,
+// Kernels/UniqueOrder.swift:117-125, Kernels/Unique.swift:46-51). order 0: first appearance, the null
+// kept once at the position of the first null row. order 1: ascending, nulls dropped.
+Array* unique_values(const Array& a, int order) {
+  if (order == 0) {
+    Array* col = const_cast<Array*>(&a);
+    Array* rows = unique_rows(&col, 1);
+    Array* out = nullptr;
+    try {
+      out = take(a, *rows);
+    } catch (...) {
+      release_array(rows);
+      throw;
+    }
+    release_array(rows);
+    return out;
+  }
+  Array* col = const_cast<Array*>(&a);
+  GroupBy* g = group_by_keys_impl(&col, 1, true);
+  Array* out = nullptr;
+  try {
+    const int64_t k = g->K - (g->has_null ? 1 : 0);   // the null group is last
+    Array* rep = new_array("I", T_U32, k);
+    rep->values = g->rep;
+    rep->null_count = 0;
+    try {
+      out = take(a, *rep);
+    } catch (...) {
+      release_array(rep);
+      throw;
+    }
+    release_array(rep);
+  } catch (...) {
     group_by_release(g);
     throw;
   }

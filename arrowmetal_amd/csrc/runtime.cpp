@@ -271,6 +271,8 @@ void batch_end() {
 
 ProfScope::ProfScope(const char* name) : name_(name) {
   if (!g_prof_on.load(std::memory_order_relaxed)) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream(), &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start_) != hipSuccess) { start_ = nullptr; return; }
   (void)hipEventRecord(start_, stream());
 }
@@ -311,9 +313,33 @@ Array* new_array(const std::string& format, int type, int64_t length) {
   a->length = length;
   return a;
 }
+Array::~Array() {
+  for (Array* c : children) release_array(c);
+}
 void retain_array(Array* a) { a->refs.fetch_add(1, std::memory_order_relaxed); }
 void release_array(Array* a) {
   if (a && a->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete a;
+}
+
+// A zero-copy window onto `a` (ArrowMetalC.swift:384-389): the buffers are shared, the offset moves.
+Array* slice(const Array& a, int64_t offset, int64_t length) {
+  if (offset < 0 || length < 0 || offset + length > a.length) fail("slice out of range");
+  if (a.type == T_STRUCT) fail("am_slice of a struct array is not supported by this build");
+  Array* out = new_array(a.format, a.type, length);
+  out->values = a.values;
+  out->offset = a.offset + offset;
+  out->null_count = 0;
+  if (a.has_nulls() && length > 0) {
+    out->validity = a.validity;
+    try {
+      out->null_count = count_nulls(*out);
+    } catch (...) {
+      release_array(out);
+      throw;
+    }
+    if (out->null_count == 0) out->validity.reset();
+  }
+  return out;
 }
 
 void check_length(int64_t n) {
@@ -486,10 +512,11 @@ Array* import_device(const ArrowSchema* schema, ArrowDeviceArray* darr) {
 namespace {
 
 struct ExportHolder {
-  std::string format;
   std::shared_ptr<std::vector<uint8_t>> values, validity;
   const void* buffers[2] = {nullptr, nullptr};
   Array* keep = nullptr;   // device export: the array whose buffers are handed out
+  std::vector<ArrowArray> kids;
+  std::vector<ArrowArray*> kid_ptrs;
   ~ExportHolder() {
     if (keep) release_array(keep);
   }
@@ -497,89 +524,127 @@ struct ExportHolder {
 
 void release_export_array(ArrowArray* a) {
   if (!a || !a->release) return;
-  delete static_cast<ExportHolder*>(a->private_data);
+  auto* h = static_cast<ExportHolder*>(a->private_data);
+  for (ArrowArray& k : h->kids)   // children not moved out by the consumer
+    if (k.release) k.release(&k);
+  delete h;
   a->release = nullptr;
 }
+
+struct SchemaHolder {
+  std::string format, name;
+  std::vector<ArrowSchema> kids;
+  std::vector<ArrowSchema*> kid_ptrs;
+};
+
 void release_export_schema(ArrowSchema* s) {
   if (!s || !s->release) return;
-  delete static_cast<std::string*>(s->private_data);
+  auto* h = static_cast<SchemaHolder*>(s->private_data);
+  for (ArrowSchema& k : h->kids)
+    if (k.release) k.release(&k);
+  delete h;
   s->release = nullptr;
 }
 
-void fill_schema(const Array& a, ArrowSchema* schema) {
-  auto* f = new std::string(a.format);
+void fill_schema(const Array& a, const std::string& name, ArrowSchema* schema) {
+  auto* h = new SchemaHolder();
+  h->format = a.format;
+  h->name = name;
+  h->kids.resize(a.children.size());
+  for (size_t i = 0; i < a.children.size(); i++) {
+    fill_schema(*a.children[i], i < a.child_names.size() ? a.child_names[i] : std::string(), &h->kids[i]);
+    h->kid_ptrs.push_back(&h->kids[i]);
+  }
   std::memset(schema, 0, sizeof *schema);
-  schema->format = f->c_str();
-  schema->name = "";
+  schema->format = h->format.c_str();
+  schema->name = h->name.c_str();
   schema->metadata = nullptr;
   schema->flags = ARROW_FLAG_NULLABLE;
-  schema->n_children = 0;
-  schema->children = nullptr;
+  schema->n_children = int64_t(h->kids.size());
+  schema->children = h->kids.empty() ? nullptr : h->kid_ptrs.data();
   schema->dictionary = nullptr;
   schema->release = release_export_schema;
-  schema->private_data = f;
+  schema->private_data = h;
+}
+
+// Copies the array's buffers to host memory once and keeps them with the array.
+void cache_host_copy(Array* a) {
+  if (a->host_values || a->type == T_STRUCT) return;
+  int w = a->width();
+  auto hv = std::make_shared<std::vector<uint8_t>>(size_t(std::max<int64_t>(a->length, 1)) * w);
+  hipStream_t s = stream();
+  if (a->length > 0) AM_HIP(hipMemcpyAsync(hv->data(), a->data(), size_t(a->length) * w, hipMemcpyDeviceToHost, s));
+  std::shared_ptr<std::vector<uint8_t>> hb;
+  if (a->has_nulls() && a->length > 0) {
+    int64_t first_byte = a->offset / 8;
+    int64_t nbytes = (a->offset + a->length + 7) / 8 - first_byte;
+    std::vector<uint8_t> raw(static_cast<size_t>(nbytes));
+    AM_HIP(hipMemcpyAsync(raw.data(), a->valid_bits() + first_byte, size_t(nbytes), hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    hb = std::make_shared<std::vector<uint8_t>>(shift_bitmap(raw.data(), a->offset % 8, a->length, nullptr));
+  }
+  AM_HIP(hipStreamSynchronize(s));
+  a->host_values = hv;
+  a->host_validity = hb;
+}
+
+// Fills `array` with the host copy (device == false) or the device buffers (device == true) of `a`,
+// children included.
+void fill_array(Array* a, bool device, ArrowArray* array) {
+  auto* h = new ExportHolder();
+  h->kids.resize(a->children.size());
+  try {
+    for (size_t i = 0; i < a->children.size(); i++) {
+      fill_array(a->children[i], device, &h->kids[i]);
+      h->kid_ptrs.push_back(&h->kids[i]);
+    }
+  } catch (...) {
+    for (ArrowArray& k : h->kids)
+      if (k.release) k.release(&k);
+    delete h;
+    throw;
+  }
+  const bool is_struct = a->type == T_STRUCT;
+  std::memset(array, 0, sizeof *array);
+  if (device) {
+    retain_array(a);
+    h->keep = a;
+    h->buffers[0] = a->has_nulls() ? a->validity->ptr : nullptr;
+    h->buffers[1] = a->values ? a->values->ptr : nullptr;
+    array->null_count = a->has_nulls() ? a->null_count : 0;
+    array->offset = a->offset;
+  } else {
+    cache_host_copy(a);
+    h->values = a->host_values;
+    h->validity = a->host_validity;
+    h->buffers[0] = h->validity ? h->validity->data() : nullptr;
+    h->buffers[1] = h->values ? h->values->data() : nullptr;
+    array->null_count = h->validity ? a->null_count : 0;
+    array->offset = 0;
+  }
+  array->length = a->length;
+  array->n_buffers = is_struct ? 1 : 2;
+  array->buffers = h->buffers;
+  array->n_children = int64_t(h->kids.size());
+  array->children = h->kids.empty() ? nullptr : h->kid_ptrs.data();
+  array->dictionary = nullptr;
+  array->release = release_export_array;
+  array->private_data = h;
 }
 
 }  // namespace
 
 void export_host(Array* a, ArrowSchema* schema, ArrowArray* array) {
   sync();   // a batch open on this thread completes before its results are read
-  if (!a->host_values) {
-    int w = a->width();
-    auto hv = std::make_shared<std::vector<uint8_t>>(size_t(std::max<int64_t>(a->length, 1)) * w);
-    hipStream_t s = stream();
-    if (a->length > 0) AM_HIP(hipMemcpyAsync(hv->data(), a->data(), size_t(a->length) * w, hipMemcpyDeviceToHost, s));
-    std::shared_ptr<std::vector<uint8_t>> hb;
-    if (a->has_nulls() && a->length > 0) {
-      int64_t first_byte = a->offset / 8;
-      int64_t nbytes = (a->offset + a->length + 7) / 8 - first_byte;
-      std::vector<uint8_t> raw(static_cast<size_t>(nbytes));
-      AM_HIP(hipMemcpyAsync(raw.data(), a->valid_bits() + first_byte, size_t(nbytes), hipMemcpyDeviceToHost, s));
-      AM_HIP(hipStreamSynchronize(s));
-      hb = std::make_shared<std::vector<uint8_t>>(shift_bitmap(raw.data(), a->offset % 8, a->length, nullptr));
-    }
-    AM_HIP(hipStreamSynchronize(s));
-    a->host_values = hv;
-    a->host_validity = hb;
-  }
-  fill_schema(*a, schema);
-  auto* h = new ExportHolder();
-  h->format = a->format;
-  h->values = a->host_values;
-  h->validity = a->host_validity;
-  h->buffers[0] = h->validity ? h->validity->data() : nullptr;
-  h->buffers[1] = h->values->data();
-  std::memset(array, 0, sizeof *array);
-  array->length = a->length;
-  array->null_count = h->validity ? a->null_count : 0;
-  array->offset = 0;
-  array->n_buffers = 2;
-  array->n_children = 0;
-  array->buffers = h->buffers;
-  array->children = nullptr;
-  array->dictionary = nullptr;
-  array->release = release_export_array;
-  array->private_data = h;
+  fill_array(a, false, array);
+  fill_schema(*a, "", schema);
 }
 
 void export_device(Array* a, ArrowSchema* schema, ArrowDeviceArray* darr) {
   sync();
-  fill_schema(*a, schema);
-  auto* h = new ExportHolder();
-  h->format = a->format;
-  retain_array(a);
-  h->keep = a;
-  h->buffers[0] = a->has_nulls() ? a->validity->ptr : nullptr;
-  h->buffers[1] = a->values ? a->values->ptr : nullptr;
   std::memset(darr, 0, sizeof *darr);
-  ArrowArray* array = &darr->array;
-  array->length = a->length;
-  array->null_count = a->has_nulls() ? a->null_count : 0;
-  array->offset = a->offset;
-  array->n_buffers = 2;
-  array->buffers = h->buffers;
-  array->release = release_export_array;
-  array->private_data = h;
+  fill_array(a, true, &darr->array);
+  fill_schema(*a, "", schema);
   darr->device_id = device().id;
   darr->device_type = ARROW_DEVICE_ROCM;
   darr->sync_event = nullptr;

@@ -88,6 +88,98 @@ def group_by_inputs(n: int, support: int = 100_000):
     return keys, ivals, fvals
 
 
+def batched_inputs(nb: int, seed: int = 0xBA7C4):
+    """Lognormal batch lengths (median 20,000 rows, sigma 1.78, capped at 4,000,000), full-range int64
+    sort keys, group keys uniform over 1,000 int64 values, int64 values uniform in [-1e9, 1e9]."""
+    z = np.sqrt(-2.0 * np.log(1.0 - unit(splitmix64(seed, nb)))) * np.cos(2 * np.pi * unit(splitmix64(seed + 1, nb)))
+    lens = np.minimum(np.exp(np.log(20_000.0) + 1.78 * z), 4_000_000).astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(offs[-1])
+    sort_keys = splitmix64(seed + 2, n).view(np.int64)
+    key_table = splitmix64(seed + 3, 1000).view(np.int64)
+    group_keys = key_table[(splitmix64(seed + 4, n) % np.uint64(1000)).astype(np.int64)]
+    values = (splitmix64(seed + 5, n) % np.uint64(2_000_000_001)).astype(np.int64) - 1_000_000_000
+    return offs, sort_keys, group_keys, values
+
+
+def batched_leg(args, log):
+    """All batches in one graph-replayed kernel sequence (am_batched_run) against the per-batch loop of
+    the plain entry points (am_slice, am_argsort_ex2, am_group_by_keys, keys, sum, count_all)."""
+    offs, sk, gk, gv = batched_inputs(args.batches)
+    n = int(offs[-1])
+    nb = len(offs) - 1
+    msk, mgk, mgv = (am.MetalArray.from_numpy(x) for x in (sk, gk, gv))
+    del sk, gk, gv
+    log(f"batched inputs resident: {nb} batches, {n} rows")
+    plan = am.Batched(offs, hash_capacity=2048)
+    t0 = time.perf_counter()
+    cfg = plan.tune(msk, mgk, mgv)
+    tune_s = time.perf_counter() - t0
+    log(f"batched tuned in {tune_s:.2f} s: {cfg}")
+    step = lambda: plan.run(msk, mgk, mgv, use_graph=True)
+    tg = timed(step, args.steps, args.warmup)
+    te = timed(lambda: plan.run(msk, mgk, mgv, use_graph=False), max(3, args.steps // 2), 1)
+    gms, ems = 1e3 * float(np.median(tg)), 1e3 * float(np.median(te))
+    am.prof_reset()
+    am.prof_enable(True)
+    plan.run(msk, mgk, mgv, use_graph=False)
+    am.prof_enable(False)
+    bprof = am.prof_report()
+    log(f"batched: graph {gms:.3f} ms, eager {ems:.3f} ms; kernels {json.dumps(bprof)}")
+
+    def loop(batches):
+        outs = []
+        for b in batches:
+            a, e = int(offs[b]), int(offs[b + 1])
+            s_k, s_g, s_v = msk.slice(a, e - a), mgk.slice(a, e - a), mgv.slice(a, e - a)
+            idx = s_k.argsort(False, "at_end", "ieee")
+            g = am.group_by([s_g])
+            outs.append((idx, g.key_arrays()[0], g.sum(s_v), g.count_all()))
+            del g
+        return outs
+
+    loop(range(min(nb, 16)))   # warm the per-call path
+    t0 = time.perf_counter()
+    outs = loop(range(nb))
+    loop_ms = 1e3 * (time.perf_counter() - t0)
+    log(f"per-batch loop: {loop_ms:.1f} ms")
+    # The batched results must equal the per-call results batch for batch (checked on a sample).
+    idx_all, _ = plan.result("indices").to_numpy()
+    keys_all, _ = plan.result("keys").to_numpy()
+    sums_all, _ = plan.result("sums").to_numpy()
+    cnts_all, _ = plan.result("counts").to_numpy()
+    go, _ = plan.result("group_offsets").to_numpy()
+    same = True
+    sample = sorted(set(np.linspace(0, nb - 1, 128).astype(int).tolist()))
+    for b in sample:
+        a, e = int(offs[b]), int(offs[b + 1])
+        lo, hi = int(go[b]), int(go[b + 1])
+        i_, k_, s_, c_ = (x.to_numpy()[0] for x in outs[b])
+        same &= bool(np.array_equal(idx_all[a:e], i_) and np.array_equal(keys_all[lo:hi], k_) and
+                     np.array_equal(sums_all[lo:hi], s_) and np.array_equal(cnts_all[lo:hi], c_))
+    del outs
+    b_alg = int((8 + (8 + 12) + 6 * 24 + 16 + 16) * n)   # sort 188 B/row + group-by 16 B/row
+    return {
+        "workload": f"batched_{nb}_lognormal_median20k_int64_argsort_plus_groupby_1000_keys",
+        "batches": nb,
+        "rows": n,
+        "ms_per_step": round(gms, 4),
+        "rows_per_s": n / (gms * 1e-3),
+        "eager_ms_per_step": round(ems, 4),
+        "per_batch_loop_ms": round(loop_ms, 3),
+        "per_batch_loop_rows_per_s": n / (loop_ms * 1e-3),
+        "speedup_vs_per_batch_loop": loop_ms / gms,
+        "step": "am_batched_run (graph replay): stable ascending argsort + group-by sum/count_all per batch",
+        "tuned": cfg,
+        "tune_s": round(tune_s, 3),
+        "b_alg_bytes": b_alg,
+        "hbm_frac_of_b_alg": b_alg / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "same_as_per_batch_loop": same,
+        "batches_compared": len(sample),
+        "kernels_ms": {kname: round(v[1] / max(v[0], 1), 4) for kname, v in bprof.items()},
+    }
+
+
 def timed(fn, steps: int, warmup: int):
     for _ in range(warmup):
         fn()
@@ -112,11 +204,12 @@ def main():
     ap.add_argument("--rows", type=int, default=50_000_000)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-groupby", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--batches", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, nothing here launches on a second GPU)",
-              file=sys.stderr)
+              "(nothing here launches on a second GPU)", file=sys.stderr)
         sys.exit(2)
     n = args.rows
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
@@ -227,6 +320,11 @@ def main():
         log(f"group-by: {gms:.3f} ms, float step {fms:.3f} ms, groups {g.group_count}")
         del g, gk, gi, gf
 
+    # ---------------------------------------------------------------- batched (2,048 record batches)
+    bt_res = None
+    if not args.skip_batched:
+        bt_res = batched_leg(args, log)
+
     b_sort = 8 * n + (8 * n + 12 * m + 4 * (n - m)) + 6 * 2 * m * 12 + m * 16 + 2 * ((n + 7) // 8)
     b_take = 20 * n
     result = {
@@ -250,7 +348,7 @@ def main():
         "b_alg_bytes": {"sort": b_sort, "gather": b_take},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "sort_scatter (middle radix pass, sort_mid<uint64,16,0>)",
+                     "kernel": "sort_scatter (a middle radix pass: read key + row, write key + row)",
                      "bytes_per_launch": bytes_per_launch, "mean_launch_ms": scat_ms,
                      "launches": scat_n},
         "kernels_ms": {kname: round(v[1] / max(v[0], 1), 4) for kname, v in prof.items()},
@@ -259,6 +357,7 @@ def main():
         "argsort_1M": {"ms": round(ms1, 4), "rows_per_s": 1e6 / (ms1 * 1e-3),
                        "workload": "stable ascending argsort of 1,000,000 full-range int64 keys"},
         "group_by": gb_res,
+        "batched": bt_res,
     }
     print(json.dumps(result), flush=True)
 

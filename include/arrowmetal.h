@@ -29,6 +29,8 @@ extern "C" {
 typedef struct am_array am_array;
 /* reference include/arrowmetal.h:900 */
 typedef struct am_groupby am_groupby;
+/* Extension of this build: a plan over many record batches (see am_batched_create). */
+typedef struct am_batched am_batched;
 
 /* reference include/arrowmetal.h:15-17 (ArrowMetalC.swift am_version / am_device_name / am_last_error) */
 const char* am_version(void);
@@ -55,6 +57,10 @@ const char* am_compute_format(am_array* a);
  * int64 or uint32; a null index gives a null; an index outside [0, length) is the error
  * "take: index out of range (array length N)". */
 int  am_take(am_array* a, am_array* indices, am_array** out);
+
+/* reference include/arrowmetal.h:94 (ArrowMetalC.swift:384-389): a zero-copy window; offset < 0,
+ * length < 0 or offset + length > am_length(a) is the error "slice out of range". */
+int  am_slice(am_array* a, int64_t offset, int64_t length, am_array** out);
 
 /* reference include/arrowmetal.h:108 (ArrowMetalC.swift:394-399): stable uint32 ("I") indices,
  * ascending or descending, nulls last, float order ieee. */
@@ -86,8 +92,11 @@ void am_group_by_release(am_groupby* gb);
  *   3 hash_mean      float64 out; correctly rounded for float64 values
  *   4 hash_min       values' type; float64 compares -0.0 < +0.0 and skips NaN
  *   5 hash_max
- * A group with no value to answer with is null. Other ops return 1 with
- * "am_group_agg_ex op N is not supported by this build". */
+ *   6 hash_min_max   struct<min, max> of the two above, the struct itself never null
+ *                    (Kernels/AggregatesExtra.swift:90-96)
+ * A group with no value to answer with is null. hash_count takes a column of any type. Ops 7-25
+ * return 1 with "am_group_agg_ex op N is not supported by this build"; any other op with
+ * "unknown am_group_agg_ex op N". */
 int  am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_array** out);
 
 /* reference include/arrowmetal.h:1331, 1348 (ArrowMetalC_Options.swift:110-119, 166-175).
@@ -95,6 +104,12 @@ int  am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_arr
  * ieee, 1 total, 2 nan_largest, any other value the error "unknown float_order N" (:52-59). */
 int  am_argsort_ex(am_array* a, int descending, int null_placement, am_array** out);
 int  am_argsort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out);
+
+/* reference include/arrowmetal.h:1359 (ArrowMetalC_Options.swift:284-297,
+ * Kernels/UniqueOrder.swift:117-125, Kernels/Unique.swift:46-51): Arrow `unique` of a numeric
+ * column. order 0 (and any value other than 1): first appearance, with the null kept once at the
+ * position of the first null row; order 1: ascending, nulls dropped. */
+int  am_unique_ex(am_array* a, int order, am_array** out);
 
 /* ---------------------------------------------------------------------------------------------
  * Extensions of this build (no counterpart in the reference header). They serve the batched and
@@ -105,6 +120,41 @@ int  am_argsort_ex2(am_array* a, int descending, int null_placement, int float_o
 /* Keep-first unique rows over `count` key columns: the lowest row of every distinct key combination,
  * ascending, as uint32 row numbers (take them with am_take). */
 int  am_unique_rows(am_array** columns, int64_t count, am_array** out);
+
+/* Batched form of am_argsort_ex2 and of am_group_by_keys + am_group_agg_ex sum / count_all, over
+ * `nbatches` record batches stored back to back: batch b is rows [offsets[b], offsets[b+1]) of each
+ * column (offsets has nbatches + 1 entries, offsets[0] == 0, at most 2^32 rows in total). The
+ * reference runs such work one call per batch (its per-thread command-buffer batching,
+ * Sources/ArrowMetal/MetalContext.swift:245-312); here all batches go through one fixed kernel
+ * sequence, captured in a hipGraph on the first run (use_graph != 0) and replayed.
+ *
+ *   am_batched_create   plan for the batch layout; hash_capacity (rounded up to a power of two, at
+ *                       most what one workgroup's LDS holds) bounds the distinct group keys per batch
+ *                       handled by the batched tables; tile_items 8, 16 or 20 keys per thread.
+ *   am_batched_tune     times the candidate tile sizes, hash capacities and group-by strategies
+ *                       (LDS table per chunk or one global table per batch) on these columns and keeps
+ *                       the fastest in the plan.
+ *   am_batched_run      sort_keys: any numeric column, or NULL; group_keys + values: integer columns,
+ *                       or both NULL. No nulls. Each batch's sort is stable, nulls-last semantics of
+ *                       am_argsort_ex2 with `descending` and `float_order` (NaN after every value under
+ *                       ieee). Each batch's groups are ascending by key; sums wrap in 64 bits. A batch
+ *                       with more distinct keys than the capacity is grouped again by the general
+ *                       hash group-by, so every result is complete.
+ *   am_batched_result   of the last run: 0 sort indices (uint32, row numbers local to their batch,
+ *                       batches back to back); 1 group keys (the key column's type); 2 sums (int64);
+ *                       3 counts (int64); 4 group offsets (int64, nbatches + 1: batch b's groups are
+ *                       rows [o[b], o[b+1]) of results 1-3).
+ *   am_batched_config   up to `count` of: tile items, hash capacity, group-by strategy, digit bits,
+ *                       batches regrouped by the general path in the last run, sort tiles. */
+int  am_batched_create(const int64_t* offsets, int64_t nbatches, int hash_capacity, int tile_items,
+                       am_batched** out);
+int  am_batched_tune(am_batched* b, am_array* sort_keys, am_array* group_keys, am_array* values,
+                     int descending, int float_order);
+int  am_batched_run(am_batched* b, am_array* sort_keys, am_array* group_keys, am_array* values,
+                    int descending, int float_order, int use_graph);
+int  am_batched_result(am_batched* b, int which, am_array** out);
+int  am_batched_config(am_batched* b, int64_t* out, int count);
+void am_batched_release(am_batched* b);
 
 /* Kernel timing of the library's own launches, measured with HIP events on the stream each kernel
  * is launched on. am_prof_report writes "name count total_ms\n" lines into buf (NUL-terminated) and

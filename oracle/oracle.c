@@ -27,6 +27,7 @@
  *   or_group_by_dense Sources/ArrowMetalC/ArrowMetalC.swift:416-469, GroupBy.swift:4-5
  *                     (dense keys; out-of-range and null keys skipped).
  *   or_unique_rows    Sources/ArrowMetal/Engine/Executor.swift:134-148 (keep-first unique rows).
+ *   or_unique_values  Sources/ArrowMetal/Kernels/UniqueOrder.swift:117-125, Kernels/Unique.swift:46-51.
  *
  * Every function here is pinned by tests/test_oracle.py against the reference's literal golden
  * vectors, pyarrow/numpy fixtures under tests/golden, math.fsum / fractions, and brute force.
@@ -513,4 +514,28 @@ int64_t or_unique_rows(int ncols, const int* types, const void* const* vals, con
     for (int64_t i = 0; i < n; i++) if (!seen[ids[i]]) { seen[ids[i]] = 1; out_rows[o++] = (uint32_t)i; }
     free(ids); free(rep); free(seen);
     return o;
+}
+
+/*
+ * Arrow `unique` of one column (ArrowMetalC_Options.swift:284-297): the rows whose values form the
+ * result, to be gathered. order 0 (Kernels/UniqueOrder.swift:117-125): the first row of every
+ * distinct value in row order, the first null row included. order 1 (Kernels/Unique.swift:46-51):
+ * one row per distinct non-null value, ascending by value (the group order), nulls dropped.
+ * Returns the number of rows written.
+ */
+int64_t or_unique_values(int type, const void* vals, const uint8_t* valid, int64_t n, int order,
+                         uint32_t* out_rows) {
+    if (n == 0) return 0;
+    const void* const v1[1] = {vals};
+    const uint8_t* const b1[1] = {valid};
+    if (order != 1) return or_unique_rows(1, &type, v1, b1, n, out_rows);
+    int32_t* ids = (int32_t*)malloc((size_t)n * 4);
+    uint32_t* rep = (uint32_t*)malloc((size_t)n * 4);
+    int64_t K = or_group_keys(1, &type, v1, b1, n, ids, rep);
+    int has_null = 0;
+    for (int64_t i = 0; i < n && valid; i++) if (!((valid[i >> 3] >> (i & 7)) & 1)) { has_null = 1; break; }
+    int64_t k = K - has_null;
+    for (int64_t g = 0; g < k; g++) out_rows[g] = rep[g];
+    free(ids); free(rep);
+    return k;
 }

@@ -38,6 +38,8 @@ def lib():
         L.or_take.argtypes = [P, c_int, P, i64, P, c_int, P, i64, P, P]
         L.or_group_keys.argtypes = [c_int, P, P, P, i64, P, P]
         L.or_group_keys.restype = i64
+        L.or_unique_values.argtypes = [c_int, P, P, i64, c_int, P]
+        L.or_unique_values.restype = i64
         L.or_group_agg.argtypes = [P, i64, i64, P, c_int, P, c_int, P, P]
         L.or_group_by_dense.argtypes = [P, c_int, P, i64, i64, P, c_int, P, c_int, P, P]
         L.or_unique_rows.argtypes = [c_int, P, P, P, i64, P]
@@ -143,4 +145,16 @@ def unique_rows(columns, valids=None):
     out = np.empty(max(n, 1), dtype=np.uint32)
     m = lib().or_unique_rows(len(columns), _ptr(types), ctypes.cast(vptrs, ctypes.c_void_p),
                              ctypes.cast(bptrs, ctypes.c_void_p), n, _ptr(out))
+    return out[:m].copy()
+
+
+def unique_values(values, valid=None, order=0):
+    """Rows whose values form Arrow `unique` (order 0 first appearance with the null once, order 1
+    ascending without nulls); gather them to get the result."""
+    values = np.ascontiguousarray(values)
+    n = len(values)
+    bm = pack_validity(valid)
+    out = np.empty(max(n, 1), dtype=np.uint32)
+    m = lib().or_unique_values(TYPE_OF[values.dtype], _ptr(values), None if bm is None else _ptr(bm), n,
+                               order, _ptr(out))
     return out[:m].copy()

@@ -265,3 +265,70 @@ def test_dense_group_by_out_of_range_and_nulls():
         keys2 = np.abs(keys).astype(it)
         gb2 = am.MetalArray.from_numpy(keys2, kv).group_by(K)
         assert_agg(gb2.sum(mv), O.group_by_dense(keys2, K, vals, 0, kv, vv), f"dense sum {it}")
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32, np.float64])
+def test_min_max_struct(dtype):
+    """op 6 hash_min_max: a struct<min, max> whose fields equal ops 4 and 5 bit for bit."""
+    pa = pytest.importorskip("pyarrow")
+    rng = np.random.default_rng(66)
+    n, span = 50_001, 900
+    keys = rng.integers(0, span, n).astype(np.int64)
+    if dtype == np.float64:
+        vals = rng.normal(0, 1e3, n)
+        vals[rng.random(n) < 0.01] = -0.0
+        vals[rng.random(n) < 0.01] = np.nan
+    else:
+        vals = rng.integers(-10**6, 10**6, n).astype(dtype)
+    vv = rng.random(n) >= 0.05
+    vv[keys == 7] = False   # one group with no value: null min and max
+    g = am.group_by([am.MetalArray.from_numpy(keys)])
+    ids_w, K, _ = O.group_keys([keys], [None])
+    st = g.min_max(am.MetalArray.from_numpy(vals, vv))
+    assert st.format == "+s" and len(st) == K and st.null_count == 0
+    arr = st.to_arrow()
+    assert isinstance(arr, pa.StructArray) and [arr.type.field(i).name for i in range(2)] == ["min", "max"]
+    for f, op in (("min", 4), ("max", 5)):
+        col = arr.field(f)
+        want_bits, want_valid = O.group_agg(ids_w, K, vals, op, vv)
+        got_valid = np.asarray(col.is_valid())
+        np.testing.assert_array_equal(got_valid, want_valid, err_msg=f)
+        got = col.fill_null(0).to_numpy(zero_copy_only=False)
+        got_bits = (got.view(np.uint64) if dtype == np.float64 else got.astype(np.int64).view(np.uint64))
+        np.testing.assert_array_equal(got_bits[want_valid], want_bits[want_valid], err_msg=f)
+
+
+@pytest.mark.parametrize("n,span,null_frac", [(0, 5, 0.0), (1, 5, 1.0), (300, 20, 0.1), (20001, 5000, 0.02),
+                                              (300_000, 100_000, 0.01), (4097, 3, 0.0)])
+@pytest.mark.parametrize("order", [0, 1])
+def test_unique_ex(n, span, null_frac, order):
+    """am_unique_ex: first appearance (the null once, at the first null row) or ascending (nulls
+    dropped), against the oracle's rows."""
+    rng = np.random.default_rng(n + span + order)
+    vals = (rng.integers(-span, span + 1, n) * 7919).astype(np.int64)
+    valid = rng.random(n) >= null_frac
+    got = am.MetalArray.from_numpy(vals, valid).unique("sorted" if order == 1 else "first_appearance")
+    rows = O.unique_values(vals, valid, order)
+    gv, gvalid = got.to_numpy()
+    assert len(gv) == len(rows)
+    want_valid = valid[rows]
+    if gvalid is None:
+        gvalid = np.ones(len(gv), dtype=bool)
+    np.testing.assert_array_equal(gvalid, want_valid)
+    np.testing.assert_array_equal(gv[want_valid], vals[rows][want_valid])
+
+
+def test_unique_ex_float_and_narrow():
+    rng = np.random.default_rng(5)
+    for vals in (rng.integers(-3, 4, 10_000).astype(np.int8),
+                 rng.integers(0, 50, 10_000).astype(np.uint16),
+                 np.round(rng.normal(0, 4, 10_000)).astype(np.float64)):
+        valid = rng.random(len(vals)) >= 0.03
+        for order in (0, 1):
+            got = am.MetalArray.from_numpy(vals, valid).unique("sorted" if order else "first_appearance")
+            rows = O.unique_values(vals, valid, order)
+            gv, gvalid = got.to_numpy()
+            if gvalid is None:
+                gvalid = np.ones(len(gv), dtype=bool)
+            np.testing.assert_array_equal(gvalid, valid[rows])
+            np.testing.assert_array_equal(gv[gvalid].view(np.uint8), vals[rows][gvalid].view(np.uint8))

@@ -445,3 +445,22 @@ def test_planted_mistake_is_caught(sort_fx):
         bad[i:j + 1] = bad[i:j + 1][::-1]
         i = j + 1
     assert not np.array_equal(bad, want)
+
+
+@pytest.mark.parametrize("n,span,null_frac", [(0, 5, 0.0), (1, 5, 1.0), (300, 20, 0.1), (20001, 5000, 0.02),
+                                              (4097, 3, 0.0)])
+def test_unique_values_matches_arrow_and_numpy(n, span, null_frac):
+    """First appearance with the null kept once is Arrow's own `unique`; sorted is numpy's `unique` of
+    the non-null values."""
+    pa = pytest.importorskip("pyarrow")
+    pc = pytest.importorskip("pyarrow.compute")
+    rng = np.random.default_rng(n + span)
+    vals = (rng.integers(-span, span + 1, n) * 7919).astype(np.int64)
+    valid = rng.random(n) >= null_frac
+    rows0 = O.unique_values(vals, valid, 0)
+    want0 = pc.unique(pa.array(vals, mask=~valid)).to_pylist()
+    got0 = [int(vals[r]) if valid[r] else None for r in rows0]
+    assert got0 == want0
+    rows1 = O.unique_values(vals, valid, 1)
+    assert vals[rows1].tolist() == np.unique(vals[valid]).tolist()
+    assert all(valid[rows1])
