@@ -108,6 +108,13 @@ _SIGNATURES = {
     "am_batched_result": ([_P, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
     "am_batched_config": ([_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
     "am_batched_release": ([_P], None),
+    "am_server_create": ([ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_server_submit": ([_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.c_int64, _P, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_int, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "am_server_wait": ([_P, ctypes.c_int64, ctypes.POINTER(_P), ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "am_server_stats": ([_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
+    "am_server_release": ([_P], None),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
     "am_prof_report": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int64),
@@ -487,6 +494,56 @@ class Batched:
         _check(lib().am_batched_config(self._h, v, 6))
         return {"tile_items": v[0], "hash_capacity": v[1], "strategy": ("lds", "global")[v[2]],
                 "digit_bits": v[3], "regrouped_batches": v[4], "sort_tiles": v[5]}
+
+
+class Server:
+    """Request server with continuous batching (`am_server_*`): `slots` workers, each on its own HIP
+    stream, admit requests in arrival order as slots and the device-memory budget free up."""
+
+    OPS = {"group_by": 0, "argsort": 1, "unique": 2}
+
+    def __init__(self, slots=4, budget_bytes=64 << 30):
+        self._h = None
+        h = _P()
+        _check(lib().am_server_create(int(slots), int(budget_bytes), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.am_server_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, op, columns, values=None, descending=False, null_placement="at_end", float_order="ieee"):
+        """Queues a request; returns its ticket. op: "group_by" (columns + values), "argsort" (one
+        column) or "unique" (columns)."""
+        if not isinstance(columns, (list, tuple)):
+            columns = [columns]
+        handles = (_P * len(columns))(*[c._h for c in columns])
+        t = ctypes.c_int64()
+        _check(lib().am_server_submit(self._h, self.OPS[op], handles, len(columns),
+                                      None if values is None else values._h, 1 if descending else 0,
+                                      NULL_PLACEMENTS.index(null_placement), _float_order(float_order),
+                                      ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, cap=8):
+        """(results, (queued_ms, running_ms, latency_ms)) of a request, blocking until it is done."""
+        outs = (_P * cap)()
+        times = (ctypes.c_double * 3)()
+        k = ctypes.c_int64()
+        _check(lib().am_server_wait(self._h, int(ticket), outs, cap, times, ctypes.byref(k)))
+        return [MetalArray(outs[i]) for i in range(min(k.value, cap))], tuple(times)
+
+    @property
+    def stats(self):
+        v = (ctypes.c_int64 * 8)()
+        _check(lib().am_server_stats(self._h, v, 8))
+        names = ("submitted", "completed", "failed", "peak_admitted_bytes", "peak_running", "budget_bytes",
+                 "queued", "slots")
+        return dict(zip(names, list(v)))
 
 
 class batch:

@@ -72,6 +72,7 @@ int type_of_format(const std::string& fmt);
 struct DevBlock {
   void* ptr = nullptr;
   size_t bytes = 0;
+  hipStream_t owner = nullptr;          // stream of the thread that took the block from the pool
   std::function<void(void*)> on_free;   // empty: the block belongs to the pool
   ~DevBlock();
 };

@@ -30,6 +30,13 @@ void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, c
                   int forder);
 void batched_config(const Batched* b, int64_t* out, int count);
 Array* batched_result(Batched* b, int which);
+struct Server;
+Server* server_create(int slots, int64_t budget_bytes);
+void server_release(Server* s);
+int64_t server_submit(Server* s, int op, Array* const* cols, int64_t ncols, Array* values, int desc, int placement,
+                      int forder);
+int64_t server_wait(Server* s, int64_t ticket, Array** out, int64_t cap, double* times);
+void server_stats(Server* s, int64_t* out, int count);
 }  // namespace am
 
 using am::Array;
@@ -225,6 +232,43 @@ int am_batched_result(am_batched* b, int which, am_array** out) {
 void am_batched_release(am_batched* b) {
   try {
     am::batched_release(B(b));
+  } catch (...) {
+  }
+}
+
+int am_server_create(int slots, int64_t budget_bytes, am_server** out) {
+  if (!out) return 2;
+  return guarded([&] { *out = reinterpret_cast<am_server*>(am::server_create(slots, budget_bytes)); });
+}
+
+int am_server_submit(am_server* s, int op, am_array** columns, int64_t count, am_array* values, int descending,
+                     int null_placement, int float_order, int64_t* ticket) {
+  if (!s || !columns || !ticket || count < 1) return 2;
+  for (int64_t i = 0; i < count; i++)
+    if (!columns[i]) return 2;
+  return guarded([&] {
+    *ticket = am::server_submit(reinterpret_cast<am::Server*>(s), op, reinterpret_cast<Array* const*>(columns),
+                                count, values ? A(values) : nullptr, descending, null_placement, float_order);
+  });
+}
+
+int am_server_wait(am_server* s, int64_t ticket, am_array** out, int64_t cap, double* times_ms, int64_t* count) {
+  if (!s || (!out && cap > 0)) return 2;
+  return guarded([&] {
+    int64_t k = am::server_wait(reinterpret_cast<am::Server*>(s), ticket, reinterpret_cast<Array**>(out), cap,
+                                times_ms);
+    if (count) *count = k;
+  });
+}
+
+int am_server_stats(am_server* s, int64_t* out, int count) {
+  if (!s || !out) return 2;
+  return guarded([&] { am::server_stats(reinterpret_cast<am::Server*>(s), out, count); });
+}
+
+void am_server_release(am_server* s) {
+  try {
+    am::server_release(reinterpret_cast<am::Server*>(s));
   } catch (...) {
   }
 }

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <unordered_map>
 
 namespace am {
@@ -79,9 +80,15 @@ const DeviceInfo& device() {
 
 namespace {
 
+// Free blocks are kept per stream. A block freed while kernels queued on its stream may still use
+// it (a temporary dropped before the call's final sync) goes back to that stream's list, where only
+// later work of the same stream, ordered after those kernels, can take it. Another stream takes it
+// only once the owner stream has drained (hipStreamQuery), so concurrent threads never share a block
+// that is still in use.
 struct Pool {
   std::mutex mu;
-  std::map<size_t, std::vector<void*>> free_lists;   // size class -> cached blocks
+  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> free_lists;   // (stream, class) -> blocks
+  std::set<hipStream_t> live;   // streams of running threads; only these are ever queried
   size_t in_use = 0, cached = 0;
 };
 Pool& pool() {
@@ -99,35 +106,84 @@ size_t size_class(size_t bytes) {
   return p;
 }
 
-void pool_free(void* ptr, size_t cls) {
+void pool_free(void* ptr, size_t cls, hipStream_t owner) {
   Pool& P = pool();
   std::lock_guard<std::mutex> g(P.mu);
-  P.free_lists[cls].push_back(ptr);
+  // A block that outlives its thread (a result released after the thread ended) is idle: its stream
+  // was drained before it was destroyed.
+  if (owner && !P.live.count(owner)) owner = nullptr;
+  P.free_lists[{owner, cls}].push_back(ptr);
   P.in_use -= cls;
   P.cached += cls;
 }
 
+// A cached block of class `cls` that stream `s` may use now, or null. Caller holds the lock.
+void* pool_take(Pool& P, hipStream_t s, size_t cls) {
+  auto it = P.free_lists.find({s, cls});
+  if (it == P.free_lists.end() || it->second.empty()) {
+    it = P.free_lists.end();
+    for (auto jt = P.free_lists.begin(); jt != P.free_lists.end(); ++jt) {
+      if (jt->first.second != cls || jt->second.empty()) continue;
+      hipStream_t other = jt->first.first;
+      if (other && P.live.count(other)) {   // otherwise the stream of a finished thread, drained before it ended
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(other, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) continue;
+        if (hipStreamQuery(other) != hipSuccess) {   // still running work that may use the block
+          (void)hipGetLastError();
+          continue;
+        }
+      }
+      it = jt;
+      break;
+    }
+    if (it == P.free_lists.end()) return nullptr;
+  }
+  void* ptr = it->second.back();
+  it->second.pop_back();
+  P.cached -= cls;
+  P.in_use += cls;
+  return ptr;
+}
+
 }  // namespace
+
+void pool_register_stream(hipStream_t s) {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  P.live.insert(s);
+}
+
+// A thread's stream is about to be destroyed: drain it and hand its free blocks to every stream.
+void pool_retire_stream(hipStream_t s) {
+  (void)hipStreamSynchronize(s);
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  P.live.erase(s);
+  for (auto it = P.free_lists.begin(); it != P.free_lists.end();) {
+    if (it->first.first != s) {
+      ++it;
+      continue;
+    }
+    auto& dst = P.free_lists[{nullptr, it->first.second}];
+    dst.insert(dst.end(), it->second.begin(), it->second.end());
+    it = P.free_lists.erase(it);
+  }
+}
 
 DevBlock::~DevBlock() {
   if (!ptr) return;
   if (on_free) on_free(ptr);
-  else pool_free(ptr, bytes);
+  else pool_free(ptr, bytes, owner);
 }
 
 DevBuf device_alloc(size_t bytes) {
   size_t cls = size_class(bytes ? bytes : 1);
   Pool& P = pool();
+  hipStream_t s = stream();
   void* ptr = nullptr;
   {
     std::lock_guard<std::mutex> g(P.mu);
-    auto it = P.free_lists.find(cls);
-    if (it != P.free_lists.end() && !it->second.empty()) {
-      ptr = it->second.back();
-      it->second.pop_back();
-      P.cached -= cls;
-      P.in_use += cls;
-    }
+    ptr = pool_take(P, s, cls);
   }
   if (!ptr) {
     device();
@@ -147,6 +203,7 @@ DevBuf device_alloc(size_t bytes) {
   auto b = std::make_shared<DevBlock>();
   b->ptr = ptr;
   b->bytes = cls;
+  b->owner = s;
   retain(b);   // inside a batch, scratch outlives the queued kernels that use it
   return b;
 }
@@ -169,7 +226,8 @@ size_t pool_bytes_cached() {
 }
 void pool_trim() {
   Pool& P = pool();
-  std::map<size_t, std::vector<void*>> lists;
+  (void)hipDeviceSynchronize();   // blocks of other streams may still be read by queued kernels
+  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> lists;
   {
     std::lock_guard<std::mutex> g(P.mu);
     lists.swap(P.free_lists);
@@ -190,7 +248,10 @@ struct ThreadState {
   std::vector<std::function<void()>> deferred;
   ~ThreadState() {
     // Streams are left to the driver at process exit; a live thread's stream is reused.
-    if (s && batch_depth == 0) (void)hipStreamDestroy(s);
+    if (s && batch_depth == 0) {
+      pool_retire_stream(s);
+      (void)hipStreamDestroy(s);
+    }
   }
 };
 thread_local ThreadState t_state;
@@ -224,6 +285,7 @@ hipStream_t stream() {
   if (!t_state.s) {
     device();
     AM_HIP(hipStreamCreateWithFlags(&t_state.s, hipStreamNonBlocking));
+    pool_register_stream(t_state.s);
   }
   return t_state.s;
 }

@@ -180,6 +180,139 @@ def batched_leg(args, log):
     }
 
 
+def serving_inputs(n: int = 50_000_000, seed: int = 0x5E4E):
+    """Resident columns of the served mix: group keys Zipf(1.1) over 200, 10,000 and 1,000,000
+    distinct int64 values; int64 values uniform in [-1e9, 1e9]; full-range int64 sort keys; the two
+    unique keys uniform over 10^6 and 10^3 (10M rows)."""
+    gkeys = {}
+    for support in (200, 10_000, 1_000_000):
+        table = splitmix64(seed + support, support).view(np.int64)
+        gkeys[support] = table[zipf_ranks(n, support, 1.1, seed + 7 * support)]
+    gvals = (splitmix64(seed + 1, n) % np.uint64(2_000_000_001)).astype(np.int64) - 1_000_000_000
+    skeys = splitmix64(seed + 2, n).view(np.int64)
+    nu = 10_000_000
+    ua = (splitmix64(seed + 3, nu) % np.uint64(1_000_000)).astype(np.int64)
+    ub = (splitmix64(seed + 4, nu) % np.uint64(1_000)).astype(np.int64)
+    return gkeys, gvals, skeys, ua, ub
+
+
+def serving_leg(args, log):
+    """Poisson arrivals of a mixed request stream on resident columns, served by am_server (4 slots,
+    admission by footprint) and, on the same schedule, by one thread making the plain calls."""
+    import threading
+    gkeys, gvals, skeys, ua, ub = serving_inputs()
+    dev_g = {k: am.MetalArray.from_numpy(v) for k, v in gkeys.items()}
+    dev_v = am.MetalArray.from_numpy(gvals)
+    dev_s = am.MetalArray.from_numpy(skeys)
+    dev_a, dev_b = am.MetalArray.from_numpy(ua), am.MetalArray.from_numpy(ub)
+    n, nu = len(gvals), len(ua)
+    del gkeys, gvals, skeys, ua, ub
+    rng = np.random.default_rng(0x5E4F)
+    count, rate = args.requests, args.rate
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for _ in range(count):
+        u = rng.random()
+        if u < 0.6:
+            rows = int(np.exp(rng.uniform(np.log(1e6), np.log(n))))
+            support = int(rng.choice([200, 10_000, 1_000_000]))
+            off = int(rng.integers(0, n - rows + 1))
+            reqs.append(("group_by", [dev_g[support].slice(off, rows)], dev_v.slice(off, rows), rows))
+        elif u < 0.9:
+            rows = int(np.exp(rng.uniform(np.log(250_000), np.log(n))))
+            off = int(rng.integers(0, n - rows + 1))
+            reqs.append(("argsort", [dev_s.slice(off, rows)], None, rows))
+        else:
+            rows = int(np.exp(rng.uniform(np.log(1e6), np.log(nu))))
+            off = int(rng.integers(0, nu - rows + 1))
+            reqs.append(("unique", [dev_a.slice(off, rows), dev_b.slice(off, rows)], None, rows))
+    total_rows = sum(r[3] for r in reqs)
+
+    def plain_call(kind, cols, values):
+        if kind == "argsort":
+            return [cols[0].argsort()]
+        if kind == "group_by":
+            g = am.group_by(cols)
+            return g.key_arrays() + [g.sum(values), g.count_all()]
+        rows = am.unique_rows(cols)
+        return [rows] + [c.take(rows) for c in cols]
+
+    for kind, cols, values, _ in reqs[:20]:   # warm both paths (pool, code objects)
+        plain_call(kind, cols, values)
+
+    # Plain: one caller, in arrival order, each call synchronous.
+    lat_plain = []
+    plain_out = []
+    t0 = time.perf_counter()
+    for i, ((kind, cols, values, _), at) in enumerate(zip(reqs, arrivals)):
+        wait = at - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        res = plain_call(kind, cols, values)
+        lat_plain.append(time.perf_counter() - t0 - at)
+        plain_out.append(res if i % 7 == 0 else None)   # a sample is kept for the comparison
+        del res
+    plain_span = time.perf_counter() - t0
+
+    srv = am.Server(slots=args.slots, budget_bytes=args.budget_gb << 30)
+    warm = [srv.submit(k, c, v) for k, c, v, _ in reqs[:20]]
+    for t in warm:
+        srv.wait(t)
+    tickets = [None] * count
+    lateness = [0.0] * count
+
+    def submitter():
+        for i, ((kind, cols, values, _), at) in enumerate(zip(reqs, arrivals)):
+            wait = at - (time.perf_counter() - t1)
+            if wait > 0:
+                time.sleep(wait)
+            lateness[i] = max(0.0, time.perf_counter() - t1 - at)
+            tickets[i] = srv.submit(kind, cols, values)
+
+    t1 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat_served, served_out = [], []
+    for i in range(count):
+        while tickets[i] is None:
+            time.sleep(0.0001)
+        res, times = srv.wait(tickets[i])
+        lat_served.append(lateness[i] + times[2] * 1e-3)
+        served_out.append(res if i % 7 == 0 else None)
+        del res
+    th.join()
+    served_span = time.perf_counter() - t1
+    st = srv.stats
+    srv.close()
+    # Every served result must equal the plain call's.
+    identical = True
+    for a_list, b_list in zip(served_out[::7], plain_out[::7]):
+        for a, b in zip(a_list, b_list):
+            va, ma = a.to_numpy()
+            vb, mb = b.to_numpy()
+            identical &= bool(np.array_equal(va, vb)) and ((ma is None) == (mb is None))
+    pct = lambda xs, q: 1e3 * float(np.percentile(np.asarray(xs), q))
+    res = {
+        "workload": f"served_mix_{count}_requests_poisson_{rate:g}_per_s",
+        "requests": count,
+        "rate_per_s": rate,
+        "mix": "60% group-by sum/count (1M-50M rows, Zipf(1.1) over 200 / 10k / 1M keys), 30% int64 argsort "
+               "(250k-50M rows), 10% unique over two int64 keys (1M-10M rows, 10^6 x 10^3)",
+        "total_rows": total_rows,
+        "served": {"p50_ms": pct(lat_served, 50), "p99_ms": pct(lat_served, 99), "mean_ms": 1e3 * float(np.mean(lat_served)),
+                   "rows_per_s": total_rows / served_span, "span_s": served_span,
+                   "slots": args.slots, "budget_bytes": args.budget_gb << 30,
+                   "peak_admitted_bytes": st["peak_admitted_bytes"], "peak_running": st["peak_running"]},
+        "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99), "mean_ms": 1e3 * float(np.mean(lat_plain)),
+                  "rows_per_s": total_rows / plain_span, "span_s": plain_span},
+        "served_equals_plain": identical,
+        "compared_requests": len(served_out[::7]),
+    }
+    log(f"serving: served p50 {res['served']['p50_ms']:.2f} p99 {res['served']['p99_ms']:.2f} ms; "
+        f"plain p50 {res['plain']['p50_ms']:.2f} p99 {res['plain']['p99_ms']:.2f} ms; stats {st}")
+    return res
+
+
 def timed(fn, steps: int, warmup: int):
     for _ in range(warmup):
         fn()
@@ -206,6 +339,11 @@ def main():
     ap.add_argument("--skip-groupby", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batches", type=int, default=2048)
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--rate", type=float, default=400.0, help="Poisson arrival rate of the served mix (1/s)")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -325,6 +463,10 @@ def main():
     if not args.skip_batched:
         bt_res = batched_leg(args, log)
 
+    sv_res = None
+    if not args.skip_serving:
+        sv_res = serving_leg(args, log)
+
     b_sort = 8 * n + (8 * n + 12 * m + 4 * (n - m)) + 6 * 2 * m * 12 + m * 16 + 2 * ((n + 7) // 8)
     b_take = 20 * n
     result = {
@@ -358,6 +500,7 @@ def main():
                        "workload": "stable ascending argsort of 1,000,000 full-range int64 keys"},
         "group_by": gb_res,
         "batched": bt_res,
+        "serving": sv_res,
     }
     print(json.dumps(result), flush=True)
 

@@ -31,6 +31,8 @@ typedef struct am_array am_array;
 typedef struct am_groupby am_groupby;
 /* Extension of this build: a plan over many record batches (see am_batched_create). */
 typedef struct am_batched am_batched;
+/* Extension of this build: a request server (see am_server_create). */
+typedef struct am_server am_server;
 
 /* reference include/arrowmetal.h:15-17 (ArrowMetalC.swift am_version / am_device_name / am_last_error) */
 const char* am_version(void);
@@ -155,6 +157,34 @@ int  am_batched_run(am_batched* b, am_array* sort_keys, am_array* group_keys, am
 int  am_batched_result(am_batched* b, int which, am_array** out);
 int  am_batched_config(am_batched* b, int64_t* out, int count);
 void am_batched_release(am_batched* b);
+
+/* Request server with continuous batching over HBM-resident columns. The reference serves one
+ * synchronous call at a time on the caller's thread; here `slots` worker threads (1..16), each with its
+ * own HIP stream, take requests from one FIFO queue. A request is admitted when a worker is free and
+ * its estimated device footprint fits in what `budget_bytes` has left (one larger than the whole budget
+ * runs alone); requests that arrive while others run join them and their kernels overlap. Admission
+ * keeps arrival order. Each request runs the code of the plain calls, so its results are those of the
+ * plain calls, alone or in the mix.
+ *
+ *   am_server_submit   op 0: group-by of `columns` with `values`: results keys (one array per key
+ *                      column, ascending groups), int64 sum (hash_sum), int64 count (hash_count_all);
+ *                      op 1: argsort of columns[0] with descending / null_placement / float_order as
+ *                      am_argsort_ex2: result uint32 indices; op 2: keep-first unique rows over
+ *                      `columns`: results the uint32 rows (am_unique_rows) and each column gathered at
+ *                      them. The server holds a reference to every array until the request is done.
+ *   am_server_wait     blocks until the request is done, hands over up to `cap` results (new
+ *                      references; *count gets how many there are) and writes times_ms[3]: queued,
+ *                      running, submit to done. A failed request returns 1 with its error.
+ *   am_server_stats    up to `count` of: submitted, completed, failed, peak admitted footprint (bytes),
+ *                      peak requests running together, budget, queued now, slots.
+ *   am_server_release  finishes the queued requests, then stops the workers. */
+int  am_server_create(int slots, int64_t budget_bytes, am_server** out);
+int  am_server_submit(am_server* s, int op, am_array** columns, int64_t count, am_array* values,
+                      int descending, int null_placement, int float_order, int64_t* ticket);
+int  am_server_wait(am_server* s, int64_t ticket, am_array** out, int64_t cap, double* times_ms,
+                    int64_t* count);
+int  am_server_stats(am_server* s, int64_t* out, int count);
+void am_server_release(am_server* s);
 
 /* Kernel timing of the library's own launches, measured with HIP events on the stream each kernel
  * is launched on. am_prof_report writes "name count total_ms\n" lines into buf (NUL-terminated) and
