@@ -324,6 +324,23 @@ def timed(fn, steps: int, warmup: int):
     return ts
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "orderby_groupby_50M_pmc_traffic.txt")
+
+
+def committed_traffic(prefix: str):
+    """HBM bytes per launch of the first kernel whose name starts with `prefix`, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or (None, None)."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic " + prefix):
+                    name, value = line[len("traffic "):].rsplit(" ", 1)
+                    return int(value), name
+    except OSError:
+        pass
+    return None, None
+
+
 def kernel_ms(name: str, prof: dict):
     c, ms = prof.get(name, (0, 0.0))
     return (ms / c if c else None), c
@@ -381,6 +398,7 @@ def main():
     log("kernels:", json.dumps(prof))
     scat_ms, scat_n = kernel_ms("sort_scatter", prof)
     bytes_per_launch = 24 * m
+    traffic, traffic_kernel = committed_traffic("sort_mid<u64, ")
     achieved = bytes_per_launch / (scat_ms * 1e-3) / 1e9 if scat_ms else None
 
     # Parity of the timed step against the CPU oracle (and the CPU baseline timing).
@@ -489,7 +507,9 @@ def main():
         "step_hbm_frac_of_b_alg": (b_sort + b_take) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "b_alg_bytes": {"sort": b_sort, "gather": b_take},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "traffic_source": (f"{os.path.relpath(PMC_SUMMARY, ROOT)} ({traffic_kernel}, FETCH_SIZE + "
+                                        "WRITE_SIZE scaled by a 1 GiB copy)") if traffic else None,
                      "kernel": "sort_scatter (a middle radix pass: read key + row, write key + row)",
                      "bytes_per_launch": bytes_per_launch, "mean_launch_ms": scat_ms,
                      "launches": scat_n},
