@@ -58,6 +58,14 @@ __host__ __device__ inline int type_width(int t) {
     default: return 0;
   }
 }
+// The XCD (accelerator complex die) a wave runs on, from the XCC_ID hardware register. Each XCD has
+// its own L2: memory private to one XCD can be updated with workgroup-scope atomics, which the XCD's
+// L2 executes, instead of device-scope ones, which must go to memory.
+constexpr int NXCD = 8;
+__device__ __forceinline__ uint32_t xcd_id() {
+  return uint32_t(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & (NXCD - 1);
+}
+
 __host__ __device__ inline bool type_is_float(int t) { return t == T_F32 || t == T_F64; }
 __host__ __device__ inline bool type_is_signed(int t) { return t >= T_I8 && t <= T_I64; }
 __host__ __device__ inline bool type_is_int(int t) { return t >= T_I8 && t <= T_U64; }

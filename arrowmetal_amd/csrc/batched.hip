@@ -189,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
   const int shift = 8 * dpass;
 
   K keys[ITEMS];
-  uint32_t rows[ITEMS], bins[ITEMS], lrank[ITEMS];
+  uint32_t rows[ITEMS], bins[ITEMS];
   const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
@@ -208,8 +208,8 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
       bins[j] = uint32_t(keys[j] >> shift) & 0xFFu;
     }
   }
-  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, lrank, wc[w], lane);
-  else rank_tile<ITEMS, 9>(bins, lrank, wc[w], lane);
+  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
   __syncthreads();
   uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
   look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
@@ -217,11 +217,11 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
   bin_base[threadIdx.x] += base[(size_t(t.seg) * P + dpass) * RADIX + threadIdx.x];
   __syncthreads();
   if (last) {
-    scatter_values<K, ITEMS, false>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                    vcount, nullptr, out_idx);
+    scatter_values<K, ITEMS, false>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                    nullptr, out_idx);
   } else {
-    scatter_values<K, ITEMS, true>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                   vcount, out_keys, out_rows);
+    scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                   out_keys, out_rows);
   }
 }
 

@@ -50,23 +50,30 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
 
 // The 64-bit grouping key of a non-null element: integers widen (sign-extended when signed), floats
 // are normalised so that bit equality is Arrow value equality (HashTable.swift:290-302).
+// Column reads of the row passes are streaming (non-temporal): each row is read once, and the hash
+// and aggregate tables stay resident in L2 instead.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const void* p, int64_t i) {
+  return __builtin_nontemporal_load(static_cast<const T*>(p) + i);
+}
+
 __device__ __forceinline__ unsigned long long group_key(const void* vals, int64_t i, int type) {
   switch (type) {
-    case T_I64: case T_U64: return static_cast<const unsigned long long*>(vals)[i];
-    case T_I32: return (unsigned long long)(long long)static_cast<const int32_t*>(vals)[i];
-    case T_I16: return (unsigned long long)(long long)static_cast<const int16_t*>(vals)[i];
-    case T_I8: return (unsigned long long)(long long)static_cast<const int8_t*>(vals)[i];
-    case T_U32: return static_cast<const uint32_t*>(vals)[i];
-    case T_U16: return static_cast<const uint16_t*>(vals)[i];
-    case T_U8: return static_cast<const uint8_t*>(vals)[i];
+    case T_I64: case T_U64: return ld_stream<unsigned long long>(vals, i);
+    case T_I32: return (unsigned long long)(long long)ld_stream<int32_t>(vals, i);
+    case T_I16: return (unsigned long long)(long long)ld_stream<int16_t>(vals, i);
+    case T_I8: return (unsigned long long)(long long)ld_stream<int8_t>(vals, i);
+    case T_U32: return ld_stream<uint32_t>(vals, i);
+    case T_U16: return ld_stream<uint16_t>(vals, i);
+    case T_U8: return ld_stream<uint8_t>(vals, i);
     case T_F64: {
-      unsigned long long b = static_cast<const unsigned long long*>(vals)[i];
+      unsigned long long b = ld_stream<unsigned long long>(vals, i);
       if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
       if ((b & 0x7FFFFFFFFFFFFFFFull) == 0ull) return 0ull;
       return b;
     }
     default: {   // T_F32
-      uint32_t b = static_cast<const uint32_t*>(vals)[i];
+      uint32_t b = ld_stream<uint32_t>(vals, i);
       if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000u;
       if ((b & 0x7FFFFFFFu) == 0u) return 0u;
       return b;
@@ -85,10 +92,11 @@ __device__ __forceinline__ unsigned long long order_key(unsigned long long gk, i
   return gk;
 }
 
-struct Slot {
-  unsigned long long key;
-  uint32_t first;   // lowest row of the slot's key
-  uint32_t pad;
+// The hash table is two arrays of cap + 1 entries: the keys (EMPTY when free; entry cap holds the
+// EMPTY key itself) and, for builds that track them, the lowest row of each key.
+struct Table {
+  unsigned long long* key;
+  uint32_t* first;   // null when not tracked
 };
 
 struct BuildStatus {
@@ -99,13 +107,12 @@ struct BuildStatus {
   unsigned long long nulls;
 };
 
-__global__ void gb_table_init(Slot* table, size_t slots, BuildStatus* st) {
+__global__ void gb_table_init(Table table, size_t slots, BuildStatus* st) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t stride = size_t(gridDim.x) * blockDim.x;
   for (; i < slots; i += stride) {
-    table[i].key = EMPTY;
-    table[i].first = 0xFFFFFFFFu;
-    table[i].pad = 0;
+    table.key[i] = EMPTY;
+    if (table.first) table.first[i] = 0xFFFFFFFFu;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->overflow = 0;
@@ -124,20 +131,72 @@ __device__ __forceinline__ void lower_first(uint32_t* p, uint32_t row) {
 
 // Probes (and inserts) key k, starting at its hash slot. Returns its slot, or NULL_SLOT when the probe
 // budget ran out (the table is too full: the caller grows it and builds again).
-__device__ __forceinline__ uint32_t probe_insert(Slot* table, unsigned long long k, uint32_t s, unsigned long long cur,
-                                                 uint32_t cap_mask, uint32_t budget) {
+__device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsigned long long k, uint32_t s,
+                                                 unsigned long long cur, uint32_t cap_mask, uint32_t budget) {
   uint32_t probes = 0;
   while (true) {
     if (cur == k) return s;
     if (cur == EMPTY) {
-      unsigned long long prev = atomicCAS(&table[s].key, EMPTY, k);
+      unsigned long long prev = atomicCAS(&tkey[s], EMPTY, k);
       if (prev == EMPTY || prev == k) return s;
     }
     if (++probes >= budget) return NULL_SLOT;
     s = (s + 1u) & cap_mask;
     // A slot's key goes from EMPTY to its key once and never changes again, so a plain read is
     // enough: a stale EMPTY only sends the row to the CAS, which returns the settled key.
-    cur = table[s].key;
+    cur = tkey[s];
+  }
+}
+
+// Distinct-key estimate (HyperLogLog, 2^12 registers) that sizes the hash table before the build, so
+// the table is about twice the distinct keys rather than twice the rows, and stays in L2.
+constexpr int HLL_BITS = 12;
+constexpr uint32_t HLL_M = 1u << HLL_BITS;
+
+__global__ __launch_bounds__(256) void gb_hll(const void* vals, int type, const uint8_t* valid, int64_t voff,
+                                              uint32_t n, uint32_t* regs) {
+  __shared__ uint32_t r[HLL_M];
+  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) r[j] = 0u;
+  __syncthreads();
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gsize) {
+    if (valid && !bit_get(valid, voff + int64_t(i))) continue;
+    const unsigned long long h = mix64(group_key(vals, int64_t(i), type) ^ 0x2545F4914F6CDD1Dull);
+    const uint32_t idx = uint32_t(h >> (64 - HLL_BITS));
+    const unsigned long long w = h << HLL_BITS;
+    const uint32_t rank = w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
+    if (rank > r[idx]) atomicMax(&r[idx], rank);
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x)
+    if (r[j]) atomicMax(&regs[j], r[j]);
+}
+
+// One block: the HyperLogLog estimate (with the linear-counting correction for small counts).
+__global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, unsigned long long* est) {
+  __shared__ float s_sum[256];
+  __shared__ uint32_t s_zero[256];
+  float sum = 0.f;
+  uint32_t zeros = 0;
+  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) {
+    sum += __builtin_ldexpf(1.0f, -int(regs[j]));
+    zeros += regs[j] == 0u ? 1u : 0u;
+  }
+  s_sum[threadIdx.x] = sum;
+  s_zero[threadIdx.x] = zeros;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
+      s_zero[threadIdx.x] += s_zero[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = double(HLL_M);
+    double e = (0.7213 / (1.0 + 1.079 / m)) * m * m / double(s_sum[0]);
+    if (e <= 2.5 * m && s_zero[0] > 0) e = m * log(m / double(s_zero[0]));
+    *est = (unsigned long long)(e + 0.5);
   }
 }
 
@@ -149,7 +208,7 @@ constexpr int BUILD_ROWS = 8;
 
 template <bool TRACK_FIRST>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                                uint32_t n, Slot* table, uint32_t cap_mask, uint32_t budget,
+                                                uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st) {
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,7 +234,7 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
       s[r] = uint32_t(mix64(k[r])) & cap_mask;
     }
 #pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? table[s[r]].key : 0ull;
+    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? table.key[s[r]] : 0ull;
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
@@ -189,11 +248,11 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
         slot = special;
         if (!st->special_used) atomicExch(&st->special_used, 1u);
       } else {
-        slot = probe_insert(table, k[r], s[r], cur[r], cap_mask, budget);
+        slot = probe_insert(table.key, k[r], s[r], cur[r], cap_mask, budget);
         if (slot == NULL_SLOT) overflow = true;
       }
-      slot_of[i] = slot;
-      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&table[slot].first, i);
+      __builtin_nontemporal_store(slot, slot_of + i);
+      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&table.first[slot], i);
     }
   }
   if (overflow) atomicOr(&st->overflow, 1u);
@@ -204,7 +263,7 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
 // Occupied slots -> (order key, slot), in slot order. One block per contiguous range of slots: a
 // counting sweep, one atomic for the block's base, then the writing sweep. The special slot C is
 // occupied when the EMPTY key was seen.
-__global__ __launch_bounds__(256) void gb_compact(const Slot* table, uint32_t cap, int type, BuildStatus* st,
+__global__ __launch_bounds__(256) void gb_compact(const unsigned long long* tkey, uint32_t cap, int type, BuildStatus* st,
                                                   unsigned long long* okeys, uint32_t* slots, uint32_t per_block) {
   __shared__ uint32_t wave_cnt[4];
   __shared__ uint32_t s_base;
@@ -213,7 +272,7 @@ __global__ __launch_bounds__(256) void gb_compact(const Slot* table, uint32_t ca
   const uint32_t end = min(cap + 1u, begin + per_block);
   auto occupied = [&](uint32_t x, unsigned long long& key) -> bool {
     if (x < cap) {
-      key = table[x].key;
+      key = tkey[x];
       return key != EMPTY;
     }
     key = EMPTY;
@@ -256,13 +315,13 @@ __global__ __launch_bounds__(256) void gb_compact(const Slot* table, uint32_t ca
   }
 }
 
-__global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t D, const Slot* table,
+__global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t D, const uint32_t* tfirst,
                            uint32_t* gid_of_slot, uint32_t* rep, const BuildStatus* st, int has_null, int track) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < D) {
     uint32_t s = slots[perm[r]];
     gid_of_slot[s] = r;
-    if (track) rep[r] = table[s].first;
+    if (track) rep[r] = tfirst[s];
   } else if (r == D && has_null && track) {
     rep[D] = st->null_first;
   }
@@ -277,12 +336,12 @@ __global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, con
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint64_t i = base + uint64_t(r) * gsize + tid;
-      sl[r] = i < n ? slot_of[i] : NULL_SLOT;
+      sl[r] = i < n ? __builtin_nontemporal_load(slot_of + i) : NULL_SLOT;
     }
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint64_t i = base + uint64_t(r) * gsize + tid;
-      if (i < n) slot_of[i] = sl[r] == NULL_SLOT ? null_gid : gid_of_slot[sl[r]];
+      if (i < n) __builtin_nontemporal_store(sl[r] == NULL_SLOT ? null_gid : gid_of_slot[sl[r]], slot_of + i);
     }
   }
 }
@@ -334,9 +393,9 @@ struct IdSource {
 
 __device__ __forceinline__ long long load_gid(const IdSource& s, int64_t i, long long K) {
   long long g;
-  if (s.type == T_I32) g = static_cast<const int32_t*>(s.ids)[i];
-  else if (s.type == T_U32) g = static_cast<const uint32_t*>(s.ids)[i];
-  else g = static_cast<const long long*>(s.ids)[i];
+  if (s.type == T_I32) g = ld_stream<int32_t>(s.ids, i);
+  else if (s.type == T_U32) g = ld_stream<uint32_t>(s.ids, i);
+  else g = ld_stream<long long>(s.ids, i);
   if (s.valid && !bit_get(s.valid, s.voff + i)) return -1;
   return (g < 0 || g >= K) ? -1 : g;
 }
@@ -350,13 +409,13 @@ struct ValSource {
 
 __device__ __forceinline__ unsigned long long raw64(const ValSource& v, int64_t i) {
   switch (v.type) {
-    case T_I64: case T_U64: case T_F64: return static_cast<const unsigned long long*>(v.vals)[i];
-    case T_I32: return (unsigned long long)(long long)static_cast<const int32_t*>(v.vals)[i];
-    case T_I16: return (unsigned long long)(long long)static_cast<const int16_t*>(v.vals)[i];
-    case T_I8: return (unsigned long long)(long long)static_cast<const int8_t*>(v.vals)[i];
-    case T_U32: case T_F32: return static_cast<const uint32_t*>(v.vals)[i];
-    case T_U16: return static_cast<const uint16_t*>(v.vals)[i];
-    default: return static_cast<const uint8_t*>(v.vals)[i];
+    case T_I64: case T_U64: case T_F64: return ld_stream<unsigned long long>(v.vals, i);
+    case T_I32: return (unsigned long long)(long long)ld_stream<int32_t>(v.vals, i);
+    case T_I16: return (unsigned long long)(long long)ld_stream<int16_t>(v.vals, i);
+    case T_I8: return (unsigned long long)(long long)ld_stream<int8_t>(v.vals, i);
+    case T_U32: case T_F32: return ld_stream<uint32_t>(v.vals, i);
+    case T_U16: return ld_stream<uint16_t>(v.vals, i);
+    default: return ld_stream<uint8_t>(v.vals, i);
   }
 }
 
@@ -506,13 +565,6 @@ struct F64BP {
 
 __device__ __forceinline__ unsigned long long identity(int op) { return op == W_MIN ? ~0ull : 0ull; }
 
-// The XCD (accelerator complex die) this wave runs on, from the XCC_ID hardware register. Each XCD
-// has its own L2: a table private to one XCD can be updated with workgroup-scope atomics, which the
-// XCD's L2 executes, instead of device-scope ones, which must go to memory.
-constexpr int NXCD = 8;
-__device__ __forceinline__ uint32_t xcd_id() {
-  return uint32_t(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & (NXCD - 1);
-}
 
 // One word update in LDS or in this XCD's private global table; workgroup-scope atomics in both
 // (ds_* in LDS, L2-resident in global memory). MIN / MAX / OR read first and skip the atomic when it
@@ -744,13 +796,17 @@ __global__ void pack_validity(const uint8_t* vb, long long K, unsigned long long
 
 // kind: 0 sum (int64 / uint64 bits), 1 integer mean (signed), 2 integer mean (unsigned),
 // 3 count, 4 min, 5 max (elements written at `width` bytes, from the unkeyed order key).
+// Finalize kinds: 0 integer sum, 1 signed mean, 2 unsigned mean, 3 count (word 0), 4 min, 5 max, and
+// the count of non-null values kept in word 1 of a sum table.
+constexpr int KIND_SUM_COUNT = 6;
+
 __global__ void gb_finalize_simple(const unsigned long long* gtab, int W, long long K, int kind, int vtype,
                                    void* out, uint8_t* vb) {
   long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   const unsigned long long* t = gtab + k * W;
-  if (kind == 3) {
-    static_cast<unsigned long long*>(out)[k] = t[0];
+  if (kind == 3 || kind == KIND_SUM_COUNT) {   // a count: word 0 of a count table, word 1 of a sum table
+    static_cast<unsigned long long*>(out)[k] = kind == 3 ? t[0] : t[1];
     vb[k] = 1;
     return;
   }
@@ -995,7 +1051,7 @@ struct GroupBy {
   uint64_t fx_col = 0;
   DevBuf fx_sum, fx_mean, fx_valid;
   uint64_t cnt_col = 0;
-  DevBuf cnt_tab;          // K x 1 count of non-null values of column cnt_col
+  DevBuf cnt_tab;          // K x 2 integer sum and count of non-null values of column cnt_col
   DevBuf count_all_tab;    // K x 1 rows per group
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
@@ -1021,17 +1077,44 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
   DevBuf stb = device_alloc(sizeof(BuildStatus));
   auto* st = static_cast<BuildStatus*>(stb->ptr);
-  // Capacity: two slots per row up to 2^21 slots; beyond that, grow on overflow.
-  uint64_t want = std::max<uint64_t>(1024, 2 * uint64_t(n));
+  // Capacity: about 2.5 slots per distinct key, from a HyperLogLog estimate on inputs large enough
+  // for the table size to matter; otherwise two slots per row. Never more than two slots per row, and
+  // grown (eightfold) when a build runs out of probes.
+  const uint64_t want = std::max<uint64_t>(1024, 2 * uint64_t(n));
+  uint64_t full = 1024;
+  while (full < want) full <<= 1;
   uint64_t cap = 1024;
-  while (cap < want && cap < (1ull << 21)) cap <<= 1;
+  if (n >= (int64_t(1) << 16)) {
+    DevBuf regs = device_alloc(HLL_M * 4);
+    DevBuf est = device_alloc(8);
+    AM_HIP(hipMemsetAsync(regs->ptr, 0, HLL_M * 4, s));
+    {
+      ProfScope ps("gb_hll");
+      hipLaunchKernelGGL(gb_hll, dim3(grid_for(n, 4)), dim3(256), 0, s, vals, type, valid, voff, un,
+                         static_cast<uint32_t*>(regs->ptr));
+      hipLaunchKernelGGL(gb_hll_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(regs->ptr),
+                         static_cast<unsigned long long*>(est->ptr));
+      AM_HIP(hipGetLastError());
+    }
+    unsigned long long d = 0;
+    AM_HIP(hipMemcpyAsync(&d, est->ptr, 8, hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    const uint64_t target = std::max<uint64_t>(1024, d * 2 + d / 2);
+    while (cap < target) cap <<= 1;
+  } else {
+    while (cap < want && cap < (1ull << 21)) cap <<= 1;
+  }
+  cap = std::min(cap, full);
   BuildStatus host{};
-  DevBuf table;
+  DevBuf tkeys, tfirst;
+  Table T{};
   const int build_grid = grid_for((n + BUILD_ROWS - 1) / BUILD_ROWS, 8);
   for (int attempt = 0;; attempt++) {
-    bool last_try = cap >= want;
-    table = device_alloc(size_t(cap + 1) * sizeof(Slot));
-    auto* T = static_cast<Slot*>(table->ptr);
+    bool last_try = cap >= full;
+    tkeys = device_alloc(size_t(cap + 1) * 8);
+    if (track) tfirst = device_alloc(size_t(cap + 1) * 4);
+    T.key = static_cast<unsigned long long*>(tkeys->ptr);
+    T.first = track ? static_cast<uint32_t*>(tfirst->ptr) : nullptr;
     hipLaunchKernelGGL(gb_table_init, dim3(grid_for(int64_t(cap + 1), 4)), dim3(256), 0, s, T, size_t(cap + 1), st);
     AM_HIP(hipGetLastError());
     {
@@ -1049,9 +1132,8 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
     AM_HIP(hipStreamSynchronize(s));
     if (!host.overflow) break;
     if (attempt > 8) fail("group-by: the key table could not be sized");
-    cap = std::min<uint64_t>(cap * 8, [&] { uint64_t c = 1024; while (c < want) c <<= 1; return c; }());
+    cap = std::min<uint64_t>(cap * 8, full);
   }
-  auto* T = static_cast<Slot*>(table->ptr);
   // Compact the occupied slots and sort their keys.
   out.okeys = device_alloc(size_t(cap + 1) * 8);
   DevBuf slots = device_alloc(size_t(cap + 1) * 4);
@@ -1059,7 +1141,7 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
     ProfScope ps("gb_compact");
     uint32_t blocks = uint32_t(std::min<uint64_t>((cap + 1 + 4095) / 4096, uint64_t(device().cu_count) * 4));
     uint32_t per_block = uint32_t((cap + 1 + blocks - 1) / blocks);
-    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, T, uint32_t(cap), type, st,
+    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, T.key, uint32_t(cap), type, st,
                        static_cast<unsigned long long*>(out.okeys->ptr), static_cast<uint32_t*>(slots->ptr), per_block);
     AM_HIP(hipGetLastError());
   }
@@ -1076,7 +1158,7 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   {
     ProfScope ps("gb_relabel");
     hipLaunchKernelGGL(gb_relabel, dim3(uint32_t((out.D + 1 + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const uint32_t*>(out.perm->ptr), static_cast<const uint32_t*>(slots->ptr), out.D, T,
+                       static_cast<const uint32_t*>(out.perm->ptr), static_cast<const uint32_t*>(slots->ptr), out.D, T.first,
                        static_cast<uint32_t*>(gid_of_slot->ptr), static_cast<uint32_t*>(out.rep->ptr), st,
                        out.has_null ? 1 : 0, track ? 1 : 0);
     AM_HIP(hipGetLastError());
@@ -1258,7 +1340,7 @@ Array* finalize_simple(const DevBuf& gtab, int W, long long K, int kind, int vty
                          static_cast<uint8_t*>(vb->ptr));
       AM_HIP(hipGetLastError());
     }
-    if (kind != 3 && K > 0) {
+    if (kind != 3 && kind != KIND_SUM_COUNT && K > 0) {
       out->validity = device_alloc(size_t((K + 63) / 64) * 8);
       DevBuf nz = device_alloc(8);
       hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, static_cast<unsigned long long*>(nz->ptr), 1);
@@ -1375,14 +1457,22 @@ Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::
 Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, int op, GroupBy* cache) {
   hipStream_t s = stream();
   if (op == 1) {
-    AggArgs a = make_args(ids, nullptr, n, K);
-    DevBuf t = run_agg<CountP<false>>(a, s, "gb_count");
+    DevBuf t;
+    if (cache && cache->count_all_tab) {
+      t = cache->count_all_tab;
+    } else {
+      AggArgs a = make_args(ids, nullptr, n, K);
+      t = run_agg<CountP<false>>(a, s, "gb_count");
+      if (cache) cache->count_all_tab = t;
+    }
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
   if (!v) fail("am_group_agg_ex op " + std::to_string(op) + " needs a value column");
   if (v->length != n) fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(v->length));
   // hash_count needs only the validity, so it takes a column of any type.
   if (op == 2) {
+    // An integer sum of the same column already counted its non-null values (word 1).
+    if (cache && cache->cnt_col == v->uid && cache->cnt_tab) return finalize_simple(cache->cnt_tab, 2, K, KIND_SUM_COUNT, T_I64, "l", T_I64);
     AggArgs a = make_args(ids, v, n, K);
     DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
@@ -1402,7 +1492,13 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
       return bits_array(op == 0 ? sum : mean, vb, K, "g", T_F64);
     }
     if (v->type == T_F32) fail("float32 grouped sum / mean is not supported by this build");
-    DevBuf t = run_agg<SumCountP>(a, s, "gb_sum");
+    DevBuf t;
+    if (cache && cache->cnt_col == v->uid && cache->cnt_tab) {
+      t = cache->cnt_tab;
+    } else {
+      t = run_agg<SumCountP>(a, s, "gb_sum");
+      if (cache) { cache->cnt_col = v->uid; cache->cnt_tab = t; }
+    }
     if (op == 0) {
       bool u64 = v->type == T_U64;
       return finalize_simple(t, 2, K, 0, v->type, u64 ? "L" : "l", u64 ? T_U64 : T_I64);

@@ -37,6 +37,11 @@ struct SortCtl {
   uint32_t digit_base[8][RADIX];    // exclusive digit prefix inside the value block, per pass
   uint32_t active_mask, first_active, last_active, pad1;
   uint32_t value_base, nan_base, null_base, pad2;
+  // Per-XCD partials of the upsweep (updated by workgroup-scope atomics in each XCD's own L2), folded
+  // into the fields above by sort_plan.
+  uint32_t xhist[NXCD][8][RADIX];
+  uint32_t xm[NXCD], xq[NXCD], xz[NXCD], pad3[NXCD];
+  unsigned long long xor_[NXCD], xand[NXCD];
 };
 
 // ---------------------------------------------------------------- kernels
@@ -47,6 +52,7 @@ __global__ void sort_init(SortCtl* ctl) {
   for (int i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0u;
   __syncthreads();
   if (threadIdx.x == 0) ctl->key_and = ~0ull;
+  if (threadIdx.x < NXCD) ctl->xand[threadIdx.x] = ~0ull;
 }
 
 // Copies the plan (active mask, first and last active pass, value rows) to pinned host memory.
@@ -122,24 +128,50 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
     atomicAnd(&s_and, a64);
   }
   __syncthreads();
+  const uint32_t x = xcd_id();
+  uint32_t* xh = &ctl->xhist[x][0][0];
   for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) {
     uint32_t v = (&h[0][0])[i];
-    if (v) atomicAdd(&(&ctl->hist[0][0])[i], v);
+    if (v) __hip_atomic_fetch_add(xh + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   if (threadIdx.x == 0) {
     if (cnt[0]) {
-      atomicAdd(&ctl->m, cnt[0]);
-      atomicOr(&ctl->key_or, s_or);
-      atomicAnd(&ctl->key_and, s_and);
+      __hip_atomic_fetch_add(&ctl->xm[x], cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_or(&ctl->xor_[x], s_or, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_and(&ctl->xand[x], s_and, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    if (cnt[1]) atomicAdd(&ctl->q, cnt[1]);
-    if (cnt[2]) atomicAdd(&ctl->z, cnt[2]);
+    if (cnt[1]) __hip_atomic_fetch_add(&ctl->xq[x], cnt[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cnt[2]) __hip_atomic_fetch_add(&ctl->xz[x], cnt[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
 template <int P>
 __global__ __launch_bounds__(BLOCK) void sort_plan(SortCtl* ctl, int nulls_first) {
   __shared__ uint32_t tmp[WAVES];
+  // Fold the per-XCD partials of the upsweep.
+  for (int p = 0; p < P; p++) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int x = 0; x < NXCD; x++) v += ctl->xhist[x][p][threadIdx.x];
+    ctl->hist[p][threadIdx.x] = v;
+  }
+  if (threadIdx.x == 0) {
+    uint32_t m = 0, q = 0, z = 0;
+    unsigned long long o = 0ull, a = ~0ull;
+    for (int x = 0; x < NXCD; x++) {
+      m += ctl->xm[x];
+      q += ctl->xq[x];
+      z += ctl->xz[x];
+      o |= ctl->xor_[x];
+      a &= ctl->xand[x];
+    }
+    ctl->m = m;
+    ctl->q = q;
+    ctl->z = z;
+    ctl->key_or = o;
+    ctl->key_and = a;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long diff = ctl->key_or ^ ctl->key_and;
     uint32_t mask = 0;
@@ -170,7 +202,7 @@ __global__ __launch_bounds__(BLOCK) void sort_plan(SortCtl* ctl, int nulls_first
 }
 
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
+__global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
                                                     uint32_t n, int type, int forder, int desc,
                                                     SortCtl* ctl, unsigned long long* status, uint32_t epoch,
                                                     K* out_keys, uint32_t* out_rows, uint32_t* out_idx) {
@@ -194,12 +226,12 @@ __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint
   const int shift = int(8 * first);
 
   K keys[ITEMS];
-  uint32_t rows[ITEMS], bins[ITEMS], lrank[ITEMS];
+  uint32_t bins[ITEMS];
+  const uint32_t rows[1] = {0};   // rows are base_e + 64 * j (ROWS_IOTA)
   const uint32_t base_e = tile * uint32_t(TILE) + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
     uint32_t e = base_e + uint32_t(j) * 64u;
-    rows[j] = e;
     keys[j] = 0;
     uint32_t b = BIN_NONE;
     if (e < n) {
@@ -214,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint
     }
     bins[j] = b;
   }
-  rank_tile<ITEMS, 9>(bins, lrank, wc[w], lane);
+  rank_tile<ITEMS, 9>(bins, wc[w], lane);
   __syncthreads();
   uint32_t vcount = tile_bin_layout<NBINS_FIRST>(wc, tile_total, bin_start, tmp);
   look_back<NBINS_FIRST>(status, tile, epoch, tile_total, bin_base);
@@ -230,15 +262,15 @@ __global__ __launch_bounds__(BLOCK) void sort_first(const void* vals, const uint
   // NaN and null rows go straight to their final slots, in input order.
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
-    uint32_t b = bins[j];
-    if (b == BIN_NAN || b == BIN_NULL) out_idx[bin_base[b] + wc[w][b] + lrank[j]] = rows[j];
+    uint32_t b = bin_of(bins[j]);
+    if (b == BIN_NAN || b == BIN_NULL) out_idx[bin_base[b] + wc[w][b] + rank_of(bins[j])] = base_e + uint32_t(j) * 64u;
   }
   if (last) {
-    scatter_values<K, ITEMS, false>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                    vcount, nullptr, out_idx);
+    scatter_values<K, ITEMS, false, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                          nullptr, out_idx, base_e);
   } else {
-    scatter_values<K, ITEMS, true>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                   vcount, out_keys, out_rows);
+    scatter_values<K, ITEMS, true, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                         out_keys, out_rows, base_e);
   }
 }
 
@@ -280,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
   const int shift = 8 * pass;
 
   K keys[ITEMS];
-  uint32_t rows[ITEMS], bins[ITEMS], lrank[ITEMS];
+  uint32_t rows[ITEMS], bins[ITEMS];
   const uint32_t base_e = tile * uint32_t(TILE) + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
@@ -296,8 +328,8 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
     }
   }
   // A full tile has no lanes past the end, so its bins need 8 bits; the last tile needs the 9th.
-  if ((tile + 1u) * uint32_t(TILE) <= m) rank_tile<ITEMS, 8>(bins, lrank, wc[w], lane);
-  else rank_tile<ITEMS, 9>(bins, lrank, wc[w], lane);
+  if ((tile + 1u) * uint32_t(TILE) <= m) rank_tile<ITEMS, 8>(bins, wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
   __syncthreads();
   uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
   look_back<RADIX>(status, tile, epoch, tile_total, bin_base);
@@ -306,11 +338,11 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
   bin_base[threadIdx.x] += ctl->digit_base[pass][threadIdx.x] + vb;
   __syncthreads();
   if (last) {
-    scatter_values<K, ITEMS, false>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                    vcount, nullptr, out_idx);
+    scatter_values<K, ITEMS, false>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                    nullptr, out_idx);
   } else {
-    scatter_values<K, ITEMS, true>(keys, rows, bins, lrank, wc[w], bin_start, bin_base, s_digit, s_stage,
-                                   vcount, out_keys, out_rows);
+    scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                   out_keys, out_rows);
   }
 }
 
@@ -321,10 +353,15 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
               int placement, int forder, uint32_t* out_idx, hipStream_t s) {
   constexpr int P = int(sizeof(K));
   constexpr int TILE = BLOCK * ITEMS;
+  // The first pass keeps at most 16 keys per thread: its raw-column loads and 258-bin classification
+  // need more registers, and 16 keeps it at three workgroups per CU.
+  constexpr int FITEMS = ITEMS > 16 ? 16 : ITEMS;
+  constexpr int FTILE = BLOCK * FITEMS;
   const uint32_t tiles = (n + TILE - 1) / TILE;
+  const uint32_t ftiles = (n + FTILE - 1) / FTILE;
   DevBuf ctl_buf = device_alloc(sizeof(SortCtl));
   SortCtl* ctl = static_cast<SortCtl*>(ctl_buf->ptr);
-  uint32_t epoch = reserve_status(size_t(tiles) * STATUS_STRIDE, s);
+  uint32_t epoch = reserve_status(size_t(std::max(tiles, ftiles)) * STATUS_STRIDE, s);
   unsigned long long* status = static_cast<unsigned long long*>(t_ws.status->ptr);
   retain(t_ws.status);
   DevBuf k0 = device_alloc(size_t(n) * sizeof(K)), k1 = device_alloc(size_t(n) * sizeof(K));
@@ -363,7 +400,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   }
   {
     ProfScope ps("sort_first");
-    hipLaunchKernelGGL((sort_first<K, ITEMS>), dim3(tiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+    hipLaunchKernelGGL((sort_first<K, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
                        dsc, ctl, status, epoch, static_cast<K*>(k0->ptr), static_cast<uint32_t*>(r0->ptr),
                        out_idx);
     AM_HIP(hipGetLastError());

@@ -5,6 +5,11 @@
 
 #include "am_internal.hpp"
 
+// Predecessor status words a look-back step reads at once (a build-time tuning constant).
+#ifndef AM_LB_WINDOW
+#define AM_LB_WINDOW 4
+#endif
+
 #include <algorithm>
 
 namespace am {
@@ -116,14 +121,19 @@ __device__ __forceinline__ void status_store(unsigned long long* p, unsigned lon
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A ranked element packs its bin (9 bits) and its rank inside its wave (the bits above).
+constexpr uint32_t BIN_BITS = 9;
+__device__ __forceinline__ uint32_t bin_of(uint32_t packed) { return packed & ((1u << BIN_BITS) - 1u); }
+__device__ __forceinline__ uint32_t rank_of(uint32_t packed) { return packed >> BIN_BITS; }
+
 // Ranks a tile held in registers. Element (w, j, lane) of the tile is tile_start + w*64*ITEMS + j*64
 // + lane; its stable rank among the tile's elements of the same bin is
 //   (count in earlier waves) + (count in this wave at earlier items) + (lanes below it in its row).
 // The last two come from here: a 9-bit match over the 64 lanes with ballots, and one LDS counter per
-// (wave, bin) that only its own wave touches. On return wc[w][bin] holds the wave's bin totals.
+// (wave, bin) that only its own wave touches. On return wc[w][bin] holds the wave's bin totals and
+// each bins[j] is packed with the element's rank inside its wave (bin_of / rank_of).
 template <int ITEMS, int NBITS>
-__device__ __forceinline__ void rank_tile(const uint32_t (&bins)[ITEMS], uint32_t (&lrank)[ITEMS],
-                                          uint32_t* wc_row, int lane) {
+__device__ __forceinline__ void rank_tile(uint32_t (&bins)[ITEMS], uint32_t* wc_row, int lane) {
   const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
@@ -144,7 +154,7 @@ __device__ __forceinline__ void rank_tile(const uint32_t (&bins)[ITEMS], uint32_
     __builtin_amdgcn_wave_barrier();
     if (below == 0) wc_row[b] = base + uint32_t(__popcll(peers));
     __builtin_amdgcn_wave_barrier();
-    lrank[j] = base + below;
+    bins[j] = b | ((base + below) << BIN_BITS);
   }
 }
 
@@ -152,17 +162,20 @@ __device__ __forceinline__ void rank_tile(const uint32_t (&bins)[ITEMS], uint32_
 // Stages the tile's value elements in LDS in bin order (keys first, then rows, through one buffer)
 // and writes them out as runs: element at staged position `pos` with digit d goes to
 // bin_base[d] + pos - bin_start[d].
-template <typename K, int ITEMS, bool WRITE_KEYS>
-__device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uint32_t (&rows)[ITEMS],
-                                               const uint32_t (&bins)[ITEMS], const uint32_t (&lrank)[ITEMS],
-                                               const uint32_t* wc_row, const uint32_t* bin_start,
-                                               uint32_t* bin_base, uint8_t* s_digit, K* s_stage,
-                                               uint32_t vcount, K* out_keys, uint32_t* out_rows) {
+// ROWS_IOTA: the row of element j is row_base + 64 * j (a first pass over the raw column), so no
+// row registers are kept; otherwise rows[j].
+template <typename K, int ITEMS, bool WRITE_KEYS, bool ROWS_IOTA = false>
+__device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uint32_t* rows,
+                                               const uint32_t (&packed)[ITEMS], const uint32_t* wc_row,
+                                               const uint32_t* bin_start, uint32_t* bin_base, uint8_t* s_digit,
+                                               K* s_stage, uint32_t vcount, K* out_keys, uint32_t* out_rows,
+                                               uint32_t row_base = 0) {
+  auto row = [&](int j) { return ROWS_IOTA ? row_base + uint32_t(j) * 64u : rows[j]; };
   uint32_t pos[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
-    uint32_t b = bins[j];
-    pos[j] = b < RADIX ? bin_start[b] + wc_row[b] + lrank[j] : 0xFFFFFFFFu;
+    uint32_t b = bin_of(packed[j]);
+    pos[j] = b < RADIX ? bin_start[b] + wc_row[b] + rank_of(packed[j]) : 0xFFFFFFFFu;
     if (b < RADIX) s_digit[pos[j]] = uint8_t(b);
   }
   // Each staged position's destination is delta[digit] + position; computed once, used by both
@@ -176,7 +189,7 @@ __device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uin
     uint32_t* s_rows = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int j = 0; j < ITEMS; j++)
-      if (pos[j] != 0xFFFFFFFFu) s_rows[pos[j]] = rows[j];
+      if (pos[j] != 0xFFFFFFFFu) s_rows[pos[j]] = row(j);
   }
   __syncthreads();
   uint32_t dest[ITEMS];
@@ -195,7 +208,7 @@ __device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uin
     uint32_t* s_rows = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int j = 0; j < ITEMS; j++)
-      if (pos[j] != 0xFFFFFFFFu) s_rows[pos[j]] = rows[j];
+      if (pos[j] != 0xFFFFFFFFu) s_rows[pos[j]] = row(j);
     __syncthreads();
   }
   const uint32_t* s_rows = reinterpret_cast<const uint32_t*>(s_stage);
@@ -247,31 +260,56 @@ __device__ __forceinline__ void look_back(unsigned long long* status, uint32_t t
   // The walk reads LB_WINDOW predecessors per step (independent loads, one memory latency), then
   // consumes them in order: aggregates are added, the first inclusive prefix ends the walk, and a
   // predecessor that has published nothing yet is polled again from where the walk stopped. The
-  // chain's head publishes its prefix at once, so no walk goes below it.
-  constexpr int LB_WINDOW = 4;
+  // chain's head publishes its prefix at once, so no walk goes below it. A thread that owns two bins
+  // (bins BLOCK.. of the first pass: NaN and null) walks both chains in the same steps.
+  constexpr int LB_WINDOW = AM_LB_WINDOW;
+  constexpr int PER = (NB + BLOCK - 1) / BLOCK;
+  static_assert(PER <= 2, "at most two bins per thread");
   const int64_t lo = int64_t(first_tile);
-  for (int b = threadIdx.x; b < NB; b += BLOCK) {
-    uint32_t sum = 0;
-    int64_t t = int64_t(tile) - 1;
-    bool done = false;
-    while (!done) {
-      unsigned long long s[LB_WINDOW];
+  int bin[PER];
+  uint32_t sum[PER];
+  int64_t t[PER];
+  bool done[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    bin[q] = int(threadIdx.x) + q * BLOCK;
+    sum[q] = 0;
+    t[q] = int64_t(tile) - 1;
+    done[q] = bin[q] >= NB;
+  }
+  while (true) {
+    bool all = true;
+#pragma unroll
+    for (int q = 0; q < PER; q++) all = all && done[q];
+    if (all) break;
+    unsigned long long sv[PER][LB_WINDOW];
+#pragma unroll
+    for (int q = 0; q < PER; q++)
 #pragma unroll
       for (int k = 0; k < LB_WINDOW; k++)
-        s[k] = (t - k >= lo) ? status_load(status + size_t(t - k) * STATUS_STRIDE + b) : 0ull;
+        sv[q][k] = (!done[q] && t[q] - k >= lo) ? status_load(status + size_t(t[q] - k) * STATUS_STRIDE + bin[q]) : 0ull;
+    bool progressed = false;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      if (done[q]) continue;
       int used = 0;
 #pragma unroll
       for (int k = 0; k < LB_WINDOW; k++) {
-        if (done || used != k || t - k < lo || uint32_t(s[k] >> 34) != epoch) continue;
-        sum += uint32_t(s[k]);
+        if (done[q] || used != k || t[q] - k < lo || uint32_t(sv[q][k] >> 34) != epoch) continue;
+        sum[q] += uint32_t(sv[q][k]);
         used = k + 1;
-        if (((s[k] >> 32) & 3u) == FLAG_PREFIX) done = true;
+        if (((sv[q][k] >> 32) & 3u) == FLAG_PREFIX) done[q] = true;
       }
-      t -= used;
-      if (!done && used == 0) __builtin_amdgcn_s_sleep(1);
+      t[q] -= used;
+      progressed = progressed || used > 0;
     }
-    excl[b] = sum;
-    status_store(mine + b, ep | ((unsigned long long)FLAG_PREFIX << 32) | (sum + tile_total[b]));
+    if (!progressed) __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    if (bin[q] >= NB) continue;
+    excl[bin[q]] = sum[q];
+    status_store(mine + bin[q], ep | ((unsigned long long)FLAG_PREFIX << 32) | (sum[q] + tile_total[bin[q]]));
   }
 }
 

@@ -5,7 +5,10 @@ last, then the float64 gather), two group-by steps over 50M Zipf(1.1) keys from 
 (keys, int64 sum, count_all, count), and three device-to-device copies of 1 GiB, which calibrate the
 FETCH_SIZE / WRITE_SIZE counters against a kernel of known traffic (read 1 GiB, write 1 GiB).
 
-Usage: python3 -m arrowmetal_amd.profile_driver [--rows N]
+With --workload batched it instead runs the batched plan (2,048 lognormal batches, ~191M rows: tune,
+then three graph replays of the per-batch argsort + group-by sum / count_all).
+
+Usage: python3 arrowmetal_amd/profile_driver.py [--rows N] [--workload orderby_groupby|batched]
 """
 from __future__ import annotations
 
@@ -39,7 +42,17 @@ def device_copy(nbytes: int, times: int) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", type=int, default=50_000_000)
+    ap.add_argument("--workload", default="orderby_groupby", choices=["orderby_groupby", "batched"])
     args = ap.parse_args()
+    if args.workload == "batched":
+        offs, sk, gk, gv = bench.batched_inputs(2048)
+        msk, mgk, mgv = (am.MetalArray.from_numpy(x) for x in (sk, gk, gv))
+        plan = am.Batched(offs, hash_capacity=2048)
+        print("tuned", plan.tune(msk, mgk, mgv), flush=True)
+        for i in range(3):
+            plan.run(msk, mgk, mgv, use_graph=True)
+            print(f"batched step {i} done", flush=True)
+        return
     n = args.rows
     keys, valid, payload = bench.order_by_inputs(n)
     k = am.MetalArray.from_numpy(keys, valid)

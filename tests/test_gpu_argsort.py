@@ -203,3 +203,21 @@ def test_order_by_gather_config_shape():
     want_idx = O.argsort(keys, valid, True, 0)
     np.testing.assert_array_equal(idx.to_numpy()[0], want_idx)
     np.testing.assert_array_equal(got, payload[want_idx])
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_large_tiles_first_pass_and_nulls(order):
+    """Above 8.4M rows the scatter passes take 20 keys per thread and the first pass 16: both tile
+    shapes (and their look-back chains, NaN and null bins included) against the oracle at 9M rows."""
+    n = 9_000_001
+    print(f"large sort n={n} order={order}", flush=True)
+    bits = O.splitmix64(0x1A2B + order, n)
+    valid = (O.splitmix64(0x3C4D, n) >> np.uint64(11)).astype(np.float64) / 2.0**53 < 0.97
+    if order == 0:
+        vals = (bits >> np.uint64(1)).view(np.int64) - (1 << 62)
+    else:
+        vals = ((bits >> np.uint64(11)).astype(np.float64) / 2.0**53 - 0.5) * 1e6
+        vals[(bits & np.uint64(1023)) == 7] = np.nan
+        vals[(bits & np.uint64(1023)) == 9] = -0.0
+    for desc, place in ((True, 0), (False, 1)):
+        check(vals, valid, desc, place, order, label="large")

@@ -72,7 +72,11 @@ def check_grouping(key_cols, key_valids, values=None, vvalid=None, label=""):
                                       err_msg=f"{label}: keys")
     if values is not None:
         mv = am.MetalArray.from_numpy(values, vvalid)
-        ops = [1, 2, 4, 5] + ([0, 3] if values.dtype != np.float32 else [])
+        # Integer columns: the sum runs first, so the count after it is the one taken from the sum's
+        # own count of non-null values; float columns count on their own.
+        ops = [1, 4, 5, 2] if values.dtype.kind == "f" else [1, 0, 2, 3, 4, 5]
+        if values.dtype == np.float64:
+            ops += [0, 3]
         for op in ops:
             name = {0: "sum", 1: "count_all", 2: "count", 3: "mean", 4: "min", 5: "max"}[op]
             got = g.count_all() if op == 1 else getattr(g, name)(mv)
@@ -332,3 +336,15 @@ def test_unique_ex_float_and_narrow():
                 gvalid = np.ones(len(gv), dtype=bool)
             np.testing.assert_array_equal(gvalid, valid[rows])
             np.testing.assert_array_equal(gv[gvalid].view(np.uint8), vals[rows][gvalid].view(np.uint8))
+
+
+@pytest.mark.parametrize("n,span", [(3_000_001, 4_000_000), (1_000_000, 3), (700_000, 65_536)])
+def test_table_sizing_from_distinct_estimate(n, span):
+    """Inputs of 2^16 rows and more size the hash table from a HyperLogLog estimate of the distinct
+    keys: nearly-all-distinct, tiny and mid-size key sets must all group exactly."""
+    print(f"sizing n={n} span={span}", flush=True)
+    rng = np.random.default_rng(n ^ span)
+    keys = (rng.integers(0, span, n).astype(np.int64) * 7_777_777_777) ^ 0x5555
+    kv = rng.random(n) >= 0.01
+    vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    check_grouping([keys], [kv], vals, None, f"n={n} span={span}")
