@@ -43,9 +43,9 @@ struct SegChunk {
 // A value key of a batched sort: NaN (ieee order) becomes the largest key after the descending
 // flip, which keeps every NaN after the values, in input order, in both directions.
 template <typename K>
-__device__ __forceinline__ K seg_key(const void* vals, uint32_t i, int type, int forder, int desc) {
+__device__ __forceinline__ K seg_key(uint64_t raw, int type, int forder, int desc) {
   K k;
-  if (classify<K>(vals, nullptr, 0, i, type, forder, desc, k) != 0) k = K(~K(0));
+  if (classify_raw<K>(raw, true, type, forder, desc, k) != 0) k = K(~K(0));
   return k;
 }
 
@@ -65,7 +65,7 @@ __global__ void seg_init(SegCtl* ctl, uint32_t* hist, size_t hist_words, uint32_
   }
 }
 
-template <typename K>
+template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void seg_upsweep(const void* vals, int type, int forder, int desc,
                                                      const SegChunk* chunks, SegCtl* ctl, uint32_t* hist) {
   constexpr int P = int(sizeof(K));
@@ -82,10 +82,14 @@ __global__ __launch_bounds__(BLOCK) void seg_upsweep(const void* vals, int type,
   constexpr int UP_ITEMS = 8;
   for (uint32_t base = c.start; base < c.end; base += BLOCK * UP_ITEMS) {
     K kk[UP_ITEMS];
+    uint64_t raw[UP_ITEMS];
+#pragma unroll
+    for (int u = 0; u < UP_ITEMS; u++)   // every load first (rows past the chunk read its last row)
+      raw[u] = load_raw<W>(vals, min(base + uint32_t(u) * BLOCK + threadIdx.x, c.end - 1u));
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
       uint32_t i = base + uint32_t(u) * BLOCK + threadIdx.x;
-      kk[u] = i < c.end ? seg_key<K>(vals, i, type, forder, desc) : K(0);
+      kk[u] = i < c.end ? seg_key<K>(raw[u], type, forder, desc) : K(0);
     }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(BLOCK) void seg_plan(const uint32_t* hist, const ui
 // One segmented scatter pass. FIRST: the first active pass, reading the raw column (row numbers start
 // local to their batch). Otherwise pass `pass` over the ping-pong buffers; every digit is launched
 // and an inactive one exits at once, so the sequence needs no host decision.
-template <typename K, int ITEMS, bool FIRST>
+template <typename K, int ITEMS, bool FIRST, int W>
 __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, int type, int forder, int desc,
                                                   const SegTile* tiles, const uint32_t* seg_off,
                                                   const uint32_t* base, SegCtl* ctl, unsigned long long* status,
@@ -191,21 +195,35 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
   K keys[ITEMS];
   uint32_t rows[ITEMS], bins[ITEMS];
   const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+  if constexpr (FIRST) {
+    uint64_t raw[ITEMS];
 #pragma unroll
-  for (int j = 0; j < ITEMS; j++) {
-    uint32_t e = base_e + uint32_t(j) * 64u;
-    keys[j] = 0;
-    rows[j] = 0;
-    bins[j] = BIN_NONE;
-    if (e < t.end) {
-      if (FIRST) {
-        keys[j] = seg_key<K>(vals, e, type, forder, desc);
+    for (int j = 0; j < ITEMS; j++)   // every load first (lanes past the tile read its last row)
+      raw[j] = t.end > t.start ? load_raw<W>(vals, min(base_e + uint32_t(j) * 64u, t.end - 1u)) : 0ull;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      uint32_t e = base_e + uint32_t(j) * 64u;
+      keys[j] = 0;
+      rows[j] = 0;
+      bins[j] = BIN_NONE;
+      if (e < t.end) {
+        keys[j] = seg_key<K>(raw[j], type, forder, desc);
         rows[j] = e - seg_start;
-      } else {
+        bins[j] = uint32_t(keys[j] >> shift) & 0xFFu;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      uint32_t e = base_e + uint32_t(j) * 64u;
+      keys[j] = 0;
+      rows[j] = 0;
+      bins[j] = BIN_NONE;
+      if (e < t.end) {
         keys[j] = in_keys[e];
         rows[j] = in_rows[e];
+        bins[j] = uint32_t(keys[j] >> shift) & 0xFFu;
       }
-      bins[j] = uint32_t(keys[j] >> shift) & 0xFFu;
     }
   }
   if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
@@ -554,7 +572,7 @@ struct Batched {
 
 namespace {
 
-template <typename K, int ITEMS>
+template <typename K, int W, int ITEMS>
 void seg_sort_launch(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
   constexpr int P = int(sizeof(K));
   auto* ctl = static_cast<SegCtl*>(b.ctl->ptr);
@@ -563,7 +581,7 @@ void seg_sort_launch(Batched& b, const void* vals, int type, int desc, int forde
                      dim3(256), 0, s, ctl, static_cast<uint32_t*>(b.hist->ptr), hist_words,
                      static_cast<uint32_t*>(b.epoch->ptr));
   if (b.nchunks > 0)
-    hipLaunchKernelGGL(seg_upsweep<K>, dim3(b.nchunks), dim3(BLOCK), 0, s, vals, type, forder, desc,
+    hipLaunchKernelGGL((seg_upsweep<K, W>), dim3(b.nchunks), dim3(BLOCK), 0, s, vals, type, forder, desc,
                        static_cast<const SegChunk*>(b.chunks->ptr), ctl, static_cast<uint32_t*>(b.hist->ptr));
   hipLaunchKernelGGL(seg_plan<P>, dim3(b.nseg), dim3(BLOCK), 0, s, static_cast<const uint32_t*>(b.hist->ptr),
                      static_cast<const uint32_t*>(b.seg_off->ptr), static_cast<uint32_t*>(b.base->ptr), ctl);
@@ -578,26 +596,30 @@ void seg_sort_launch(Batched& b, const void* vals, int type, int desc, int forde
   const uint32_t* base = static_cast<const uint32_t*>(b.base->ptr);
   {
     ProfScope ps("seg_first");
-    hipLaunchKernelGGL((seg_pass<K, ITEMS, true>), dim3(b.ntiles), dim3(BLOCK), 0, s, 0, vals, type, forder, desc,
+    hipLaunchKernelGGL((seg_pass<K, ITEMS, true, W>), dim3(b.ntiles), dim3(BLOCK), 0, s, 0, vals, type, forder, desc,
                        tiles, so, base, ctl, st, kp0, rp0, kp1, rp1, out);
   }
   for (int p = 0; p < P; p++) {
     ProfScope ps("seg_pass");
-    hipLaunchKernelGGL((seg_pass<K, ITEMS, false>), dim3(b.ntiles), dim3(BLOCK), 0, s, p, vals, type, forder, desc,
+    hipLaunchKernelGGL((seg_pass<K, ITEMS, false, 0>), dim3(b.ntiles), dim3(BLOCK), 0, s, p, vals, type, forder, desc,
                        tiles, so, base, ctl, st, kp0, rp0, kp1, rp1, out);
   }
   AM_HIP(hipGetLastError());
 }
 
+template <typename K, int W>
+void seg_sort_items(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
+  if (b.items == 8) seg_sort_launch<K, W, 8>(b, vals, type, desc, forder, s);
+  else if (b.items == 20) seg_sort_launch<K, W, 20>(b, vals, type, desc, forder, s);
+  else seg_sort_launch<K, W, 16>(b, vals, type, desc, forder, s);
+}
+
 void seg_sort(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
-  if (type_width(type) == 8) {
-    if (b.items == 8) seg_sort_launch<unsigned long long, 8>(b, vals, type, desc, forder, s);
-    else if (b.items == 20) seg_sort_launch<unsigned long long, 20>(b, vals, type, desc, forder, s);
-    else seg_sort_launch<unsigned long long, 16>(b, vals, type, desc, forder, s);
-  } else {
-    if (b.items == 8) seg_sort_launch<uint32_t, 8>(b, vals, type, desc, forder, s);
-    else if (b.items == 20) seg_sort_launch<uint32_t, 20>(b, vals, type, desc, forder, s);
-    else seg_sort_launch<uint32_t, 16>(b, vals, type, desc, forder, s);
+  switch (type_width(type)) {
+    case 8: seg_sort_items<unsigned long long, 8>(b, vals, type, desc, forder, s); break;
+    case 4: seg_sort_items<uint32_t, 4>(b, vals, type, desc, forder, s); break;
+    case 2: seg_sort_items<uint32_t, 2>(b, vals, type, desc, forder, s); break;
+    default: seg_sort_items<uint32_t, 1>(b, vals, type, desc, forder, s); break;
   }
 }
 

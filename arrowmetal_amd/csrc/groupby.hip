@@ -57,6 +57,47 @@ __device__ __forceinline__ T ld_stream(const void* p, int64_t i) {
   return __builtin_nontemporal_load(static_cast<const T*>(p) + i);
 }
 
+// The row passes issue all their loads before using any: raw bits of a W-byte element, loaded
+// unconditionally (a type switch around the loads would make the compiler wait on each at the branch
+// join), and a validity byte from a one-byte all-valid constant when a column has no bitmap.
+__device__ const uint8_t kAllValid = 0xFFu;
+
+template <int W>
+__device__ __forceinline__ unsigned long long ld_raw(const void* p, int64_t i) {
+  if constexpr (W == 8) return ld_stream<unsigned long long>(p, i);
+  else if constexpr (W == 4) return ld_stream<uint32_t>(p, i);
+  else if constexpr (W == 2) return ld_stream<uint16_t>(p, i);
+  else return ld_stream<uint8_t>(p, i);
+}
+
+// Whether row i is valid, from an unconditional load.
+__device__ __forceinline__ bool ld_valid(const uint8_t* valid, int64_t voff, int64_t i) {
+  const int64_t b = voff + i;
+  const uint8_t* p = valid ? valid + (b >> 3) : &kAllValid;
+  return (*p >> (valid ? (b & 7) : 0)) & 1;
+}
+
+// The group key of raw element bits: integers widened with their sign, float NaN made canonical and
+// -0.0 made +0.0 (registers only).
+__device__ __forceinline__ unsigned long long key_of_raw(unsigned long long r, int type) {
+  switch (type) {
+    case T_I32: return (unsigned long long)(long long)int32_t(uint32_t(r));
+    case T_I16: return (unsigned long long)(long long)int16_t(uint16_t(r));
+    case T_I8: return (unsigned long long)(long long)int8_t(uint8_t(r));
+    case T_F64:
+      if ((r & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
+      if ((r & 0x7FFFFFFFFFFFFFFFull) == 0ull) return 0ull;
+      return r;
+    case T_F32: {
+      uint32_t b = uint32_t(r);
+      if ((b & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FC00000u;
+      if ((b & 0x7FFFFFFFu) == 0u) return 0u;
+      return b;
+    }
+    default: return r;   // unsigned integers and int64: the bits themselves
+  }
+}
+
 __device__ __forceinline__ unsigned long long group_key(const void* vals, int64_t i, int type) {
   switch (type) {
     case T_I64: case T_U64: return ld_stream<unsigned long long>(vals, i);
@@ -129,6 +170,10 @@ __device__ __forceinline__ void lower_first(uint32_t* p, uint32_t row) {
   if (row < *p) atomicMin(p, row);
 }
 
+// A probe's read of a slot key, through this XCD's L2. A stale EMPTY only sends the row to the CAS,
+// which returns the settled key (device-scope reads measured slower on MI355X).
+__device__ __forceinline__ unsigned long long probe_load(const unsigned long long* p) { return *p; }
+
 // Probes (and inserts) key k, starting at its hash slot. Returns its slot, or NULL_SLOT when the probe
 // budget ran out (the table is too full: the caller grows it and builds again).
 __device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsigned long long k, uint32_t s,
@@ -144,7 +189,7 @@ __device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsig
     s = (s + 1u) & cap_mask;
     // A slot's key goes from EMPTY to its key once and never changes again, so a plain read is
     // enough: a stale EMPTY only sends the row to the CAS, which returns the settled key.
-    cur = tkey[s];
+    cur = probe_load(&tkey[s]);
   }
 }
 
@@ -153,6 +198,7 @@ __device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsig
 constexpr int HLL_BITS = 12;
 constexpr uint32_t HLL_M = 1u << HLL_BITS;
 
+template <int W>
 __global__ __launch_bounds__(256) void gb_hll(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                               uint32_t n, uint32_t* regs) {
   __shared__ uint32_t r[HLL_M];
@@ -160,8 +206,9 @@ __global__ __launch_bounds__(256) void gb_hll(const void* vals, int type, const 
   __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gsize) {
-    if (valid && !bit_get(valid, voff + int64_t(i))) continue;
-    const unsigned long long h = mix64(group_key(vals, int64_t(i), type) ^ 0x2545F4914F6CDD1Dull);
+    const unsigned long long raw = ld_raw<W>(vals, int64_t(i));
+    if (!ld_valid(valid, voff, int64_t(i))) continue;
+    const unsigned long long h = mix64(key_of_raw(raw, type) ^ 0x2545F4914F6CDD1Dull);
     const uint32_t idx = uint32_t(h >> (64 - HLL_BITS));
     const unsigned long long w = h << HLL_BITS;
     const uint32_t rank = w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
@@ -206,7 +253,7 @@ __global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, uns
 // every slot's lowest row (the representative rows of float and multi-column keys).
 constexpr int BUILD_ROWS = 8;
 
-template <bool TRACK_FIRST>
+template <bool TRACK_FIRST, int W>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st) {
@@ -219,22 +266,33 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
     unsigned long long k[BUILD_ROWS], cur[BUILD_ROWS];
     uint32_t s[BUILD_ROWS];
     int cat[BUILD_ROWS];   // 0 key, 1 null, 2 the EMPTY key, 3 past the end
+    bool vok[BUILD_ROWS];
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {   // every load first (rows past the end read the last row)
+      const uint64_t iu = base + uint64_t(r) * gsize + tid;
+      const int64_t ii = int64_t(iu < uint64_t(n) ? iu : uint64_t(n) - 1u);
+      k[r] = ld_raw<W>(vals, ii);
+      vok[r] = ld_valid(valid, voff, ii);
+    }
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint64_t i = base + uint64_t(r) * gsize + tid;
       cat[r] = 3;
-      k[r] = 0;
       if (i < n) {
-        if (valid && !bit_get(valid, voff + int64_t(i))) cat[r] = 1;
-        else {
-          k[r] = group_key(vals, int64_t(i), type);
+        if (!vok[r]) {
+          cat[r] = 1;
+          k[r] = 0;
+        } else {
+          k[r] = key_of_raw(k[r], type);
           cat[r] = k[r] == EMPTY ? 2 : 0;
         }
+      } else {
+        k[r] = 0;
       }
       s[r] = uint32_t(mix64(k[r])) & cap_mask;
     }
 #pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? table.key[s[r]] : 0ull;
+    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? probe_load(&table.key[s[r]]) : 0ull;
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
@@ -423,6 +481,17 @@ __device__ __forceinline__ bool val_valid(const ValSource& v, int64_t i) {
   return !v.valid || bit_get(v.valid, v.voff + i);
 }
 
+// A value's 64-bit word from its raw bits (what raw64 reads): signed integers sign-extended, the rest
+// zero-extended (registers only).
+__device__ __forceinline__ unsigned long long val_of_raw(unsigned long long r, int type) {
+  switch (type) {
+    case T_I32: return (unsigned long long)(long long)int32_t(uint32_t(r));
+    case T_I16: return (unsigned long long)(long long)int16_t(uint16_t(r));
+    case T_I8: return (unsigned long long)(long long)int8_t(uint8_t(r));
+    default: return r;
+  }
+}
+
 // Aggregate families. Each has W 64-bit words per group with an op per word, an identity per word,
 // and a per-row contribution (false when the row contributes nothing).
 struct AggArgs {
@@ -440,7 +509,9 @@ struct CountP {
   static constexpr int W = 1;
   static constexpr bool SKIP_NULL = VALID_ONLY;
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
-  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[1]) {
+  static constexpr bool USES_VALUE = false;
+  static constexpr int FIXED_VW = 0;
+  __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long, unsigned long long (&c)[1]) {
     c[0] = 1;
     return true;
   }
@@ -451,8 +522,10 @@ struct SumCountP {
   static constexpr int W = 2;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
-  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
-    c[0] = raw64(a.v, i);
+  static constexpr bool USES_VALUE = true;
+  static constexpr int FIXED_VW = 0;
+  __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
+    c[0] = x;
     c[1] = 1;
     return true;
   }
@@ -464,8 +537,10 @@ struct MinMaxP {
   static constexpr int W = 2;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MIN : W_MAX; }
-  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[2]) {
-    unsigned long long b = raw64(a.v, i), key;
+  static constexpr bool USES_VALUE = true;
+  static constexpr int FIXED_VW = 0;
+  __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
+    unsigned long long b = x, key;
     switch (a.v.type) {
       case T_F64:
         if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return false;
@@ -493,8 +568,9 @@ struct F64AP {
   static constexpr int W = 4;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MAX : (w == 1 ? W_OR : (w == 2 ? W_ADD : W_MIN)); }
-  __device__ static bool contrib(const AggArgs& a, int64_t i, long long, unsigned long long (&c)[4]) {
-    unsigned long long v = raw64(a.v, i);
+  static constexpr bool USES_VALUE = true;
+  static constexpr int FIXED_VW = 8;
+  __device__ static bool contrib(const AggArgs&, int64_t i, long long, unsigned long long v, unsigned long long (&c)[4]) {
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     unsigned long long mag = v & 0x7FFFFFFFFFFFFFFFull;
     c[0] = 0;
@@ -542,8 +618,9 @@ struct F64BP {
   static constexpr int W = 3;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_ADD128 : (w == 1 ? W_HI : W_ADD); }
-  __device__ static bool contrib(const AggArgs& a, int64_t i, long long g, unsigned long long (&c)[3]) {
-    unsigned long long v = raw64(a.v, i);
+  static constexpr bool USES_VALUE = true;
+  static constexpr int FIXED_VW = 8;
+  __device__ static bool contrib(const AggArgs& a, int64_t, long long g, unsigned long long v, unsigned long long (&c)[3]) {
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) return false;
     const unsigned long long* st = a.aux + size_t(g) * 4;
@@ -631,8 +708,14 @@ constexpr int AGG_ROWS = 4;   // rows per thread whose loads are issued together
 // groups, claimed by CAS on a key word, in front of this XCD's table in global memory (L2); a row
 // whose group cannot be cached within 8 probes goes to that table directly. The tables of the
 // eight XCDs are merged by gb_agg_merge.
-template <class P>
-__global__ __launch_bounds__(256) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rows_per_block,
+// 1024 threads per workgroup and one workgroup per CU: the whole LDS budget goes to one table.
+#ifndef AM_AGG_THREADS
+#define AM_AGG_THREADS 1024
+#endif
+// IDW: byte width of the group ids (4: int32 / uint32, 8: int64); VW: byte width of the values (0 when
+// the family reads none).
+template <class P, int IDW, int VW>
+__global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rows_per_block,
                                               unsigned long long* gtab8) {
   extern __shared__ unsigned long long lds[];
   unsigned long long* lacc = lds;                                     // [cap][W]
@@ -649,15 +732,26 @@ __global__ __launch_bounds__(256) void gb_agg(AggArgs a, int mode, uint32_t cap,
   for (int64_t base = begin; base < end; base += int64_t(blockDim.x) * AGG_ROWS) {
     long long gs[AGG_ROWS];
     unsigned long long cs[AGG_ROWS][P::W];
+    unsigned long long idr[AGG_ROWS], vr[AGG_ROWS];
+    bool iok[AGG_ROWS], vok[AGG_ROWS];
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {   // every load first (rows past the end read the last row)
+      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+      const int64_t ii = i < end ? i : end - 1;
+      idr[r] = ld_raw<IDW>(a.ids.ids, ii);
+      iok[r] = ld_valid(a.ids.valid, a.ids.voff, ii);
+      vr[r] = 0;
+      if constexpr (VW != 0) vr[r] = ld_raw<VW>(a.v.vals, ii);
+      vok[r] = P::SKIP_NULL ? ld_valid(a.v.valid, a.v.voff, ii) : true;
+    }
 #pragma unroll
     for (int r = 0; r < AGG_ROWS; r++) {
-      int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-      long long g = -1;
-      if (i < end) {
-        g = load_gid(a.ids, i, a.K);
-        if (P::SKIP_NULL && g >= 0 && !val_valid(a.v, i)) g = -1;
-        if (g >= 0 && !P::contrib(a, i, g, cs[r])) g = -1;
-      }
+      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+      long long g;
+      if constexpr (IDW == 8) g = (long long)idr[r];
+      else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(idr[r])) : (long long)uint32_t(idr[r]);
+      if (i >= end || !iok[r] || g < 0 || g >= a.K || !vok[r]) g = -1;
+      if (g >= 0 && !P::contrib(a, i, g, val_of_raw(vr[r], a.v.type), cs[r])) g = -1;
       gs[r] = g;
     }
 #pragma unroll
@@ -731,6 +825,34 @@ __global__ void gb_agg_merge(const unsigned long long* gtab8, long long K, unsig
   }
 }
 
+template <class P, int IDW, int VW>
+void launch_agg(int blocks, size_t shmem, hipStream_t s, const AggArgs& a, int mode, uint32_t cap, int64_t rpb,
+                unsigned long long* g8) {
+  if (shmem > 65536)
+    AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_agg<P, IDW, VW>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+  hipLaunchKernelGGL((gb_agg<P, IDW, VW>), dim3(blocks), dim3(AM_AGG_THREADS), shmem, s, a, mode, cap, rpb, g8);
+}
+
+// The instantiation for the values' width: none for families that read no value, the fixed width of
+// the float64 families, otherwise 1, 2, 4 or 8 bytes.
+template <class P, int IDW>
+void launch_agg_vw(int vw, int blocks, size_t shmem, hipStream_t s, const AggArgs& a, int mode, uint32_t cap,
+                   int64_t rpb, unsigned long long* g8) {
+  if constexpr (!P::USES_VALUE) {
+    launch_agg<P, IDW, 0>(blocks, shmem, s, a, mode, cap, rpb, g8);
+  } else if constexpr (P::FIXED_VW != 0) {
+    launch_agg<P, IDW, P::FIXED_VW>(blocks, shmem, s, a, mode, cap, rpb, g8);
+  } else {
+    switch (vw) {
+      case 8: launch_agg<P, IDW, 8>(blocks, shmem, s, a, mode, cap, rpb, g8); break;
+      case 4: launch_agg<P, IDW, 4>(blocks, shmem, s, a, mode, cap, rpb, g8); break;
+      case 2: launch_agg<P, IDW, 2>(blocks, shmem, s, a, mode, cap, rpb, g8); break;
+      default: launch_agg<P, IDW, 1>(blocks, shmem, s, a, mode, cap, rpb, g8); break;
+    }
+  }
+}
+
 // Runs family P over the rows into a fresh global table of K x W words.
 template <class P>
 DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
@@ -747,7 +869,10 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
   hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(NXCD * K * P::W, 4)), dim3(256), 0, s, g8, NXCD * K);
   AM_HIP(hipGetLastError());
   // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
-  const size_t budget = 48 * 1024;
+#ifndef AM_AGG_LDS
+#define AM_AGG_LDS (96 * 1024)
+#endif
+  const size_t budget = std::min<size_t>(AM_AGG_LDS, device().lds_per_block);
   const uint32_t per_slot = P::W * 8;
   uint32_t dense_cap = uint32_t(budget / per_slot);
   int mode;
@@ -761,12 +886,15 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
     while (cap * 2 * (per_slot + 4) <= budget) cap *= 2;
   }
   size_t shmem = size_t(cap) * per_slot + (mode == 1 ? size_t(cap) * 4 : 0);
-  int blocks = int(std::min<int64_t>((a.n + 4095) / 4096, int64_t(device().cu_count) * 4));
+  int blocks = int(std::min<int64_t>((a.n + 4095) / 4096, int64_t(device().cu_count) * 4 * 256 / AM_AGG_THREADS));
   blocks = std::max(blocks, 1);
   int64_t rpb = (a.n + blocks - 1) / blocks;
   {
     ProfScope ps(name);
-    hipLaunchKernelGGL(gb_agg<P>, dim3(blocks), dim3(256), shmem, s, a, mode, cap, rpb, g8);
+    const int idw = a.ids.type == T_I64 ? 8 : 4;
+    const int vw = !P::USES_VALUE ? 0 : (P::FIXED_VW ? P::FIXED_VW : type_width(a.v.type));
+    if (idw == 8) launch_agg_vw<P, 8>(vw, blocks, shmem, s, a, mode, cap, rpb, g8);
+    else launch_agg_vw<P, 4>(vw, blocks, shmem, s, a, mode, cap, rpb, g8);
     AM_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(gb_agg_merge<P>, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s, g8, K, o);
@@ -1068,6 +1196,18 @@ struct Mapped {
   DevBuf ids, rep, okeys, perm;
 };
 
+// gb_build for the key column's element width.
+template <bool TRACK>
+void launch_build(int w, int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
+                  uint32_t n, Table T, uint32_t mask, uint32_t budget, uint32_t* slot_of, BuildStatus* st) {
+  switch (w) {
+    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
+    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
+    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
+    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
+  }
+}
+
 // Maps one key column (values, type, validity) to dense ascending ids; with `track` it also finds
 // the lowest row of every group.
 Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track) {
@@ -1090,8 +1230,14 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
     AM_HIP(hipMemsetAsync(regs->ptr, 0, HLL_M * 4, s));
     {
       ProfScope ps("gb_hll");
-      hipLaunchKernelGGL(gb_hll, dim3(grid_for(n, 4)), dim3(256), 0, s, vals, type, valid, voff, un,
-                         static_cast<uint32_t*>(regs->ptr));
+      auto* rg = static_cast<uint32_t*>(regs->ptr);
+      const dim3 hg(grid_for(n, 4));
+      switch (type_width(type)) {
+        case 8: hipLaunchKernelGGL(gb_hll<8>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
+        case 4: hipLaunchKernelGGL(gb_hll<4>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
+        case 2: hipLaunchKernelGGL(gb_hll<2>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
+        default: hipLaunchKernelGGL(gb_hll<1>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
+      }
       hipLaunchKernelGGL(gb_hll_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(regs->ptr),
                          static_cast<unsigned long long*>(est->ptr));
       AM_HIP(hipGetLastError());
@@ -1121,11 +1267,11 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
       ProfScope ps("gb_build");
       uint32_t budget = last_try ? uint32_t(cap) : 128u;
       if (track)
-        hipLaunchKernelGGL(gb_build<true>, dim3(build_grid), dim3(256), 0, s, vals, type, valid, voff, un, T,
-                           uint32_t(cap - 1), budget, static_cast<uint32_t*>(out.ids->ptr), st);
+        launch_build<true>(type_width(type), build_grid, s, vals, type, valid, voff, un, T, uint32_t(cap - 1), budget,
+                           static_cast<uint32_t*>(out.ids->ptr), st);
       else
-        hipLaunchKernelGGL(gb_build<false>, dim3(build_grid), dim3(256), 0, s, vals, type, valid, voff, un, T,
-                           uint32_t(cap - 1), budget, static_cast<uint32_t*>(out.ids->ptr), st);
+        launch_build<false>(type_width(type), build_grid, s, vals, type, valid, voff, un, T, uint32_t(cap - 1), budget,
+                            static_cast<uint32_t*>(out.ids->ptr), st);
       AM_HIP(hipGetLastError());
     }
     AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));

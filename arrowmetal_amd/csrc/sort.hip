@@ -64,7 +64,7 @@ __global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
 }
 
 
-template <typename K>
+template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const uint8_t* valid, int64_t voff,
                                                       uint32_t n, int type, int forder, int desc,
                                                       uint32_t rows_per_block, SortCtl* ctl) {
@@ -86,12 +86,19 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
   for (uint32_t base = begin; base < end; base += BLOCK * UP_ITEMS) {
     K kk[UP_ITEMS];
     int cc[UP_ITEMS];
+    uint64_t raw[UP_ITEMS];
+    uint32_t vb[UP_ITEMS], bit[UP_ITEMS];
+#pragma unroll
+    for (int u = 0; u < UP_ITEMS; u++) {   // every load first (rows past the end read the last row)
+      const uint32_t i = min(base + uint32_t(u) * BLOCK + threadIdx.x, end - 1u);
+      raw[u] = load_raw<W>(vals, i);
+      vb[u] = load_valid_byte(valid, voff, i, bit[u]);
+    }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
-      uint32_t i = base + uint32_t(u) * BLOCK + threadIdx.x;
-      cc[u] = 3;
+      const uint32_t i = base + uint32_t(u) * BLOCK + threadIdx.x;
       kk[u] = 0;
-      if (i < end) cc[u] = classify<K>(vals, valid, voff, i, type, forder, desc, kk[u]);
+      cc[u] = i < end ? classify_raw<K>(raw[u], (vb[u] >> bit[u]) & 1u, type, forder, desc, kk[u]) : 3;
     }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(BLOCK) void sort_plan(SortCtl* ctl, int nulls_first
   }
 }
 
-template <typename K, int ITEMS>
+template <typename K, int W, int ITEMS>
 __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
                                                     uint32_t n, int type, int forder, int desc,
                                                     SortCtl* ctl, unsigned long long* status, uint32_t epoch,
@@ -229,14 +236,25 @@ __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const u
   uint32_t bins[ITEMS];
   const uint32_t rows[1] = {0};   // rows are base_e + 64 * j (ROWS_IOTA)
   const uint32_t base_e = tile * uint32_t(TILE) + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+  // Every load first (lanes past the end read the last row), then the mapping on registers.
+  uint32_t vbit[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t e = min(base_e + uint32_t(j) * 64u, n - 1u);
+    uint32_t bit;
+    const uint32_t vb = load_valid_byte(valid, voff, e, bit);
+    vbit[j] = (vb >> bit) & 1u;
+    keys[j] = K(load_raw<W>(vals, e));
+  }
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
     uint32_t e = base_e + uint32_t(j) * 64u;
-    keys[j] = 0;
     uint32_t b = BIN_NONE;
+    const uint64_t raw = uint64_t(keys[j]);
+    keys[j] = 0;
     if (e < n) {
       K k;
-      int c = classify<K>(vals, valid, voff, e, type, forder, desc, k);
+      int c = classify_raw<K>(raw, vbit[j] != 0u, type, forder, desc, k);
       if (c == 0) {
         keys[j] = k;
         b = uint32_t(k >> shift) & 0xFFu;
@@ -348,7 +366,7 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
 
 // ---------------------------------------------------------------- host side
 
-template <typename K, int ITEMS>
+template <typename K, int W, int ITEMS>
 void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
               int placement, int forder, uint32_t* out_idx, hipStream_t s) {
   constexpr int P = int(sizeof(K));
@@ -378,7 +396,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     uint32_t rpb = ((n + blocks - 1) / blocks + 2047u) / 2048u * 2048u;
     blocks = (n + rpb - 1) / rpb;
     ProfScope ps("sort_upsweep");
-    hipLaunchKernelGGL(sort_upsweep<K>, dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+    hipLaunchKernelGGL((sort_upsweep<K, W>), dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
                        dsc, rpb, ctl);
     AM_HIP(hipGetLastError());
   }
@@ -400,7 +418,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   }
   {
     ProfScope ps("sort_first");
-    hipLaunchKernelGGL((sort_first<K, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+    hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
                        dsc, ctl, status, epoch, static_cast<K*>(k0->ptr), static_cast<uint32_t*>(r0->ptr),
                        out_idx);
     AM_HIP(hipGetLastError());
@@ -436,15 +454,15 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   }
 }
 
-template <typename K>
+template <typename K, int W>
 void dispatch_items(int items, const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n,
                     bool desc, int placement, int forder, uint32_t* out_idx, hipStream_t s) {
   switch (items) {
-    case 4: run_sort<K, 4>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
-    case 8: run_sort<K, 8>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
-    case 12: run_sort<K, 12>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
-    case 20: run_sort<K, 20>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
-    default: run_sort<K, 16>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    case 4: run_sort<K, W, 4>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    case 8: run_sort<K, W, 8>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    case 12: run_sort<K, W, 12>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    case 20: run_sort<K, W, 20>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
+    default: run_sort<K, W, 16>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
   }
 }
 
@@ -458,12 +476,24 @@ void argsort_device(const void* vals, int type, const uint8_t* validity, int64_t
   if (w == 0) fail("argsort: unsupported element type");
   int kb = w == 8 ? 8 : 4;
   SortTune t = sort_tune(n, kb);
-  if (kb == 8)
-    dispatch_items<unsigned long long>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending,
-                                       null_placement, float_order, out_idx, s);
-  else
-    dispatch_items<uint32_t>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending, null_placement,
-                             float_order, out_idx, s);
+  switch (w) {
+    case 8:
+      dispatch_items<unsigned long long, 8>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending,
+                                            null_placement, float_order, out_idx, s);
+      break;
+    case 4:
+      dispatch_items<uint32_t, 4>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending, null_placement,
+                                  float_order, out_idx, s);
+      break;
+    case 2:
+      dispatch_items<uint32_t, 2>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending, null_placement,
+                                  float_order, out_idx, s);
+      break;
+    default:
+      dispatch_items<uint32_t, 1>(t.items, vals, type, validity, vbit_off, uint32_t(n), descending, null_placement,
+                                  float_order, out_idx, s);
+      break;
+  }
 }
 
 Array* argsort(const Array& a, bool descending, int null_placement, int float_order) {

@@ -33,22 +33,42 @@ constexpr uint64_t SIGN64 = 0x8000000000000000ull;
 constexpr uint64_t ABS64 = 0x7FFFFFFFFFFFFFFFull;
 constexpr uint64_t INF64 = 0x7FF0000000000000ull;
 
-// Category of row i (0 value, 1 NaN in its own block, 2 null) and, for a value, its ascending
-// order-preserving key with the descending flip applied (SortSource.swift:131-190).
-// float_order: 0 ieee (NaN apart, -0.0 ties +0.0), 1 total (raw sign flip), 2 nan_largest.
+// Rows are read in two steps so that a thread's loads are all issued before any is used: the raw
+// bits of a W-byte element and its validity byte are loaded unconditionally (the byte comes from a
+// one-byte all-valid constant when the column has no bitmap), and the type-dependent mapping below
+// runs on registers only. A type switch around the loads themselves would make the compiler wait on
+// every load at the branch join.
+__device__ const uint8_t kAllValid = 0xFFu;
+
+template <int W>
+__device__ __forceinline__ uint64_t load_raw(const void* vals, uint32_t i) {
+  if constexpr (W == 8) return static_cast<const uint64_t*>(vals)[i];
+  else if constexpr (W == 4) return static_cast<const uint32_t*>(vals)[i];
+  else if constexpr (W == 2) return static_cast<const uint16_t*>(vals)[i];
+  else return static_cast<const uint8_t*>(vals)[i];
+}
+
+// The validity byte of row i and the bit of row i in it.
+__device__ __forceinline__ uint32_t load_valid_byte(const uint8_t* valid, int64_t voff, uint32_t i, uint32_t& bit) {
+  const int64_t b = voff + int64_t(i);
+  const uint8_t* p = valid ? valid + (b >> 3) : &kAllValid;
+  bit = valid ? uint32_t(b & 7) : 0u;
+  return *p;
+}
+
+// Category of a row (0 value, 1 NaN in its own block, 2 null) from its raw bits and validity and,
+// for a value, its ascending order-preserving key with the descending flip applied
+// (SortSource.swift:131-190). float_order: 0 ieee (NaN apart, -0.0 ties +0.0), 1 total (raw sign
+// flip), 2 nan_largest. Registers only.
 template <typename K>
-__device__ __forceinline__ int classify(const void* vals, const uint8_t* valid, int64_t voff, uint32_t i,
-                                        int type, int forder, int desc, K& key) {
-  if (valid) {
-    int64_t b = voff + int64_t(i);
-    if (!((valid[b >> 3] >> (b & 7)) & 1)) return 2;
-  }
+__device__ __forceinline__ int classify_raw(uint64_t raw, bool is_valid, int type, int forder, int desc, K& key) {
+  if (!is_valid) return 2;
   uint64_t k;
   switch (type) {
-    case T_I64: k = uint64_t(static_cast<const int64_t*>(vals)[i]) ^ SIGN64; break;
-    case T_U64: k = static_cast<const uint64_t*>(vals)[i]; break;
+    case T_I64: k = raw ^ SIGN64; break;
+    case T_U64: k = raw; break;
     case T_F64: {
-      uint64_t x = static_cast<const uint64_t*>(vals)[i];
+      uint64_t x = raw;
       if (forder != 1) {
         if ((x & ABS64) > INF64) {
           if (forder == 0) return 1;
@@ -60,14 +80,12 @@ __device__ __forceinline__ int classify(const void* vals, const uint8_t* valid, 
       k = (x & SIGN64) ? ~x : (x | SIGN64);
       break;
     }
-    case T_I32: k = uint32_t(static_cast<const int32_t*>(vals)[i]) ^ 0x80000000u; break;
-    case T_I16: k = uint32_t(int32_t(static_cast<const int16_t*>(vals)[i])) ^ 0x80000000u; break;
-    case T_I8: k = uint32_t(int32_t(static_cast<const int8_t*>(vals)[i])) ^ 0x80000000u; break;
-    case T_U32: k = static_cast<const uint32_t*>(vals)[i]; break;
-    case T_U16: k = static_cast<const uint16_t*>(vals)[i]; break;
-    case T_U8: k = static_cast<const uint8_t*>(vals)[i]; break;
+    case T_I32: k = uint32_t(raw) ^ 0x80000000u; break;
+    case T_I16: k = uint32_t(int32_t(int16_t(uint16_t(raw)))) ^ 0x80000000u; break;
+    case T_I8: k = uint32_t(int32_t(int8_t(uint8_t(raw)))) ^ 0x80000000u; break;
+    case T_U32: case T_U16: case T_U8: k = raw; break;
     default: {   // T_F32
-      uint32_t x = static_cast<const uint32_t*>(vals)[i];
+      uint32_t x = uint32_t(raw);
       if (forder != 1) {
         if ((x & 0x7FFFFFFFu) > 0x7F800000u) {
           if (forder == 0) return 1;
@@ -83,6 +101,15 @@ __device__ __forceinline__ int classify(const void* vals, const uint8_t* valid, 
   if (desc) k = ~k;
   key = K(k);
   return 0;
+}
+
+// Both steps for one row (rows whose loads need not be batched).
+template <typename K, int W>
+__device__ __forceinline__ int classify(const void* vals, const uint8_t* valid, int64_t voff, uint32_t i,
+                                        int type, int forder, int desc, K& key) {
+  uint32_t bit;
+  const uint32_t vb = load_valid_byte(valid, voff, i, bit);
+  return classify_raw<K>(load_raw<W>(vals, i), (vb >> bit) & 1u, type, forder, desc, key);
 }
 
 // ---------------------------------------------------------------- block helpers
