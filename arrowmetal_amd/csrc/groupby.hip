@@ -146,6 +146,7 @@ struct BuildStatus {
   uint32_t distinct;
   uint32_t special_used;   // the EMPTY key has been seen: slot C holds it
   unsigned long long nulls;
+  uint32_t xspecial[NXCD]; // the same, per XCD-private table
 };
 
 __global__ void gb_table_init(Table table, size_t slots, BuildStatus* st) {
@@ -155,11 +156,12 @@ __global__ void gb_table_init(Table table, size_t slots, BuildStatus* st) {
     table.key[i] = EMPTY;
     if (table.first) table.first[i] = 0xFFFFFFFFu;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (st && blockIdx.x == 0 && threadIdx.x == 0) {
     st->overflow = 0;
     st->null_first = 0xFFFFFFFFu;
     st->distinct = 0;
     st->special_used = 0;
+    for (int x = 0; x < NXCD; x++) st->xspecial[x] = 0;
     st->nulls = 0;
   }
 }
@@ -256,10 +258,18 @@ constexpr int BUILD_ROWS = 8;
 template <bool TRACK_FIRST, int W>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
-                                                uint32_t* slot_of, BuildStatus* st) {
+                                                uint32_t* slot_of, BuildStatus* st, int private_tables) {
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t special = cap_mask + 1u;
+  // With private tables each XCD inserts into its own copy (cap + 1 entries each), so a slot line in
+  // an XCD's L2 is only ever changed by that XCD's own atomics; rows store the slot's index over all
+  // the copies.
+  const uint32_t tbl = private_tables ? xcd_id() : 0u;
+  const uint32_t tbase = tbl * (cap_mask + 2u);
+  unsigned long long* tkey = table.key + tbase;
+  uint32_t* tfirst = TRACK_FIRST ? table.first + tbase : nullptr;
+  uint32_t* special_flag = private_tables ? &st->xspecial[tbl] : &st->special_used;
   unsigned long long nulls = 0;
   bool overflow = false;
   for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
       s[r] = uint32_t(mix64(k[r])) & cap_mask;
     }
 #pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? probe_load(&table.key[s[r]]) : 0ull;
+    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? probe_load(&tkey[s[r]]) : 0ull;
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
@@ -304,18 +314,56 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
         if (TRACK_FIRST) lower_first(&st->null_first, i);
       } else if (cat[r] == 2) {
         slot = special;
-        if (!st->special_used) atomicExch(&st->special_used, 1u);
+        if (!*special_flag) atomicExch(special_flag, 1u);
       } else {
-        slot = probe_insert(table.key, k[r], s[r], cur[r], cap_mask, budget);
+        slot = probe_insert(tkey, k[r], s[r], cur[r], cap_mask, budget);
         if (slot == NULL_SLOT) overflow = true;
       }
-      __builtin_nontemporal_store(slot, slot_of + i);
-      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&table.first[slot], i);
+      __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : tbase + slot, slot_of + i);
+      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&tfirst[slot], i);
     }
   }
   if (overflow) atomicOr(&st->overflow, 1u);
   for (int off = 32; off > 0; off >>= 1) nulls += __shfl_down(nulls, off, 64);
   if ((threadIdx.x & 63) == 0 && nulls) atomicAdd(&st->nulls, nulls);
+}
+
+// Folds the eight XCD-private tables into one table of cap + 1 entries: every occupied private slot
+// inserts its key (a few per distinct key, not per row) and records where it went in xmap; lowest
+// rows merge by minimum.
+__global__ __launch_bounds__(256) void gb_merge_tables(const unsigned long long* xkeys, const uint32_t* xfirst,
+                                                       uint32_t cap, BuildStatus* st, Table g, uint32_t* xmap) {
+  const uint32_t per = cap + 1u;
+  const uint32_t total = uint32_t(NXCD) * per;
+  bool overflow = false;
+  for (uint32_t xs = blockIdx.x * blockDim.x + threadIdx.x; xs < total; xs += gridDim.x * blockDim.x) {
+    const uint32_t t = xs / per, sl = xs - t * per;
+    uint32_t gs = NULL_SLOT;
+    if (sl == cap) {
+      if (st->xspecial[t]) {
+        gs = cap;
+        if (!st->special_used) atomicExch(&st->special_used, 1u);
+      }
+    } else {
+      const unsigned long long k = xkeys[xs];
+      if (k != EMPTY) {
+        const uint32_t h = uint32_t(mix64(k)) & (cap - 1u);
+        gs = probe_insert(g.key, k, h, probe_load(&g.key[h]), cap - 1u, cap);
+        if (gs == NULL_SLOT) overflow = true;
+      }
+    }
+    xmap[xs] = gs;
+    if (xfirst && gs != NULL_SLOT) lower_first(&g.first[gs], xfirst[xs]);
+  }
+  if (overflow) atomicOr(&st->overflow, 1u);
+}
+
+// Group id of every private slot, through the merged table.
+__global__ void gb_compose(const uint32_t* xmap, uint32_t total, const uint32_t* gid_of_slot, uint32_t* gid_x) {
+  for (uint32_t xs = blockIdx.x * blockDim.x + threadIdx.x; xs < total; xs += gridDim.x * blockDim.x) {
+    const uint32_t gs = xmap[xs];
+    gid_x[xs] = gs == NULL_SLOT ? NULL_SLOT : gid_of_slot[gs];
+  }
 }
 
 // Occupied slots -> (order key, slot), in slot order. One block per contiguous range of slots: a
@@ -1199,12 +1247,13 @@ struct Mapped {
 // gb_build for the key column's element width.
 template <bool TRACK>
 void launch_build(int w, int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
-                  uint32_t n, Table T, uint32_t mask, uint32_t budget, uint32_t* slot_of, BuildStatus* st) {
+                  uint32_t n, Table T, uint32_t mask, uint32_t budget, uint32_t* slot_of, BuildStatus* st, bool priv) {
+  const int p = priv ? 1 : 0;
   switch (w) {
-    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
-    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
-    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
-    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st); break;
+    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
   }
 }
 
@@ -1252,26 +1301,48 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   }
   cap = std::min(cap, full);
   BuildStatus host{};
-  DevBuf tkeys, tfirst;
-  Table T{};
+  DevBuf tkeys, tfirst, gkeys, gfirst, xmap;
+  Table T{};          // the table the rows were inserted into (eight private copies, or one)
+  Table G{};          // the merged table of distinct keys
+  bool priv = false;
   const int build_grid = grid_for((n + BUILD_ROWS - 1) / BUILD_ROWS, 8);
   for (int attempt = 0;; attempt++) {
     bool last_try = cap >= full;
-    tkeys = device_alloc(size_t(cap + 1) * 8);
-    if (track) tfirst = device_alloc(size_t(cap + 1) * 4);
+    // Tables up to 2^20 slots are private to each XCD (eight copies): a key line is then only changed
+    // by atomics of the XCD whose L2 holds it. Larger ones are shared.
+    priv = cap <= (1u << 20);
+    const size_t ntab = priv ? NXCD : 1;
+    tkeys = device_alloc(ntab * size_t(cap + 1) * 8);
+    if (track) tfirst = device_alloc(ntab * size_t(cap + 1) * 4);
     T.key = static_cast<unsigned long long*>(tkeys->ptr);
     T.first = track ? static_cast<uint32_t*>(tfirst->ptr) : nullptr;
-    hipLaunchKernelGGL(gb_table_init, dim3(grid_for(int64_t(cap + 1), 4)), dim3(256), 0, s, T, size_t(cap + 1), st);
+    hipLaunchKernelGGL(gb_table_init, dim3(grid_for(int64_t(ntab * (cap + 1)), 4)), dim3(256), 0, s, T,
+                       ntab * size_t(cap + 1), st);
     AM_HIP(hipGetLastError());
     {
       ProfScope ps("gb_build");
       uint32_t budget = last_try ? uint32_t(cap) : 128u;
       if (track)
         launch_build<true>(type_width(type), build_grid, s, vals, type, valid, voff, un, T, uint32_t(cap - 1), budget,
-                           static_cast<uint32_t*>(out.ids->ptr), st);
+                           static_cast<uint32_t*>(out.ids->ptr), st, priv);
       else
         launch_build<false>(type_width(type), build_grid, s, vals, type, valid, voff, un, T, uint32_t(cap - 1), budget,
-                            static_cast<uint32_t*>(out.ids->ptr), st);
+                            static_cast<uint32_t*>(out.ids->ptr), st, priv);
+      AM_HIP(hipGetLastError());
+    }
+    G = T;
+    if (priv) {
+      gkeys = device_alloc(size_t(cap + 1) * 8);
+      if (track) gfirst = device_alloc(size_t(cap + 1) * 4);
+      xmap = device_alloc(size_t(NXCD) * size_t(cap + 1) * 4);
+      G.key = static_cast<unsigned long long*>(gkeys->ptr);
+      G.first = track ? static_cast<uint32_t*>(gfirst->ptr) : nullptr;
+      hipLaunchKernelGGL(gb_table_init, dim3(grid_for(int64_t(cap + 1), 4)), dim3(256), 0, s, G, size_t(cap + 1),
+                         static_cast<BuildStatus*>(nullptr));
+      ProfScope ps("gb_merge");
+      hipLaunchKernelGGL(gb_merge_tables, dim3(grid_for(int64_t(NXCD) * (cap + 1), 4)), dim3(256), 0, s,
+                         static_cast<const unsigned long long*>(T.key), static_cast<const uint32_t*>(T.first),
+                         uint32_t(cap), st, G, static_cast<uint32_t*>(xmap->ptr));
       AM_HIP(hipGetLastError());
     }
     AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
@@ -1287,7 +1358,7 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
     ProfScope ps("gb_compact");
     uint32_t blocks = uint32_t(std::min<uint64_t>((cap + 1 + 4095) / 4096, uint64_t(device().cu_count) * 4));
     uint32_t per_block = uint32_t((cap + 1 + blocks - 1) / blocks);
-    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, T.key, uint32_t(cap), type, st,
+    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, G.key, uint32_t(cap), type, st,
                        static_cast<unsigned long long*>(out.okeys->ptr), static_cast<uint32_t*>(slots->ptr), per_block);
     AM_HIP(hipGetLastError());
   }
@@ -1304,15 +1375,26 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   {
     ProfScope ps("gb_relabel");
     hipLaunchKernelGGL(gb_relabel, dim3(uint32_t((out.D + 1 + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const uint32_t*>(out.perm->ptr), static_cast<const uint32_t*>(slots->ptr), out.D, T.first,
+                       static_cast<const uint32_t*>(out.perm->ptr), static_cast<const uint32_t*>(slots->ptr), out.D, G.first,
                        static_cast<uint32_t*>(gid_of_slot->ptr), static_cast<uint32_t*>(out.rep->ptr), st,
                        out.has_null ? 1 : 0, track ? 1 : 0);
     AM_HIP(hipGetLastError());
   }
+  // Rows hold private-table slots: map those through the merged table first.
+  DevBuf gid_x;
+  const uint32_t* row_gid = static_cast<const uint32_t*>(gid_of_slot->ptr);
+  if (priv) {
+    const uint32_t total = uint32_t(NXCD) * uint32_t(cap + 1);
+    gid_x = device_alloc(size_t(total) * 4);
+    hipLaunchKernelGGL(gb_compose, dim3(grid_for(total, 4)), dim3(256), 0, s, static_cast<const uint32_t*>(xmap->ptr),
+                       total, row_gid, static_cast<uint32_t*>(gid_x->ptr));
+    AM_HIP(hipGetLastError());
+    row_gid = static_cast<const uint32_t*>(gid_x->ptr);
+  }
   if (n > 0) {
     ProfScope ps("gb_ids");
-    hipLaunchKernelGGL(gb_ids, dim3(build_grid), dim3(256), 0, s, static_cast<uint32_t*>(out.ids->ptr), un,
-                       static_cast<const uint32_t*>(gid_of_slot->ptr), out.D);
+    hipLaunchKernelGGL(gb_ids, dim3(build_grid), dim3(256), 0, s, static_cast<uint32_t*>(out.ids->ptr), un, row_gid,
+                       out.D);
     AM_HIP(hipGetLastError());
   }
   AM_HIP(hipStreamSynchronize(s));
