@@ -207,14 +207,27 @@ __global__ __launch_bounds__(256) void gb_hll(const void* vals, int type, const 
   for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) r[j] = 0u;
   __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gsize) {
-    const unsigned long long raw = ld_raw<W>(vals, int64_t(i));
-    if (!ld_valid(valid, voff, int64_t(i))) continue;
-    const unsigned long long h = mix64(key_of_raw(raw, type) ^ 0x2545F4914F6CDD1Dull);
-    const uint32_t idx = uint32_t(h >> (64 - HLL_BITS));
-    const unsigned long long w = h << HLL_BITS;
-    const uint32_t rank = w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
-    if (rank > r[idx]) atomicMax(&r[idx], rank);
+  constexpr int U = 8;   // rows per thread per step, loaded together
+  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * U) {
+    unsigned long long raw[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t iu = base + uint64_t(u) * gsize;
+      const int64_t ii = int64_t(iu < n ? iu : uint64_t(n) - 1u);
+      raw[u] = ld_raw<W>(vals, ii);
+      const bool v = ld_valid(valid, voff, ii);
+      ok[u] = v && iu < n;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!ok[u]) continue;
+      const unsigned long long h = mix64(key_of_raw(raw[u], type) ^ 0x2545F4914F6CDD1Dull);
+      const uint32_t idx = uint32_t(h >> (64 - HLL_BITS));
+      const unsigned long long w = h << HLL_BITS;
+      const uint32_t rank = w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
+      if (rank > r[idx]) atomicMax(&r[idx], rank);
+    }
   }
   __syncthreads();
   for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x)
@@ -811,7 +824,7 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
         continue;
       }
       uint32_t h = uint32_t(g) * 2654435761u;
-      uint32_t sl = (h ^ (h >> 15)) & (cap - 1u);
+      uint32_t sl = __umulhi(h ^ (h >> 15), cap);   // [0, cap) for any cap
       bool done = false;
       for (int probe = 0; probe < 8; probe++) {
         unsigned int cur = lkey[sl];
@@ -824,7 +837,7 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
           done = true;
           break;
         }
-        sl = (sl + 1u) & (cap - 1u);
+        sl = sl + 1u == cap ? 0u : sl + 1u;
       }
       if (!done) apply<P>(&gtab[size_t(g) * P::W], cs[r]);
     }
@@ -918,7 +931,7 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
   AM_HIP(hipGetLastError());
   // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
 #ifndef AM_AGG_LDS
-#define AM_AGG_LDS (96 * 1024)
+#define AM_AGG_LDS (144 * 1024)
 #endif
   const size_t budget = std::min<size_t>(AM_AGG_LDS, device().lds_per_block);
   const uint32_t per_slot = P::W * 8;
@@ -930,8 +943,7 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
     cap = uint32_t(a.K);
   } else {
     mode = 1;
-    cap = 1;
-    while (cap * 2 * (per_slot + 4) <= budget) cap *= 2;
+    cap = uint32_t(budget / (per_slot + 4));
   }
   size_t shmem = size_t(cap) * per_slot + (mode == 1 ? size_t(cap) * 4 : 0);
   int blocks = int(std::min<int64_t>((a.n + 4095) / 4096, int64_t(device().cu_count) * 4 * 256 / AM_AGG_THREADS));
