@@ -5,7 +5,7 @@ last, then the float64 gather), two group-by steps over 50M Zipf(1.1) keys from 
 (keys, int64 sum, count_all, count), and three device-to-device copies of 1 GiB, which calibrate the
 FETCH_SIZE / WRITE_SIZE counters against a kernel of known traffic (read 1 GiB, write 1 GiB).
 
-With --workload batched it instead runs the batched plan (2,048 lognormal batches, ~191M rows: tune,
+With --workload orderby it stops after the ORDER BY steps; with --workload batched it instead runs the batched plan (2,048 lognormal batches, ~191M rows: tune,
 then three graph replays of the per-batch argsort + group-by sum / count_all).
 
 Usage: python3 arrowmetal_amd/profile_driver.py [--rows N] [--workload orderby_groupby|batched]
@@ -42,7 +42,7 @@ def device_copy(nbytes: int, times: int) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", type=int, default=50_000_000)
-    ap.add_argument("--workload", default="orderby_groupby", choices=["orderby_groupby", "batched"])
+    ap.add_argument("--workload", default="orderby_groupby", choices=["orderby_groupby", "orderby", "batched"])
     args = ap.parse_args()
     if args.workload == "batched":
         offs, sk, gk, gv = bench.batched_inputs(2048)
@@ -63,6 +63,8 @@ def main():
         del idx, out
         print(f"order-by step {i} done", flush=True)
     del k, p, keys, valid, payload
+    if args.workload == "orderby":
+        return
     gkeys, ivals, _ = bench.group_by_inputs(n)
     gk = am.MetalArray.from_numpy(gkeys)
     gi = am.MetalArray.from_numpy(ivals)

@@ -299,6 +299,7 @@ def serving_leg(args, log):
         "mix": "60% group-by sum/count (1M-50M rows, Zipf(1.1) over 200 / 10k / 1M keys), 30% int64 argsort "
                "(250k-50M rows), 10% unique over two int64 keys (1M-10M rows, 10^6 x 10^3)",
         "total_rows": total_rows,
+        "offered_rows_per_s": total_rows / float(arrivals[-1]),
         "served": {"p50_ms": pct(lat_served, 50), "p99_ms": pct(lat_served, 99), "mean_ms": 1e3 * float(np.mean(lat_served)),
                    "rows_per_s": total_rows / served_span, "span_s": served_span,
                    "slots": args.slots, "budget_bytes": args.budget_gb << 30,
@@ -357,8 +358,8 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batches", type=int, default=2048)
     ap.add_argument("--skip-serving", action="store_true")
-    ap.add_argument("--requests", type=int, default=600)
-    ap.add_argument("--rate", type=float, default=400.0, help="Poisson arrival rate of the served mix (1/s)")
+    ap.add_argument("--requests", type=int, default=1500)
+    ap.add_argument("--rate", type=float, default=700.0, help="Poisson arrival rate of the served mix (1/s)")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=int, default=8)
     args = ap.parse_args()
