@@ -56,11 +56,19 @@ __global__ void sort_init(SortCtl* ctl) {
 }
 
 // Copies the plan (active mask, first and last active pass, value rows) to pinned host memory.
-__global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
+__global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out, int P) {
   out[0] = ctl->active_mask;
   out[1] = ctl->first_active;
   out[2] = ctl->last_active;
   out[3] = ctl->m;
+  // The largest digit count of the top two digits (the hybrid sort's bucket estimate).
+  uint32_t a = 0, b = 0;
+  for (int d = 0; d < RADIX; d++) {
+    a = max(a, ctl->hist[P - 1][d]);
+    b = max(b, ctl->hist[P - 2][d]);
+  }
+  out[4] = a;
+  out[5] = b;
 }
 
 
@@ -212,7 +220,8 @@ template <typename K, int W, int ITEMS>
 __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
                                                     uint32_t n, int type, int forder, int desc,
                                                     SortCtl* ctl, unsigned long long* status, uint32_t epoch,
-                                                    K* out_keys, uint32_t* out_rows, uint32_t* out_idx) {
+                                                    K* out_keys, uint32_t* out_rows, uint32_t* out_idx,
+                                                    int msd_digit) {
   constexpr int TILE = BLOCK * ITEMS;
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
@@ -225,11 +234,13 @@ __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const u
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[0], 1u);
+  // msd_digit >= 0: the first pass of the hybrid sort partitions by that (top) digit and always writes
+  // keys; it counts its tiles apart from the LSD first pass.
+  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[msd_digit >= 0 ? 9 : 0], 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t first = ctl->first_active;
-  const bool last = first == ctl->last_active;
+  const uint32_t first = msd_digit >= 0 ? uint32_t(msd_digit) : ctl->first_active;
+  const bool last = msd_digit < 0 && first == ctl->last_active;
   const int shift = int(8 * first);
 
   K keys[ITEMS];
@@ -364,7 +375,271 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
   }
 }
 
+// ---------------------------------------------------------------- hybrid MSD sort (64-bit keys)
+//
+// For large inputs of 64-bit keys whose top two digits spread the rows well: the first pass
+// partitions the value rows stably by the top digit (sort_first with msd_digit); a segmented pass
+// partitions each top bucket stably by the second digit; every resulting bucket (a few hundred rows
+// at 50M uniform keys, at most LOCAL_MAX) is then sorted in LDS by the remaining digits and its rows
+// written to their final slots. Three passes over the rows instead of eight.
+
+constexpr uint32_t HYB_CHUNK = 65536;   // rows per histogram chunk of the segmented pass
+constexpr uint32_t LOCAL_MAX = 2048;    // largest bucket the LDS sort takes
+
+struct HybChunk {
+  uint32_t seg, start, end, pad;
+};
+struct HybTile {
+  uint32_t seg, start, end, head;   // head: the segment's first tile
+};
+
+// One workgroup: the chunk and tile tables of the segmented pass, one segment per top digit value.
+// hs[0] = chunks, hs[1] = tiles, hs[2] = largest bucket (filled later).
+template <int P, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void hyb_layout(const SortCtl* ctl, HybChunk* chunks, HybTile* tiles,
+                                                    uint32_t* hs, uint32_t* seg_hist) {
+  __shared__ uint32_t tmp[WAVES];
+  for (uint32_t i = threadIdx.x; i < RADIX * RADIX; i += BLOCK) seg_hist[i] = 0u;
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t b = threadIdx.x;
+  const uint32_t size = ctl->hist[P - 1][b];
+  const uint32_t start = ctl->digit_base[P - 1][b];
+  const uint32_t nc = (size + HYB_CHUNK - 1) / HYB_CHUNK;
+  const uint32_t nt = (size + TILE - 1) / TILE;
+  uint32_t tc, tt;
+  const uint32_t coff = block_excl_scan(nc, tmp, tc);
+  const uint32_t toff = block_excl_scan(nt, tmp, tt);
+  for (uint32_t c = 0; c < nc; c++)
+    chunks[coff + c] = {b, start + c * HYB_CHUNK, start + min(size, (c + 1u) * HYB_CHUNK), 0u};
+  for (uint32_t t = 0; t < nt; t++) tiles[toff + t] = {b, start + t * TILE, start + min(size, (t + 1u) * TILE), toff};
+  if (b == 0) {
+    hs[0] = tc;
+    hs[1] = tt;
+    hs[2] = 0;
+  }
+}
+
+// Second-digit histogram of each top bucket, one chunk per workgroup (surplus workgroups exit).
+template <typename K, int P>
+__global__ __launch_bounds__(BLOCK) void hyb_hist(const K* keys, const HybChunk* chunks, const uint32_t* hs,
+                                                  uint32_t* seg_hist) {
+  __shared__ uint32_t h[RADIX];
+  if (blockIdx.x >= hs[0]) return;
+  const HybChunk c = chunks[blockIdx.x];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  constexpr int U = 8;
+  for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
+    K k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) k[u] = keys[min(base + uint32_t(u) * BLOCK, c.end - 1u)];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * (P - 2))) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  const uint32_t v = h[threadIdx.x];
+  if (v) atomicAdd(&seg_hist[c.seg * RADIX + threadIdx.x], v);
+}
+
+// One workgroup per top bucket: the absolute start of each of its second-digit buckets, and the
+// largest bucket.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void hyb_seg_plan(const SortCtl* ctl, const uint32_t* seg_hist,
+                                                      uint32_t* seg_base, uint32_t* hs) {
+  __shared__ uint32_t tmp[WAVES];
+  const uint32_t seg = blockIdx.x;
+  const uint32_t v = seg_hist[seg * RADIX + threadIdx.x];
+  uint32_t total;
+  const uint32_t ex = block_excl_scan(v, tmp, total);
+  seg_base[seg * RADIX + threadIdx.x] = ctl->digit_base[P - 1][seg] + ex;
+  if (v) atomicMax(&hs[2], v);
+}
+
+// The segmented pass: every top bucket partitioned stably by the second digit, from (k0, r0) into
+// (k1, r1). Tile ids come from a counter; blocks past the last tile exit.
+template <typename K, int P, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void hyb_seg_pass(SortCtl* ctl, const HybTile* tiles, const uint32_t* hs,
+                                                      const uint32_t* seg_base, unsigned long long* status,
+                                                      uint32_t epoch, const K* in_keys, const uint32_t* in_rows,
+                                                      K* out_keys, uint32_t* out_rows) {
+  constexpr int TILE = BLOCK * ITEMS;
+  __shared__ uint32_t wc[WAVES][WC_STRIDE];
+  __shared__ uint32_t tile_total[WC_STRIDE];
+  __shared__ uint32_t bin_start[RADIX];
+  __shared__ uint32_t bin_base[WC_STRIDE];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ uint8_t s_digit[TILE];
+  __shared__ K s_stage[TILE];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[10], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  if (tile >= hs[1]) return;   // uniform over the workgroup
+  const HybTile t = tiles[tile];
+  const int shift = 8 * (P - 2);
+  K keys[ITEMS];
+  uint32_t rows[ITEMS], bins[ITEMS];
+  const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the tile read its last row)
+    const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
+    keys[j] = in_keys[e];
+    rows[j] = in_rows[e];
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++)
+    bins[j] = base_e + uint32_t(j) * 64u < t.end ? uint32_t(keys[j] >> shift) & 0xFFu : uint32_t(BIN_NONE);
+  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
+  __syncthreads();
+  const uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
+  look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
+  __syncthreads();
+  bin_base[threadIdx.x] += seg_base[t.seg * RADIX + threadIdx.x];
+  __syncthreads();
+  scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount, out_keys,
+                                 out_rows);
+}
+
+// Sorts one bucket of at most LI * BLOCK rows in LDS by the digits below the top two (the active
+// ones, least significant first) and writes its rows to the output.
+template <typename K, int LI>
+__device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t* rows_in, uint32_t off,
+                                               uint32_t size, uint32_t low_mask, uint32_t* out_idx,
+                                               uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
+                                               uint32_t* tmp, K* s_keys, uint32_t* s_rows) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
+  K keys[LI];
+  uint32_t rows[LI], bins[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
+    keys[j] = keys_in[off + e];
+    rows[j] = rows_in[off + e];
+  }
+  for (int d = 0; d < 8; d++) {
+    if (!((low_mask >> d) & 1u)) continue;   // uniform
+    for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LI; j++)
+      bins[j] = base_e + uint32_t(j) * 64u < size ? uint32_t(keys[j] >> (8 * d)) & 0xFFu : uint32_t(BIN_NONE);
+    rank_tile<LI, 9>(bins, wc[w], lane);
+    __syncthreads();
+    tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LI; j++) {
+      const uint32_t b = bin_of(bins[j]);
+      if (b < RADIX) {
+        const uint32_t pos = bin_start[b] + wc[w][b] + rank_of(bins[j]);
+        s_keys[pos] = keys[j];
+        s_rows[pos] = rows[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LI; j++) {
+      const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
+      keys[j] = s_keys[e];
+      rows[j] = s_rows[e];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t e = base_e + uint32_t(j) * 64u;
+    if (e < size) out_idx[off + e] = rows[j];
+  }
+}
+
+// One workgroup per (top digit, second digit) bucket.
+template <typename K, int P>
+__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
+                                                   const uint32_t* seg_base, const K* keys, const uint32_t* rows,
+                                                   uint32_t* out_idx) {
+  __shared__ uint32_t wc[WAVES][WC_STRIDE];
+  __shared__ uint32_t tile_total[WC_STRIDE];
+  __shared__ uint32_t bin_start[RADIX];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ K s_keys[LOCAL_MAX];
+  __shared__ uint32_t s_rows[LOCAL_MAX];
+  const uint32_t size = seg_hist[blockIdx.x];
+  if (size == 0u || size > LOCAL_MAX) return;   // the host has checked that no bucket is larger
+  const uint32_t off = seg_base[blockIdx.x];
+  const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
+  uint32_t* out = out_idx + ctl->value_base;
+  if (size <= 4u * BLOCK)
+    hyb_local_body<K, 4>(keys, rows, off, size, low_mask, out, wc, tile_total, bin_start, tmp, s_keys, s_rows);
+  else
+    hyb_local_body<K, 8>(keys, rows, off, size, low_mask, out, wc, tile_total, bin_start, tmp, s_keys, s_rows);
+}
+
 // ---------------------------------------------------------------- host side
+
+// The hybrid sort of 64-bit keys (the analysed plan says the top two digits vary). Returns false,
+// with nothing written to out_idx that the LSD passes do not write again, when a bucket turns out
+// larger than the LDS sort takes.
+template <typename K, int W, int FITEMS, int ITEMS>
+bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
+                SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, K* k0, uint32_t* r0, K* k1,
+                uint32_t* r1, uint32_t* out_idx, hipStream_t s) {
+  constexpr int P = int(sizeof(K));
+  constexpr uint32_t FTILE = BLOCK * FITEMS, TILE = BLOCK * ITEMS;
+  const uint32_t ftiles = (n + FTILE - 1) / FTILE;
+  const uint32_t max_chunks = (m + HYB_CHUNK - 1) / HYB_CHUNK + uint32_t(RADIX);
+  const uint32_t max_tiles = (m + TILE - 1) / TILE + uint32_t(RADIX);
+  DevBuf chunks = device_alloc(size_t(max_chunks) * sizeof(HybChunk));
+  DevBuf tiles = device_alloc(size_t(max_tiles) * sizeof(HybTile));
+  DevBuf seg = device_alloc(size_t(RADIX) * RADIX * 4 * 2 + 64);
+  uint32_t* seg_hist = static_cast<uint32_t*>(seg->ptr);
+  uint32_t* seg_base = seg_hist + RADIX * RADIX;
+  uint32_t* hs = seg_base + RADIX * RADIX;
+  HybChunk* cp = static_cast<HybChunk*>(chunks->ptr);
+  HybTile* tp = static_cast<HybTile*>(tiles->ptr);
+  {
+    ProfScope ps("sort_msd_first");
+    hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+                       dsc, ctl, status, epoch + 10u, k0, r0, out_idx, P - 1);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_layout");
+    hipLaunchKernelGGL((hyb_layout<P, ITEMS>), dim3(1), dim3(BLOCK), 0, s, ctl, cp, tp, hs, seg_hist);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_hist");
+    hipLaunchKernelGGL((hyb_hist<K, P>), dim3(max_chunks), dim3(BLOCK), 0, s, k0, cp, hs, seg_hist);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_plan");
+    hipLaunchKernelGGL((hyb_seg_plan<P>), dim3(RADIX), dim3(BLOCK), 0, s, ctl, seg_hist, seg_base, hs);
+    AM_HIP(hipGetLastError());
+  }
+  uint32_t* hp = host_plan_words();
+  AM_HIP(hipMemcpyAsync(hp, hs + 2, 4, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  if (hp[0] > LOCAL_MAX) return false;
+  {
+    ProfScope ps("sort_msd_scatter");
+    hipLaunchKernelGGL((hyb_seg_pass<K, P, ITEMS>), dim3(max_tiles), dim3(BLOCK), 0, s, ctl, tp, hs, seg_base, status,
+                       epoch + 11u, k0, r0, k1, r1);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_local");
+    hipLaunchKernelGGL((hyb_local<K, P>), dim3(RADIX * RADIX), dim3(BLOCK), 0, s, ctl, seg_hist, seg_base, k1, r1,
+                       out_idx);
+    AM_HIP(hipGetLastError());
+  }
+  return true;
+}
 
 template <typename K, int W, int ITEMS>
 void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
@@ -379,7 +654,9 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   const uint32_t ftiles = (n + FTILE - 1) / FTILE;
   DevBuf ctl_buf = device_alloc(sizeof(SortCtl));
   SortCtl* ctl = static_cast<SortCtl*>(ctl_buf->ptr);
-  uint32_t epoch = reserve_status(size_t(std::max(tiles, ftiles)) * STATUS_STRIDE, s);
+  // Status rows for every tile of a pass; the hybrid sort's segmented pass has up to one more tile
+  // per top digit value.
+  uint32_t epoch = reserve_status(size_t(std::max(tiles + uint32_t(RADIX), ftiles)) * STATUS_STRIDE, s);
   unsigned long long* status = static_cast<unsigned long long*>(t_ws.status->ptr);
   retain(t_ws.status);
   DevBuf k0 = device_alloc(size_t(n) * sizeof(K)), k1 = device_alloc(size_t(n) * sizeof(K));
@@ -407,26 +684,35 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   }
   // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
   // that only active passes are launched, over the value rows only.
-  struct Plan { uint32_t mask, first, last, m; } plan{0xFFu, 0, uint32_t(P - 1), n};
+  struct Plan { uint32_t mask, first, last, m, top_max, second_max; } plan{0xFFu, 0, uint32_t(P - 1), n, 0, 0};
   const bool analyse = n >= (1u << 18) && !capturing(s);
   if (analyse) {
     Plan* hp = reinterpret_cast<Plan*>(host_plan_words());
-    hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp));
+    hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp), P);
     AM_HIP(hipGetLastError());
     AM_HIP(hipStreamSynchronize(s));
     plan = *hp;
-  }
-  {
-    ProfScope ps("sort_first");
-    hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, ctl, status, epoch, static_cast<K*>(k0->ptr), static_cast<uint32_t*>(r0->ptr),
-                       out_idx);
-    AM_HIP(hipGetLastError());
   }
   K* kp0 = static_cast<K*>(k0->ptr);
   K* kp1 = static_cast<K*>(k1->ptr);
   uint32_t* rp0 = static_cast<uint32_t*>(r0->ptr);
   uint32_t* rp1 = static_cast<uint32_t*>(r1->ptr);
+  if constexpr (P == 8) {
+    // The hybrid sort, when the top two digits are expected to cut the value rows into buckets the
+    // LDS sort takes (the estimate assumes the two digits independent; the exact largest bucket is
+    // checked before the local sort, and the LSD passes below run instead when it is too large).
+    if (analyse && n >= (1u << 22) && ((plan.mask >> 7) & 1u) && ((plan.mask >> 6) & 1u) &&
+        uint64_t(plan.top_max) * plan.second_max <= uint64_t(LOCAL_MAX) * 3u / 4u * plan.m &&
+        run_hybrid<K, W, FITEMS, ITEMS>(vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, kp0, rp0,
+                                         kp1, rp1, out_idx, s))
+      return;
+  }
+  {
+    ProfScope ps("sort_first");
+    hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+                       dsc, ctl, status, epoch, kp0, rp0, out_idx, -1);
+    AM_HIP(hipGetLastError());
+  }
   if (analyse) {
     // The host read the plan back: launch only the active passes, each over the value rows, with
     // the last one's kernel known (no key writes).

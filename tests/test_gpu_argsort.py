@@ -221,3 +221,66 @@ def test_large_tiles_first_pass_and_nulls(order):
         vals[(bits & np.uint64(1023)) == 9] = -0.0
     for desc, place in ((True, 0), (False, 1)):
         check(vals, valid, desc, place, order, label="large")
+
+
+def sort_with_kernels(vals, valid=None, desc=False, place=0, order=0):
+    """The GPU argsort with kernel timing on: (indices, names of the kernels it launched)."""
+    am.prof_reset()
+    am.prof_enable(True)
+    try:
+        got = gpu_argsort(vals, valid, desc, place, order)
+    finally:
+        am.prof_enable(False)
+    names = set(am.prof_report())
+    am.prof_reset()
+    return got, names
+
+
+def msd_case(case, n):
+    bits = O.splitmix64(0x7E57 + len(case), n)
+    if case == "uniform":
+        return bits.view(np.int64), None, "hybrid"
+    if case == "low_digits_constant":   # only the top two digits vary: the local sort has no pass
+        return ((bits & np.uint64(0xFFFF000000000000)) | np.uint64(0x1234)).view(np.int64), None, "hybrid"
+    if case == "correlated_top":        # second digit equals the top one: buckets of n/256 rows
+        top = bits >> np.uint64(56)
+        k = (top << np.uint64(56)) | (top << np.uint64(48)) | (bits & np.uint64((1 << 48) - 1))
+        return k.view(np.int64), None, "fallback"
+    if case == "duplicates":            # one key on 0.5% of the rows: one bucket above the LDS sort's limit
+        k = bits.copy()
+        k[(bits % np.uint64(200)) == 3] = np.uint64(0x0123456789ABCDEF)
+        return k.view(np.int64), None, "fallback"
+    if case == "float_bits":            # random bit patterns: every exponent, NaNs, infinities, -0.0
+        v = bits.view(np.float64).copy()
+        # Equal keys must fit one LDS bucket (2048 rows): about 450 rows of -0.0, and a quarter of
+        # the 1-in-2048 NaN patterns kept (nan_largest orders every NaN as one key).
+        v[(bits % np.uint64(9973)) == 5] = -0.0
+        drop = np.isnan(v) & ((bits >> np.uint64(3)) % np.uint64(4) != 0)
+        v[drop] = (bits[drop] >> np.uint64(11)).astype(np.float64)
+        valid = (bits >> np.uint64(20)) % np.uint64(33) != 0
+        return v, valid, "hybrid"
+    if case == "skewed":                # top digit nearly constant: the estimate sends it to the LSD passes
+        return (bits >> np.uint64(10)).view(np.int64), None, "lsd"
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["uniform", "low_digits_constant", "correlated_top", "duplicates",
+                                  "float_bits", "skewed"])
+def test_msd_hybrid_and_fallback(case):
+    """64-bit keys above 4M rows whose top two digits vary take the hybrid sort (a stable partition by
+    the top digit, a segmented partition by the second, an LDS sort of each bucket by the rest). A
+    bucket above the LDS sort's 2048 rows sends the sort back to the LSD passes; a skewed top digit
+    never starts the hybrid. Each case checks the path it took and the order against the oracle."""
+    n = 4_500_000
+    vals, valid, path = msd_case(case, n)
+    combos = [(False, 0, 0), (True, 1, 0)] + ([(False, 1, 1), (True, 0, 2)] if vals.dtype == np.float64 else [])
+    for desc, place, order in combos:
+        print(f"msd case={case} n={n} desc={desc} place={place} order={order}", flush=True)
+        got, names = sort_with_kernels(vals, valid, desc, place, order)
+        want = O.argsort(vals, valid, desc, place, order)
+        if not np.array_equal(got, want):
+            bad = int(np.nonzero(got != want)[0][0])
+            raise AssertionError(f"{case} desc={desc} place={place} order={order}: first difference at {bad}: "
+                                 f"got {got[bad]} want {want[bad]}")
+        took = ("hybrid" if "sort_msd_local" in names else "fallback" if "sort_msd_first" in names else "lsd")
+        assert took == path, f"{case}: took the {took} path, kernels {sorted(names)}"
