@@ -504,79 +504,120 @@ __global__ __launch_bounds__(BLOCK) void hyb_seg_pass(SortCtl* ctl, const HybTil
                                  out_rows);
 }
 
-// Sorts one bucket of at most LI * BLOCK rows in LDS by the digits below the top two (the active
-// ones, least significant first) and writes its rows to the output.
+constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
+
+// One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
+// in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
+template <typename K, int LI>
+__device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t (&pos)[LI], uint32_t base_e,
+                                            uint32_t size, int d, uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
+                                            uint32_t* bin_start, uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+  __syncthreads();
+  uint32_t bins[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    bins[j] = base_e + uint32_t(j) * 64u < size ? uint32_t(keys[j] >> (8 * d)) & 0xFFu : uint32_t(BIN_NONE);
+  rank_tile<LI, 9>(bins, wc[w], lane);
+  __syncthreads();
+  tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t b = bin_of(bins[j]);
+    if (b < RADIX) {
+      const uint32_t at = bin_start[b] + wc[w][b] + rank_of(bins[j]);
+      s_keys[at] = keys[j];
+      s_pos[at] = uint16_t(pos[j]);
+    }
+  }
+}
+
+// Sorts one bucket of at most LI * BLOCK rows by the digits below the top two and writes its rows to
+// the output. The rows of a bucket arrive in input order (both partitions before were stable), so the
+// stable order is the order of (key, position in the bucket) pairs, which are all distinct, and rows
+// are read only for the final write. One LDS pass splits the bucket by its most significant varying
+// digit; when every resulting sub-bucket has at most SUB_MAX rows, each row's place inside its
+// sub-bucket is the number of pairs there below its own. Otherwise (many equal keys) stable LSD
+// passes over every varying digit sort the bucket in LDS.
 template <typename K, int LI>
 __device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t* rows_in, uint32_t off,
                                                uint32_t size, uint32_t low_mask, uint32_t* out_idx,
                                                uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
-                                               uint32_t* tmp, K* s_keys, uint32_t* s_rows) {
+                                               uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
+  if (low_mask == 0u) {   // only the top two digits vary: the bucket is sorted already
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out_idx[off + p] = rows_in[off + p];
+    return;
+  }
   K keys[LI];
-  uint32_t rows[LI], bins[LI];
+  uint32_t pos[LI];
 #pragma unroll
   for (int j = 0; j < LI; j++) {
-    const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
-    keys[j] = keys_in[off + e];
-    rows[j] = rows_in[off + e];
+    pos[j] = base_e + uint32_t(j) * 64u;
+    keys[j] = keys_in[off + min(pos[j], size - 1u)];
   }
+  const int top = 31 - __builtin_clz(low_mask);
+  local_split<K, LI>(keys, pos, base_e, size, top, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+  if (!__syncthreads_or(tile_total[threadIdx.x] > SUB_MAX ? 1 : 0)) {
+    // Staged positions, one per thread in turn: consecutive lanes share sub-buckets.
+#pragma unroll 1
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) {
+      const K k = s_keys[p];
+      const uint32_t e = s_pos[p];
+      const uint32_t b = uint32_t(k >> (8 * top)) & 0xFFu;
+      const uint32_t lo = bin_start[b], hi = lo + tile_total[b];
+      uint32_t below = 0;
+      for (uint32_t q = lo; q < hi; q++) {
+        const K kq = s_keys[q];
+        below += (kq < k || (kq == k && s_pos[q] < e)) ? 1u : 0u;
+      }
+      out_idx[off + lo + below] = rows_in[off + e];
+    }
+    return;
+  }
+  // Many equal keys: stable LSD passes over every varying digit, from the input order.
   for (int d = 0; d < 8; d++) {
     if (!((low_mask >> d) & 1u)) continue;   // uniform
-    for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < LI; j++)
-      bins[j] = base_e + uint32_t(j) * 64u < size ? uint32_t(keys[j] >> (8 * d)) & 0xFFu : uint32_t(BIN_NONE);
-    rank_tile<LI, 9>(bins, wc[w], lane);
-    __syncthreads();
-    tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < LI; j++) {
-      const uint32_t b = bin_of(bins[j]);
-      if (b < RADIX) {
-        const uint32_t pos = bin_start[b] + wc[w][b] + rank_of(bins[j]);
-        s_keys[pos] = keys[j];
-        s_rows[pos] = rows[j];
-      }
-    }
+    local_split<K, LI>(keys, pos, base_e, size, d, wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < LI; j++) {
       const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
       keys[j] = s_keys[e];
-      rows[j] = s_rows[e];
+      pos[j] = s_pos[e];
     }
     __syncthreads();
   }
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t e = base_e + uint32_t(j) * 64u;
-    if (e < size) out_idx[off + e] = rows[j];
+    if (e < size) out_idx[off + e] = rows_in[off + pos[j]];
   }
 }
 
-// One workgroup per (top digit, second digit) bucket.
-template <typename K, int P>
-__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
+// One workgroup per (top digit, second digit) bucket; LI keys per thread, so buckets of at most
+// LI * BLOCK rows (the host picks LI from the largest bucket). LDS for the bucket is dynamic:
+// LI * BLOCK * (sizeof(K) + 2) bytes.
+template <typename K, int P, int LI>
+__global__ __launch_bounds__(BLOCK, LI <= 4 ? 8 : 4) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
                                                    const uint32_t* seg_base, const K* keys, const uint32_t* rows,
                                                    uint32_t* out_idx) {
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
   __shared__ uint32_t bin_start[RADIX];
   __shared__ uint32_t tmp[WAVES];
-  __shared__ K s_keys[LOCAL_MAX];
-  __shared__ uint32_t s_rows[LOCAL_MAX];
+  extern __shared__ __align__(16) unsigned char local_lds[];
+  K* s_keys = reinterpret_cast<K*>(local_lds);
+  uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
   const uint32_t size = seg_hist[blockIdx.x];
-  if (size == 0u || size > LOCAL_MAX) return;   // the host has checked that no bucket is larger
+  if (size == 0u || size > uint32_t(LI * BLOCK)) return;   // the host has checked that no bucket is larger
   const uint32_t off = seg_base[blockIdx.x];
   const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
-  uint32_t* out = out_idx + ctl->value_base;
-  if (size <= 4u * BLOCK)
-    hyb_local_body<K, 4>(keys, rows, off, size, low_mask, out, wc, tile_total, bin_start, tmp, s_keys, s_rows);
-  else
-    hyb_local_body<K, 8>(keys, rows, off, size, low_mask, out, wc, tile_total, bin_start, tmp, s_keys, s_rows);
+  hyb_local_body<K, LI>(keys, rows, off, size, low_mask, out_idx + ctl->value_base, wc, tile_total, bin_start, tmp,
+                        s_keys, s_pos);
 }
 
 // ---------------------------------------------------------------- host side
@@ -634,8 +675,16 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   }
   {
     ProfScope ps("sort_msd_local");
-    hipLaunchKernelGGL((hyb_local<K, P>), dim3(RADIX * RADIX), dim3(BLOCK), 0, s, ctl, seg_hist, seg_base, k1, r1,
-                       out_idx);
+    const uint32_t biggest = hp[0];
+    if (biggest <= 2u * BLOCK)
+      hipLaunchKernelGGL((hyb_local<K, P, 2>), dim3(RADIX * RADIX), dim3(BLOCK), 2 * BLOCK * (sizeof(K) + 2), s, ctl,
+                         seg_hist, seg_base, k1, r1, out_idx);
+    else if (biggest <= 4u * BLOCK)
+      hipLaunchKernelGGL((hyb_local<K, P, 4>), dim3(RADIX * RADIX), dim3(BLOCK), 4 * BLOCK * (sizeof(K) + 2), s, ctl,
+                         seg_hist, seg_base, k1, r1, out_idx);
+    else
+      hipLaunchKernelGGL((hyb_local<K, P, 8>), dim3(RADIX * RADIX), dim3(BLOCK), 8 * BLOCK * (sizeof(K) + 2), s, ctl,
+                         seg_hist, seg_base, k1, r1, out_idx);
     AM_HIP(hipGetLastError());
   }
   return true;
