@@ -508,7 +508,11 @@ constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
 
 // One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
 // in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
-template <typename K, int LI>
+// PACK (d at most 5): instead, one 64-bit word per row, digit << 56 | (key bits below digit d) <<
+// POS_BITS | position, lands in s_keys: inside one digit run, words order as (key, position) pairs do.
+constexpr int POS_BITS = 11;   // positions in a bucket of at most LOCAL_MAX rows
+static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
+template <typename K, int LI, bool PACK = false>
 __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t (&pos)[LI], uint32_t base_e,
                                             uint32_t size, int d, uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
                                             uint32_t* bin_start, uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
@@ -528,8 +532,13 @@ __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t 
     const uint32_t b = bin_of(bins[j]);
     if (b < RADIX) {
       const uint32_t at = bin_start[b] + wc[w][b] + rank_of(bins[j]);
-      s_keys[at] = keys[j];
-      s_pos[at] = uint16_t(pos[j]);
+      if (PACK) {
+        const unsigned long long low = (unsigned long long)keys[j] & ((1ull << (8 * d)) - 1ull);
+        s_keys[at] = K((static_cast<unsigned long long>(b) << 56) | (low << POS_BITS) | pos[j]);
+      } else {
+        s_keys[at] = keys[j];
+        s_pos[at] = uint16_t(pos[j]);
+      }
     }
   }
 }
@@ -559,23 +568,34 @@ __device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t*
     pos[j] = base_e + uint32_t(j) * 64u;
     keys[j] = keys_in[off + min(pos[j], size - 1u)];
   }
-  const int top = 31 - __builtin_clz(low_mask);
-  local_split<K, LI>(keys, pos, base_e, size, top, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+  const int top = 31 - __builtin_clz(low_mask);   // at most 5: 8 + 40 + POS_BITS bits per packed word
+  local_split<K, LI, true>(keys, pos, base_e, size, top, wc, tile_total, bin_start, tmp, s_keys, s_pos);
   if (!__syncthreads_or(tile_total[threadIdx.x] > SUB_MAX ? 1 : 0)) {
-    // Staged positions, one per thread in turn: consecutive lanes share sub-buckets.
-#pragma unroll 1
-    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) {
-      const K k = s_keys[p];
-      const uint32_t e = s_pos[p];
-      const uint32_t b = uint32_t(k >> (8 * top)) & 0xFFu;
-      const uint32_t lo = bin_start[b], hi = lo + tile_total[b];
-      uint32_t below = 0;
-      for (uint32_t q = lo; q < hi; q++) {
-        const K kq = s_keys[q];
-        below += (kq < k || (kq == k && s_pos[q] < e)) ? 1u : 0u;
+    // Staged positions, one per thread in turn (consecutive lanes share sub-buckets); every row load
+    // is issued before the first store.
+    uint32_t dest[LI], src[LI];
+#pragma unroll
+    for (int j = 0; j < LI; j++) {
+      const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+      dest[j] = ~0u;
+      src[j] = off;
+      if (p < size) {
+        const K word = s_keys[p];
+        const uint32_t b = uint32_t(word >> 56);
+        const uint32_t lo = bin_start[b], hi = lo + tile_total[b];
+        uint32_t below = 0;
+#pragma unroll 4
+        for (uint32_t q = lo; q < hi; q++) below += s_keys[q] < word ? 1u : 0u;
+        dest[j] = off + lo + below;
+        src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
       }
-      out_idx[off + lo + below] = rows_in[off + e];
     }
+    uint32_t r[LI];
+#pragma unroll
+    for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];
+#pragma unroll
+    for (int j = 0; j < LI; j++)
+      if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
     return;
   }
   // Many equal keys: stable LSD passes over every varying digit, from the input order.
@@ -591,10 +611,13 @@ __device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t*
     }
     __syncthreads();
   }
+  uint32_t r[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) r[j] = rows_in[off + min(pos[j], size - 1u)];
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t e = base_e + uint32_t(j) * 64u;
-    if (e < size) out_idx[off + e] = rows_in[off + pos[j]];
+    if (e < size) out_idx[off + e] = r[j];
   }
 }
 

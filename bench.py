@@ -10,9 +10,9 @@ Also measured and reported beside it (same JSON line):
   * group_by_50M_zipf_100k: am_group_by_keys over 50M int64 keys drawn Zipf(1.1) from 100,000
     distinct values, then int64 sum, count_all and count (am_group_agg_ex ops 0, 1, 2), and the float64
     min / max / sum of a second value column;
-  * roofline: the middle scatter pass of the argsort (the dominant kernel), algorithmic bytes per
-    launch (24 B per value row: read key + row, write key + row) over its mean launch time, measured
-    with HIP events on the library's own stream (am_prof_*);
+  * roofline: the dominant kernel of the step (the longest mean launch), its algorithmic bytes per
+    launch (orderby_alg_bytes) over its mean launch time, measured with HIP events on the library's
+    own stream (am_prof_*); kernels_roofline gives the same for every kernel of the step;
   * cpu_baseline: the CPU oracle's ORDER BY (an 8-bit LSD radix argsort + gather, one core) on the
     same 50M rows, timed on this host.
 
@@ -342,6 +342,38 @@ def committed_traffic(prefix: str):
     return None, None
 
 
+def orderby_alg_bytes(n: int, m: int):
+    """Algorithmic HBM bytes per launch of each kernel of the ORDER BY step (n rows, m non-null): every
+    input byte read once and every output byte written once, with a description."""
+    vb = (n + 7) // 8
+    return {
+        "take": (20 * n, "gather: read u32 index + f64 value, write f64; 20 B/row"),
+        "sort_upsweep": (8 * n + vb, "digit histograms: read key + validity; 8.125 B/row"),
+        "sort_msd_first": (8 * n + vb + 12 * m + 4 * (n - m),
+                           "top-digit partition: read key + validity, write key + row (null rows: index); ~20 B/row"),
+        "sort_msd_hist": (8 * m, "second-digit histograms of the partitioned keys: read key; 8 B/row"),
+        "sort_msd_scatter": (24 * m, "segmented second-digit partition: read key + row, write key + row; 24 B/row"),
+        "sort_msd_local": (16 * m, "LDS bucket sort: read key + row, write index; 16 B/row"),
+        "sort_first": (8 * n + vb + 12 * m + 4 * (n - m), "LSD first pass: read key + validity, write key + row"),
+        "sort_scatter": (24 * m, "LSD middle pass: read key + row, write key + row; 24 B/row"),
+        "sort_scatter_last": (16 * m, "LSD last pass: read key + row, write index; 16 B/row"),
+    }
+
+
+# rocprofv3's kernel names (as shortened in the committed PMC summary) of the profiled launch names.
+ROCPROF_NAME = {
+    "take": "take_kernel<u64, u32>",
+    "sort_upsweep": "sort_upsweep<u64, 8>",
+    "sort_msd_first": "sort_first<u64, 8, 16>",
+    "sort_msd_hist": "hyb_hist<u64, 8>",
+    "sort_msd_scatter": "hyb_seg_pass<u64, 8, 20>",
+    "sort_msd_local": "hyb_local<u64, 8, 4>",
+    "sort_first": "sort_first<u64, 8, 16>",
+    "sort_scatter": "sort_mid<u64, 20, 0>",
+    "sort_scatter_last": "sort_mid<u64, 20, 1>",
+}
+
+
 def kernel_ms(name: str, prof: dict):
     c, ms = prof.get(name, (0, 0.0))
     return (ms / c if c else None), c
@@ -397,10 +429,20 @@ def main():
     am.prof_enable(False)
     prof = am.prof_report()
     log("kernels:", json.dumps(prof))
-    scat_ms, scat_n = kernel_ms("sort_scatter", prof)
-    bytes_per_launch = 24 * m
-    traffic, traffic_kernel = committed_traffic("sort_mid<u64, ")
-    achieved = bytes_per_launch / (scat_ms * 1e-3) / 1e9 if scat_ms else None
+    # Roofline of the dominant kernel (the longest mean launch of the step), and of every other kernel
+    # of the step with an algorithmic byte count.
+    alg = orderby_alg_bytes(n, m)
+    rows_roof = {}
+    for kname, (count, total) in prof.items():
+        if kname in alg and count:
+            mean_ms = total / count
+            rows_roof[kname] = {"mean_ms": round(mean_ms, 4), "alg_bytes": alg[kname][0],
+                                "achieved_gbs": round(alg[kname][0] / (mean_ms * 1e-3) / 1e9, 1)}
+    dom = max(rows_roof, key=lambda kname: rows_roof[kname]["mean_ms"]) if rows_roof else None
+    dom_ms, dom_n = kernel_ms(dom, prof) if dom else (None, 0)
+    bytes_per_launch = alg[dom][0] if dom else None
+    traffic, traffic_kernel = committed_traffic(ROCPROF_NAME[dom]) if dom in ROCPROF_NAME else (None, None)
+    achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9 if dom_ms else None
 
     # Parity of the timed step against the CPU oracle (and the CPU baseline timing).
     idx, _ = k.argsort(True, "at_end", "ieee").to_numpy()
@@ -486,8 +528,10 @@ def main():
     if not args.skip_serving:
         sv_res = serving_leg(args, log)
 
-    b_sort = 8 * n + (8 * n + 12 * m + 4 * (n - m)) + 6 * 2 * m * 12 + m * 16 + 2 * ((n + 7) // 8)
+    # Algorithmic bytes of one step: the sort kernels the profiled steps launched (3 steps), and the gather.
+    b_sort = sum(alg[kname][0] * prof[kname][0] // 3 for kname in rows_roof if kname != "take")
     b_take = 20 * n
+    hybrid = "sort_msd_local" in prof
     result = {
         "metric": METRIC,
         "value": n / (ms * 1e-3),
@@ -504,16 +548,19 @@ def main():
         "data": "synthetic (SplitMix64 streams; keys uniform over [-2^62, 2^62), 2% nulls, float64 payload)",
         "config": {"workload": "order_by_50M_int64_desc_nulls_last_then_gather_f64", "rows": n,
                    "null_fraction": round(1 - m / n, 5), "descending": True, "null_placement": "at_end",
-                   "float_order": "ieee", "digit_bits": 8, "passes": 8},
+                   "float_order": "ieee", "digit_bits": 8,
+                   "sort": ("hybrid: top-digit partition, segmented second-digit partition, LDS bucket sort"
+                            if hybrid else "LSD onesweep, 8 passes")},
         "step_hbm_frac_of_b_alg": (b_sort + b_take) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "b_alg_bytes": {"sort": b_sort, "gather": b_take},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "traffic_source": (f"{os.path.relpath(PMC_SUMMARY, ROOT)} ({traffic_kernel}, FETCH_SIZE + "
                                         "WRITE_SIZE scaled by a 1 GiB copy)") if traffic else None,
-                     "kernel": "sort_scatter (a middle radix pass: read key + row, write key + row)",
-                     "bytes_per_launch": bytes_per_launch, "mean_launch_ms": scat_ms,
-                     "launches": scat_n},
+                     "kernel": f"{dom} ({alg[dom][1]})" if dom else None,
+                     "bytes_per_launch": bytes_per_launch, "mean_launch_ms": dom_ms,
+                     "launches": dom_n},
+        "kernels_roofline": rows_roof,
         "kernels_ms": {kname: round(v[1] / max(v[0], 1), 4) for kname, v in prof.items()},
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,
