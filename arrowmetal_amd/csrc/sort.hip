@@ -72,15 +72,26 @@ __global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out, int P) {
 }
 
 
+// Rows per unit of the top-digit partition (a power of two): the upsweep counts the top digit, NaN
+// and null rows per unit when given unit_hist ([NBINS_FIRST][nunits], bin-major), so that the
+// partition knows every unit's destinations before it starts.
+constexpr uint32_t UNIT_SHIFT = 14, UNIT_ROWS = 1u << UNIT_SHIFT;
+constexpr uint32_t UNITS_PER_BLOCK = 8;   // most units one upsweep workgroup counts
+
 template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const uint8_t* valid, int64_t voff,
                                                       uint32_t n, int type, int forder, int desc,
-                                                      uint32_t rows_per_block, SortCtl* ctl) {
+                                                      uint32_t rows_per_block, SortCtl* ctl, uint32_t* unit_hist,
+                                                      uint32_t nunits) {
   constexpr int P = int(sizeof(K));
   __shared__ uint32_t h[P][RADIX];
+  __shared__ uint32_t hu[UNITS_PER_BLOCK][NBINS_FIRST];
   __shared__ uint32_t cnt[3];
   __shared__ unsigned long long s_or, s_and;
+  const bool units = unit_hist != nullptr;   // uniform
   for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) (&h[0][0])[i] = 0u;
+  if (units)
+    for (int i = threadIdx.x; i < int(UNITS_PER_BLOCK * NBINS_FIRST); i += BLOCK) (&hu[0][0])[i] = 0u;
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0u;
   if (threadIdx.x == 0) { s_or = 0ull; s_and = ~0ull; }
   __syncthreads();
@@ -110,17 +121,22 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
     }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
+      const uint32_t ui = (base + uint32_t(u) * BLOCK + threadIdx.x - begin) >> UNIT_SHIFT;
       if (cc[u] == 0) {
         K k = kk[u];
         cm++;
         o |= k;
         a &= k;
 #pragma unroll
-        for (int p = 0; p < P; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+        for (int p = 0; p < P - 1; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+        const uint32_t top = uint32_t(k >> (8 * (P - 1))) & 0xFFu;
+        atomicAdd(units ? &hu[ui & (UNITS_PER_BLOCK - 1)][top] : &h[P - 1][top], 1u);
       } else if (cc[u] == 1) {
         cq++;
+        if (units) atomicAdd(&hu[ui & (UNITS_PER_BLOCK - 1)][BIN_NAN], 1u);
       } else if (cc[u] == 2) {
         cz++;
+        if (units) atomicAdd(&hu[ui & (UNITS_PER_BLOCK - 1)][BIN_NULL], 1u);
       }
     }
   }
@@ -143,6 +159,20 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
     atomicAnd(&s_and, a64);
   }
   __syncthreads();
+  if (units) {
+    // The block's top-digit histogram is the sum of its units'; every unit's counts go out bin-major.
+    const uint32_t u0 = begin >> UNIT_SHIFT;
+    const uint32_t nu = (end - begin + UNIT_ROWS - 1) >> UNIT_SHIFT;
+    for (uint32_t b = threadIdx.x; b < uint32_t(NBINS_FIRST); b += BLOCK) {
+      uint32_t sum = 0;
+      for (uint32_t ui = 0; ui < nu; ui++) {
+        sum += hu[ui][b];
+        unit_hist[size_t(b) * nunits + u0 + ui] = hu[ui][b];
+      }
+      if (b < uint32_t(RADIX)) h[P - 1][b] = sum;
+    }
+    __syncthreads();
+  }
   const uint32_t x = xcd_id();
   uint32_t* xh = &ctl->xhist[x][0][0];
   for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) {
@@ -375,6 +405,113 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
   }
 }
 
+// ---------------------------------------------------------------- top-digit partition by units
+//
+// The hybrid sort's first pass, without look-back: the upsweep counted every unit's top digits, NaN
+// and null rows, so one workgroup per bin scans those counts into every unit's destinations
+// (msd_unit_scan), and persistent workgroups then partition whole units, tile after tile, keeping a
+// running destination per bin in LDS (msd_part). Units are taken from a counter, in any order.
+
+// unit_base[b][u] = the first destination of bin b's rows of unit u: inside the value block for
+// digits (positions in the partitioned key array), absolute output slots for NaN and null rows.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void msd_unit_scan(const SortCtl* ctl, const uint32_t* unit_hist,
+                                                       uint32_t nunits, uint32_t* unit_base) {
+  __shared__ uint32_t tmp[WAVES];
+  const uint32_t b = blockIdx.x;
+  uint32_t carry = b < uint32_t(RADIX) ? ctl->digit_base[P - 1][b] : (b == uint32_t(BIN_NAN) ? ctl->nan_base
+                                                                                                 : ctl->null_base);
+  for (uint32_t u0 = 0; u0 < nunits; u0 += BLOCK) {
+    const uint32_t u = u0 + threadIdx.x;
+    const uint32_t v = u < nunits ? unit_hist[size_t(b) * nunits + u] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(v, tmp, total);
+    if (u < nunits) unit_base[size_t(b) * nunits + u] = carry + ex;
+    carry += total;
+  }
+}
+
+template <typename K, int W, int ITEMS>
+__global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uint8_t* valid, int64_t voff,
+                                                     uint32_t n, int type, int forder, int desc, SortCtl* ctl,
+                                                     const uint32_t* unit_base, uint32_t nunits, K* out_keys,
+                                                     uint32_t* out_rows, uint32_t* out_idx) {
+  constexpr int P = int(sizeof(K));
+  constexpr int TILE = BLOCK * ITEMS;
+  static_assert(UNIT_ROWS % TILE == 0, "units hold whole tiles");
+  __shared__ uint32_t wc[WAVES][WC_STRIDE];
+  __shared__ uint32_t tile_total[WC_STRIDE];
+  __shared__ uint32_t bin_start[RADIX];
+  __shared__ uint32_t bin_base[WC_STRIDE];
+  __shared__ uint32_t running[WC_STRIDE];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ uint32_t s_unit;
+  __shared__ uint8_t s_digit[TILE];
+  __shared__ K s_stage[TILE];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int shift = 8 * (P - 1);
+  const uint32_t rows[1] = {0};   // rows are base_e + 64 * j (ROWS_IOTA)
+  while (true) {
+    if (threadIdx.x == 0) s_unit = atomicAdd(&ctl->tile_counter[11], 1u);
+    __syncthreads();
+    const uint32_t unit = s_unit;
+    if (unit >= nunits) return;   // uniform: every workgroup ends here once the units run out
+    for (int b = threadIdx.x; b < NBINS_FIRST; b += BLOCK) running[b] = unit_base[size_t(b) * nunits + unit];
+    const uint32_t begin = unit << UNIT_SHIFT;
+    const uint32_t end = min(n, begin + UNIT_ROWS);
+    for (uint32_t t0 = begin; t0 < end; t0 += TILE) {
+      for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+      __syncthreads();
+      K keys[ITEMS];
+      uint32_t bins[ITEMS], vbit[ITEMS];
+      const uint32_t base_e = t0 + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+#pragma unroll
+      for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the unit read its last row)
+        const uint32_t e = min(base_e + uint32_t(j) * 64u, end - 1u);
+        uint32_t bit;
+        const uint32_t vb = load_valid_byte(valid, voff, e, bit);
+        vbit[j] = (vb >> bit) & 1u;
+        keys[j] = K(load_raw<W>(vals, e));
+      }
+#pragma unroll
+      for (int j = 0; j < ITEMS; j++) {
+        const uint32_t e = base_e + uint32_t(j) * 64u;
+        uint32_t b = BIN_NONE;
+        const uint64_t raw = uint64_t(keys[j]);
+        keys[j] = 0;
+        if (e < end) {
+          K k;
+          const int c = classify_raw<K>(raw, vbit[j] != 0u, type, forder, desc, k);
+          if (c == 0) {
+            keys[j] = k;
+            b = uint32_t(k >> shift) & 0xFFu;
+          } else {
+            b = c == 1 ? BIN_NAN : BIN_NULL;
+          }
+        }
+        bins[j] = b;
+      }
+      rank_tile<ITEMS, 9>(bins, wc[w], lane);
+      __syncthreads();
+      const uint32_t vcount = tile_bin_layout<NBINS_FIRST>(wc, tile_total, bin_start, tmp);
+      for (int b = threadIdx.x; b < NBINS_FIRST; b += BLOCK) {
+        bin_base[b] = running[b];
+        running[b] += tile_total[b];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < ITEMS; j++) {   // NaN and null rows go straight to their final slots
+        const uint32_t b = bin_of(bins[j]);
+        if (b == BIN_NAN || b == BIN_NULL)
+          out_idx[bin_base[b] + wc[w][b] + rank_of(bins[j])] = base_e + uint32_t(j) * 64u;
+      }
+      scatter_values<K, ITEMS, true, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
+                                           out_keys, out_rows, base_e);
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- hybrid MSD sort (64-bit keys)
 //
 // For large inputs of 64-bit keys whose top two digits spread the rows well: the first pass
@@ -383,6 +520,9 @@ __global__ __launch_bounds__(BLOCK) void sort_mid(int pass, SortCtl* ctl, unsign
 // at 50M uniform keys, at most LOCAL_MAX) is then sorted in LDS by the remaining digits and its rows
 // written to their final slots. Three passes over the rows instead of eight.
 
+#ifndef AM_MSD_FITEMS
+#define AM_MSD_FITEMS 16   // keys per thread of the top-digit partition
+#endif
 constexpr uint32_t HYB_CHUNK = 65536;   // rows per histogram chunk of the segmented pass
 constexpr uint32_t LOCAL_MAX = 2048;    // largest bucket the LDS sort takes
 
@@ -651,10 +791,11 @@ __global__ __launch_bounds__(BLOCK, LI <= 4 ? 8 : 4) void hyb_local(const SortCt
 template <typename K, int W, int FITEMS, int ITEMS>
 bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
                 SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, K* k0, uint32_t* r0, K* k1,
-                uint32_t* r1, uint32_t* out_idx, hipStream_t s) {
+                uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base, uint32_t nunits,
+                hipStream_t s) {
   constexpr int P = int(sizeof(K));
-  constexpr uint32_t FTILE = BLOCK * FITEMS, TILE = BLOCK * ITEMS;
-  const uint32_t ftiles = (n + FTILE - 1) / FTILE;
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  (void)status;
   const uint32_t max_chunks = (m + HYB_CHUNK - 1) / HYB_CHUNK + uint32_t(RADIX);
   const uint32_t max_tiles = (m + TILE - 1) / TILE + uint32_t(RADIX);
   DevBuf chunks = device_alloc(size_t(max_chunks) * sizeof(HybChunk));
@@ -666,9 +807,15 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   HybChunk* cp = static_cast<HybChunk*>(chunks->ptr);
   HybTile* tp = static_cast<HybTile*>(tiles->ptr);
   {
+    ProfScope ps("sort_msd_unit_scan");
+    hipLaunchKernelGGL(msd_unit_scan<P>, dim3(NBINS_FIRST), dim3(BLOCK), 0, s, ctl, unit_hist, nunits, unit_base);
+    AM_HIP(hipGetLastError());
+  }
+  {
     ProfScope ps("sort_msd_first");
-    hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, ctl, status, epoch + 10u, k0, r0, out_idx, P - 1);
+    const uint32_t workers = std::min(nunits, uint32_t(device().cu_count) * 3u);
+    hipLaunchKernelGGL((msd_part<K, W, FITEMS>), dim3(workers), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+                       dsc, ctl, unit_base, nunits, k0, r0, out_idx);
     AM_HIP(hipGetLastError());
   }
   {
@@ -739,14 +886,26 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     hipLaunchKernelGGL(sort_init, dim3(1), dim3(BLOCK), 0, s, ctl);
     AM_HIP(hipGetLastError());
   }
+  // Large 64-bit inputs may take the hybrid sort: the upsweep then also counts its units.
+  const bool hybrid_ok = P == 8 && n >= (1u << 22) && !capturing(s);
+  const uint32_t nunits = hybrid_ok ? (n + UNIT_ROWS - 1) / UNIT_ROWS : 0u;
+  DevBuf units_buf;
+  uint32_t* unit_hist = nullptr;
+  if (hybrid_ok) {
+    units_buf = device_alloc(size_t(nunits) * NBINS_FIRST * 4 * 2);
+    unit_hist = static_cast<uint32_t*>(units_buf->ptr);
+  }
   {
     uint32_t max_blocks = uint32_t(device().cu_count) * 4u;
     uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 8191u) / 8192u));
-    uint32_t rpb = ((n + blocks - 1) / blocks + 2047u) / 2048u * 2048u;
+    // Block ranges are whole units when units are counted, at most UNITS_PER_BLOCK of them.
+    const uint32_t align = hybrid_ok ? UNIT_ROWS : 2048u;
+    uint32_t rpb = ((n + blocks - 1) / blocks + align - 1) / align * align;
+    if (hybrid_ok) rpb = std::min(rpb, UNITS_PER_BLOCK * UNIT_ROWS);
     blocks = (n + rpb - 1) / rpb;
     ProfScope ps("sort_upsweep");
     hipLaunchKernelGGL((sort_upsweep<K, W>), dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, rpb, ctl);
+                       dsc, rpb, ctl, unit_hist, nunits);
     AM_HIP(hipGetLastError());
   }
   {
@@ -773,10 +932,11 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     // The hybrid sort, when the top two digits are expected to cut the value rows into buckets the
     // LDS sort takes (the estimate assumes the two digits independent; the exact largest bucket is
     // checked before the local sort, and the LSD passes below run instead when it is too large).
-    if (analyse && n >= (1u << 22) && ((plan.mask >> 7) & 1u) && ((plan.mask >> 6) & 1u) &&
+    if (hybrid_ok && analyse && ((plan.mask >> 7) & 1u) && ((plan.mask >> 6) & 1u) &&
         uint64_t(plan.top_max) * plan.second_max <= uint64_t(LOCAL_MAX) * 3u / 4u * plan.m &&
-        run_hybrid<K, W, FITEMS, ITEMS>(vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, kp0, rp0,
-                                         kp1, rp1, out_idx, s))
+        run_hybrid<K, W, AM_MSD_FITEMS, ITEMS>(
+            vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, kp0, rp0, kp1, rp1, out_idx,
+            unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, s))
       return;
   }
   {
