@@ -33,7 +33,7 @@ struct SortCtl {
   uint32_t hist[8][RADIX];          // digit histograms of the value keys, one per pass
   uint32_t m, q, z, pad0;           // value rows, NaN rows (separate block), null rows
   unsigned long long key_or, key_and;
-  uint32_t tile_counter[16];        // [0] first pass, [1 + p] pass p
+  uint32_t tile_counter[16];        // [0] first pass, [1 + p] pass p, [10] segmented pass, [11] units
   uint32_t digit_base[8][RADIX];    // exclusive digit prefix inside the value block, per pass
   uint32_t active_mask, first_active, last_active, pad1;
   uint32_t value_base, nan_base, null_base, pad2;
@@ -250,8 +250,7 @@ template <typename K, int W, int ITEMS>
 __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const uint8_t* valid, int64_t voff,
                                                     uint32_t n, int type, int forder, int desc,
                                                     SortCtl* ctl, unsigned long long* status, uint32_t epoch,
-                                                    K* out_keys, uint32_t* out_rows, uint32_t* out_idx,
-                                                    int msd_digit) {
+                                                    K* out_keys, uint32_t* out_rows, uint32_t* out_idx) {
   constexpr int TILE = BLOCK * ITEMS;
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
@@ -264,13 +263,11 @@ __global__ __launch_bounds__(BLOCK, 3) void sort_first(const void* vals, const u
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  // msd_digit >= 0: the first pass of the hybrid sort partitions by that (top) digit and always writes
-  // keys; it counts its tiles apart from the LSD first pass.
-  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[msd_digit >= 0 ? 9 : 0], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[0], 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t first = msd_digit >= 0 ? uint32_t(msd_digit) : ctl->first_active;
-  const bool last = msd_digit < 0 && first == ctl->last_active;
+  const uint32_t first = ctl->first_active;
+  const bool last = first == ctl->last_active;
   const int shift = int(8 * first);
 
   K keys[ITEMS];
@@ -515,10 +512,11 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 // ---------------------------------------------------------------- hybrid MSD sort (64-bit keys)
 //
 // For large inputs of 64-bit keys whose top two digits spread the rows well: the first pass
-// partitions the value rows stably by the top digit (sort_first with msd_digit); a segmented pass
-// partitions each top bucket stably by the second digit; every resulting bucket (a few hundred rows
-// at 50M uniform keys, at most LOCAL_MAX) is then sorted in LDS by the remaining digits and its rows
-// written to their final slots. Three passes over the rows instead of eight.
+// partitions the value rows stably by the top digit (msd_part, above); a segmented pass partitions
+// each top bucket stably by the second digit (onesweep tiles whose look-back stops at their
+// segment's first tile); every resulting bucket (a few hundred rows at 50M uniform keys, at most
+// LOCAL_MAX) is then sorted in LDS by the remaining digits and its rows written to their final
+// slots. Three passes over the rows instead of eight.
 
 #ifndef AM_MSD_FITEMS
 #define AM_MSD_FITEMS 16   // keys per thread of the top-digit partition
@@ -942,7 +940,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   {
     ProfScope ps("sort_first");
     hipLaunchKernelGGL((sort_first<K, W, FITEMS>), dim3(ftiles), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, ctl, status, epoch, kp0, rp0, out_idx, -1);
+                       dsc, ctl, status, epoch, kp0, rp0, out_idx);
     AM_HIP(hipGetLastError());
   }
   if (analyse) {
