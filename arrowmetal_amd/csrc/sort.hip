@@ -522,7 +522,7 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 #define AM_MSD_FITEMS 16   // keys per thread of the top-digit partition
 #endif
 constexpr uint32_t HYB_CHUNK = 65536;   // rows per histogram chunk of the segmented pass
-constexpr uint32_t LOCAL_MAX = 2048;    // largest bucket the LDS sort takes
+constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes
 
 struct HybChunk {
   uint32_t seg, start, end, pad;
@@ -532,7 +532,7 @@ struct HybTile {
 };
 
 // One workgroup: the chunk and tile tables of the segmented pass, one segment per top digit value.
-// hs[0] = chunks, hs[1] = tiles, hs[2] = largest bucket (filled later).
+// hs[0] = chunks, hs[1] = tiles, hs[2] = largest bucket (filled later), hs[3..4] = the LSD queue.
 template <int P, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void hyb_layout(const SortCtl* ctl, HybChunk* chunks, HybTile* tiles,
                                                     uint32_t* hs, uint32_t* seg_hist) {
@@ -554,6 +554,8 @@ __global__ __launch_bounds__(BLOCK) void hyb_layout(const SortCtl* ctl, HybChunk
     hs[0] = tc;
     hs[1] = tt;
     hs[2] = 0;
+    hs[3] = 0;   // buckets queued for the LSD bucket sort
+    hs[4] = 0;   // the next queued bucket to take
   }
 }
 
@@ -646,11 +648,7 @@ constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
 
 // One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
 // in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
-// PACK (d at most 5): instead, one 64-bit word per row, digit << 56 | (key bits below digit d) <<
-// POS_BITS | position, lands in s_keys: inside one digit run, words order as (key, position) pairs do.
-constexpr int POS_BITS = 11;   // positions in a bucket of at most LOCAL_MAX rows
-static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
-template <typename K, int LI, bool PACK = false>
+template <typename K, int LI>
 __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t (&pos)[LI], uint32_t base_e,
                                             uint32_t size, int d, uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
                                             uint32_t* bin_start, uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
@@ -670,35 +668,21 @@ __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t 
     const uint32_t b = bin_of(bins[j]);
     if (b < RADIX) {
       const uint32_t at = bin_start[b] + wc[w][b] + rank_of(bins[j]);
-      if (PACK) {
-        const unsigned long long low = (unsigned long long)keys[j] & ((1ull << (8 * d)) - 1ull);
-        s_keys[at] = K((static_cast<unsigned long long>(b) << 56) | (low << POS_BITS) | pos[j]);
-      } else {
-        s_keys[at] = keys[j];
-        s_pos[at] = uint16_t(pos[j]);
-      }
+      s_keys[at] = keys[j];
+      s_pos[at] = uint16_t(pos[j]);
     }
   }
 }
 
-// Sorts one bucket of at most LI * BLOCK rows by the digits below the top two and writes its rows to
-// the output. The rows of a bucket arrive in input order (both partitions before were stable), so the
-// stable order is the order of (key, position in the bucket) pairs, which are all distinct, and rows
-// are read only for the final write. One LDS pass splits the bucket by its most significant varying
-// digit; when every resulting sub-bucket has at most SUB_MAX rows, each row's place inside its
-// sub-bucket is the number of pairs there below its own. Otherwise (many equal keys) stable LSD
-// passes over every varying digit sort the bucket in LDS.
+// Stable LSD passes over every varying digit below the top two, in LDS, for one bucket of at most
+// LI * BLOCK rows (rows in input order), then its rows written to their final slots.
 template <typename K, int LI>
-__device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t* rows_in, uint32_t off,
-                                               uint32_t size, uint32_t low_mask, uint32_t* out_idx,
-                                               uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
-                                               uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
+__device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
+                                          uint32_t low_mask, uint32_t* out_idx, uint32_t (*wc)[WC_STRIDE],
+                                          uint32_t* tile_total, uint32_t* bin_start, uint32_t* tmp, K* s_keys,
+                                          uint16_t* s_pos) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
-  if (low_mask == 0u) {   // only the top two digits vary: the bucket is sorted already
-    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out_idx[off + p] = rows_in[off + p];
-    return;
-  }
   K keys[LI];
   uint32_t pos[LI];
 #pragma unroll
@@ -706,37 +690,6 @@ __device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t*
     pos[j] = base_e + uint32_t(j) * 64u;
     keys[j] = keys_in[off + min(pos[j], size - 1u)];
   }
-  const int top = 31 - __builtin_clz(low_mask);   // at most 5: 8 + 40 + POS_BITS bits per packed word
-  local_split<K, LI, true>(keys, pos, base_e, size, top, wc, tile_total, bin_start, tmp, s_keys, s_pos);
-  if (!__syncthreads_or(tile_total[threadIdx.x] > SUB_MAX ? 1 : 0)) {
-    // Staged positions, one per thread in turn (consecutive lanes share sub-buckets); every row load
-    // is issued before the first store.
-    uint32_t dest[LI], src[LI];
-#pragma unroll
-    for (int j = 0; j < LI; j++) {
-      const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
-      dest[j] = ~0u;
-      src[j] = off;
-      if (p < size) {
-        const K word = s_keys[p];
-        const uint32_t b = uint32_t(word >> 56);
-        const uint32_t lo = bin_start[b], hi = lo + tile_total[b];
-        uint32_t below = 0;
-#pragma unroll 4
-        for (uint32_t q = lo; q < hi; q++) below += s_keys[q] < word ? 1u : 0u;
-        dest[j] = off + lo + below;
-        src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
-      }
-    }
-    uint32_t r[LI];
-#pragma unroll
-    for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];
-#pragma unroll
-    for (int j = 0; j < LI; j++)
-      if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
-    return;
-  }
-  // Many equal keys: stable LSD passes over every varying digit, from the input order.
   for (int d = 0; d < 8; d++) {
     if (!((low_mask >> d) & 1u)) continue;   // uniform
     local_split<K, LI>(keys, pos, base_e, size, d, wc, tile_total, bin_start, tmp, s_keys, s_pos);
@@ -759,26 +712,118 @@ __device__ __forceinline__ void hyb_local_body(const K* keys_in, const uint32_t*
   }
 }
 
-// One workgroup per (top digit, second digit) bucket; LI keys per thread, so buckets of at most
-// LI * BLOCK rows (the host picks LI from the largest bucket). LDS for the bucket is dynamic:
-// LI * BLOCK * (sizeof(K) + 2) bytes.
+// The bucket sort: one workgroup per (top digit, second digit) bucket of at most LI * BLOCK rows
+// (the host picks LI from the largest bucket). The rows of a bucket arrive in input order (both
+// partitions before were stable), so the stable order is the order of (key, position in the bucket)
+// pairs, which are all distinct. The bucket is split by its most significant varying digit with LDS
+// counters (any order inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the
+// digit) << POS_BITS | position, so that inside one digit run words order as those pairs do; each
+// row's place in its run is the number of words there below its own. A bucket with a run longer
+// than SUB_MAX rows (many equal keys) is queued for the stable LSD passes of hyb_local_lsd instead.
+constexpr int POS_BITS = 12;   // positions in a bucket of at most LOCAL_MAX rows
+static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
 template <typename K, int P, int LI>
-__global__ __launch_bounds__(BLOCK, LI <= 4 ? 8 : 4) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
-                                                   const uint32_t* seg_base, const K* keys, const uint32_t* rows,
-                                                   uint32_t* out_idx) {
+__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
+                                                   const uint32_t* seg_base, const K* keys_in, const uint32_t* rows_in,
+                                                   uint32_t* out_idx, uint32_t* queue, uint32_t* queue_len) {
+  __shared__ uint32_t cnt[RADIX], start[RADIX], cursor[RADIX];
+  __shared__ uint32_t tmp[WAVES];
+  extern __shared__ __align__(16) unsigned char local_lds[];
+  K* s_words = reinterpret_cast<K*>(local_lds);   // LI * BLOCK words
+  const uint32_t bucket = blockIdx.x;
+  const uint32_t size = seg_hist[bucket];
+  if (size == 0u || size > uint32_t(LI * BLOCK)) return;   // the host has checked that no bucket is larger
+  const uint32_t off = seg_base[bucket];
+  const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
+  uint32_t* out = out_idx + ctl->value_base;
+  if (low_mask == 0u) {   // only the top two digits vary: the bucket is sorted already
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
+    return;
+  }
+  const int shift = 8 * (31 - __builtin_clz(low_mask));   // at most 40: 8 + 40 + POS_BITS bits per word
+  cnt[threadIdx.x] = 0u;
+  K keys[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&cnt[uint32_t(keys[j] >> shift) & 0xFFu], 1u);
+  __syncthreads();
+  const uint32_t c = cnt[threadIdx.x];
+  uint32_t total;
+  const uint32_t st = block_excl_scan(c, tmp, total);
+  start[threadIdx.x] = st;
+  cursor[threadIdx.x] = st;
+  if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
+    if (threadIdx.x == 0) queue[atomicAdd(queue_len, 1u)] = bucket;
+    return;
+  }
+  const unsigned long long low_bits = (1ull << shift) - 1ull;
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    if (p < size) {
+      const uint32_t d = uint32_t(keys[j] >> shift) & 0xFFu;
+      const uint32_t slot = atomicAdd(&cursor[d], 1u);
+      s_words[slot] = K((static_cast<unsigned long long>(d) << 56) |
+                        ((static_cast<unsigned long long>(keys[j]) & low_bits) << POS_BITS) | p);
+    }
+  }
+  __syncthreads();
+  uint32_t dest[LI], src[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    dest[j] = ~0u;
+    src[j] = off;
+    if (p < size) {
+      const K word = s_words[p];
+      const uint32_t d = uint32_t(word >> 56);
+      const uint32_t lo = start[d], hi = lo + cnt[d];
+      uint32_t below = 0;
+#pragma unroll 4
+      for (uint32_t q = lo; q < hi; q++) below += s_words[q] < word ? 1u : 0u;
+      dest[j] = off + lo + below;
+      src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
+    }
+  }
+  uint32_t r[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (dest[j] != ~0u) out[dest[j]] = r[j];
+}
+
+// The queued buckets, each by stable LSD passes in LDS: persistent workgroups take them from a
+// counter (qctl[0] = queued buckets, qctl[1] = the next one to take) until none is left. LDS for the
+// bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
+template <typename K, int P, int LI>
+__global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const SortCtl* ctl, const uint32_t* seg_hist,
+                                                       const uint32_t* seg_base, const K* keys, const uint32_t* rows,
+                                                       uint32_t* out_idx, const uint32_t* queue, uint32_t* qctl) {
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
   __shared__ uint32_t bin_start[RADIX];
   __shared__ uint32_t tmp[WAVES];
+  __shared__ uint32_t s_q;
   extern __shared__ __align__(16) unsigned char local_lds[];
   K* s_keys = reinterpret_cast<K*>(local_lds);
   uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
-  const uint32_t size = seg_hist[blockIdx.x];
-  if (size == 0u || size > uint32_t(LI * BLOCK)) return;   // the host has checked that no bucket is larger
-  const uint32_t off = seg_base[blockIdx.x];
+  const uint32_t queued = qctl[0];
   const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
-  hyb_local_body<K, LI>(keys, rows, off, size, low_mask, out_idx + ctl->value_base, wc, tile_total, bin_start, tmp,
-                        s_keys, s_pos);
+  while (true) {
+    if (threadIdx.x == 0) s_q = atomicAdd(&qctl[1], 1u);
+    __syncthreads();
+    const uint32_t qi = s_q;
+    if (qi >= queued) return;   // uniform: every workgroup ends here once the queue is empty
+    const uint32_t bucket = queue[qi];
+    const uint32_t size = seg_hist[bucket];
+    local_lsd<K, LI>(keys, rows, seg_base[bucket], size, low_mask, out_idx + ctl->value_base, wc, tile_total,
+                     bin_start, tmp, s_keys, s_pos);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -798,10 +843,11 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   const uint32_t max_tiles = (m + TILE - 1) / TILE + uint32_t(RADIX);
   DevBuf chunks = device_alloc(size_t(max_chunks) * sizeof(HybChunk));
   DevBuf tiles = device_alloc(size_t(max_tiles) * sizeof(HybTile));
-  DevBuf seg = device_alloc(size_t(RADIX) * RADIX * 4 * 2 + 64);
+  DevBuf seg = device_alloc(size_t(RADIX) * RADIX * 4 * 3 + 64);
   uint32_t* seg_hist = static_cast<uint32_t*>(seg->ptr);
   uint32_t* seg_base = seg_hist + RADIX * RADIX;
-  uint32_t* hs = seg_base + RADIX * RADIX;
+  uint32_t* queue = seg_base + RADIX * RADIX;
+  uint32_t* hs = queue + RADIX * RADIX;
   HybChunk* cp = static_cast<HybChunk*>(chunks->ptr);
   HybTile* tp = static_cast<HybTile*>(tiles->ptr);
   {
@@ -841,18 +887,36 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
                        epoch + 11u, k0, r0, k1, r1);
     AM_HIP(hipGetLastError());
   }
+  // Keys per thread from the largest bucket; the LSD kernel takes the buckets the counting sort
+  // queued (its workgroups exit at once when none was).
+  const uint32_t biggest = hp[0];
+  const int li = biggest <= 4u * BLOCK ? 4 : biggest <= 8u * BLOCK ? 8 : 16;
+  const dim3 grid(RADIX * RADIX), lsd_grid(uint32_t(device().cu_count) * 2u);
+  const size_t lds = size_t(li) * BLOCK * sizeof(K), lsd_lds = size_t(li) * BLOCK * (sizeof(K) + 2);
   {
     ProfScope ps("sort_msd_local");
-    const uint32_t biggest = hp[0];
-    if (biggest <= 2u * BLOCK)
-      hipLaunchKernelGGL((hyb_local<K, P, 2>), dim3(RADIX * RADIX), dim3(BLOCK), 2 * BLOCK * (sizeof(K) + 2), s, ctl,
-                         seg_hist, seg_base, k1, r1, out_idx);
-    else if (biggest <= 4u * BLOCK)
-      hipLaunchKernelGGL((hyb_local<K, P, 4>), dim3(RADIX * RADIX), dim3(BLOCK), 4 * BLOCK * (sizeof(K) + 2), s, ctl,
-                         seg_hist, seg_base, k1, r1, out_idx);
+    if (li == 4)
+      hipLaunchKernelGGL((hyb_local<K, P, 4>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
+                         queue, hs + 3);
+    else if (li == 8)
+      hipLaunchKernelGGL((hyb_local<K, P, 8>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
+                         queue, hs + 3);
     else
-      hipLaunchKernelGGL((hyb_local<K, P, 8>), dim3(RADIX * RADIX), dim3(BLOCK), 8 * BLOCK * (sizeof(K) + 2), s, ctl,
-                         seg_hist, seg_base, k1, r1, out_idx);
+      hipLaunchKernelGGL((hyb_local<K, P, 16>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
+                         queue, hs + 3);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_local_lsd");
+    if (li == 4)
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1, r1,
+                         out_idx, queue, hs + 3);
+    else if (li == 8)
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1, r1,
+                         out_idx, queue, hs + 3);
+    else
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1,
+                         r1, out_idx, queue, hs + 3);
     AM_HIP(hipGetLastError());
   }
   return true;

@@ -250,9 +250,20 @@ def msd_case(case, n):
         k = bits.copy()
         k[(bits % np.uint64(200)) == 3] = np.uint64(0x0123456789ABCDEF)
         return k.view(np.int64), None, "fallback"
+    if case == "repeats":               # 2,000 keys repeated 60 times: runs the bucket LSD queue
+        k = bits.copy()
+        pick = (bits >> np.uint64(7)) % np.uint64(16) == 0
+        k[pick] = O.splitmix64(0x5EED, 2000)[(bits[pick] >> np.uint64(20)) % np.uint64(2000)]
+        return k.view(np.int64), None, "hybrid"
+    if case == "wide_buckets":          # top 16 bits from 1,500 values: ~3,000-row buckets, 16 keys per thread
+        tops = np.unique(O.splitmix64(0x70B5, 1600) >> np.uint64(48))[:1500] << np.uint64(48)
+        k = tops[(bits >> np.uint64(3)) % np.uint64(len(tops))] | (bits & np.uint64(0x0000FFFFFFFFFFFF))
+        return k.view(np.int64), None, "hybrid"
+    if case == "half_range":            # the ORDER BY keys: [-2^62, 2^62), 128 top digits, ~1,500-row buckets
+        return ((bits >> np.uint64(1)).view(np.int64) - (1 << 62)), None, "hybrid"
     if case == "float_bits":            # random bit patterns: every exponent, NaNs, infinities, -0.0
         v = bits.view(np.float64).copy()
-        # Equal keys must fit one LDS bucket (2048 rows): about 450 rows of -0.0, and a quarter of
+        # Equal keys must fit one LDS bucket (4096 rows): about 450 rows of -0.0, and a quarter of
         # the 1-in-2048 NaN patterns kept (nan_largest orders every NaN as one key).
         v[(bits % np.uint64(9973)) == 5] = -0.0
         drop = np.isnan(v) & ((bits >> np.uint64(3)) % np.uint64(4) != 0)
@@ -265,12 +276,13 @@ def msd_case(case, n):
 
 
 @pytest.mark.parametrize("case", ["uniform", "low_digits_constant", "correlated_top", "duplicates",
-                                  "float_bits", "skewed"])
+                                  "repeats", "wide_buckets", "half_range", "float_bits", "skewed"])
 def test_msd_hybrid_and_fallback(case):
     """64-bit keys above 4M rows whose top two digits vary take the hybrid sort (a stable partition by
     the top digit, a segmented partition by the second, an LDS sort of each bucket by the rest). A
-    bucket above the LDS sort's 2048 rows sends the sort back to the LSD passes; a skewed top digit
-    never starts the hybrid. Each case checks the path it took and the order against the oracle."""
+    bucket above the LDS sort's 4096 rows sends the sort back to the LSD passes; a bucket with many
+    equal keys is sorted by LSD passes in LDS; a skewed top digit never starts the hybrid. Each case
+    checks the path it took and the order against the oracle."""
     n = 4_500_000
     vals, valid, path = msd_case(case, n)
     combos = [(False, 0, 0), (True, 1, 0)] + ([(False, 1, 1), (True, 0, 2)] if vals.dtype == np.float64 else [])
