@@ -82,7 +82,7 @@ template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const uint8_t* valid, int64_t voff,
                                                       uint32_t n, int type, int forder, int desc,
                                                       uint32_t rows_per_block, SortCtl* ctl, uint32_t* unit_hist,
-                                                      uint32_t nunits) {
+                                                      uint32_t nunits, int all_digits) {
   constexpr int P = int(sizeof(K));
   __shared__ uint32_t h[P][RADIX];
   __shared__ uint32_t hu[UNITS_PER_BLOCK][NBINS_FIRST];
@@ -127,8 +127,12 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
         cm++;
         o |= k;
         a &= k;
+        // all_digits == 0 (the hybrid sort's analysis): only the top two digits' histograms.
+        if (all_digits) {
 #pragma unroll
-        for (int p = 0; p < P - 1; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+          for (int p = 0; p < P - 2; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+        }
+        if (P >= 2) atomicAdd(&h[P - 2][uint32_t(k >> (8 * (P - 2))) & 0xFFu], 1u);
         const uint32_t top = uint32_t(k >> (8 * (P - 1))) & 0xFFu;
         atomicAdd(units ? &hu[ui & (UNITS_PER_BLOCK - 1)][top] : &h[P - 1][top], 1u);
       } else if (cc[u] == 1) {
@@ -957,24 +961,27 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     units_buf = device_alloc(size_t(nunits) * NBINS_FIRST * 4 * 2);
     unit_hist = static_cast<uint32_t*>(units_buf->ptr);
   }
-  {
+  // The upsweep and the plan. With units (the hybrid sort's analysis) the upsweep counts only the top
+  // two digits; if the LSD passes run after all, the analysis is redone with every digit.
+  auto analyse_column = [&](bool units) {
     uint32_t max_blocks = uint32_t(device().cu_count) * 4u;
     uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 8191u) / 8192u));
     // Block ranges are whole units when units are counted, at most UNITS_PER_BLOCK of them.
-    const uint32_t align = hybrid_ok ? UNIT_ROWS : 2048u;
+    const uint32_t align = units ? UNIT_ROWS : 2048u;
     uint32_t rpb = ((n + blocks - 1) / blocks + align - 1) / align * align;
-    if (hybrid_ok) rpb = std::min(rpb, UNITS_PER_BLOCK * UNIT_ROWS);
+    if (units) rpb = std::min(rpb, UNITS_PER_BLOCK * UNIT_ROWS);
     blocks = (n + rpb - 1) / rpb;
-    ProfScope ps("sort_upsweep");
-    hipLaunchKernelGGL((sort_upsweep<K, W>), dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, rpb, ctl, unit_hist, nunits);
-    AM_HIP(hipGetLastError());
-  }
-  {
+    {
+      ProfScope ps("sort_upsweep");
+      hipLaunchKernelGGL((sort_upsweep<K, W>), dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+                         dsc, rpb, ctl, units ? unit_hist : nullptr, nunits, units ? 0 : 1);
+      AM_HIP(hipGetLastError());
+    }
     ProfScope ps("sort_plan");
     hipLaunchKernelGGL(sort_plan<P>, dim3(1), dim3(BLOCK), 0, s, ctl, placement == 1 ? 1 : 0);
     AM_HIP(hipGetLastError());
-  }
+  };
+  analyse_column(hybrid_ok);
   // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
   // that only active passes are launched, over the value rows only.
   struct Plan { uint32_t mask, first, last, m, top_max, second_max; } plan{0xFFu, 0, uint32_t(P - 1), n, 0, 0};
@@ -1000,6 +1007,16 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
             vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, kp0, rp0, kp1, rp1, out_idx,
             unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, s))
       return;
+  }
+  if (hybrid_ok) {
+    // The LSD passes need every digit's histogram: the analysis again, from a cleared control block
+    // (the plan read back above stays the same).
+    {
+      ProfScope ps("sort_init");
+      hipLaunchKernelGGL(sort_init, dim3(1), dim3(BLOCK), 0, s, ctl);
+      AM_HIP(hipGetLastError());
+    }
+    analyse_column(false);
   }
   {
     ProfScope ps("sort_first");
