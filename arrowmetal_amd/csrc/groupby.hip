@@ -1240,6 +1240,7 @@ struct GroupBy {
   DevBuf fx_sum, fx_mean, fx_valid;
   uint64_t cnt_col = 0;
   DevBuf cnt_tab;          // K x 2 integer sum and count of non-null values of column cnt_col
+  bool cnt_all_rows = false;   // column cnt_col had no nulls: its counts are the rows per group
   DevBuf count_all_tab;    // K x 1 rows per group
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
@@ -1698,6 +1699,9 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   hipStream_t s = stream();
   if (op == 1) {
     DevBuf t;
+    // An integer sum of a column without nulls already counted every row of each group (word 1).
+    if (cache && !cache->count_all_tab && cache->cnt_tab && cache->cnt_all_rows)
+      return finalize_simple(cache->cnt_tab, 2, K, KIND_SUM_COUNT, T_I64, "l", T_I64);
     if (cache && cache->count_all_tab) {
       t = cache->count_all_tab;
     } else {
@@ -1713,6 +1717,8 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   if (op == 2) {
     // An integer sum of the same column already counted its non-null values (word 1).
     if (cache && cache->cnt_col == v->uid && cache->cnt_tab) return finalize_simple(cache->cnt_tab, 2, K, KIND_SUM_COUNT, T_I64, "l", T_I64);
+    // A column without nulls has as many non-null values per group as rows.
+    if (cache && cache->count_all_tab && !v->has_nulls()) return finalize_simple(cache->count_all_tab, 1, K, 3, T_I64, "l", T_I64);
     AggArgs a = make_args(ids, v, n, K);
     DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
@@ -1737,7 +1743,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
       t = cache->cnt_tab;
     } else {
       t = run_agg<SumCountP>(a, s, "gb_sum");
-      if (cache) { cache->cnt_col = v->uid; cache->cnt_tab = t; }
+      if (cache) { cache->cnt_col = v->uid; cache->cnt_tab = t; cache->cnt_all_rows = !v->has_nulls(); }
     }
     if (op == 0) {
       bool u64 = v->type == T_U64;

@@ -348,3 +348,24 @@ def test_table_sizing_from_distinct_estimate(n, span):
     kv = rng.random(n) >= 0.01
     vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
     check_grouping([keys], [kv], vals, None, f"n={n} span={span}")
+
+
+@pytest.mark.parametrize("with_nulls", [False, True])
+def test_count_reuse_in_any_op_order(with_nulls):
+    """count_all after an integer sum of a column without nulls, and count after count_all, reuse the
+    table already built; with nulls in the value column they must not. Every order of sum / count_all /
+    count gives the oracle's answers, with null keys present."""
+    rng = np.random.default_rng(17 + with_nulls)
+    n = 300_007
+    keys = rng.integers(0, 5_000, n).astype(np.int64)
+    kv = rng.random(n) >= 0.02
+    vals = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    vv = (rng.random(n) >= 0.1) if with_nulls else None
+    ids_w, K, _ = O.group_keys([keys], [kv])
+    mk, mv = am.MetalArray.from_numpy(keys, kv), am.MetalArray.from_numpy(vals, vv)
+    for order in ([0, 1, 2], [1, 2, 0], [2, 1, 0], [1, 0, 2]):
+        g = am.group_by([mk])
+        for op in order:
+            name = {0: "sum", 1: "count_all", 2: "count"}[op]
+            got = g.count_all() if op == 1 else getattr(g, name)(mv)
+            assert_agg(got, O.group_agg(ids_w, K, vals, op, vv), f"order {order} {name} nulls={with_nulls}")
