@@ -7,9 +7,11 @@ One step = argsort + gather over the HBM-resident columns; rows/s = rows / step 
 
 Also measured and reported beside it (same JSON line):
   * argsort_1M: the plain ascending argsort of 1,000,000 full-range int64 keys;
+  * order_by_f64_total: the same ORDER BY with the keys as float64, 0.1% NaN and 0.1% -0.0, under
+    the total order, with its own parity check against the oracle;
   * group_by_50M_zipf_100k: am_group_by_keys over 50M int64 keys drawn Zipf(1.1) from 100,000
     distinct values, then int64 sum, count_all and count (am_group_agg_ex ops 0, 1, 2), and the float64
-    min / max / sum of a second value column;
+    min / max / sum of a second value column; the same integer step over 200 uniform distinct keys;
   * roofline: the dominant kernel of the step (the longest mean launch), its algorithmic bytes per
     launch (orderby_alg_bytes) over its mean launch time, measured with HIP events on the library's
     own stream (am_prof_*); kernels_roofline gives the same for every kernel of the step;
@@ -387,6 +389,7 @@ def main():
     ap.add_argument("--rows", type=int, default=50_000_000)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-groupby", action="store_true")
+    ap.add_argument("--skip-f64", action="store_true", help="omit the float64 total-order ORDER BY leg")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batches", type=int, default=2048)
     ap.add_argument("--skip-serving", action="store_true")
@@ -470,6 +473,42 @@ def main():
     ms1 = 1e3 * float(np.median(ts1))
     del k1
 
+    # ---------------------------------------------------------------- ORDER BY, float64 keys, total order
+    # The same keys as float64 with 0.1% NaN and 0.1% -0.0 (seeded positions), total order (-NaN < -inf
+    # < ... < -0.0 < +0.0 < ... < +inf < +NaN), descending, nulls last, then the same gather.
+    f64_res = None
+    if not args.skip_f64:
+        sel = unit(splitmix64(0xA11CE504, n))
+        kf = keys.astype(np.float64)
+        kf[sel < 0.001] = np.nan
+        kf[(sel >= 0.001) & (sel < 0.002)] = -0.0
+        kfm = am.MetalArray.from_numpy(kf, valid)
+
+        def order_by_f64_step():
+            idx = kfm.argsort(True, "at_end", "total")
+            out = p.take(idx)
+            del idx, out
+
+        tf64 = timed(order_by_f64_step, args.steps, args.warmup)
+        f64_ms = 1e3 * float(np.median(tf64))
+        f64_parity = None
+        if not args.skip_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_py as O   # the checker only
+            got, _ = kfm.argsort(True, "at_end", "total").to_numpy()
+            f64_parity = bool(np.array_equal(got, O.argsort(kf, valid, True, 0, 1)))
+        am.prof_reset()
+        am.prof_enable(True)
+        order_by_f64_step()
+        am.prof_enable(False)
+        f64_prof = am.prof_report()
+        f64_res = {"workload": "order_by_50M_float64_total_order_0.1pct_nan_0.1pct_negzero_desc_nulls_last_then_gather",
+                   "ms_per_step": round(f64_ms, 4), "rows_per_s": n / (f64_ms * 1e-3),
+                   "parity_vs_oracle": f64_parity,
+                   "kernels_ms": {kname: round(v[1] / max(v[0], 1), 4) for kname, v in f64_prof.items()}}
+        log(f"order-by float64 total order: {f64_ms:.3f} ms, parity {f64_parity}")
+        del kfm, kf, sel
+
     # ---------------------------------------------------------------- group-by 50M, Zipf over 100k
     gb_res = None
     if not args.skip_groupby:
@@ -517,7 +556,16 @@ def main():
             "kernels_ms": {kname: round(v[1] / max(v[0], 1), 4) for kname, v in gprof.items()},
         }
         log(f"group-by: {gms:.3f} ms, float step {fms:.3f} ms, groups {g.group_count}")
-        del g, gk, gi, gf
+        del g, gk, gf
+        # The second run: 200 distinct keys, uniform.
+        table200 = splitmix64(0x6B0DE6, 200).view(np.int64)
+        gk = am.MetalArray.from_numpy(table200[(splitmix64(0x21F1, n) % np.uint64(200)).astype(np.int64)])
+        t200 = timed(gb_step, args.steps, args.warmup)
+        ms200 = 1e3 * float(np.median(t200))
+        gb_res["uniform_200_keys"] = {"ms_per_step": round(ms200, 4), "rows_per_s": n / (ms200 * 1e-3),
+                                      "step": "the same step over 50M int64 keys uniform over 200 distinct values"}
+        log(f"group-by, 200 uniform keys: {ms200:.3f} ms")
+        del gk, gi
 
     # ---------------------------------------------------------------- batched (2,048 record batches)
     bt_res = None
@@ -566,6 +614,7 @@ def main():
         "cpu_baseline": cpu,
         "argsort_1M": {"ms": round(ms1, 4), "rows_per_s": 1e6 / (ms1 * 1e-3),
                        "workload": "stable ascending argsort of 1,000,000 full-range int64 keys"},
+        "order_by_f64_total": f64_res,
         "group_by": gb_res,
         "batched": bt_res,
         "serving": sv_res,
