@@ -28,8 +28,11 @@ namespace {
 struct SegCtl {
   unsigned long long key_or, key_and;
   uint32_t active_mask, first_active, last_active, epoch;
-  uint32_t tile_counter[16];
-  uint32_t chunk_counter, pad[3];
+  // Bumped by every workgroup of a kernel, so on cache lines of their own: sharing a line with the
+  // plan words those workgroups read slows the pass down.
+  alignas(128) uint32_t tile_counter[16];
+  alignas(128) uint32_t chunk_counter;
+  uint32_t pad[31];
 };
 
 struct SegTile {

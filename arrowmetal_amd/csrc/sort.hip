@@ -33,8 +33,10 @@ struct SortCtl {
   uint32_t hist[8][RADIX];          // digit histograms of the value keys, one per pass
   uint32_t m, q, z, pad0;           // value rows, NaN rows (separate block), null rows
   unsigned long long key_or, key_and;
-  uint32_t tile_counter[16];        // [0] first pass, [1 + p] pass p, [10] segmented pass, [11] units
-  uint32_t digit_base[8][RADIX];    // exclusive digit prefix inside the value block, per pass
+  // Bumped by every workgroup of a pass, so on cache lines of their own: a line that also holds
+  // words the same workgroups read slows the pass down.
+  alignas(128) uint32_t tile_counter[16];   // [0] first pass, [1 + p] pass p, [11] units
+  alignas(128) uint32_t digit_base[8][RADIX];   // exclusive digit prefix inside the value block, per pass
   uint32_t active_mask, first_active, last_active, pad1;
   uint32_t value_base, nan_base, null_base, pad2;
   // Per-XCD partials of the upsweep (updated by workgroup-scope atomics in each XCD's own L2), folded
@@ -56,19 +58,11 @@ __global__ void sort_init(SortCtl* ctl) {
 }
 
 // Copies the plan (active mask, first and last active pass, value rows) to pinned host memory.
-__global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out, int P) {
+__global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
   out[0] = ctl->active_mask;
   out[1] = ctl->first_active;
   out[2] = ctl->last_active;
   out[3] = ctl->m;
-  // The largest digit count of the top two digits (the hybrid sort's bucket estimate).
-  uint32_t a = 0, b = 0;
-  for (int d = 0; d < RADIX; d++) {
-    a = max(a, ctl->hist[P - 1][d]);
-    b = max(b, ctl->hist[P - 2][d]);
-  }
-  out[4] = a;
-  out[5] = b;
 }
 
 
@@ -515,61 +509,114 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 
 // ---------------------------------------------------------------- hybrid MSD sort (64-bit keys)
 //
-// For large inputs of 64-bit keys whose top two digits spread the rows well: the first pass
-// partitions the value rows stably by the top digit (msd_part, above); a segmented pass partitions
-// each top bucket stably by the second digit (onesweep tiles whose look-back stops at their
-// segment's first tile); every resulting bucket (a few hundred rows at 50M uniform keys, at most
-// LOCAL_MAX) is then sorted in LDS by the remaining digits and its rows written to their final
-// slots. Three passes over the rows instead of eight.
+// For large inputs of 64-bit keys whose top digit varies: the first pass partitions the value rows
+// stably by the top digit (msd_part, above). Then, level by level down the active digits, every
+// bucket still larger than LOCAL_MAX rows is partitioned stably by the next digit (onesweep tiles
+// whose look-back stops at their bucket's first tile), into the other key / row buffer at the same
+// positions; a bucket whose digit is the same on all its rows is carried to the next level without
+// moving. Buckets of at most LOCAL_MAX rows leave for the bucket sort, which sorts each in LDS by its
+// remaining digits and writes its rows to their final slots. After the lowest active digit every
+// bucket holds equal keys, in input order, and is copied. Uniform keys take three passes over the
+// rows (top digit, second digit, bucket sort) instead of eight.
 
 #ifndef AM_MSD_FITEMS
 #define AM_MSD_FITEMS 16   // keys per thread of the top-digit partition
 #endif
-constexpr uint32_t HYB_CHUNK = 65536;   // rows per histogram chunk of the segmented pass
+constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
 constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes
 
-struct HybChunk {
-  uint32_t seg, start, end, pad;
+// A bucket: rows [start, start + size) of the value block, in key / row buffer `buf` (0 or 1), all
+// sharing the digits from the top down to `level`.
+struct Bucket {
+  uint32_t start, size, buf, level;
 };
-struct HybTile {
-  uint32_t seg, start, end, head;   // head: the segment's first tile
+struct LvlChunk {
+  uint32_t bucket, start, end, pad;
+};
+struct LvlTile {
+  uint32_t bucket, start, end, head;   // head: the bucket's first tile
 };
 
-// One workgroup: the chunk and tile tables of the segmented pass, one segment per top digit value.
-// hs[0] = chunks, hs[1] = tiles, hs[2] = largest bucket (filled later), hs[3..4] = the LSD queue.
-template <int P, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void hyb_layout(const SortCtl* ctl, HybChunk* chunks, HybTile* tiles,
-                                                    uint32_t* hs, uint32_t* seg_hist) {
-  __shared__ uint32_t tmp[WAVES];
-  for (uint32_t i = threadIdx.x; i < RADIX * RADIX; i += BLOCK) seg_hist[i] = 0u;
-  constexpr uint32_t TILE = BLOCK * ITEMS;
+// Device counters of the levels (one small array). The host reads back the first LC_READ words. The
+// two counters that every workgroup of a kernel bumps each sit on a cache line of their own: sharing a
+// line with words the same workgroups read (the tile count) made the level pass 1.8x slower.
+constexpr int LC_LIST = 0;        // [0], [1]: entries of the two large-bucket lists
+constexpr int LC_LOCAL = 2;       // entries of the bucket sort's list
+constexpr int LC_CHUNKS = 3;      // chunks of the current level
+constexpr int LC_TILES = 4;       // tiles of the current level
+constexpr int LC_MAXLOCAL = 5;    // largest bucket the bucket sort sorts (not counting copies)
+constexpr int LC_OVERFLOW = 6;    // the bucket sort's list ran out of room
+constexpr int LC_QLEN = 7;        // buckets queued for the LSD bucket sort
+constexpr int LC_READ = 16;
+constexpr int LC_NEXT_TILE = 32;  // the next tile of the current level to take (own 128-byte line)
+constexpr int LC_QNEXT = 64;      // the next queued bucket to take (own 128-byte line)
+constexpr int LC_WORDS = 96;
+
+// Appends one bucket to the bucket sort's list. `sorted` = its rows need no sorting (equal keys).
+__device__ __forceinline__ void push_local(Bucket* local, uint32_t local_cap, uint32_t* lc, Bucket b, bool sorted) {
+  const uint32_t i = atomicAdd(&lc[LC_LOCAL], 1u);
+  if (i < local_cap) local[i] = b;
+  else atomicOr(&lc[LC_OVERFLOW], 1u);
+  if (!sorted) atomicMax(&lc[LC_MAXLOCAL], b.size);
+}
+
+// One workgroup: the top-digit buckets msd_part made, into the first large-bucket list or the bucket
+// sort's list; the counters cleared. `low` = the active digits below the top one.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, Bucket* large, Bucket* local,
+                                                        uint32_t local_cap, uint32_t* lc) {
+  if (threadIdx.x < LC_WORDS) lc[threadIdx.x] = 0u;
+  __syncthreads();
   const uint32_t b = threadIdx.x;
   const uint32_t size = ctl->hist[P - 1][b];
-  const uint32_t start = ctl->digit_base[P - 1][b];
-  const uint32_t nc = (size + HYB_CHUNK - 1) / HYB_CHUNK;
-  const uint32_t nt = (size + TILE - 1) / TILE;
-  uint32_t tc, tt;
-  const uint32_t coff = block_excl_scan(nc, tmp, tc);
-  const uint32_t toff = block_excl_scan(nt, tmp, tt);
-  for (uint32_t c = 0; c < nc; c++)
-    chunks[coff + c] = {b, start + c * HYB_CHUNK, start + min(size, (c + 1u) * HYB_CHUNK), 0u};
-  for (uint32_t t = 0; t < nt; t++) tiles[toff + t] = {b, start + t * TILE, start + min(size, (t + 1u) * TILE), toff};
-  if (b == 0) {
-    hs[0] = tc;
-    hs[1] = tt;
-    hs[2] = 0;
-    hs[3] = 0;   // buckets queued for the LSD bucket sort
-    hs[4] = 0;   // the next queued bucket to take
+  if (size == 0u) return;
+  const Bucket bk{ctl->digit_base[P - 1][b], size, 0u, uint32_t(P - 1)};
+  const uint32_t low = ctl->active_mask & ((1u << (P - 1)) - 1u);
+  if (size > LOCAL_MAX && low != 0u) large[atomicAdd(&lc[LC_LIST], 1u)] = bk;
+  else push_local(local, local_cap, lc, bk, low == 0u);
+}
+
+// One workgroup: the chunk and tile tables of a level over its large buckets (list `cur`), in list
+// order; the bucket histograms cleared; the other list's counter and the tile counter reset.
+template <int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, LvlChunk* chunks, LvlTile* tiles,
+                                                    uint32_t* bucket_hist, uint32_t* lc) {
+  __shared__ uint32_t tmp[WAVES];
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t nb = lc[LC_LIST + cur];
+  uint32_t coff = 0, toff = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += BLOCK) {
+    const uint32_t b = b0 + threadIdx.x;
+    Bucket bk{0, 0, 0, 0};
+    if (b < nb) bk = large[b];
+    const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
+    uint32_t tc, tt;
+    const uint32_t c0 = coff + block_excl_scan(nc, tmp, tc);
+    const uint32_t t0 = toff + block_excl_scan(nt, tmp, tt);
+    for (uint32_t c = 0; c < nc; c++)
+      chunks[c0 + c] = {b, bk.start + c * LVL_CHUNK, bk.start + min(bk.size, (c + 1u) * LVL_CHUNK), 0u};
+    for (uint32_t t = 0; t < nt; t++) tiles[t0 + t] = {b, bk.start + t * TILE, bk.start + min(bk.size, (t + 1u) * TILE), t0};
+    coff += tc;
+    toff += tt;
+  }
+  for (uint32_t i = threadIdx.x; i < nb * RADIX; i += BLOCK) bucket_hist[i] = 0u;
+  if (threadIdx.x == 0) {
+    lc[LC_CHUNKS] = coff;
+    lc[LC_TILES] = toff;
+    lc[LC_NEXT_TILE] = 0u;
+    lc[LC_LIST + (cur ^ 1)] = 0u;
   }
 }
 
-// Second-digit histogram of each top bucket, one chunk per workgroup (surplus workgroups exit).
-template <typename K, int P>
-__global__ __launch_bounds__(BLOCK) void hyb_hist(const K* keys, const HybChunk* chunks, const uint32_t* hs,
-                                                  uint32_t* seg_hist) {
+// Digit d's histogram of every large bucket, one chunk per workgroup (surplus workgroups exit).
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1, const Bucket* large,
+                                                  const LvlChunk* chunks, const uint32_t* lc, int d,
+                                                  uint32_t* bucket_hist) {
   __shared__ uint32_t h[RADIX];
-  if (blockIdx.x >= hs[0]) return;
-  const HybChunk c = chunks[blockIdx.x];
+  if (blockIdx.x >= lc[LC_CHUNKS]) return;
+  const LvlChunk c = chunks[blockIdx.x];
+  const K* keys = large[c.bucket].buf ? keys1 : keys0;
   h[threadIdx.x] = 0u;
   __syncthreads();
   constexpr int U = 8;
@@ -579,34 +626,46 @@ __global__ __launch_bounds__(BLOCK) void hyb_hist(const K* keys, const HybChunk*
     for (int u = 0; u < U; u++) k[u] = keys[min(base + uint32_t(u) * BLOCK, c.end - 1u)];
 #pragma unroll
     for (int u = 0; u < U; u++)
-      if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * (P - 2))) & 0xFFu], 1u);
+      if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * d)) & 0xFFu], 1u);
   }
   __syncthreads();
   const uint32_t v = h[threadIdx.x];
-  if (v) atomicAdd(&seg_hist[c.seg * RADIX + threadIdx.x], v);
+  if (v) atomicAdd(&bucket_hist[size_t(c.bucket) * RADIX + threadIdx.x], v);
 }
 
-// One workgroup per top bucket: the absolute start of each of its second-digit buckets, and the
-// largest bucket.
-template <int P>
-__global__ __launch_bounds__(BLOCK) void hyb_seg_plan(const SortCtl* ctl, const uint32_t* seg_hist,
-                                                      uint32_t* seg_base, uint32_t* hs) {
+// One workgroup per large bucket: the start of each of its digit-d sub-buckets (bucket_base), and
+// where each goes: to the bucket sort's list (at most LOCAL_MAX rows, or equal keys after the lowest
+// active digit `last`), or to the next level's list. A bucket whose rows all share digit d is carried
+// (carried[b] = 1): it keeps its buffer and the pass skips it.
+__global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist, int d,
+                                                  int last, uint32_t* bucket_base, uint32_t* carried,
+                                                  Bucket* next, Bucket* local, uint32_t local_cap, uint32_t* lc) {
   __shared__ uint32_t tmp[WAVES];
-  const uint32_t seg = blockIdx.x;
-  const uint32_t v = seg_hist[seg * RADIX + threadIdx.x];
+  const uint32_t b = blockIdx.x;
+  if (b >= lc[LC_LIST + cur]) return;   // uniform over the workgroup
+  const Bucket bk = large[b];
+  const uint32_t v = bucket_hist[size_t(b) * RADIX + threadIdx.x];
   uint32_t total;
-  const uint32_t ex = block_excl_scan(v, tmp, total);
-  seg_base[seg * RADIX + threadIdx.x] = ctl->digit_base[P - 1][seg] + ex;
-  if (v) atomicMax(&hs[2], v);
+  const uint32_t base = bk.start + block_excl_scan(v, tmp, total);
+  bucket_base[size_t(b) * RADIX + threadIdx.x] = base;
+  const int bins = __syncthreads_count(v != 0u ? 1 : 0);
+  const bool carry = bins == 1;
+  if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
+  if (v == 0u) return;
+  const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), uint32_t(d)};
+  if (last) push_local(local, local_cap, lc, sub, true);
+  else if (v <= LOCAL_MAX) push_local(local, local_cap, lc, sub, false);
+  else next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
 }
 
-// The segmented pass: every top bucket partitioned stably by the second digit, from (k0, r0) into
-// (k1, r1). Tile ids come from a counter; blocks past the last tile exit.
-template <typename K, int P, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void hyb_seg_pass(SortCtl* ctl, const HybTile* tiles, const uint32_t* hs,
-                                                      const uint32_t* seg_base, unsigned long long* status,
-                                                      uint32_t epoch, const K* in_keys, const uint32_t* in_rows,
-                                                      K* out_keys, uint32_t* out_rows) {
+// A level's pass: every large bucket that is not carried partitioned stably by digit d from its buffer
+// into the other one, at the same positions. Tile ids come from a counter; tiles of carried buckets
+// (whose whole look-back chain is skipped) and blocks past the last tile exit.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_pass(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
+                                                  const uint32_t* carried, const uint32_t* bucket_base,
+                                                  unsigned long long* status, uint32_t epoch, int d, K* keys0,
+                                                  uint32_t* rows0, K* keys1, uint32_t* rows1) {
   constexpr int TILE = BLOCK * ITEMS;
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
@@ -618,12 +677,18 @@ __global__ __launch_bounds__(BLOCK) void hyb_seg_pass(SortCtl* ctl, const HybTil
   __shared__ K s_stage[TILE];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  if (threadIdx.x == 0) s_tile = atomicAdd(&ctl->tile_counter[10], 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  if (tile >= hs[1]) return;   // uniform over the workgroup
-  const HybTile t = tiles[tile];
-  const int shift = 8 * (P - 2);
+  if (tile >= lc[LC_TILES]) return;   // uniform over the workgroup
+  const LvlTile t = tiles[tile];
+  if (carried[t.bucket]) return;
+  const bool from1 = large[t.bucket].buf != 0u;
+  const K* in_keys = from1 ? keys1 : keys0;
+  const uint32_t* in_rows = from1 ? rows1 : rows0;
+  K* out_keys = from1 ? keys0 : keys1;
+  uint32_t* out_rows = from1 ? rows0 : rows1;
+  const int shift = 8 * d;
   K keys[ITEMS];
   uint32_t rows[ITEMS], bins[ITEMS];
   const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
@@ -642,7 +707,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_seg_pass(SortCtl* ctl, const HybTil
   const uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
   look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
   __syncthreads();
-  bin_base[threadIdx.x] += seg_base[t.seg * RADIX + threadIdx.x];
+  bin_base[threadIdx.x] += bucket_base[size_t(t.bucket) * RADIX + threadIdx.x];
   __syncthreads();
   scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount, out_keys,
                                  out_rows);
@@ -716,35 +781,43 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
   }
 }
 
-// The bucket sort: one workgroup per (top digit, second digit) bucket of at most LI * BLOCK rows
-// (the host picks LI from the largest bucket). The rows of a bucket arrive in input order (both
-// partitions before were stable), so the stable order is the order of (key, position in the bucket)
-// pairs, which are all distinct. The bucket is split by its most significant varying digit with LDS
-// counters (any order inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the
-// digit) << POS_BITS | position, so that inside one digit run words order as those pairs do; each
-// row's place in its run is the number of words there below its own. A bucket with a run longer
-// than SUB_MAX rows (many equal keys) is queued for the stable LSD passes of hyb_local_lsd instead.
+// The bucket sort: one workgroup per entry of its list (a bucket of at most LI * BLOCK rows; the host
+// picks LI from the largest). The rows of a bucket arrive in input order (every partition before was
+// stable), so the stable order is the order of (key, position in the bucket) pairs, which are all
+// distinct. The bucket is split by its most significant varying digit with LDS counters (any order
+// inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the digit) << POS_BITS |
+// position, so that inside one digit run words order as those pairs do; each row's place in its run
+// is the number of words there below its own. A bucket whose varying digits do not fit that word (a
+// top-digit bucket), or with a run longer than SUB_MAX rows (many equal keys), is queued for the
+// stable LSD passes of hyb_local_lsd instead. A bucket of equal keys (no varying digit left), of any
+// size, is copied.
 constexpr int POS_BITS = 12;   // positions in a bucket of at most LOCAL_MAX rows
 static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
 template <typename K, int P, int LI>
-__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uint32_t* seg_hist,
-                                                   const uint32_t* seg_base, const K* keys_in, const uint32_t* rows_in,
-                                                   uint32_t* out_idx, uint32_t* queue, uint32_t* queue_len) {
+__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const Bucket* list, const K* keys0,
+                                                   const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
+                                                   uint32_t* out_idx, uint32_t* queue, uint32_t* lc) {
   __shared__ uint32_t cnt[RADIX], start[RADIX], cursor[RADIX];
   __shared__ uint32_t tmp[WAVES];
   extern __shared__ __align__(16) unsigned char local_lds[];
   K* s_words = reinterpret_cast<K*>(local_lds);   // LI * BLOCK words
-  const uint32_t bucket = blockIdx.x;
-  const uint32_t size = seg_hist[bucket];
-  if (size == 0u || size > uint32_t(LI * BLOCK)) return;   // the host has checked that no bucket is larger
-  const uint32_t off = seg_base[bucket];
-  const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
+  const Bucket bk = list[blockIdx.x];
+  const K* keys_in = bk.buf ? keys1 : keys0;
+  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+  const uint32_t size = bk.size, off = bk.start;
+  const uint32_t low_mask = ctl->active_mask & ((1u << bk.level) - 1u);
   uint32_t* out = out_idx + ctl->value_base;
-  if (low_mask == 0u) {   // only the top two digits vary: the bucket is sorted already
+  if (low_mask == 0u) {   // equal keys: the bucket is in order already
     for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
     return;
   }
-  const int shift = 8 * (31 - __builtin_clz(low_mask));   // at most 40: 8 + 40 + POS_BITS bits per word
+  if (size > uint32_t(LI * BLOCK)) return;   // never: the host sized LI for every bucket to sort
+  const int top = 31 - __builtin_clz(low_mask);
+  if (top > 5) {   // 8 + 8 * top + POS_BITS bits would not fit one word
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = blockIdx.x;
+    return;
+  }
+  const int shift = 8 * top;   // at most 40: 8 + 40 + POS_BITS bits per word
   cnt[threadIdx.x] = 0u;
   K keys[LI];
 #pragma unroll
@@ -760,7 +833,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uin
   start[threadIdx.x] = st;
   cursor[threadIdx.x] = st;
   if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
-    if (threadIdx.x == 0) queue[atomicAdd(queue_len, 1u)] = bucket;
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = blockIdx.x;
     return;
   }
   const unsigned long long low_bits = (1ull << shift) - 1ull;
@@ -801,12 +874,11 @@ __global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const uin
 }
 
 // The queued buckets, each by stable LSD passes in LDS: persistent workgroups take them from a
-// counter (qctl[0] = queued buckets, qctl[1] = the next one to take) until none is left. LDS for the
-// bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
+// counter until none is left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
 template <typename K, int P, int LI>
-__global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const SortCtl* ctl, const uint32_t* seg_hist,
-                                                       const uint32_t* seg_base, const K* keys, const uint32_t* rows,
-                                                       uint32_t* out_idx, const uint32_t* queue, uint32_t* qctl) {
+__global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const SortCtl* ctl, const Bucket* list, const K* keys0,
+                                                       const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
+                                                       uint32_t* out_idx, const uint32_t* queue, uint32_t* lc) {
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
   __shared__ uint32_t bin_start[RADIX];
@@ -815,45 +887,50 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const SortCtl* ctl, const
   extern __shared__ __align__(16) unsigned char local_lds[];
   K* s_keys = reinterpret_cast<K*>(local_lds);
   uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
-  const uint32_t queued = qctl[0];
-  const uint32_t low_mask = ctl->active_mask & ((1u << (P - 2)) - 1u);
+  const uint32_t queued = lc[LC_QLEN];
   while (true) {
-    if (threadIdx.x == 0) s_q = atomicAdd(&qctl[1], 1u);
+    if (threadIdx.x == 0) s_q = atomicAdd(&lc[LC_QNEXT], 1u);
     __syncthreads();
     const uint32_t qi = s_q;
     if (qi >= queued) return;   // uniform: every workgroup ends here once the queue is empty
-    const uint32_t bucket = queue[qi];
-    const uint32_t size = seg_hist[bucket];
-    local_lsd<K, LI>(keys, rows, seg_base[bucket], size, low_mask, out_idx + ctl->value_base, wc, tile_total,
-                     bin_start, tmp, s_keys, s_pos);
+    const Bucket bk = list[queue[qi]];
+    const uint32_t low_mask = ctl->active_mask & ((1u << bk.level) - 1u);
+    local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask,
+                     out_idx + ctl->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------- host side
 
-// The hybrid sort of 64-bit keys (the analysed plan says the top two digits vary). Returns false,
-// with nothing written to out_idx that the LSD passes do not write again, when a bucket turns out
-// larger than the LDS sort takes.
+// The hybrid sort of 64-bit keys (the analysed plan says the top digit varies; `mask` = the active
+// digits). Returns false, with nothing written to out_idx that the LSD passes do not write again, when
+// the bucket sort's list runs out of room.
 template <typename K, int W, int FITEMS, int ITEMS>
 bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
-                SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, K* k0, uint32_t* r0, K* k1,
-                uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base, uint32_t nunits,
-                hipStream_t s) {
+                SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, uint32_t mask, K* k0,
+                uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base,
+                uint32_t nunits, hipStream_t s) {
   constexpr int P = int(sizeof(K));
   constexpr uint32_t TILE = BLOCK * ITEMS;
-  (void)status;
-  const uint32_t max_chunks = (m + HYB_CHUNK - 1) / HYB_CHUNK + uint32_t(RADIX);
-  const uint32_t max_tiles = (m + TILE - 1) / TILE + uint32_t(RADIX);
-  DevBuf chunks = device_alloc(size_t(max_chunks) * sizeof(HybChunk));
-  DevBuf tiles = device_alloc(size_t(max_tiles) * sizeof(HybTile));
-  DevBuf seg = device_alloc(size_t(RADIX) * RADIX * 4 * 3 + 64);
-  uint32_t* seg_hist = static_cast<uint32_t*>(seg->ptr);
-  uint32_t* seg_base = seg_hist + RADIX * RADIX;
-  uint32_t* queue = seg_base + RADIX * RADIX;
-  uint32_t* hs = queue + RADIX * RADIX;
-  HybChunk* cp = static_cast<HybChunk*>(chunks->ptr);
-  HybTile* tp = static_cast<HybTile*>(tiles->ptr);
+  // Capacities: buckets above LOCAL_MAX rows at one level; entries of the bucket sort's list (every
+  // entry has a row, and runs of many tiny buckets are what fills it); chunks and tiles of a level.
+  const uint32_t large_cap = m / LOCAL_MAX + uint32_t(RADIX);
+  const uint32_t local_cap = m / 32u + (1u << 17);
+  const uint32_t max_chunks = m / LVL_CHUNK + large_cap + 1u;
+  const uint32_t max_tiles = m / TILE + large_cap + 1u;
+  DevBuf lists = device_alloc((size_t(large_cap) * 2 + local_cap) * sizeof(Bucket));
+  DevBuf tabs = device_alloc(size_t(max_chunks) * sizeof(LvlChunk) + size_t(max_tiles) * sizeof(LvlTile));
+  DevBuf words = device_alloc((size_t(large_cap) * (2 * RADIX + 1) + local_cap + LC_WORDS) * 4);
+  Bucket* large[2] = {static_cast<Bucket*>(lists->ptr), static_cast<Bucket*>(lists->ptr) + large_cap};
+  Bucket* local = large[1] + large_cap;
+  LvlChunk* chunks = static_cast<LvlChunk*>(tabs->ptr);
+  LvlTile* tiles = reinterpret_cast<LvlTile*>(chunks + max_chunks);
+  uint32_t* lc = static_cast<uint32_t*>(words->ptr);   // first, so that its lines are its own
+  uint32_t* bucket_hist = lc + LC_WORDS;
+  uint32_t* bucket_base = bucket_hist + size_t(large_cap) * RADIX;
+  uint32_t* carried = bucket_base + size_t(large_cap) * RADIX;
+  uint32_t* queue = carried + large_cap;
   {
     ProfScope ps("sort_msd_unit_scan");
     hipLaunchKernelGGL(msd_unit_scan<P>, dim3(NBINS_FIRST), dim3(BLOCK), 0, s, ctl, unit_hist, nunits, unit_base);
@@ -867,60 +944,89 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
     AM_HIP(hipGetLastError());
   }
   {
-    ProfScope ps("sort_msd_layout");
-    hipLaunchKernelGGL((hyb_layout<P, ITEMS>), dim3(1), dim3(BLOCK), 0, s, ctl, cp, tp, hs, seg_hist);
+    ProfScope ps("sort_msd_lists");
+    hipLaunchKernelGGL(msd_init_lists<P>, dim3(1), dim3(BLOCK), 0, s, ctl, large[0], local, local_cap, lc);
     AM_HIP(hipGetLastError());
   }
-  {
-    ProfScope ps("sort_msd_hist");
-    hipLaunchKernelGGL((hyb_hist<K, P>), dim3(max_chunks), dim3(BLOCK), 0, s, k0, cp, hs, seg_hist);
-    AM_HIP(hipGetLastError());
-  }
-  {
-    ProfScope ps("sort_msd_plan");
-    hipLaunchKernelGGL((hyb_seg_plan<P>), dim3(RADIX), dim3(BLOCK), 0, s, ctl, seg_hist, seg_base, hs);
-    AM_HIP(hipGetLastError());
-  }
+  // Levels down the active digits while large buckets are left. The first level's buckets are the top
+  // digit's (at most RADIX); later counts come back with one 64-byte read per level.
   uint32_t* hp = host_plan_words();
-  AM_HIP(hipMemcpyAsync(hp, hs + 2, 4, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
-  if (hp[0] > LOCAL_MAX) return false;
-  {
-    ProfScope ps("sort_msd_scatter");
-    hipLaunchKernelGGL((hyb_seg_pass<K, P, ITEMS>), dim3(max_tiles), dim3(BLOCK), 0, s, ctl, tp, hs, seg_base, status,
-                       epoch + 11u, k0, r0, k1, r1);
-    AM_HIP(hipGetLastError());
+  auto next_digit = [&](int d) {
+    while (--d >= 0 && !((mask >> d) & 1u)) {}
+    return d;
+  };
+  uint32_t bound = uint32_t(RADIX);
+  int cur = 0;
+  bool read = false;
+  for (int d = next_digit(P - 1); d >= 0 && bound > 0; d = next_digit(d)) {
+    const int last = next_digit(d) < 0 ? 1 : 0;
+    {
+      ProfScope ps("sort_msd_layout");
+      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, large[cur], cur, chunks, tiles, bucket_hist,
+                         lc);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_hist");
+      hipLaunchKernelGGL(lvl_hist<K>, dim3(m / LVL_CHUNK + bound + 1u), dim3(BLOCK), 0, s, k0, k1, large[cur],
+                         chunks, lc, d, bucket_hist);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_plan");
+      hipLaunchKernelGGL(lvl_plan, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, bucket_hist, d, last, bucket_base,
+                         carried, large[cur ^ 1], local, local_cap, lc);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_scatter");
+      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(m / TILE + bound + 1u), dim3(BLOCK), 0, s, large[cur], tiles, lc,
+                         carried, bucket_base, status, epoch + 8u + uint32_t(P - 1 - d), d, k0, r0, k1, r1);
+      AM_HIP(hipGetLastError());
+    }
+    AM_HIP(hipMemcpyAsync(hp, lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    read = true;
+    cur ^= 1;
+    bound = hp[LC_LIST + cur];
+    if (bound > large_cap) fail("argsort: bucket list overflow");   // cannot happen: each holds > LOCAL_MAX rows
   }
-  // Keys per thread from the largest bucket; the LSD kernel takes the buckets the counting sort
-  // queued (its workgroups exit at once when none was).
-  const uint32_t biggest = hp[0];
+  if (!read) {   // no level ran (no active digit below the top one)
+    AM_HIP(hipMemcpyAsync(hp, lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+  }
+  if (hp[LC_OVERFLOW]) return false;
+  const uint32_t nlocal = hp[LC_LOCAL], biggest = hp[LC_MAXLOCAL];
+  if (nlocal == 0u) return true;
+  // Keys per thread from the largest bucket to sort; the LSD kernel takes the buckets the counting
+  // sort queued (its workgroups exit at once when none was).
   const int li = biggest <= 4u * BLOCK ? 4 : biggest <= 8u * BLOCK ? 8 : 16;
-  const dim3 grid(RADIX * RADIX), lsd_grid(uint32_t(device().cu_count) * 2u);
+  const dim3 grid(nlocal), lsd_grid(uint32_t(device().cu_count) * 2u);
   const size_t lds = size_t(li) * BLOCK * sizeof(K), lsd_lds = size_t(li) * BLOCK * (sizeof(K) + 2);
   {
     ProfScope ps("sort_msd_local");
     if (li == 4)
-      hipLaunchKernelGGL((hyb_local<K, P, 4>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
-                         queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local<K, P, 4>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
+                         lc);
     else if (li == 8)
-      hipLaunchKernelGGL((hyb_local<K, P, 8>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
-                         queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local<K, P, 8>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
+                         lc);
     else
-      hipLaunchKernelGGL((hyb_local<K, P, 16>), grid, dim3(BLOCK), lds, s, ctl, seg_hist, seg_base, k1, r1, out_idx,
-                         queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local<K, P, 16>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
+                         lc);
     AM_HIP(hipGetLastError());
   }
   {
     ProfScope ps("sort_msd_local_lsd");
     if (li == 4)
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1, r1,
-                         out_idx, queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
+                         out_idx, queue, lc);
     else if (li == 8)
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1, r1,
-                         out_idx, queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
+                         out_idx, queue, lc);
     else
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, seg_hist, seg_base, k1,
-                         r1, out_idx, queue, hs + 3);
+      hipLaunchKernelGGL((hyb_local_lsd<K, P, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
+                         out_idx, queue, lc);
     AM_HIP(hipGetLastError());
   }
   return true;
@@ -939,9 +1045,10 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   const uint32_t ftiles = (n + FTILE - 1) / FTILE;
   DevBuf ctl_buf = device_alloc(sizeof(SortCtl));
   SortCtl* ctl = static_cast<SortCtl*>(ctl_buf->ptr);
-  // Status rows for every tile of a pass; the hybrid sort's segmented pass has up to one more tile
-  // per top digit value.
-  uint32_t epoch = reserve_status(size_t(std::max(tiles + uint32_t(RADIX), ftiles)) * STATUS_STRIDE, s);
+  // Status rows for every tile of a pass; a level of the hybrid sort has up to one more tile per large
+  // bucket (at most m / 4096 + 256 of them).
+  const uint32_t hyb_tiles = tiles + n / 4096u + 2u * uint32_t(RADIX);
+  uint32_t epoch = reserve_status(size_t(std::max(hyb_tiles, ftiles)) * STATUS_STRIDE, s);
   unsigned long long* status = static_cast<unsigned long long*>(t_ws.status->ptr);
   retain(t_ws.status);
   DevBuf k0 = device_alloc(size_t(n) * sizeof(K)), k1 = device_alloc(size_t(n) * sizeof(K));
@@ -984,11 +1091,11 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   analyse_column(hybrid_ok);
   // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
   // that only active passes are launched, over the value rows only.
-  struct Plan { uint32_t mask, first, last, m, top_max, second_max; } plan{0xFFu, 0, uint32_t(P - 1), n, 0, 0};
+  struct Plan { uint32_t mask, first, last, m; } plan{0xFFu, 0, uint32_t(P - 1), n};
   const bool analyse = n >= (1u << 18) && !capturing(s);
   if (analyse) {
     Plan* hp = reinterpret_cast<Plan*>(host_plan_words());
-    hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp), P);
+    hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp));
     AM_HIP(hipGetLastError());
     AM_HIP(hipStreamSynchronize(s));
     plan = *hp;
@@ -998,14 +1105,12 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   uint32_t* rp0 = static_cast<uint32_t*>(r0->ptr);
   uint32_t* rp1 = static_cast<uint32_t*>(r1->ptr);
   if constexpr (P == 8) {
-    // The hybrid sort, when the top two digits are expected to cut the value rows into buckets the
-    // LDS sort takes (the estimate assumes the two digits independent; the exact largest bucket is
-    // checked before the local sort, and the LSD passes below run instead when it is too large).
-    if (hybrid_ok && analyse && ((plan.mask >> 7) & 1u) && ((plan.mask >> 6) & 1u) &&
-        uint64_t(plan.top_max) * plan.second_max <= uint64_t(LOCAL_MAX) * 3u / 4u * plan.m &&
+    // The hybrid sort when the top digit varies; the LSD passes below run instead if its bucket list
+    // runs out of room.
+    if (hybrid_ok && analyse && ((plan.mask >> (P - 1)) & 1u) &&
         run_hybrid<K, W, AM_MSD_FITEMS, ITEMS>(
-            vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, kp0, rp0, kp1, rp1, out_idx,
-            unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, s))
+            vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, plan.mask, kp0, rp0, kp1, rp1,
+            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, s))
       return;
   }
   if (hybrid_ok) {

@@ -242,14 +242,20 @@ def msd_case(case, n):
         return bits.view(np.int64), None, "hybrid"
     if case == "low_digits_constant":   # only the top two digits vary: the local sort has no pass
         return ((bits & np.uint64(0xFFFF000000000000)) | np.uint64(0x1234)).view(np.int64), None, "hybrid"
-    if case == "correlated_top":        # second digit equals the top one: buckets of n/256 rows
+    if case == "correlated_top":        # second digit equals the top one: every top bucket is carried a level
         top = bits >> np.uint64(56)
         k = (top << np.uint64(56)) | (top << np.uint64(48)) | (bits & np.uint64((1 << 48) - 1))
-        return k.view(np.int64), None, "fallback"
-    if case == "duplicates":            # one key on 0.5% of the rows: one bucket above the LDS sort's limit
+        return k.view(np.int64), None, "hybrid"
+    if case == "duplicates":            # one key on 0.5% of the rows: a bucket that goes down every level
         k = bits.copy()
         k[(bits % np.uint64(200)) == 3] = np.uint64(0x0123456789ABCDEF)
-        return k.view(np.int64), None, "fallback"
+        return k.view(np.int64), None, "hybrid"
+    if case == "few_distinct":          # 3,000 distinct keys: buckets of equal keys, carried, then copied
+        return O.splitmix64(0xD15, 3000)[(bits >> np.uint64(5)) % np.uint64(3000)].view(np.int64), None, "hybrid"
+    if case == "narrow_floats":         # float64 in [1, 2) and [-2, -1): two top digits, the levels go deeper
+        v = 1.0 + (bits >> np.uint64(12)).astype(np.float64) / 2.0**52
+        v[(bits & np.uint64(1)) == 1] *= -1.0
+        return v, None, "hybrid"
     if case == "repeats":               # 2,000 keys repeated 60 times: runs the bucket LSD queue
         k = bits.copy()
         pick = (bits >> np.uint64(7)) % np.uint64(16) == 0
@@ -276,13 +282,15 @@ def msd_case(case, n):
 
 
 @pytest.mark.parametrize("case", ["uniform", "low_digits_constant", "correlated_top", "duplicates",
-                                  "repeats", "wide_buckets", "half_range", "float_bits", "skewed"])
+                                  "few_distinct", "narrow_floats", "repeats", "wide_buckets", "half_range",
+                                  "float_bits", "skewed"])
 def test_msd_hybrid_and_fallback(case):
-    """64-bit keys above 4M rows whose top two digits vary take the hybrid sort (a stable partition by
-    the top digit, a segmented partition by the second, an LDS sort of each bucket by the rest). A
-    bucket above the LDS sort's 4096 rows sends the sort back to the LSD passes; a bucket with many
-    equal keys is sorted by LSD passes in LDS; a skewed top digit never starts the hybrid. Each case
-    checks the path it took and the order against the oracle."""
+    """64-bit keys above 4M rows whose top digit varies take the hybrid sort: a stable partition by the
+    top digit, then level by level a segmented partition of every bucket above 4096 rows by the next
+    digit (buckets whose rows share the digit carried without moving), then an LDS sort of each
+    bucket by its remaining digits (LSD passes in LDS for buckets with many equal keys, a copy for
+    buckets of equal keys). A constant top digit keeps the LSD passes. Each case checks the path it
+    took and the order against the oracle."""
     n = 4_500_000
     vals, valid, path = msd_case(case, n)
     combos = [(False, 0, 0), (True, 1, 0)] + ([(False, 1, 1), (True, 0, 2)] if vals.dtype == np.float64 else [])
