@@ -353,8 +353,8 @@ def orderby_alg_bytes(n: int, m: int):
         "sort_upsweep": (8 * n + vb, "digit histograms: read key + validity; 8.125 B/row"),
         "sort_msd_first": (8 * n + vb + 12 * m + 4 * (n - m),
                            "top-digit partition: read key + validity, write key + row (null rows: index); ~20 B/row"),
-        "sort_msd_hist": (8 * m, "second-digit histograms of the partitioned keys: read key; 8 B/row"),
-        "sort_msd_scatter": (24 * m, "segmented second-digit partition: read key + row, write key + row; 24 B/row"),
+        "sort_msd_hist": (8 * m, "one level's digit histograms of the large buckets: read key; 8 B/row"),
+        "sort_msd_scatter": (24 * m, "one level's segmented partition: read key + row, write key + row; 24 B/row"),
         "sort_msd_local": (16 * m, "LDS bucket sort: read key + row, write index; 16 B/row"),
         "sort_first": (8 * n + vb + 12 * m + 4 * (n - m), "LSD first pass: read key + validity, write key + row"),
         "sort_scatter": (24 * m, "LSD middle pass: read key + row, write key + row; 24 B/row"),
@@ -367,8 +367,8 @@ ROCPROF_NAME = {
     "take": "take_kernel<u64, u32>",
     "sort_upsweep": "sort_upsweep<u64, 8>",
     "sort_msd_first": "sort_first<u64, 8, 16>",
-    "sort_msd_hist": "hyb_hist<u64, 8>",
-    "sort_msd_scatter": "hyb_seg_pass<u64, 8, 20>",
+    "sort_msd_hist": "lvl_hist<u64>",
+    "sort_msd_scatter": "lvl_pass<u64, 20>",
     "sort_msd_local": "hyb_local<u64, 8, 4>",
     "sort_first": "sort_first<u64, 8, 16>",
     "sort_scatter": "sort_mid<u64, 20, 0>",
