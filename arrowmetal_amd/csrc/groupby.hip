@@ -266,7 +266,21 @@ __global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, uns
 // BUILD_ROWS rows in flight: their keys are loaded, hashed and their first slots read together, so a
 // wave waits on one memory latency per BUILD_ROWS rows rather than per row. TRACK_FIRST also keeps
 // every slot's lowest row (the representative rows of float and multi-column keys).
+//
+// Each workgroup keeps the slots of the keys it has seen in an LDS cache (BUILD_CACHE entries, direct
+// mapped by high hash bits, each written once: the key by CAS, then its slot). A row whose key is
+// cached never reads the table: with skewed keys, the few hottest slot lines would otherwise take a
+// large share of all table reads, on one L2 channel each.
 constexpr int BUILD_ROWS = 8;
+constexpr uint32_t BUILD_CACHE = 2048;
+
+// Caches key -> slot in the workgroup's LDS if the key's entry is still free. The slot is written after
+// the key; a reader that finds the key with no slot yet reads the table instead.
+__device__ __forceinline__ void cache_slot(unsigned long long* ckey, uint32_t* cslot, unsigned long long k,
+                                           uint32_t slot) {
+  const uint32_t c = uint32_t(mix64(k) >> 40) & (BUILD_CACHE - 1u);
+  if (ckey[c] == EMPTY && atomicCAS(&ckey[c], EMPTY, k) == EMPTY) cslot[c] = slot;
+}
 
 template <bool TRACK_FIRST, int W>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
@@ -283,6 +297,13 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
   unsigned long long* tkey = table.key + tbase;
   uint32_t* tfirst = TRACK_FIRST ? table.first + tbase : nullptr;
   uint32_t* special_flag = private_tables ? &st->xspecial[tbl] : &st->special_used;
+  __shared__ unsigned long long ckey[BUILD_CACHE];
+  __shared__ uint32_t cslot[BUILD_CACHE];
+  for (uint32_t j = threadIdx.x; j < BUILD_CACHE; j += blockDim.x) {
+    ckey[j] = EMPTY;   // never a cached key: rows with the EMPTY key take the side slot
+    cslot[j] = NULL_SLOT;
+  }
+  __syncthreads();
   unsigned long long nulls = 0;
   bool overflow = false;
   for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
@@ -312,10 +333,24 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
       } else {
         k[r] = 0;
       }
-      s[r] = uint32_t(mix64(k[r])) & cap_mask;
+      const unsigned long long h = mix64(k[r]);
+      s[r] = uint32_t(h) & cap_mask;
+      // A cached key takes its slot from LDS (cat 4).
+      if (cat[r] == 0) {
+        const uint32_t c = uint32_t(h >> 40) & (BUILD_CACHE - 1u);
+        const uint32_t cs = cslot[c];
+        if (ckey[c] == k[r] && cs != NULL_SLOT) {
+          cat[r] = 4;
+          s[r] = cs;
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? probe_load(&tkey[s[r]]) : 0ull;
+    // Rows whose key already sits in its home slot (most rows of a repeated key) finish here; the
+    // others (a first insert or a collision) go through the probe loop one at a time below, with the
+    // key read again, so that the loop is not inlined per row.
+    uint32_t slow = 0;
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
@@ -328,10 +363,27 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
       } else if (cat[r] == 2) {
         slot = special;
         if (!*special_flag) atomicExch(special_flag, 1u);
+      } else if (cat[r] == 4) {
+        slot = s[r];
+      } else if (cur[r] == k[r]) {
+        slot = s[r];
+        cache_slot(ckey, cslot, k[r], slot);
       } else {
-        slot = probe_insert(tkey, k[r], s[r], cur[r], cap_mask, budget);
-        if (slot == NULL_SLOT) overflow = true;
+        slow |= 1u << r;
+        continue;
       }
+      __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : tbase + slot, slot_of + i);
+      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&tfirst[slot], i);
+    }
+    while (slow) {
+      const int r = __builtin_ctz(slow);
+      slow &= slow - 1u;
+      const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
+      const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), type);
+      const uint32_t ss = uint32_t(mix64(kk)) & cap_mask;
+      const uint32_t slot = probe_insert(tkey, kk, ss, probe_load(&tkey[ss]), cap_mask, budget);
+      if (slot == NULL_SLOT) overflow = true;
+      else cache_slot(ckey, cslot, kk, slot);
       __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : tbase + slot, slot_of + i);
       if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&tfirst[slot], i);
     }
