@@ -498,21 +498,43 @@ __global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t
   }
 }
 
+// Rows' slots to group ids, in place. Each workgroup caches slot -> id pairs in LDS (one 64-bit word
+// per entry, (slot + 1) | id << 32, direct mapped; any word a reader finds is a whole pair), so rows
+// of hot groups do not all read the same few lines of the id table.
+constexpr uint32_t IDS_CACHE = 2048;   // (slot * 2654435761) >> 21 indexes it
+static_assert(IDS_CACHE == (1u << 11), "the entry hash gives 11 bits");
 __global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot,
                                               uint32_t null_gid) {
+  __shared__ unsigned long long cache[IDS_CACHE];
+  for (uint32_t j = threadIdx.x; j < IDS_CACHE; j += blockDim.x) cache[j] = 0ull;
+  __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  auto entry = [](uint32_t sl) { return (sl * 2654435761u) >> 21; };   // [0, IDS_CACHE)
   for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
-    uint32_t sl[BUILD_ROWS];
+    uint32_t sl[BUILD_ROWS], g[BUILD_ROWS];
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint64_t i = base + uint64_t(r) * gsize + tid;
       sl[r] = i < n ? __builtin_nontemporal_load(slot_of + i) : NULL_SLOT;
     }
+    bool miss[BUILD_ROWS];
+#pragma unroll
+    for (int r = 0; r < BUILD_ROWS; r++) {
+      miss[r] = false;
+      if (sl[r] == NULL_SLOT) {
+        g[r] = null_gid;
+        continue;
+      }
+      const unsigned long long e = cache[entry(sl[r])];
+      miss[r] = uint32_t(e) != sl[r] + 1u;
+      g[r] = miss[r] ? gid_of_slot[sl[r]] : uint32_t(e >> 32);
+    }
 #pragma unroll
     for (int r = 0; r < BUILD_ROWS; r++) {
       uint64_t i = base + uint64_t(r) * gsize + tid;
-      if (i < n) __builtin_nontemporal_store(sl[r] == NULL_SLOT ? null_gid : gid_of_slot[sl[r]], slot_of + i);
+      if (miss[r]) cache[entry(sl[r])] = (unsigned long long)(sl[r] + 1u) | ((unsigned long long)g[r] << 32);
+      if (i < n) __builtin_nontemporal_store(g[r], slot_of + i);
     }
   }
 }
