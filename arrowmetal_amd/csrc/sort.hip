@@ -69,14 +69,16 @@ __global__ void sort_read_plan(const SortCtl* ctl, uint32_t* out) {
 // Rows per unit of the top-digit partition (a power of two): the upsweep counts the top digit, NaN
 // and null rows per unit when given unit_hist ([NBINS_FIRST][nunits], bin-major), so that the
 // partition knows every unit's destinations before it starts.
-constexpr uint32_t UNIT_SHIFT = 14, UNIT_ROWS = 1u << UNIT_SHIFT;
+// Units are 2^unit_shift rows: 16K for 4M rows and more, 4K (one tile of the partition) below, so
+// that a 1M-row input still has ~250 units to spread over the CUs.
+constexpr uint32_t UNIT_SHIFT_LARGE = 14, UNIT_SHIFT_SMALL = 12;
 constexpr uint32_t UNITS_PER_BLOCK = 8;   // most units one upsweep workgroup counts
 
 template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const uint8_t* valid, int64_t voff,
                                                       uint32_t n, int type, int forder, int desc,
                                                       uint32_t rows_per_block, SortCtl* ctl, uint32_t* unit_hist,
-                                                      uint32_t nunits, int all_digits) {
+                                                      uint32_t nunits, uint32_t unit_shift, int all_digits) {
   constexpr int P = int(sizeof(K));
   __shared__ uint32_t h[P][RADIX];
   __shared__ uint32_t hu[UNITS_PER_BLOCK][NBINS_FIRST];
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
     }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
-      const uint32_t ui = (base + uint32_t(u) * BLOCK + threadIdx.x - begin) >> UNIT_SHIFT;
+      const uint32_t ui = (base + uint32_t(u) * BLOCK + threadIdx.x - begin) >> unit_shift;
       if (cc[u] == 0) {
         K k = kk[u];
         cm++;
@@ -159,8 +161,8 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
   __syncthreads();
   if (units) {
     // The block's top-digit histogram is the sum of its units'; every unit's counts go out bin-major.
-    const uint32_t u0 = begin >> UNIT_SHIFT;
-    const uint32_t nu = (end - begin + UNIT_ROWS - 1) >> UNIT_SHIFT;
+    const uint32_t u0 = begin >> unit_shift;
+    const uint32_t nu = (end - begin + (1u << unit_shift) - 1u) >> unit_shift;
     for (uint32_t b = threadIdx.x; b < uint32_t(NBINS_FIRST); b += BLOCK) {
       uint32_t sum = 0;
       for (uint32_t ui = 0; ui < nu; ui++) {
@@ -429,11 +431,12 @@ __global__ __launch_bounds__(BLOCK) void msd_unit_scan(const SortCtl* ctl, const
 template <typename K, int W, int ITEMS>
 __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uint8_t* valid, int64_t voff,
                                                      uint32_t n, int type, int forder, int desc, SortCtl* ctl,
-                                                     const uint32_t* unit_base, uint32_t nunits, K* out_keys,
-                                                     uint32_t* out_rows, uint32_t* out_idx) {
+                                                     const uint32_t* unit_base, uint32_t nunits,
+                                                     uint32_t unit_shift, K* out_keys, uint32_t* out_rows,
+                                                     uint32_t* out_idx) {
   constexpr int P = int(sizeof(K));
   constexpr int TILE = BLOCK * ITEMS;
-  static_assert(UNIT_ROWS % TILE == 0, "units hold whole tiles");
+  static_assert((1u << UNIT_SHIFT_SMALL) % TILE == 0, "units hold whole tiles");
   __shared__ uint32_t wc[WAVES][WC_STRIDE];
   __shared__ uint32_t tile_total[WC_STRIDE];
   __shared__ uint32_t bin_start[RADIX];
@@ -452,8 +455,8 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
     const uint32_t unit = s_unit;
     if (unit >= nunits) return;   // uniform: every workgroup ends here once the units run out
     for (int b = threadIdx.x; b < NBINS_FIRST; b += BLOCK) running[b] = unit_base[size_t(b) * nunits + unit];
-    const uint32_t begin = unit << UNIT_SHIFT;
-    const uint32_t end = min(n, begin + UNIT_ROWS);
+    const uint32_t begin = unit << unit_shift;
+    const uint32_t end = min(n, begin + (1u << unit_shift));
     for (uint32_t t0 = begin; t0 < end; t0 += TILE) {
       for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
       __syncthreads();
@@ -910,7 +913,7 @@ template <typename K, int W, int FITEMS, int ITEMS>
 bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
                 SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, uint32_t mask, K* k0,
                 uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base,
-                uint32_t nunits, hipStream_t s) {
+                uint32_t nunits, uint32_t unit_shift, hipStream_t s) {
   constexpr int P = int(sizeof(K));
   constexpr uint32_t TILE = BLOCK * ITEMS;
   // Capacities: buckets above LOCAL_MAX rows at one level; entries of the bucket sort's list (every
@@ -940,7 +943,7 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
     ProfScope ps("sort_msd_first");
     const uint32_t workers = std::min(nunits, uint32_t(device().cu_count) * 3u);
     hipLaunchKernelGGL((msd_part<K, W, FITEMS>), dim3(workers), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, ctl, unit_base, nunits, k0, r0, out_idx);
+                       dsc, ctl, unit_base, nunits, unit_shift, k0, r0, out_idx);
     AM_HIP(hipGetLastError());
   }
   {
@@ -1060,8 +1063,10 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     AM_HIP(hipGetLastError());
   }
   // Large 64-bit inputs may take the hybrid sort: the upsweep then also counts its units.
-  const bool hybrid_ok = P == 8 && n >= (1u << 22) && !capturing(s);
-  const uint32_t nunits = hybrid_ok ? (n + UNIT_ROWS - 1) / UNIT_ROWS : 0u;
+  const bool hybrid_ok = P == 8 && n >= (1u << 18) && !capturing(s);
+  const uint32_t unit_shift = n >= (1u << 22) ? UNIT_SHIFT_LARGE : UNIT_SHIFT_SMALL;
+  const uint32_t unit_rows = 1u << unit_shift;
+  const uint32_t nunits = hybrid_ok ? (n + unit_rows - 1) / unit_rows : 0u;
   DevBuf units_buf;
   uint32_t* unit_hist = nullptr;
   if (hybrid_ok) {
@@ -1074,14 +1079,14 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     uint32_t max_blocks = uint32_t(device().cu_count) * 4u;
     uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 8191u) / 8192u));
     // Block ranges are whole units when units are counted, at most UNITS_PER_BLOCK of them.
-    const uint32_t align = units ? UNIT_ROWS : 2048u;
+    const uint32_t align = units ? unit_rows : 2048u;
     uint32_t rpb = ((n + blocks - 1) / blocks + align - 1) / align * align;
-    if (units) rpb = std::min(rpb, UNITS_PER_BLOCK * UNIT_ROWS);
+    if (units) rpb = std::min(rpb, UNITS_PER_BLOCK * unit_rows);
     blocks = (n + rpb - 1) / rpb;
     {
       ProfScope ps("sort_upsweep");
       hipLaunchKernelGGL((sort_upsweep<K, W>), dim3(blocks), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                         dsc, rpb, ctl, units ? unit_hist : nullptr, nunits, units ? 0 : 1);
+                         dsc, rpb, ctl, units ? unit_hist : nullptr, nunits, unit_shift, units ? 0 : 1);
       AM_HIP(hipGetLastError());
     }
     ProfScope ps("sort_plan");
@@ -1110,7 +1115,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     if (hybrid_ok && analyse && ((plan.mask >> (P - 1)) & 1u) &&
         run_hybrid<K, W, AM_MSD_FITEMS, ITEMS>(
             vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, plan.mask, kp0, rp0, kp1, rp1,
-            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, s))
+            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, unit_shift, s))
       return;
   }
   if (hybrid_ok) {

@@ -206,19 +206,25 @@ def test_order_by_gather_config_shape():
 
 
 @pytest.mark.parametrize("order", [0, 1])
-def test_large_tiles_first_pass_and_nulls(order):
-    """Above 8.4M rows the scatter passes take 20 keys per thread and the first pass 16: both tile
-    shapes (and their look-back chains, NaN and null bins included) against the oracle at 9M rows."""
+@pytest.mark.parametrize("width", [8, 4])
+def test_large_tiles_first_pass_and_nulls(order, width):
+    """9M rows, NaN and null bins included, against the oracle. 64-bit keys take the hybrid sort;
+    32-bit keys keep the LSD passes, whose scatter passes take 20 keys per thread above 8.4M rows and
+    whose first pass takes 16: both tile shapes and their look-back chains."""
     n = 9_000_001
-    print(f"large sort n={n} order={order}", flush=True)
+    print(f"large sort n={n} order={order} width={width}", flush=True)
     bits = O.splitmix64(0x1A2B + order, n)
     valid = (O.splitmix64(0x3C4D, n) >> np.uint64(11)).astype(np.float64) / 2.0**53 < 0.97
     if order == 0:
         vals = (bits >> np.uint64(1)).view(np.int64) - (1 << 62)
+        if width == 4:
+            vals = (vals >> 32).astype(np.int32)
     else:
         vals = ((bits >> np.uint64(11)).astype(np.float64) / 2.0**53 - 0.5) * 1e6
         vals[(bits & np.uint64(1023)) == 7] = np.nan
         vals[(bits & np.uint64(1023)) == 9] = -0.0
+        if width == 4:
+            vals = vals.astype(np.float32)
     for desc, place in ((True, 0), (False, 1)):
         check(vals, valid, desc, place, order, label="large")
 
@@ -304,3 +310,20 @@ def test_msd_hybrid_and_fallback(case):
                                  f"got {got[bad]} want {want[bad]}")
         took = ("hybrid" if "sort_msd_local" in names else "fallback" if "sort_msd_first" in names else "lsd")
         assert took == path, f"{case}: took the {took} path, kernels {sorted(names)}"
+
+
+@pytest.mark.parametrize("n", [1 << 18, 300_007, 1_000_000, (1 << 22) - 1])
+@pytest.mark.parametrize("case", ["uniform", "float_bits", "half_range", "duplicates"])
+def test_msd_hybrid_small_inputs(n, case):
+    """From 2^18 rows, 64-bit keys take the hybrid sort with 4K-row units (16K from 2^22 rows): units
+    that end inside a tile, top-digit buckets that go straight to the bucket sort, and the levels
+    when a bucket is larger."""
+    vals, valid, _ = msd_case(case, n)
+    for desc, place, order in ((False, 0, 0), (True, 1, 1)):
+        print(f"small msd case={case} n={n} desc={desc} place={place} order={order}", flush=True)
+        got, names = sort_with_kernels(vals, valid, desc, place, order)
+        want = O.argsort(vals, valid, desc, place, order)
+        if not np.array_equal(got, want):
+            bad = int(np.nonzero(got != want)[0][0])
+            raise AssertionError(f"{case} n={n}: first difference at {bad}: got {got[bad]} want {want[bad]}")
+        assert "sort_msd_local" in names, f"{case} n={n}: kernels {sorted(names)}"
