@@ -550,6 +550,7 @@ constexpr int LC_TILES = 4;       // tiles of the current level
 constexpr int LC_MAXLOCAL = 5;    // largest bucket the bucket sort sorts (not counting copies)
 constexpr int LC_OVERFLOW = 6;    // the bucket sort's list ran out of room
 constexpr int LC_QLEN = 7;        // buckets queued for the LSD bucket sort
+constexpr int LC_ROWS = 8;        // rows in the next level's large buckets (bounds its grids)
 constexpr int LC_READ = 16;
 constexpr int LC_NEXT_TILE = 32;  // the next tile of the current level to take (own 128-byte line)
 constexpr int LC_QNEXT = 64;      // the next queued bucket to take (own 128-byte line)
@@ -579,70 +580,121 @@ __global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, Buck
   else push_local(local, local_cap, lc, bk, low == 0u);
 }
 
-// One workgroup: the chunk and tile tables of a level over its large buckets (list `cur`), in list
-// order; the bucket histograms cleared; the other list's counter and the tile counter reset.
+// One workgroup: where each large bucket's chunks and tiles start in a level's tables (list `cur`
+// order; bofs[2b], bofs[2b + 1]); the bucket histograms and OR / AND cleared; the other list's
+// counter, the tile counter and the row count reset. lvl_fill writes the tables.
 template <int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, LvlChunk* chunks, LvlTile* tiles,
-                                                    uint32_t* bucket_hist, uint32_t* lc) {
+__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, uint32_t* bofs,
+                                                    uint32_t* bucket_hist, unsigned long long* bucket_or,
+                                                    unsigned long long* bucket_and, uint32_t* lc) {
   __shared__ uint32_t tmp[WAVES];
   constexpr uint32_t TILE = BLOCK * ITEMS;
   const uint32_t nb = lc[LC_LIST + cur];
   uint32_t coff = 0, toff = 0;
   for (uint32_t b0 = 0; b0 < nb; b0 += BLOCK) {
     const uint32_t b = b0 + threadIdx.x;
-    Bucket bk{0, 0, 0, 0};
-    if (b < nb) bk = large[b];
-    const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
+    const uint32_t size = b < nb ? large[b].size : 0u;
+    const uint32_t nc = (size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (size + TILE - 1) / TILE;
     uint32_t tc, tt;
     const uint32_t c0 = coff + block_excl_scan(nc, tmp, tc);
     const uint32_t t0 = toff + block_excl_scan(nt, tmp, tt);
-    for (uint32_t c = 0; c < nc; c++)
-      chunks[c0 + c] = {b, bk.start + c * LVL_CHUNK, bk.start + min(bk.size, (c + 1u) * LVL_CHUNK), 0u};
-    for (uint32_t t = 0; t < nt; t++) tiles[t0 + t] = {b, bk.start + t * TILE, bk.start + min(bk.size, (t + 1u) * TILE), t0};
+    if (b < nb) {
+      bofs[2 * b] = c0;
+      bofs[2 * b + 1] = t0;
+    }
     coff += tc;
     toff += tt;
   }
   for (uint32_t i = threadIdx.x; i < nb * RADIX; i += BLOCK) bucket_hist[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) {
+    bucket_or[i] = 0ull;
+    bucket_and[i] = ~0ull;
+  }
   if (threadIdx.x == 0) {
     lc[LC_CHUNKS] = coff;
     lc[LC_TILES] = toff;
     lc[LC_NEXT_TILE] = 0u;
     lc[LC_LIST + (cur ^ 1)] = 0u;
+    lc[LC_ROWS] = 0u;
   }
 }
 
-// Digit d's histogram of every large bucket, one chunk per workgroup (surplus workgroups exit).
+// One workgroup per large bucket (surplus workgroups exit): its chunks and tiles, written by all threads.
+template <int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_fill(const Bucket* large, int cur, const uint32_t* lc,
+                                                  const uint32_t* bofs, LvlChunk* chunks, LvlTile* tiles) {
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t b = blockIdx.x;
+  if (b >= lc[LC_LIST + cur]) return;
+  const Bucket bk = large[b];
+  const uint32_t c0 = bofs[2 * b], t0 = bofs[2 * b + 1];
+  const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
+  for (uint32_t c = threadIdx.x; c < nc; c += BLOCK)
+    chunks[c0 + c] = {b, bk.start + c * LVL_CHUNK, bk.start + min(bk.size, (c + 1u) * LVL_CHUNK), 0u};
+  for (uint32_t t = threadIdx.x; t < nt; t += BLOCK)
+    tiles[t0 + t] = {b, bk.start + t * TILE, bk.start + min(bk.size, (t + 1u) * TILE), t0};
+}
+
+// Digit d's histogram, and the OR / AND of the keys, of every large bucket, one chunk per workgroup
+// (surplus workgroups exit).
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1, const Bucket* large,
                                                   const LvlChunk* chunks, const uint32_t* lc, int d,
-                                                  uint32_t* bucket_hist) {
+                                                  uint32_t* bucket_hist, unsigned long long* bucket_or,
+                                                  unsigned long long* bucket_and) {
   __shared__ uint32_t h[RADIX];
+  __shared__ unsigned long long s_or, s_and;
   if (blockIdx.x >= lc[LC_CHUNKS]) return;
   const LvlChunk c = chunks[blockIdx.x];
   const K* keys = large[c.bucket].buf ? keys1 : keys0;
   h[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    s_or = 0ull;
+    s_and = ~0ull;
+  }
   __syncthreads();
+  unsigned long long o = 0ull, a = ~0ull;
   constexpr int U = 8;
   for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
     K k[U];
 #pragma unroll
     for (int u = 0; u < U; u++) k[u] = keys[min(base + uint32_t(u) * BLOCK, c.end - 1u)];
 #pragma unroll
-    for (int u = 0; u < U; u++)
+    for (int u = 0; u < U; u++) {
+      // Rows past the end read the chunk's last row, which does not change the OR / AND.
+      o |= (unsigned long long)k[u];
+      a &= (unsigned long long)k[u];
       if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * d)) & 0xFFu], 1u);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    o |= __shfl_xor(o, off, 64);
+    a &= __shfl_xor(a, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && c.start + threadIdx.x < c.end) {   // waves that read at least one row
+    atomicOr(&s_or, o);
+    atomicAnd(&s_and, a);
   }
   __syncthreads();
   const uint32_t v = h[threadIdx.x];
   if (v) atomicAdd(&bucket_hist[size_t(c.bucket) * RADIX + threadIdx.x], v);
+  if (threadIdx.x == 0) {
+    atomicOr(&bucket_or[c.bucket], s_or);
+    atomicAnd(&bucket_and[c.bucket], s_and);
+  }
 }
 
 // One workgroup per large bucket: the start of each of its digit-d sub-buckets (bucket_base), and
-// where each goes: to the bucket sort's list (at most LOCAL_MAX rows, or equal keys after the lowest
-// active digit `last`), or to the next level's list. A bucket whose rows all share digit d is carried
-// (carried[b] = 1): it keeps its buffer and the pass skips it.
-__global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist, int d,
-                                                  int last, uint32_t* bucket_base, uint32_t* carried,
-                                                  Bucket* next, Bucket* local, uint32_t local_cap, uint32_t* lc) {
+// where each goes: to the bucket sort's list (at most LOCAL_MAX rows; or, of any size, equal keys: the
+// bucket's keys differ in no bit below digit d, or d is the lowest active digit `last`), or to the
+// next level's list. A bucket whose rows all share digit d is carried (carried[b] = 1): it keeps its
+// buffer and the pass skips it.
+__global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist,
+                                                  const unsigned long long* bucket_or,
+                                                  const unsigned long long* bucket_and, int d, int last,
+                                                  uint32_t* bucket_base, uint32_t* carried, Bucket* next,
+                                                  Bucket* local, uint32_t local_cap, uint32_t* lc) {
   __shared__ uint32_t tmp[WAVES];
   const uint32_t b = blockIdx.x;
   if (b >= lc[LC_LIST + cur]) return;   // uniform over the workgroup
@@ -655,10 +707,17 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
   const bool carry = bins == 1;
   if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
   if (v == 0u) return;
-  const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), uint32_t(d)};
-  if (last) push_local(local, local_cap, lc, sub, true);
-  else if (v <= LOCAL_MAX) push_local(local, local_cap, lc, sub, false);
-  else next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
+  const bool equal_below = ((bucket_or[b] ^ bucket_and[b]) & ((1ull << (8 * d)) - 1ull)) == 0ull;
+  // A sub-bucket of equal keys is copied: level 0 leaves it no digit to sort by.
+  const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), equal_below ? 0u : uint32_t(d)};
+  if (last || equal_below) {
+    push_local(local, local_cap, lc, sub, true);
+  } else if (v <= LOCAL_MAX) {
+    push_local(local, local_cap, lc, sub, false);
+  } else {
+    next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
+    atomicAdd(&lc[LC_ROWS], v);
+  }
 }
 
 // A level's pass: every large bucket that is not carried partitioned stably by digit d from its buffer
@@ -934,6 +993,10 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   uint32_t* bucket_base = bucket_hist + size_t(large_cap) * RADIX;
   uint32_t* carried = bucket_base + size_t(large_cap) * RADIX;
   uint32_t* queue = carried + large_cap;
+  DevBuf orand = device_alloc(size_t(large_cap) * 24);
+  unsigned long long* bucket_or = static_cast<unsigned long long*>(orand->ptr);
+  unsigned long long* bucket_and = bucket_or + large_cap;
+  uint32_t* bofs = reinterpret_cast<uint32_t*>(bucket_and + large_cap);   // 2 words per bucket
   {
     ProfScope ps("sort_msd_unit_scan");
     hipLaunchKernelGGL(msd_unit_scan<P>, dim3(NBINS_FIRST), dim3(BLOCK), 0, s, ctl, unit_hist, nunits, unit_base);
@@ -958,32 +1021,33 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
     while (--d >= 0 && !((mask >> d) & 1u)) {}
     return d;
   };
-  uint32_t bound = uint32_t(RADIX);
+  uint32_t bound = uint32_t(RADIX), rows = m;   // large buckets and their rows (bounds of the grids)
   int cur = 0;
   bool read = false;
   for (int d = next_digit(P - 1); d >= 0 && bound > 0; d = next_digit(d)) {
     const int last = next_digit(d) < 0 ? 1 : 0;
     {
       ProfScope ps("sort_msd_layout");
-      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, large[cur], cur, chunks, tiles, bucket_hist,
-                         lc);
+      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, large[cur], cur, bofs, bucket_hist,
+                         bucket_or, bucket_and, lc);
+      hipLaunchKernelGGL(lvl_fill<ITEMS>, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, lc, bofs, chunks, tiles);
       AM_HIP(hipGetLastError());
     }
     {
       ProfScope ps("sort_msd_hist");
-      hipLaunchKernelGGL(lvl_hist<K>, dim3(m / LVL_CHUNK + bound + 1u), dim3(BLOCK), 0, s, k0, k1, large[cur],
-                         chunks, lc, d, bucket_hist);
+      hipLaunchKernelGGL(lvl_hist<K>, dim3(rows / LVL_CHUNK + bound + 1u), dim3(BLOCK), 0, s, k0, k1, large[cur],
+                         chunks, lc, d, bucket_hist, bucket_or, bucket_and);
       AM_HIP(hipGetLastError());
     }
     {
       ProfScope ps("sort_msd_plan");
-      hipLaunchKernelGGL(lvl_plan, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, bucket_hist, d, last, bucket_base,
-                         carried, large[cur ^ 1], local, local_cap, lc);
+      hipLaunchKernelGGL(lvl_plan, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, bucket_hist, bucket_or,
+                         bucket_and, d, last, bucket_base, carried, large[cur ^ 1], local, local_cap, lc);
       AM_HIP(hipGetLastError());
     }
     {
       ProfScope ps("sort_msd_scatter");
-      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(m / TILE + bound + 1u), dim3(BLOCK), 0, s, large[cur], tiles, lc,
+      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(rows / TILE + bound + 1u), dim3(BLOCK), 0, s, large[cur], tiles, lc,
                          carried, bucket_base, status, epoch + 8u + uint32_t(P - 1 - d), d, k0, r0, k1, r1);
       AM_HIP(hipGetLastError());
     }
@@ -992,6 +1056,7 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
     read = true;
     cur ^= 1;
     bound = hp[LC_LIST + cur];
+    rows = hp[LC_ROWS];
     if (bound > large_cap) fail("argsort: bucket list overflow");   // cannot happen: each holds > LOCAL_MAX rows
   }
   if (!read) {   // no level ran (no active digit below the top one)
