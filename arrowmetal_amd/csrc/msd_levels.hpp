@@ -1,0 +1,647 @@
+// The levels of the adaptive MSD radix argsort of 64-bit keys, shared by the plain argsort (sort.hip)
+// and the batched plan (batched.hip).
+//
+// Rows are in buckets (Bucket: a range of the value block, in one of two key / row buffers, whose keys
+// share their digits from the top down to `level`). Level by level down the digits, every bucket
+// larger than `local_max` rows is partitioned stably by the next digit (onesweep tiles whose
+// look-back stops at their bucket's first tile) into the other buffer at the same positions; a bucket
+// whose rows all share the digit is carried without moving; a bucket whose keys differ in no lower
+// bit leaves as a copy. Buckets of at most `local_max` rows leave for the bucket sort (hyb_local),
+// which sorts each in LDS by its remaining digits and writes its rows to their final slots.
+//
+// The kernels take their counts from device memory (the `lc` counters and an MsdCtl), so a level can
+// run from host read-backs (the plain argsort sizes each launch from them) or blind, inside a graph
+// capture (the batched plan launches every digit's level with grids sized for the worst case; work-
+// groups past the work exit, and a level of an inactive digit carries every bucket).
+#pragma once
+
+#include "sort_common.hpp"
+
+namespace am {
+namespace {
+
+constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
+constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes (local_max is at most this)
+
+// A bucket: rows [start, start + size) of the value block, in key / row buffer `buf` (0 or 1), all
+// sharing the digits from the top down to `level`.
+struct Bucket {
+  uint32_t start, size, buf, level;
+};
+struct LvlChunk {
+  uint32_t bucket, start, end, pad;
+};
+struct LvlTile {
+  uint32_t bucket, start, end, head;   // head: the bucket's first tile
+};
+// What the level kernels read besides their lists: the active digits (bit d: digit d varies over
+// the value keys), where the value block starts in the output, and the run's look-back epoch.
+struct MsdCtl {
+  uint32_t active_mask, value_base, epoch, pad;
+};
+
+// Device counters of the levels (one small array). The host reads back the first LC_READ words. The
+// two counters that every workgroup of a kernel bumps each sit on a cache line of their own: sharing a
+// line with words the same workgroups read (the tile count) made the level pass 1.8x slower.
+constexpr int LC_LIST = 0;        // [0], [1]: entries of the two large-bucket lists
+constexpr int LC_LOCAL = 2;       // entries of the bucket sort's list
+constexpr int LC_CHUNKS = 3;      // chunks of the current level
+constexpr int LC_TILES = 4;       // tiles of the current level
+constexpr int LC_MAXLOCAL = 5;    // largest bucket the bucket sort sorts (not counting copies)
+constexpr int LC_OVERFLOW = 6;    // the bucket sort's list ran out of room
+constexpr int LC_QLEN = 7;        // buckets queued for the LSD bucket sort
+constexpr int LC_ROWS = 8;        // rows in the next level's large buckets (bounds its grids)
+constexpr int LC_READ = 16;
+constexpr int LC_NEXT_TILE = 32;  // the next tile of the current level to take (own 128-byte line)
+constexpr int LC_QNEXT = 64;      // the next queued bucket to take (own 128-byte line)
+constexpr int LC_WORDS = 96;
+
+// Appends one bucket to the bucket sort's list. `sorted` = its rows need no sorting (equal keys).
+__device__ __forceinline__ void push_local(Bucket* local, uint32_t local_cap, uint32_t* lc, Bucket b, bool sorted) {
+  const uint32_t i = atomicAdd(&lc[LC_LOCAL], 1u);
+  if (i < local_cap) local[i] = b;
+  else atomicOr(&lc[LC_OVERFLOW], 1u);
+  if (!sorted) atomicMax(&lc[LC_MAXLOCAL], b.size);
+}
+
+// One workgroup: where each large bucket's chunks and tiles start in a level's tables (list `cur`
+// order; bofs[2b], bofs[2b + 1]); the bucket histograms and OR / AND cleared; the other list's
+// counter, the tile counter and the row count reset. lvl_fill writes the tables.
+template <int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, uint32_t* bofs,
+                                                    uint32_t* bucket_hist, unsigned long long* bucket_or,
+                                                    unsigned long long* bucket_and, uint32_t* lc) {
+  __shared__ uint32_t tmp[WAVES];
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t nb = lc[LC_LIST + cur];
+  uint32_t coff = 0, toff = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += BLOCK) {
+    const uint32_t b = b0 + threadIdx.x;
+    const uint32_t size = b < nb ? large[b].size : 0u;
+    const uint32_t nc = (size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (size + TILE - 1) / TILE;
+    uint32_t tc, tt;
+    const uint32_t c0 = coff + block_excl_scan(nc, tmp, tc);
+    const uint32_t t0 = toff + block_excl_scan(nt, tmp, tt);
+    if (b < nb) {
+      bofs[2 * b] = c0;
+      bofs[2 * b + 1] = t0;
+    }
+    coff += tc;
+    toff += tt;
+  }
+  for (uint32_t i = threadIdx.x; i < nb * RADIX; i += BLOCK) bucket_hist[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) {
+    bucket_or[i] = 0ull;
+    bucket_and[i] = ~0ull;
+  }
+  if (threadIdx.x == 0) {
+    lc[LC_CHUNKS] = coff;
+    lc[LC_TILES] = toff;
+    lc[LC_NEXT_TILE] = 0u;
+    lc[LC_LIST + (cur ^ 1)] = 0u;
+    lc[LC_ROWS] = 0u;
+  }
+}
+
+// The chunks and tiles of each large bucket, a bucket per workgroup in turn, written by all threads.
+template <int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_fill(const Bucket* large, int cur, const uint32_t* lc,
+                                                  const uint32_t* bofs, LvlChunk* chunks, LvlTile* tiles) {
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t nb = lc[LC_LIST + cur];
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const Bucket bk = large[b];
+    const uint32_t c0 = bofs[2 * b], t0 = bofs[2 * b + 1];
+    const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
+    for (uint32_t c = threadIdx.x; c < nc; c += BLOCK)
+      chunks[c0 + c] = {b, bk.start + c * LVL_CHUNK, bk.start + min(bk.size, (c + 1u) * LVL_CHUNK), 0u};
+    for (uint32_t t = threadIdx.x; t < nt; t += BLOCK)
+      tiles[t0 + t] = {b, bk.start + t * TILE, bk.start + min(bk.size, (t + 1u) * TILE), t0};
+  }
+}
+
+// Digit d's histogram, and the OR / AND of the keys, of every large bucket, a chunk per workgroup in
+// turn (nothing when digit d is inactive).
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1, const Bucket* large,
+                                                  const LvlChunk* chunks, const uint32_t* lc, const MsdCtl* mc,
+                                                  int d, uint32_t* bucket_hist, unsigned long long* bucket_or,
+                                                  unsigned long long* bucket_and) {
+  __shared__ uint32_t h[RADIX];
+  __shared__ unsigned long long s_or, s_and;
+  if (!((mc->active_mask >> d) & 1u)) return;   // uniform over the grid
+  const uint32_t nch = lc[LC_CHUNKS];
+  for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+    const LvlChunk c = chunks[ci];
+    const K* keys = large[c.bucket].buf ? keys1 : keys0;
+    h[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+      s_or = 0ull;
+      s_and = ~0ull;
+    }
+    __syncthreads();
+    unsigned long long o = 0ull, a = ~0ull;
+    constexpr int U = 8;
+    for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
+      K k[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) k[u] = keys[min(base + uint32_t(u) * BLOCK, c.end - 1u)];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        // Rows past the end read the chunk's last row, which does not change the OR / AND.
+        o |= (unsigned long long)k[u];
+        a &= (unsigned long long)k[u];
+        if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * d)) & 0xFFu], 1u);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      o |= __shfl_xor(o, off, 64);
+      a &= __shfl_xor(a, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {   // waves that read no row hold the identities
+      atomicOr(&s_or, o);
+      atomicAnd(&s_and, a);
+    }
+    __syncthreads();
+    const uint32_t v = h[threadIdx.x];
+    if (v) atomicAdd(&bucket_hist[size_t(c.bucket) * RADIX + threadIdx.x], v);
+    if (threadIdx.x == 0) {
+      atomicOr(&bucket_or[c.bucket], s_or);
+      atomicAnd(&bucket_and[c.bucket], s_and);
+    }
+    __syncthreads();
+  }
+}
+
+// Each large bucket, a workgroup per bucket in turn: the start of each of its digit-d sub-buckets
+// (bucket_base), and where each goes: to the bucket sort's list (at most local_max rows; or, of any
+// size, equal keys: the bucket's keys differ in no bit below digit d, or no active digit is left
+// below d), or to the next level's list. A bucket whose rows all share digit d (every bucket, when
+// digit d is inactive) is carried (carried[b] = 1): it keeps its buffer and the pass skips it.
+__global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist,
+                                                  const unsigned long long* bucket_or,
+                                                  const unsigned long long* bucket_and, const MsdCtl* mc, int d,
+                                                  uint32_t local_max, uint32_t* bucket_base, uint32_t* carried,
+                                                  Bucket* next, Bucket* local, uint32_t local_cap, uint32_t* lc) {
+  __shared__ uint32_t tmp[WAVES];
+  const uint32_t nb = lc[LC_LIST + cur];
+  const uint32_t mask = mc->active_mask;
+  const bool active = (mask >> d) & 1u;
+  const bool last = (mask & ((1u << d) - 1u)) == 0u;
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const Bucket bk = large[b];
+    uint32_t v, base;
+    bool carry;
+    if (active) {
+      v = bucket_hist[size_t(b) * RADIX + threadIdx.x];
+      uint32_t total;
+      base = bk.start + block_excl_scan(v, tmp, total);
+      bucket_base[size_t(b) * RADIX + threadIdx.x] = base;
+      carry = __syncthreads_count(v != 0u ? 1 : 0) == 1;
+    } else {   // one sub-bucket: the whole bucket, kept by thread 0
+      v = threadIdx.x == 0 ? bk.size : 0u;
+      base = bk.start;
+      carry = true;
+    }
+    if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
+    if (v != 0u) {
+      const unsigned long long diff = active ? (bucket_or[b] ^ bucket_and[b]) : ~0ull;
+      const bool equal_below = (diff & ((1ull << (8 * d)) - 1ull)) == 0ull;
+      // A sub-bucket of equal keys is copied: level 0 leaves it no digit to sort by.
+      const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), equal_below ? 0u : uint32_t(d)};
+      if (last || equal_below) {
+        push_local(local, local_cap, lc, sub, true);
+      } else if (v <= local_max) {
+        push_local(local, local_cap, lc, sub, false);
+      } else {
+        next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
+        atomicAdd(&lc[LC_ROWS], v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// A level's pass: every large bucket that is not carried partitioned stably by digit d from its buffer
+// into the other one, at the same positions. Tile ids come from a counter; tiles of carried buckets
+// (whose whole look-back chain is skipped) and blocks past the last tile exit.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void lvl_pass(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
+                                                  const uint32_t* carried, const uint32_t* bucket_base,
+                                                  unsigned long long* status, const MsdCtl* mc, uint32_t eoff,
+                                                  int d, K* keys0, uint32_t* rows0, K* keys1, uint32_t* rows1) {
+  constexpr int TILE = BLOCK * ITEMS;
+  __shared__ uint32_t wc[WAVES][WC_STRIDE];
+  __shared__ uint32_t tile_total[WC_STRIDE];
+  __shared__ uint32_t bin_start[RADIX];
+  __shared__ uint32_t bin_base[WC_STRIDE];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ uint8_t s_digit[TILE];
+  __shared__ K s_stage[TILE];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // Exactly the first LC_TILES workgroups take a tile each (in counter order); the rest exit at once.
+  if (blockIdx.x >= lc[LC_TILES] || !((mc->active_mask >> d) & 1u)) return;
+  const uint32_t epoch = mc->epoch + eoff;
+  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const LvlTile t = tiles[tile];
+  if (carried[t.bucket]) return;
+  const bool from1 = large[t.bucket].buf != 0u;
+  const K* in_keys = from1 ? keys1 : keys0;
+  const uint32_t* in_rows = from1 ? rows1 : rows0;
+  K* out_keys = from1 ? keys0 : keys1;
+  uint32_t* out_rows = from1 ? rows0 : rows1;
+  const int shift = 8 * d;
+  K keys[ITEMS];
+  uint32_t rows[ITEMS], bins[ITEMS];
+  const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the tile read its last row)
+    const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
+    keys[j] = in_keys[e];
+    rows[j] = in_rows[e];
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++)
+    bins[j] = base_e + uint32_t(j) * 64u < t.end ? uint32_t(keys[j] >> shift) & 0xFFu : uint32_t(BIN_NONE);
+  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
+  __syncthreads();
+  const uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
+  look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
+  __syncthreads();
+  bin_base[threadIdx.x] += bucket_base[size_t(t.bucket) * RADIX + threadIdx.x];
+  __syncthreads();
+  scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount, out_keys,
+                                 out_rows);
+}
+
+constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
+
+// One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
+// in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
+template <typename K, int LI>
+__device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t (&pos)[LI], uint32_t base_e,
+                                            uint32_t size, int d, uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
+                                            uint32_t* bin_start, uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
+  __syncthreads();
+  uint32_t bins[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    bins[j] = base_e + uint32_t(j) * 64u < size ? uint32_t(keys[j] >> (8 * d)) & 0xFFu : uint32_t(BIN_NONE);
+  rank_tile<LI, 9>(bins, wc[w], lane);
+  __syncthreads();
+  tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t b = bin_of(bins[j]);
+    if (b < RADIX) {
+      const uint32_t at = bin_start[b] + wc[w][b] + rank_of(bins[j]);
+      s_keys[at] = keys[j];
+      s_pos[at] = uint16_t(pos[j]);
+    }
+  }
+}
+
+// Stable LSD passes over every varying digit below the top two, in LDS, for one bucket of at most
+// LI * BLOCK rows (rows in input order), then its rows written to their final slots.
+template <typename K, int LI>
+__device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
+                                          uint32_t low_mask, uint32_t* out_idx, uint32_t (*wc)[WC_STRIDE],
+                                          uint32_t* tile_total, uint32_t* bin_start, uint32_t* tmp, K* s_keys,
+                                          uint16_t* s_pos) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
+  K keys[LI];
+  uint32_t pos[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    pos[j] = base_e + uint32_t(j) * 64u;
+    keys[j] = keys_in[off + min(pos[j], size - 1u)];
+  }
+  for (int d = 0; d < 8; d++) {
+    if (!((low_mask >> d) & 1u)) continue;   // uniform
+    local_split<K, LI>(keys, pos, base_e, size, d, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LI; j++) {
+      const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
+      keys[j] = s_keys[e];
+      pos[j] = s_pos[e];
+    }
+    __syncthreads();
+  }
+  uint32_t r[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) r[j] = rows_in[off + min(pos[j], size - 1u)];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t e = base_e + uint32_t(j) * 64u;
+    if (e < size) out_idx[off + e] = r[j];
+  }
+}
+
+// The bucket sort: one workgroup per entry of its list (a bucket of at most LI * BLOCK rows; the host
+// picks LI from the largest). The rows of a bucket arrive in input order (every partition before was
+// stable), so the stable order is the order of (key, position in the bucket) pairs, which are all
+// distinct. The bucket is split by its most significant varying digit with LDS counters (any order
+// inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the digit) << POS_BITS |
+// position, so that inside one digit run words order as those pairs do; each row's place in its run
+// is the number of words there below its own. A bucket whose varying digits do not fit that word (a
+// top-digit bucket), or with a run longer than SUB_MAX rows (many equal keys), is queued for the
+// stable LSD passes of hyb_local_lsd instead. A bucket of equal keys (no varying digit left), of any
+// size, is copied.
+constexpr int POS_BITS = 12;   // positions in a bucket of at most LOCAL_MAX rows
+static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
+template <typename K, int LI>
+__device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, uint32_t idx, const K* keys0,
+                                          const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
+                                          uint32_t* out_idx, uint32_t* queue, uint32_t* lc, uint32_t* cnt,
+                                          uint32_t* start, uint32_t* cursor, uint32_t* tmp, K* s_words) {
+  const Bucket bk = list[idx];
+  const K* keys_in = bk.buf ? keys1 : keys0;
+  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+  const uint32_t size = bk.size, off = bk.start;
+  const uint32_t low_mask = mc->active_mask & ((1u << bk.level) - 1u);
+  uint32_t* out = out_idx + mc->value_base;
+  if (low_mask == 0u) {   // equal keys: the bucket is in order already
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
+    return;
+  }
+  if (size > uint32_t(LI * BLOCK)) return;   // never: LI covers every bucket to sort
+  const int top = 31 - __builtin_clz(low_mask);
+  if (top > 5) {   // 8 + 8 * top + POS_BITS bits would not fit one word
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
+    return;
+  }
+  const int shift = 8 * top;   // at most 40: 8 + 40 + POS_BITS bits per word
+  cnt[threadIdx.x] = 0u;
+  K keys[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&cnt[uint32_t(keys[j] >> shift) & 0xFFu], 1u);
+  __syncthreads();
+  const uint32_t c = cnt[threadIdx.x];
+  uint32_t total;
+  const uint32_t st = block_excl_scan(c, tmp, total);
+  start[threadIdx.x] = st;
+  cursor[threadIdx.x] = st;
+  if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
+    return;
+  }
+  const unsigned long long low_bits = (1ull << shift) - 1ull;
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    if (p < size) {
+      const uint32_t dd = uint32_t(keys[j] >> shift) & 0xFFu;
+      const uint32_t slot = atomicAdd(&cursor[dd], 1u);
+      s_words[slot] = K((static_cast<unsigned long long>(dd) << 56) |
+                        ((static_cast<unsigned long long>(keys[j]) & low_bits) << POS_BITS) | p);
+    }
+  }
+  __syncthreads();
+  uint32_t dest[LI], src[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    dest[j] = ~0u;
+    src[j] = off;
+    if (p < size) {
+      const K word = s_words[p];
+      const uint32_t dd = uint32_t(word >> 56);
+      const uint32_t lo = start[dd], hi = lo + cnt[dd];
+      uint32_t below = 0;
+#pragma unroll 4
+      for (uint32_t q = lo; q < hi; q++) below += s_words[q] < word ? 1u : 0u;
+      dest[j] = off + lo + below;
+      src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
+    }
+  }
+  uint32_t r[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (dest[j] != ~0u) out[dest[j]] = r[j];
+}
+
+// The bucket sort over its list, an entry per workgroup in turn (LDS for the bucket is dynamic:
+// LI * BLOCK * sizeof(K) bytes).
+template <typename K, int LI>
+__global__ __launch_bounds__(BLOCK) void hyb_local(const MsdCtl* mc, const Bucket* list, uint32_t local_cap,
+                                                   const K* keys0, const uint32_t* rows0, const K* keys1,
+                                                   const uint32_t* rows1, uint32_t* out_idx, uint32_t* queue,
+                                                   uint32_t* lc) {
+  __shared__ uint32_t cnt[RADIX], start[RADIX], cursor[RADIX];
+  __shared__ uint32_t tmp[WAVES];
+  extern __shared__ __align__(16) unsigned char local_lds[];
+  K* s_words = reinterpret_cast<K*>(local_lds);   // LI * BLOCK words
+  const uint32_t nlocal = min(lc[LC_LOCAL], local_cap);
+  for (uint32_t i = blockIdx.x; i < nlocal; i += gridDim.x) {
+    local_one<K, LI>(mc, list, i, keys0, rows0, keys1, rows1, out_idx, queue, lc, cnt, start, cursor, tmp,
+                     s_words);
+    __syncthreads();
+  }
+}
+
+// The queued buckets, each by stable LSD passes in LDS: persistent workgroups take them from a
+// counter until none is left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
+template <typename K, int LI>
+__global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const Bucket* list, const K* keys0,
+                                                       const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
+                                                       uint32_t* out_idx, const uint32_t* queue, uint32_t* lc) {
+  __shared__ uint32_t wc[WAVES][WC_STRIDE];
+  __shared__ uint32_t tile_total[WC_STRIDE];
+  __shared__ uint32_t bin_start[RADIX];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ uint32_t s_q;
+  extern __shared__ __align__(16) unsigned char local_lds[];
+  K* s_keys = reinterpret_cast<K*>(local_lds);
+  uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
+  const uint32_t queued = lc[LC_QLEN];
+  while (true) {
+    if (threadIdx.x == 0) s_q = atomicAdd(&lc[LC_QNEXT], 1u);
+    __syncthreads();
+    const uint32_t qi = s_q;
+    if (qi >= queued) return;   // uniform: every workgroup ends here once the queue is empty
+    const Bucket bk = list[queue[qi]];
+    const uint32_t low_mask = mc->active_mask & ((1u << bk.level) - 1u);
+    local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask,
+                     out_idx + mc->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+    __syncthreads();
+  }
+}
+
+
+// ---------------------------------------------------------------- host side
+
+// The device buffers of the levels for up to m value rows, with `items` keys per thread in a pass.
+// Capacities: buckets above LOCAL_MAX / 2 rows at one level (the least local_max used); entries of the
+// bucket sort's list (every entry has a row, and runs of many tiny buckets are what fills it); chunks
+// and tiles of a level.
+struct MsdWork {
+  uint32_t large_cap = 0, local_cap = 0;
+  DevBuf lists, tabs, words, wide;
+  Bucket* large[2] = {nullptr, nullptr};
+  Bucket* local = nullptr;
+  LvlChunk* chunks = nullptr;
+  LvlTile* tiles = nullptr;
+  uint32_t *lc = nullptr, *bucket_hist = nullptr, *bucket_base = nullptr, *carried = nullptr, *queue = nullptr,
+           *bofs = nullptr;
+  unsigned long long *bucket_or = nullptr, *bucket_and = nullptr;
+  MsdCtl* mc = nullptr;
+};
+
+inline MsdWork msd_work(uint32_t m, int items, DevBuf (*alloc)(size_t)) {
+  MsdWork w;
+  const uint32_t tile = uint32_t(BLOCK * items);
+  w.large_cap = m / (LOCAL_MAX / 2) + 4096u;
+  w.local_cap = m / 32u + (1u << 17);
+  const uint32_t max_chunks = m / LVL_CHUNK + w.large_cap + 1u;
+  const uint32_t max_tiles = m / tile + w.large_cap + 1u;
+  w.lists = alloc((size_t(w.large_cap) * 2 + w.local_cap) * sizeof(Bucket));
+  w.tabs = alloc(size_t(max_chunks) * sizeof(LvlChunk) + size_t(max_tiles) * sizeof(LvlTile));
+  w.words = alloc((size_t(w.large_cap) * (2 * RADIX + 3) + w.local_cap + LC_WORDS) * 4);
+  w.wide = alloc(size_t(w.large_cap) * 16 + sizeof(MsdCtl));
+  w.large[0] = static_cast<Bucket*>(w.lists->ptr);
+  w.large[1] = w.large[0] + w.large_cap;
+  w.local = w.large[1] + w.large_cap;
+  w.chunks = static_cast<LvlChunk*>(w.tabs->ptr);
+  w.tiles = reinterpret_cast<LvlTile*>(w.chunks + max_chunks);
+  w.lc = static_cast<uint32_t*>(w.words->ptr);   // first, so that its lines are its own
+  w.bucket_hist = w.lc + LC_WORDS;
+  w.bucket_base = w.bucket_hist + size_t(w.large_cap) * RADIX;
+  w.carried = w.bucket_base + size_t(w.large_cap) * RADIX;
+  w.bofs = w.carried + w.large_cap;   // 2 words per bucket
+  w.queue = w.bofs + 2 * size_t(w.large_cap);
+  w.bucket_or = static_cast<unsigned long long*>(w.wide->ptr);
+  w.bucket_and = w.bucket_or + w.large_cap;
+  w.mc = reinterpret_cast<MsdCtl*>(w.bucket_and + w.large_cap);
+  return w;
+}
+
+// The levels below digit `top` and the bucket sort, on stream s, after the first list is built (in
+// w.large[0] with its count, the bucket sort's list started, the counters and w.mc set).
+// - Read-back form (blind = false; `mask` = the active digits, known on the host): only active
+//   digits get a level, each launch is sized from 64 bytes of counters read back after each level,
+//   and the bucket sort's keys per thread come from the largest bucket. Returns false, with no row
+//   written that a rerun does not write again, when the bucket sort's list ran out of room.
+// - Blind form (blind = true, e.g. inside a graph capture): every digit below `top` gets a level,
+//   sized for the worst case, the kernels read what to do from the device, and the bucket sort takes
+//   buckets of at most local_max = LOCAL_MAX / 2 rows. The caller must check LC_OVERFLOW itself.
+// Look-back epochs: w.mc->epoch + 9 .. + 15.
+template <typename K, int ITEMS>
+bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind, uint32_t local_max,
+                unsigned long long* status, K* k0, uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx,
+                hipStream_t s) {
+  constexpr int P = int(sizeof(K));
+  constexpr uint32_t TILE = BLOCK * ITEMS;
+  const uint32_t cus = uint32_t(device().cu_count);
+  uint32_t* hp = host_plan_words();
+  auto next_digit = [&](int d) {
+    while (--d >= 0 && !blind && !((mask >> d) & 1u)) {}
+    return d;
+  };
+  // Large buckets and their rows (bounds of the grids): at most RADIX top-digit buckets first.
+  uint32_t bound = blind ? w.large_cap : uint32_t(RADIX), rows = m;
+  int cur = 0;
+  bool read = false;
+  for (int d = next_digit(top); d >= 0 && bound > 0; d = next_digit(d)) {
+    {
+      ProfScope ps("sort_msd_layout");
+      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, w.large[cur], cur, w.bofs, w.bucket_hist,
+                         w.bucket_or, w.bucket_and, w.lc);
+      hipLaunchKernelGGL(lvl_fill<ITEMS>, dim3(std::min(bound, cus * 4u)), dim3(BLOCK), 0, s, w.large[cur], cur,
+                         w.lc, w.bofs, w.chunks, w.tiles);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_hist");
+      hipLaunchKernelGGL(lvl_hist<K>, dim3(std::min(rows / LVL_CHUNK + bound + 1u, cus * 8u)), dim3(BLOCK), 0, s,
+                         k0, k1, w.large[cur], w.chunks, w.lc, w.mc, d, w.bucket_hist, w.bucket_or, w.bucket_and);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_plan");
+      hipLaunchKernelGGL(lvl_plan, dim3(std::min(bound, cus * 4u)), dim3(BLOCK), 0, s, w.large[cur], cur,
+                         w.bucket_hist, w.bucket_or, w.bucket_and, w.mc, d, local_max, w.bucket_base, w.carried,
+                         w.large[cur ^ 1], w.local, w.local_cap, w.lc);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_scatter");
+      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(rows / TILE + bound + 1u), dim3(BLOCK), 0, s, w.large[cur],
+                         w.tiles, w.lc, w.carried, w.bucket_base, status, w.mc, 8u + uint32_t(P - 1 - d), d, k0, r0,
+                         k1, r1);
+      AM_HIP(hipGetLastError());
+    }
+    cur ^= 1;
+    if (!blind) {
+      AM_HIP(hipMemcpyAsync(hp, w.lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
+      AM_HIP(hipStreamSynchronize(s));
+      read = true;
+      bound = hp[LC_LIST + cur];
+      rows = hp[LC_ROWS];
+      if (bound > w.large_cap) fail("argsort: bucket list overflow");   // cannot happen: each holds > local_max rows
+    }
+  }
+  int li = 8;
+  uint32_t grid = cus * 8u;
+  if (!blind) {
+    if (!read) {   // no level ran (no active digit below the top one)
+      AM_HIP(hipMemcpyAsync(hp, w.lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
+      AM_HIP(hipStreamSynchronize(s));
+    }
+    if (hp[LC_OVERFLOW]) return false;
+    if (hp[LC_LOCAL] == 0u) return true;
+    const uint32_t biggest = hp[LC_MAXLOCAL];
+    li = biggest <= 4u * BLOCK ? 4 : biggest <= 8u * BLOCK ? 8 : 16;
+    grid = hp[LC_LOCAL];
+  }
+  // The LSD kernel takes the buckets the counting sort queued (its workgroups exit at once when none
+  // was).
+  const dim3 lsd_grid(cus * 2u);
+  const size_t lds = size_t(li) * BLOCK * sizeof(K), lsd_lds = size_t(li) * BLOCK * (sizeof(K) + 2);
+  {
+    ProfScope ps("sort_msd_local");
+    if (li == 4)
+      hipLaunchKernelGGL((hyb_local<K, 4>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
+                         r1, out_idx, w.queue, w.lc);
+    else if (li == 8)
+      hipLaunchKernelGGL((hyb_local<K, 8>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
+                         r1, out_idx, w.queue, w.lc);
+    else
+      hipLaunchKernelGGL((hyb_local<K, 16>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
+                         r1, out_idx, w.queue, w.lc);
+    AM_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps("sort_msd_local_lsd");
+    if (li == 4)
+      hipLaunchKernelGGL((hyb_local_lsd<K, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
+                         out_idx, w.queue, w.lc);
+    else if (li == 8)
+      hipLaunchKernelGGL((hyb_local_lsd<K, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
+                         out_idx, w.queue, w.lc);
+    else
+      hipLaunchKernelGGL((hyb_local_lsd<K, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
+                         out_idx, w.queue, w.lc);
+    AM_HIP(hipGetLastError());
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace am

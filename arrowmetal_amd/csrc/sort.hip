@@ -24,7 +24,7 @@
 // per-bin counts and look back over earlier tiles for the exclusive prefix (decoupled look-back with
 // epoch-tagged 64-bit status words, so the status table is never cleared between passes), then stage
 // the tile in LDS in digit order so that the global writes are runs of consecutive addresses.
-#include "sort_common.hpp"
+#include "msd_levels.hpp"
 
 namespace am {
 namespace {
@@ -525,51 +525,15 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 #ifndef AM_MSD_FITEMS
 #define AM_MSD_FITEMS 16   // keys per thread of the top-digit partition
 #endif
-constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
-constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes
-
-// A bucket: rows [start, start + size) of the value block, in key / row buffer `buf` (0 or 1), all
-// sharing the digits from the top down to `level`.
-struct Bucket {
-  uint32_t start, size, buf, level;
-};
-struct LvlChunk {
-  uint32_t bucket, start, end, pad;
-};
-struct LvlTile {
-  uint32_t bucket, start, end, head;   // head: the bucket's first tile
-};
-
-// Device counters of the levels (one small array). The host reads back the first LC_READ words. The
-// two counters that every workgroup of a kernel bumps each sit on a cache line of their own: sharing a
-// line with words the same workgroups read (the tile count) made the level pass 1.8x slower.
-constexpr int LC_LIST = 0;        // [0], [1]: entries of the two large-bucket lists
-constexpr int LC_LOCAL = 2;       // entries of the bucket sort's list
-constexpr int LC_CHUNKS = 3;      // chunks of the current level
-constexpr int LC_TILES = 4;       // tiles of the current level
-constexpr int LC_MAXLOCAL = 5;    // largest bucket the bucket sort sorts (not counting copies)
-constexpr int LC_OVERFLOW = 6;    // the bucket sort's list ran out of room
-constexpr int LC_QLEN = 7;        // buckets queued for the LSD bucket sort
-constexpr int LC_ROWS = 8;        // rows in the next level's large buckets (bounds its grids)
-constexpr int LC_READ = 16;
-constexpr int LC_NEXT_TILE = 32;  // the next tile of the current level to take (own 128-byte line)
-constexpr int LC_QNEXT = 64;      // the next queued bucket to take (own 128-byte line)
-constexpr int LC_WORDS = 96;
-
-// Appends one bucket to the bucket sort's list. `sorted` = its rows need no sorting (equal keys).
-__device__ __forceinline__ void push_local(Bucket* local, uint32_t local_cap, uint32_t* lc, Bucket b, bool sorted) {
-  const uint32_t i = atomicAdd(&lc[LC_LOCAL], 1u);
-  if (i < local_cap) local[i] = b;
-  else atomicOr(&lc[LC_OVERFLOW], 1u);
-  if (!sorted) atomicMax(&lc[LC_MAXLOCAL], b.size);
-}
-
-// One workgroup: the top-digit buckets msd_part made, into the first large-bucket list or the bucket
-// sort's list; the counters cleared. `low` = the active digits below the top one.
+// One workgroup: the level kernels' MsdCtl; the top-digit buckets msd_part made, into the first
+// large-bucket list or the bucket sort's list; the counters cleared. `low` = the active digits below
+// the top one.
 template <int P>
-__global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, Bucket* large, Bucket* local,
-                                                        uint32_t local_cap, uint32_t* lc) {
+__global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, MsdCtl* mc, uint32_t epoch,
+                                                        Bucket* large, Bucket* local, uint32_t local_cap,
+                                                        uint32_t* lc) {
   if (threadIdx.x < LC_WORDS) lc[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) *mc = {ctl->active_mask, ctl->value_base, epoch, 0u};
   __syncthreads();
   const uint32_t b = threadIdx.x;
   const uint32_t size = ctl->hist[P - 1][b];
@@ -578,389 +542,6 @@ __global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, Buck
   const uint32_t low = ctl->active_mask & ((1u << (P - 1)) - 1u);
   if (size > LOCAL_MAX && low != 0u) large[atomicAdd(&lc[LC_LIST], 1u)] = bk;
   else push_local(local, local_cap, lc, bk, low == 0u);
-}
-
-// One workgroup: where each large bucket's chunks and tiles start in a level's tables (list `cur`
-// order; bofs[2b], bofs[2b + 1]); the bucket histograms and OR / AND cleared; the other list's
-// counter, the tile counter and the row count reset. lvl_fill writes the tables.
-template <int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, uint32_t* bofs,
-                                                    uint32_t* bucket_hist, unsigned long long* bucket_or,
-                                                    unsigned long long* bucket_and, uint32_t* lc) {
-  __shared__ uint32_t tmp[WAVES];
-  constexpr uint32_t TILE = BLOCK * ITEMS;
-  const uint32_t nb = lc[LC_LIST + cur];
-  uint32_t coff = 0, toff = 0;
-  for (uint32_t b0 = 0; b0 < nb; b0 += BLOCK) {
-    const uint32_t b = b0 + threadIdx.x;
-    const uint32_t size = b < nb ? large[b].size : 0u;
-    const uint32_t nc = (size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (size + TILE - 1) / TILE;
-    uint32_t tc, tt;
-    const uint32_t c0 = coff + block_excl_scan(nc, tmp, tc);
-    const uint32_t t0 = toff + block_excl_scan(nt, tmp, tt);
-    if (b < nb) {
-      bofs[2 * b] = c0;
-      bofs[2 * b + 1] = t0;
-    }
-    coff += tc;
-    toff += tt;
-  }
-  for (uint32_t i = threadIdx.x; i < nb * RADIX; i += BLOCK) bucket_hist[i] = 0u;
-  for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) {
-    bucket_or[i] = 0ull;
-    bucket_and[i] = ~0ull;
-  }
-  if (threadIdx.x == 0) {
-    lc[LC_CHUNKS] = coff;
-    lc[LC_TILES] = toff;
-    lc[LC_NEXT_TILE] = 0u;
-    lc[LC_LIST + (cur ^ 1)] = 0u;
-    lc[LC_ROWS] = 0u;
-  }
-}
-
-// One workgroup per large bucket (surplus workgroups exit): its chunks and tiles, written by all threads.
-template <int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_fill(const Bucket* large, int cur, const uint32_t* lc,
-                                                  const uint32_t* bofs, LvlChunk* chunks, LvlTile* tiles) {
-  constexpr uint32_t TILE = BLOCK * ITEMS;
-  const uint32_t b = blockIdx.x;
-  if (b >= lc[LC_LIST + cur]) return;
-  const Bucket bk = large[b];
-  const uint32_t c0 = bofs[2 * b], t0 = bofs[2 * b + 1];
-  const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
-  for (uint32_t c = threadIdx.x; c < nc; c += BLOCK)
-    chunks[c0 + c] = {b, bk.start + c * LVL_CHUNK, bk.start + min(bk.size, (c + 1u) * LVL_CHUNK), 0u};
-  for (uint32_t t = threadIdx.x; t < nt; t += BLOCK)
-    tiles[t0 + t] = {b, bk.start + t * TILE, bk.start + min(bk.size, (t + 1u) * TILE), t0};
-}
-
-// Digit d's histogram, and the OR / AND of the keys, of every large bucket, one chunk per workgroup
-// (surplus workgroups exit).
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1, const Bucket* large,
-                                                  const LvlChunk* chunks, const uint32_t* lc, int d,
-                                                  uint32_t* bucket_hist, unsigned long long* bucket_or,
-                                                  unsigned long long* bucket_and) {
-  __shared__ uint32_t h[RADIX];
-  __shared__ unsigned long long s_or, s_and;
-  if (blockIdx.x >= lc[LC_CHUNKS]) return;
-  const LvlChunk c = chunks[blockIdx.x];
-  const K* keys = large[c.bucket].buf ? keys1 : keys0;
-  h[threadIdx.x] = 0u;
-  if (threadIdx.x == 0) {
-    s_or = 0ull;
-    s_and = ~0ull;
-  }
-  __syncthreads();
-  unsigned long long o = 0ull, a = ~0ull;
-  constexpr int U = 8;
-  for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
-    K k[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) k[u] = keys[min(base + uint32_t(u) * BLOCK, c.end - 1u)];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      // Rows past the end read the chunk's last row, which does not change the OR / AND.
-      o |= (unsigned long long)k[u];
-      a &= (unsigned long long)k[u];
-      if (base + uint32_t(u) * BLOCK < c.end) atomicAdd(&h[uint32_t(k[u] >> (8 * d)) & 0xFFu], 1u);
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    o |= __shfl_xor(o, off, 64);
-    a &= __shfl_xor(a, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0 && c.start + threadIdx.x < c.end) {   // waves that read at least one row
-    atomicOr(&s_or, o);
-    atomicAnd(&s_and, a);
-  }
-  __syncthreads();
-  const uint32_t v = h[threadIdx.x];
-  if (v) atomicAdd(&bucket_hist[size_t(c.bucket) * RADIX + threadIdx.x], v);
-  if (threadIdx.x == 0) {
-    atomicOr(&bucket_or[c.bucket], s_or);
-    atomicAnd(&bucket_and[c.bucket], s_and);
-  }
-}
-
-// One workgroup per large bucket: the start of each of its digit-d sub-buckets (bucket_base), and
-// where each goes: to the bucket sort's list (at most LOCAL_MAX rows; or, of any size, equal keys: the
-// bucket's keys differ in no bit below digit d, or d is the lowest active digit `last`), or to the
-// next level's list. A bucket whose rows all share digit d is carried (carried[b] = 1): it keeps its
-// buffer and the pass skips it.
-__global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist,
-                                                  const unsigned long long* bucket_or,
-                                                  const unsigned long long* bucket_and, int d, int last,
-                                                  uint32_t* bucket_base, uint32_t* carried, Bucket* next,
-                                                  Bucket* local, uint32_t local_cap, uint32_t* lc) {
-  __shared__ uint32_t tmp[WAVES];
-  const uint32_t b = blockIdx.x;
-  if (b >= lc[LC_LIST + cur]) return;   // uniform over the workgroup
-  const Bucket bk = large[b];
-  const uint32_t v = bucket_hist[size_t(b) * RADIX + threadIdx.x];
-  uint32_t total;
-  const uint32_t base = bk.start + block_excl_scan(v, tmp, total);
-  bucket_base[size_t(b) * RADIX + threadIdx.x] = base;
-  const int bins = __syncthreads_count(v != 0u ? 1 : 0);
-  const bool carry = bins == 1;
-  if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
-  if (v == 0u) return;
-  const bool equal_below = ((bucket_or[b] ^ bucket_and[b]) & ((1ull << (8 * d)) - 1ull)) == 0ull;
-  // A sub-bucket of equal keys is copied: level 0 leaves it no digit to sort by.
-  const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), equal_below ? 0u : uint32_t(d)};
-  if (last || equal_below) {
-    push_local(local, local_cap, lc, sub, true);
-  } else if (v <= LOCAL_MAX) {
-    push_local(local, local_cap, lc, sub, false);
-  } else {
-    next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
-    atomicAdd(&lc[LC_ROWS], v);
-  }
-}
-
-// A level's pass: every large bucket that is not carried partitioned stably by digit d from its buffer
-// into the other one, at the same positions. Tile ids come from a counter; tiles of carried buckets
-// (whose whole look-back chain is skipped) and blocks past the last tile exit.
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_pass(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
-                                                  const uint32_t* carried, const uint32_t* bucket_base,
-                                                  unsigned long long* status, uint32_t epoch, int d, K* keys0,
-                                                  uint32_t* rows0, K* keys1, uint32_t* rows1) {
-  constexpr int TILE = BLOCK * ITEMS;
-  __shared__ uint32_t wc[WAVES][WC_STRIDE];
-  __shared__ uint32_t tile_total[WC_STRIDE];
-  __shared__ uint32_t bin_start[RADIX];
-  __shared__ uint32_t bin_base[WC_STRIDE];
-  __shared__ uint32_t tmp[WAVES];
-  __shared__ uint32_t s_tile;
-  __shared__ uint8_t s_digit[TILE];
-  __shared__ K s_stage[TILE];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  if (threadIdx.x == 0) s_tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  if (tile >= lc[LC_TILES]) return;   // uniform over the workgroup
-  const LvlTile t = tiles[tile];
-  if (carried[t.bucket]) return;
-  const bool from1 = large[t.bucket].buf != 0u;
-  const K* in_keys = from1 ? keys1 : keys0;
-  const uint32_t* in_rows = from1 ? rows1 : rows0;
-  K* out_keys = from1 ? keys0 : keys1;
-  uint32_t* out_rows = from1 ? rows0 : rows1;
-  const int shift = 8 * d;
-  K keys[ITEMS];
-  uint32_t rows[ITEMS], bins[ITEMS];
-  const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
-#pragma unroll
-  for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the tile read its last row)
-    const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
-    keys[j] = in_keys[e];
-    rows[j] = in_rows[e];
-  }
-#pragma unroll
-  for (int j = 0; j < ITEMS; j++)
-    bins[j] = base_e + uint32_t(j) * 64u < t.end ? uint32_t(keys[j] >> shift) & 0xFFu : uint32_t(BIN_NONE);
-  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
-  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
-  __syncthreads();
-  const uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
-  look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
-  __syncthreads();
-  bin_base[threadIdx.x] += bucket_base[size_t(t.bucket) * RADIX + threadIdx.x];
-  __syncthreads();
-  scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount, out_keys,
-                                 out_rows);
-}
-
-constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
-
-// One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
-// in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
-template <typename K, int LI>
-__device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t (&pos)[LI], uint32_t base_e,
-                                            uint32_t size, int d, uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total,
-                                            uint32_t* bin_start, uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  __syncthreads();
-  uint32_t bins[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++)
-    bins[j] = base_e + uint32_t(j) * 64u < size ? uint32_t(keys[j] >> (8 * d)) & 0xFFu : uint32_t(BIN_NONE);
-  rank_tile<LI, 9>(bins, wc[w], lane);
-  __syncthreads();
-  tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < LI; j++) {
-    const uint32_t b = bin_of(bins[j]);
-    if (b < RADIX) {
-      const uint32_t at = bin_start[b] + wc[w][b] + rank_of(bins[j]);
-      s_keys[at] = keys[j];
-      s_pos[at] = uint16_t(pos[j]);
-    }
-  }
-}
-
-// Stable LSD passes over every varying digit below the top two, in LDS, for one bucket of at most
-// LI * BLOCK rows (rows in input order), then its rows written to their final slots.
-template <typename K, int LI>
-__device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
-                                          uint32_t low_mask, uint32_t* out_idx, uint32_t (*wc)[WC_STRIDE],
-                                          uint32_t* tile_total, uint32_t* bin_start, uint32_t* tmp, K* s_keys,
-                                          uint16_t* s_pos) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
-  K keys[LI];
-  uint32_t pos[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) {
-    pos[j] = base_e + uint32_t(j) * 64u;
-    keys[j] = keys_in[off + min(pos[j], size - 1u)];
-  }
-  for (int d = 0; d < 8; d++) {
-    if (!((low_mask >> d) & 1u)) continue;   // uniform
-    local_split<K, LI>(keys, pos, base_e, size, d, wc, tile_total, bin_start, tmp, s_keys, s_pos);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < LI; j++) {
-      const uint32_t e = min(base_e + uint32_t(j) * 64u, size - 1u);
-      keys[j] = s_keys[e];
-      pos[j] = s_pos[e];
-    }
-    __syncthreads();
-  }
-  uint32_t r[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) r[j] = rows_in[off + min(pos[j], size - 1u)];
-#pragma unroll
-  for (int j = 0; j < LI; j++) {
-    const uint32_t e = base_e + uint32_t(j) * 64u;
-    if (e < size) out_idx[off + e] = r[j];
-  }
-}
-
-// The bucket sort: one workgroup per entry of its list (a bucket of at most LI * BLOCK rows; the host
-// picks LI from the largest). The rows of a bucket arrive in input order (every partition before was
-// stable), so the stable order is the order of (key, position in the bucket) pairs, which are all
-// distinct. The bucket is split by its most significant varying digit with LDS counters (any order
-// inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the digit) << POS_BITS |
-// position, so that inside one digit run words order as those pairs do; each row's place in its run
-// is the number of words there below its own. A bucket whose varying digits do not fit that word (a
-// top-digit bucket), or with a run longer than SUB_MAX rows (many equal keys), is queued for the
-// stable LSD passes of hyb_local_lsd instead. A bucket of equal keys (no varying digit left), of any
-// size, is copied.
-constexpr int POS_BITS = 12;   // positions in a bucket of at most LOCAL_MAX rows
-static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
-template <typename K, int P, int LI>
-__global__ __launch_bounds__(BLOCK) void hyb_local(const SortCtl* ctl, const Bucket* list, const K* keys0,
-                                                   const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
-                                                   uint32_t* out_idx, uint32_t* queue, uint32_t* lc) {
-  __shared__ uint32_t cnt[RADIX], start[RADIX], cursor[RADIX];
-  __shared__ uint32_t tmp[WAVES];
-  extern __shared__ __align__(16) unsigned char local_lds[];
-  K* s_words = reinterpret_cast<K*>(local_lds);   // LI * BLOCK words
-  const Bucket bk = list[blockIdx.x];
-  const K* keys_in = bk.buf ? keys1 : keys0;
-  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
-  const uint32_t size = bk.size, off = bk.start;
-  const uint32_t low_mask = ctl->active_mask & ((1u << bk.level) - 1u);
-  uint32_t* out = out_idx + ctl->value_base;
-  if (low_mask == 0u) {   // equal keys: the bucket is in order already
-    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
-    return;
-  }
-  if (size > uint32_t(LI * BLOCK)) return;   // never: the host sized LI for every bucket to sort
-  const int top = 31 - __builtin_clz(low_mask);
-  if (top > 5) {   // 8 + 8 * top + POS_BITS bits would not fit one word
-    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = blockIdx.x;
-    return;
-  }
-  const int shift = 8 * top;   // at most 40: 8 + 40 + POS_BITS bits per word
-  cnt[threadIdx.x] = 0u;
-  K keys[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < LI; j++)
-    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&cnt[uint32_t(keys[j] >> shift) & 0xFFu], 1u);
-  __syncthreads();
-  const uint32_t c = cnt[threadIdx.x];
-  uint32_t total;
-  const uint32_t st = block_excl_scan(c, tmp, total);
-  start[threadIdx.x] = st;
-  cursor[threadIdx.x] = st;
-  if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
-    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = blockIdx.x;
-    return;
-  }
-  const unsigned long long low_bits = (1ull << shift) - 1ull;
-#pragma unroll
-  for (int j = 0; j < LI; j++) {
-    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
-    if (p < size) {
-      const uint32_t d = uint32_t(keys[j] >> shift) & 0xFFu;
-      const uint32_t slot = atomicAdd(&cursor[d], 1u);
-      s_words[slot] = K((static_cast<unsigned long long>(d) << 56) |
-                        ((static_cast<unsigned long long>(keys[j]) & low_bits) << POS_BITS) | p);
-    }
-  }
-  __syncthreads();
-  uint32_t dest[LI], src[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) {
-    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
-    dest[j] = ~0u;
-    src[j] = off;
-    if (p < size) {
-      const K word = s_words[p];
-      const uint32_t d = uint32_t(word >> 56);
-      const uint32_t lo = start[d], hi = lo + cnt[d];
-      uint32_t below = 0;
-#pragma unroll 4
-      for (uint32_t q = lo; q < hi; q++) below += s_words[q] < word ? 1u : 0u;
-      dest[j] = off + lo + below;
-      src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
-    }
-  }
-  uint32_t r[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
-#pragma unroll
-  for (int j = 0; j < LI; j++)
-    if (dest[j] != ~0u) out[dest[j]] = r[j];
-}
-
-// The queued buckets, each by stable LSD passes in LDS: persistent workgroups take them from a
-// counter until none is left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
-template <typename K, int P, int LI>
-__global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const SortCtl* ctl, const Bucket* list, const K* keys0,
-                                                       const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
-                                                       uint32_t* out_idx, const uint32_t* queue, uint32_t* lc) {
-  __shared__ uint32_t wc[WAVES][WC_STRIDE];
-  __shared__ uint32_t tile_total[WC_STRIDE];
-  __shared__ uint32_t bin_start[RADIX];
-  __shared__ uint32_t tmp[WAVES];
-  __shared__ uint32_t s_q;
-  extern __shared__ __align__(16) unsigned char local_lds[];
-  K* s_keys = reinterpret_cast<K*>(local_lds);
-  uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
-  const uint32_t queued = lc[LC_QLEN];
-  while (true) {
-    if (threadIdx.x == 0) s_q = atomicAdd(&lc[LC_QNEXT], 1u);
-    __syncthreads();
-    const uint32_t qi = s_q;
-    if (qi >= queued) return;   // uniform: every workgroup ends here once the queue is empty
-    const Bucket bk = list[queue[qi]];
-    const uint32_t low_mask = ctl->active_mask & ((1u << bk.level) - 1u);
-    local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask,
-                     out_idx + ctl->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
-    __syncthreads();
-  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -974,29 +555,7 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
                 uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base,
                 uint32_t nunits, uint32_t unit_shift, hipStream_t s) {
   constexpr int P = int(sizeof(K));
-  constexpr uint32_t TILE = BLOCK * ITEMS;
-  // Capacities: buckets above LOCAL_MAX rows at one level; entries of the bucket sort's list (every
-  // entry has a row, and runs of many tiny buckets are what fills it); chunks and tiles of a level.
-  const uint32_t large_cap = m / LOCAL_MAX + uint32_t(RADIX);
-  const uint32_t local_cap = m / 32u + (1u << 17);
-  const uint32_t max_chunks = m / LVL_CHUNK + large_cap + 1u;
-  const uint32_t max_tiles = m / TILE + large_cap + 1u;
-  DevBuf lists = device_alloc((size_t(large_cap) * 2 + local_cap) * sizeof(Bucket));
-  DevBuf tabs = device_alloc(size_t(max_chunks) * sizeof(LvlChunk) + size_t(max_tiles) * sizeof(LvlTile));
-  DevBuf words = device_alloc((size_t(large_cap) * (2 * RADIX + 1) + local_cap + LC_WORDS) * 4);
-  Bucket* large[2] = {static_cast<Bucket*>(lists->ptr), static_cast<Bucket*>(lists->ptr) + large_cap};
-  Bucket* local = large[1] + large_cap;
-  LvlChunk* chunks = static_cast<LvlChunk*>(tabs->ptr);
-  LvlTile* tiles = reinterpret_cast<LvlTile*>(chunks + max_chunks);
-  uint32_t* lc = static_cast<uint32_t*>(words->ptr);   // first, so that its lines are its own
-  uint32_t* bucket_hist = lc + LC_WORDS;
-  uint32_t* bucket_base = bucket_hist + size_t(large_cap) * RADIX;
-  uint32_t* carried = bucket_base + size_t(large_cap) * RADIX;
-  uint32_t* queue = carried + large_cap;
-  DevBuf orand = device_alloc(size_t(large_cap) * 24);
-  unsigned long long* bucket_or = static_cast<unsigned long long*>(orand->ptr);
-  unsigned long long* bucket_and = bucket_or + large_cap;
-  uint32_t* bofs = reinterpret_cast<uint32_t*>(bucket_and + large_cap);   // 2 words per bucket
+  const MsdWork w = msd_work(m, ITEMS, device_alloc);
   {
     ProfScope ps("sort_msd_unit_scan");
     hipLaunchKernelGGL(msd_unit_scan<P>, dim3(NBINS_FIRST), dim3(BLOCK), 0, s, ctl, unit_hist, nunits, unit_base);
@@ -1011,93 +570,11 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   }
   {
     ProfScope ps("sort_msd_lists");
-    hipLaunchKernelGGL(msd_init_lists<P>, dim3(1), dim3(BLOCK), 0, s, ctl, large[0], local, local_cap, lc);
+    hipLaunchKernelGGL(msd_init_lists<P>, dim3(1), dim3(BLOCK), 0, s, ctl, w.mc, epoch, w.large[0], w.local,
+                       w.local_cap, w.lc);
     AM_HIP(hipGetLastError());
   }
-  // Levels down the active digits while large buckets are left. The first level's buckets are the top
-  // digit's (at most RADIX); later counts come back with one 64-byte read per level.
-  uint32_t* hp = host_plan_words();
-  auto next_digit = [&](int d) {
-    while (--d >= 0 && !((mask >> d) & 1u)) {}
-    return d;
-  };
-  uint32_t bound = uint32_t(RADIX), rows = m;   // large buckets and their rows (bounds of the grids)
-  int cur = 0;
-  bool read = false;
-  for (int d = next_digit(P - 1); d >= 0 && bound > 0; d = next_digit(d)) {
-    const int last = next_digit(d) < 0 ? 1 : 0;
-    {
-      ProfScope ps("sort_msd_layout");
-      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, large[cur], cur, bofs, bucket_hist,
-                         bucket_or, bucket_and, lc);
-      hipLaunchKernelGGL(lvl_fill<ITEMS>, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, lc, bofs, chunks, tiles);
-      AM_HIP(hipGetLastError());
-    }
-    {
-      ProfScope ps("sort_msd_hist");
-      hipLaunchKernelGGL(lvl_hist<K>, dim3(rows / LVL_CHUNK + bound + 1u), dim3(BLOCK), 0, s, k0, k1, large[cur],
-                         chunks, lc, d, bucket_hist, bucket_or, bucket_and);
-      AM_HIP(hipGetLastError());
-    }
-    {
-      ProfScope ps("sort_msd_plan");
-      hipLaunchKernelGGL(lvl_plan, dim3(bound), dim3(BLOCK), 0, s, large[cur], cur, bucket_hist, bucket_or,
-                         bucket_and, d, last, bucket_base, carried, large[cur ^ 1], local, local_cap, lc);
-      AM_HIP(hipGetLastError());
-    }
-    {
-      ProfScope ps("sort_msd_scatter");
-      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(rows / TILE + bound + 1u), dim3(BLOCK), 0, s, large[cur], tiles, lc,
-                         carried, bucket_base, status, epoch + 8u + uint32_t(P - 1 - d), d, k0, r0, k1, r1);
-      AM_HIP(hipGetLastError());
-    }
-    AM_HIP(hipMemcpyAsync(hp, lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
-    read = true;
-    cur ^= 1;
-    bound = hp[LC_LIST + cur];
-    rows = hp[LC_ROWS];
-    if (bound > large_cap) fail("argsort: bucket list overflow");   // cannot happen: each holds > LOCAL_MAX rows
-  }
-  if (!read) {   // no level ran (no active digit below the top one)
-    AM_HIP(hipMemcpyAsync(hp, lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
-  }
-  if (hp[LC_OVERFLOW]) return false;
-  const uint32_t nlocal = hp[LC_LOCAL], biggest = hp[LC_MAXLOCAL];
-  if (nlocal == 0u) return true;
-  // Keys per thread from the largest bucket to sort; the LSD kernel takes the buckets the counting
-  // sort queued (its workgroups exit at once when none was).
-  const int li = biggest <= 4u * BLOCK ? 4 : biggest <= 8u * BLOCK ? 8 : 16;
-  const dim3 grid(nlocal), lsd_grid(uint32_t(device().cu_count) * 2u);
-  const size_t lds = size_t(li) * BLOCK * sizeof(K), lsd_lds = size_t(li) * BLOCK * (sizeof(K) + 2);
-  {
-    ProfScope ps("sort_msd_local");
-    if (li == 4)
-      hipLaunchKernelGGL((hyb_local<K, P, 4>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
-                         lc);
-    else if (li == 8)
-      hipLaunchKernelGGL((hyb_local<K, P, 8>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
-                         lc);
-    else
-      hipLaunchKernelGGL((hyb_local<K, P, 16>), grid, dim3(BLOCK), lds, s, ctl, local, k0, r0, k1, r1, out_idx, queue,
-                         lc);
-    AM_HIP(hipGetLastError());
-  }
-  {
-    ProfScope ps("sort_msd_local_lsd");
-    if (li == 4)
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
-                         out_idx, queue, lc);
-    else if (li == 8)
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
-                         out_idx, queue, lc);
-    else
-      hipLaunchKernelGGL((hyb_local_lsd<K, P, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, ctl, local, k0, r0, k1, r1,
-                         out_idx, queue, lc);
-    AM_HIP(hipGetLastError());
-  }
-  return true;
+  return msd_levels<K, ITEMS>(w, m, P - 1, mask, false, LOCAL_MAX, status, k0, r0, k1, r1, out_idx, s);
 }
 
 template <typename K, int W, int ITEMS>
@@ -1114,8 +591,8 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   DevBuf ctl_buf = device_alloc(sizeof(SortCtl));
   SortCtl* ctl = static_cast<SortCtl*>(ctl_buf->ptr);
   // Status rows for every tile of a pass; a level of the hybrid sort has up to one more tile per large
-  // bucket (at most m / 4096 + 256 of them).
-  const uint32_t hyb_tiles = tiles + n / 4096u + 2u * uint32_t(RADIX);
+  // bucket (fewer than the levels' bucket capacity, m / 2048 + 4096).
+  const uint32_t hyb_tiles = tiles + n / 2048u + 4096u + 2u;
   uint32_t epoch = reserve_status(size_t(std::max(hyb_tiles, ftiles)) * STATUS_STRIDE, s);
   unsigned long long* status = static_cast<unsigned long long*>(t_ws.status->ptr);
   retain(t_ws.status);
