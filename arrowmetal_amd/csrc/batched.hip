@@ -18,7 +18,7 @@
 // Semantics per batch are those of am_argsort_ex2 (nulls last; `ieee` NaN after every value in both
 // directions) and of am_group_by_keys + am_group_agg_ex sum / count over an int64-like key column
 // with int64 values (groups ascending by key; the sum wraps in 64 bits). Nulls are not taken here.
-#include "sort_common.hpp"
+#include "msd_levels.hpp"
 
 #include <vector>
 
@@ -243,6 +243,90 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
   } else {
     scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
                                    out_keys, out_rows);
+  }
+}
+
+// ---------------------------------------------------------------- batched MSD argsort (64-bit keys)
+//
+// Every batch is one bucket of the adaptive MSD radix sort of msd_levels.hpp, in its blind form: the
+// whole sequence is fixed (a map pass, then one level per digit, then the bucket sort), so it can be
+// captured in the plan's graph, and every kernel reads from device memory what it has to do.
+
+// Maps the sort keys of every batch (the same map as the LSD pass) into k0, with rows counted from
+// each batch's start in r0, and the OR / AND of the mapped keys into the control block; a 64K-row
+// chunk of one batch per workgroup.
+template <typename K, int W>
+__global__ __launch_bounds__(BLOCK) void seg_msd_map(const void* vals, int type, int forder, int desc,
+                                                     const SegChunk* chunks, const uint32_t* seg_off, SegCtl* ctl,
+                                                     K* k0, uint32_t* r0) {
+  __shared__ unsigned long long s_or, s_and;
+  if (threadIdx.x == 0) {
+    s_or = 0ull;
+    s_and = ~0ull;
+  }
+  __syncthreads();
+  const SegChunk c = chunks[blockIdx.x];
+  const uint32_t seg_start = seg_off[c.seg];
+  unsigned long long o = 0ull, a = ~0ull;
+  constexpr int U = 8;
+  for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
+    uint64_t raw[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) raw[u] = load_raw<W>(vals, min(base + uint32_t(u) * BLOCK, c.end - 1u));
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t i = base + uint32_t(u) * BLOCK;
+      const K k = seg_key<K>(raw[u], type, forder, desc);
+      o |= (unsigned long long)k;   // rows past the end repeat the chunk's last key
+      a &= (unsigned long long)k;
+      if (i < c.end) {
+        k0[i] = k;
+        r0[i] = i - seg_start;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    o |= __shfl_xor(o, off, 64);
+    a &= __shfl_xor(a, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicOr(&s_or, o);
+    atomicAnd(&s_and, a);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicOr(&ctl->key_or, s_or);
+    atomicAnd(&ctl->key_and, s_and);
+  }
+}
+
+// One workgroup: the level kernels' control words (the active digits from the OR / AND, the run's
+// epoch) and their counters cleared.
+template <int P>
+__global__ void seg_msd_ctl(const SegCtl* ctl, MsdCtl* mc, uint32_t* lc) {
+  if (threadIdx.x < LC_WORDS) lc[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    const unsigned long long diff = ctl->key_or ^ ctl->key_and;
+    uint32_t mask = 0;
+    for (int p = 0; p < P; p++)
+      if ((diff >> (8 * p)) & 0xFFull) mask |= 1u << p;
+    *mc = {mask, 0u, ctl->epoch, 0u};
+  }
+}
+
+// Every batch as a bucket of level P (no digit applied yet): into the first large-bucket list, or the
+// bucket sort's list when it has at most local_max rows or all its keys are equal.
+template <int P>
+__global__ void seg_msd_lists(const uint32_t* seg_off, uint32_t nseg, const MsdCtl* mc, uint32_t local_max,
+                              Bucket* large, Bucket* local, uint32_t local_cap, uint32_t* lc) {
+  const uint32_t mask = mc->active_mask;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nseg; b += gridDim.x * blockDim.x) {
+    const uint32_t start = seg_off[b], size = seg_off[b + 1] - start;
+    if (size == 0u) continue;
+    const Bucket bk{start, size, 0u, uint32_t(P)};
+    if (size > local_max && mask != 0u) large[atomicAdd(&lc[LC_LIST], 1u)] = bk;
+    else push_local(local, local_cap, lc, bk, mask == 0u);
   }
 }
 
@@ -549,6 +633,9 @@ struct Batched {
   DevBuf seg_off, tiles, chunks, gb_chunks, hist, base, ctl, epoch, status;
   uint32_t ntiles = 0, nchunks = 0, ngb_chunks = 0;
   DevBuf k0, k1, r0, r1, out_idx;
+  MsdWork msd;                       // the levels of the MSD argsort of 64-bit keys
+  bool msd_used = false;             // the last recorded sort was the MSD one
+  bool use_msd = true;               // 64-bit sort keys take the MSD sort (the LSD one after an overflow)
   DevBuf tkey, tsum, tcnt, overflow, skey, ssum, scnt, gcount, goff, okey, osum, ocnt;
   size_t out_slots = 0;              // capacity of okey / osum / ocnt
   // What the last run produced.
@@ -610,6 +697,33 @@ void seg_sort_launch(Batched& b, const void* vals, int type, int desc, int forde
   AM_HIP(hipGetLastError());
 }
 
+// The batched argsort of 64-bit keys as an adaptive MSD radix sort (blind levels, graph-capturable).
+// Keys per thread in a level's pass: 16 (its persistent form stays at three workgroups per CU).
+void seg_sort_msd(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
+  using K = unsigned long long;
+  constexpr int P = 8;
+  auto* ctl = static_cast<SegCtl*>(b.ctl->ptr);
+  hipLaunchKernelGGL(seg_init, dim3(1), dim3(256), 0, s, ctl, static_cast<uint32_t*>(b.hist->ptr), size_t(0),
+                     static_cast<uint32_t*>(b.epoch->ptr));
+  K* kp0 = static_cast<K*>(b.k0->ptr);
+  K* kp1 = static_cast<K*>(b.k1->ptr);
+  uint32_t* rp0 = static_cast<uint32_t*>(b.r0->ptr);
+  uint32_t* rp1 = static_cast<uint32_t*>(b.r1->ptr);
+  if (b.nchunks > 0) {
+    ProfScope ps("seg_msd_map");
+    hipLaunchKernelGGL((seg_msd_map<K, 8>), dim3(b.nchunks), dim3(BLOCK), 0, s, vals, type, forder, desc,
+                       static_cast<const SegChunk*>(b.chunks->ptr), static_cast<const uint32_t*>(b.seg_off->ptr),
+                       ctl, kp0, rp0);
+  }
+  hipLaunchKernelGGL(seg_msd_ctl<P>, dim3(1), dim3(BLOCK), 0, s, ctl, b.msd.mc, b.msd.lc);
+  hipLaunchKernelGGL(seg_msd_lists<P>, dim3(std::min<uint32_t>((b.nseg + 255) / 256, 1024)), dim3(256), 0, s,
+                     static_cast<const uint32_t*>(b.seg_off->ptr), b.nseg, b.msd.mc, LOCAL_MAX / 2, b.msd.large[0],
+                     b.msd.local, b.msd.local_cap, b.msd.lc);
+  AM_HIP(hipGetLastError());
+  msd_levels<K, 16>(b.msd, b.n, P, 0xFFu, true, LOCAL_MAX / 2, static_cast<unsigned long long*>(b.status->ptr), kp0,
+                    rp0, kp1, rp1, static_cast<uint32_t*>(b.out_idx->ptr), s);
+}
+
 template <typename K, int W>
 void seg_sort_items(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
   if (b.items == 8) seg_sort_launch<K, W, 8>(b, vals, type, desc, forder, s);
@@ -618,6 +732,11 @@ void seg_sort_items(Batched& b, const void* vals, int type, int desc, int forder
 }
 
 void seg_sort(Batched& b, const void* vals, int type, int desc, int forder, hipStream_t s) {
+  b.msd_used = type_width(type) == 8 && b.use_msd;
+  if (b.msd_used) {
+    seg_sort_msd(b, vals, type, desc, forder, s);
+    return;
+  }
   switch (type_width(type)) {
     case 8: seg_sort_items<unsigned long long, 8>(b, vals, type, desc, forder, s); break;
     case 4: seg_sort_items<uint32_t, 4>(b, vals, type, desc, forder, s); break;
@@ -698,9 +817,12 @@ void build_tiles(Batched& b, int items) {
   b.ntiles = uint32_t(tiles.size());
   b.tiles = persistent(tiles.size() * sizeof(SegTile));
   AM_HIP(hipMemcpy(b.tiles->ptr, tiles.data(), tiles.size() * sizeof(SegTile), hipMemcpyHostToDevice));
-  // Status words carry the epoch of the pass that wrote them; zero is older than every epoch.
-  b.status = persistent(size_t(b.ntiles) * STATUS_STRIDE * 8);
-  AM_HIP(hipMemset(b.status->ptr, 0, size_t(b.ntiles) * STATUS_STRIDE * 8));
+  // Status words carry the epoch of the pass that wrote them; zero is older than every epoch. Rows
+  // for the LSD passes' tiles, or for a level of the MSD sort (16 keys per thread, up to one more
+  // tile per large bucket), whichever is more.
+  const size_t rows = std::max<size_t>(b.ntiles, size_t(b.n) / 4096u + b.msd.large_cap + 1u);
+  b.status = persistent(rows * STATUS_STRIDE * 8);
+  AM_HIP(hipMemset(b.status->ptr, 0, rows * STATUS_STRIDE * 8));
   AM_HIP(hipMemset(b.epoch->ptr, 0, 4));
 }
 
@@ -858,6 +980,7 @@ Batched* batched_create(const int64_t* offsets, int64_t nbatches, int hash_capac
     b->base = persistent(size_t(b->nseg) * 8 * RADIX * 4);
     b->ctl = persistent(sizeof(SegCtl));
     b->epoch = persistent(4);
+    b->msd = msd_work(b->n, 16, persistent, b->nseg);
     build_tiles(*b, (items == 8 || items == 20) ? items : 16);
     size_t nn = std::max<size_t>(n, 1);
     b->k0 = persistent(nn * 8);
@@ -949,6 +1072,18 @@ void batched_run(Batched* b, const Array* sort_keys, const Array* group_keys, co
     }
     ProfScope pscope("batched_graph");
     AM_HIP(hipGraphLaunch(b->exec, s));
+  }
+  if (sort_keys && b->msd_used) {
+    // The MSD sort's bucket list has room for every batch plus m / 32 more buckets; if a run needed
+    // more, this plan sorts with the LSD passes from now on, and this run's sort is done again.
+    uint32_t words[LC_READ];
+    AM_HIP(hipMemcpyAsync(words, b->msd.lc, sizeof(words), hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    if (words[LC_OVERFLOW] || uint64_t(words[LC_LOCAL]) + words[LC_GROUPS] > b->msd.local_cap) {
+      b->use_msd = false;
+      b->drop_graph();
+      seg_sort(*b, sort_keys->data(), sort_keys->type, desc, forder, s);
+    }
   }
   b->ran_sort = sort_keys != nullptr;
   b->ran_group = group_keys != nullptr;

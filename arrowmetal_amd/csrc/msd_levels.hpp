@@ -22,12 +22,20 @@ namespace {
 
 constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
 constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes (local_max is at most this)
+static_assert(LOCAL_MAX <= 4096u, "positions in a bucket fit the 12 low bits of the bucket sort's words");
+constexpr uint32_t SUB_MAX = 32;    // longest digit run of one bucket the bucket sort ranks by counting
+constexpr uint32_t GROUP_SPAN = 16;    // a group's buckets have their digit in one window of this many values
+constexpr uint32_t GROUP_BINS = GROUP_SPAN * RADIX;
+constexpr uint32_t GROUP_ROWS = 2048;  // rows of a group at most (the bucket sort at 8 keys per thread)
 
 // A bucket: rows [start, start + size) of the value block, in key / row buffer `buf` (0 or 1), all
 // sharing the digits from the top down to `level`.
 struct Bucket {
   uint32_t start, size, buf, level;
 };
+constexpr uint32_t LEVEL_BITS = 0xFFu;    // Bucket::level: the level ...
+constexpr uint32_t GROUP_FLAG = 0x100u;   // ... and, in the bucket sort's list, a group of small buckets
+constexpr int SPAN_SHIFT = 12;            // ... whose digits share an aligned window of 1 << (level >> 12)
 struct LvlChunk {
   uint32_t bucket, start, end, pad;
 };
@@ -51,6 +59,7 @@ constexpr int LC_MAXLOCAL = 5;    // largest bucket the bucket sort sorts (not c
 constexpr int LC_OVERFLOW = 6;    // the bucket sort's list ran out of room
 constexpr int LC_QLEN = 7;        // buckets queued for the LSD bucket sort
 constexpr int LC_ROWS = 8;        // rows in the next level's large buckets (bounds its grids)
+constexpr int LC_GROUPS = 9;      // groups of small buckets, at the far end of the bucket sort's list
 constexpr int LC_READ = 16;
 constexpr int LC_NEXT_TILE = 32;  // the next tile of the current level to take (own 128-byte line)
 constexpr int LC_QNEXT = 64;      // the next queued bucket to take (own 128-byte line)
@@ -62,6 +71,15 @@ __device__ __forceinline__ void push_local(Bucket* local, uint32_t local_cap, ui
   if (i < local_cap) local[i] = b;
   else atomicOr(&lc[LC_OVERFLOW], 1u);
   if (!sorted) atomicMax(&lc[LC_MAXLOCAL], b.size);
+}
+// Appends one group of small buckets to the list, from its far end (entry local_cap - 1 down). The
+// list ran out of room when LC_LOCAL + LC_GROUPS > local_cap (list_overflow).
+__device__ __forceinline__ void push_group(Bucket* local, uint32_t local_cap, uint32_t* lc, Bucket b) {
+  const uint32_t g = atomicAdd(&lc[LC_GROUPS], 1u);
+  if (g < local_cap) local[local_cap - 1u - g] = b;
+}
+__device__ __forceinline__ bool list_overflow(const uint32_t* lc, uint32_t local_cap) {
+  return lc[LC_OVERFLOW] != 0u || uint64_t(lc[LC_LOCAL]) + lc[LC_GROUPS] > local_cap;
 }
 
 // One workgroup: where each large bucket's chunks and tiles start in a level's tables (list `cur`
@@ -174,21 +192,47 @@ __global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1
   }
 }
 
+// Inclusive running maximum over the threads of a workgroup (tmp: WAVES words of LDS).
+__device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* tmp) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v = max(v, u);
+  }
+  if (lane == 63) tmp[w] = v;
+  __syncthreads();
+  uint32_t pre = 0;
+#pragma unroll
+  for (int k = 0; k < WAVES; k++) pre = k < w ? max(pre, tmp[k]) : pre;
+  __syncthreads();
+  return max(v, pre);
+}
+
 // Each large bucket, a workgroup per bucket in turn: the start of each of its digit-d sub-buckets
 // (bucket_base), and where each goes: to the bucket sort's list (at most local_max rows; or, of any
 // size, equal keys: the bucket's keys differ in no bit below digit d, or no active digit is left
 // below d), or to the next level's list. A bucket whose rows all share digit d (every bucket, when
 // digit d is inactive) is carried (carried[b] = 1): it keeps its buffer and the pass skips it.
+// Neighbouring sub-buckets are adjacent in memory and in key order, so several leave as one entry:
+// - a run of sub-buckets of equal keys, as one copy;
+// - sub-buckets of at most small_max rows, as a group: the sub-buckets of the widest aligned window of
+//   digits (1, 2, 4, 8 or GROUP_SPAN values) around each that holds at most group_max rows and only
+//   such sub-buckets, sorted together by the digits from d down (level d + 1).
 __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, const uint32_t* bucket_hist,
                                                   const unsigned long long* bucket_or,
                                                   const unsigned long long* bucket_and, const MsdCtl* mc, int d,
-                                                  uint32_t local_max, uint32_t* bucket_base, uint32_t* carried,
-                                                  Bucket* next, Bucket* local, uint32_t local_cap, uint32_t* lc) {
+                                                  uint32_t local_max, uint32_t small_max, uint32_t group_max,
+                                                  uint32_t* bucket_base, uint32_t* carried, Bucket* next,
+                                                  Bucket* local, uint32_t local_cap, uint32_t* lc) {
+  enum : uint32_t { SUB_EMPTY = 0, SUB_COPY = 1, SUB_SMALL = 2, SUB_OTHER = 3 };
   __shared__ uint32_t tmp[WAVES];
+  __shared__ uint2 s_sub[RADIX];   // (rows, start) of each non-empty sub-bucket of equal keys
   const uint32_t nb = lc[LC_LIST + cur];
   const uint32_t mask = mc->active_mask;
   const bool active = (mask >> d) & 1u;
   const bool last = (mask & ((1u << d) - 1u)) == 0u;
+  const uint32_t lane = threadIdx.x & 63u;
   for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const Bucket bk = large[b];
     uint32_t v, base;
@@ -205,51 +249,95 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
       carry = true;
     }
     if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
+    const uint32_t buf = carry ? bk.buf : (bk.buf ^ 1u);
+    uint32_t kind = SUB_EMPTY;
     if (v != 0u) {
       const unsigned long long diff = active ? (bucket_or[b] ^ bucket_and[b]) : ~0ull;
       const bool equal_below = (diff & ((1ull << (8 * d)) - 1ull)) == 0ull;
-      // A sub-bucket of equal keys is copied: level 0 leaves it no digit to sort by.
-      const Bucket sub{base, v, carry ? bk.buf : (bk.buf ^ 1u), equal_below ? 0u : uint32_t(d)};
       if (last || equal_below) {
-        push_local(local, local_cap, lc, sub, true);
+        kind = SUB_COPY;
+      } else if (v <= small_max) {
+        kind = SUB_SMALL;
       } else if (v <= local_max) {
-        push_local(local, local_cap, lc, sub, false);
+        kind = SUB_OTHER;
+        push_local(local, local_cap, lc, Bucket{base, v, buf, uint32_t(d)}, false);
       } else {
-        next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = sub;
+        kind = SUB_OTHER;
+        next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = Bucket{base, v, buf, uint32_t(d)};
         atomicAdd(&lc[LC_ROWS], v);
       }
+    }
+    // Groups: the rows of each aligned window (any other sub-bucket weighs more than a group holds);
+    // the first non-empty sub-bucket of the chosen window pushes it.
+    {
+      const uint32_t w1 = kind == SUB_SMALL ? v : (v != 0u ? group_max + 1u : 0u);
+      const uint32_t w2 = w1 + __shfl_xor(w1, 1, 64);
+      const uint32_t w4 = w2 + __shfl_xor(w2, 2, 64);
+      const uint32_t w8 = w4 + __shfl_xor(w4, 4, 64);
+      const uint32_t w16 = w8 + __shfl_xor(w8, 8, 64);
+      static_assert(GROUP_SPAN == 16 && GROUP_BINS == RADIX << 4, "window sums up to GROUP_SPAN digits");
+      const unsigned long long nonempty = __ballot(v != 0u);
+      if (kind == SUB_SMALL) {
+        uint32_t span = 16, rows = w16;
+        if (rows > group_max) { span = 8; rows = w8; }
+        if (rows > group_max) { span = 4; rows = w4; }
+        if (rows > group_max) { span = 2; rows = w2; }
+        if (rows > group_max) { span = 1; rows = w1; }
+        const uint32_t first = lane & ~(span - 1u);
+        const unsigned long long earlier = nonempty & ((1ull << lane) - 1ull) & ~((1ull << first) - 1ull);
+        if (earlier == 0ull)
+          push_group(local, local_cap, lc,
+                     Bucket{base, rows, buf,
+                            (uint32_t(d) + 1u) | GROUP_FLAG | (uint32_t(__builtin_ctz(span)) << SPAN_SHIFT)});
+      }
+    }
+    // Copies: the non-empty sub-buckets compacted in digit order; a run of equal-key ones leaves as one.
+    uint32_t nne;
+    const uint32_t ci = block_excl_scan(v != 0u ? 1u : 0u, tmp, nne);
+    if (v != 0u) s_sub[ci] = make_uint2(kind == SUB_COPY ? v : 0u, base);
+    if (__syncthreads_or(kind == SUB_COPY ? 1 : 0)) {   // (orders the writes above)
+      const uint32_t i = threadIdx.x;
+      uint2 me = make_uint2(0u, 0u), prev = me, next_sub = me;
+      if (i < nne) {
+        me = s_sub[i];
+        if (i > 0) prev = s_sub[i - 1];
+        if (i + 1 < nne) next_sub = s_sub[i + 1];
+      }
+      const bool copy = me.x != 0u;
+      const bool run_start = copy && prev.x == 0u, run_end = copy && next_sub.x == 0u;
+      const uint32_t run_base = block_incl_max(run_start ? me.y + 1u : 0u, tmp) - 1u;
+      // A copy keeps level 0: no digit is left to sort by.
+      if (run_end) push_local(local, local_cap, lc, Bucket{run_base, me.y + me.x - run_base, buf, 0u}, true);
     }
     __syncthreads();
   }
 }
 
 // A level's pass: every large bucket that is not carried partitioned stably by digit d from its buffer
-// into the other one, at the same positions. Tile ids come from a counter; tiles of carried buckets
-// (whose whole look-back chain is skipped) and blocks past the last tile exit.
+// into the other one, at the same positions (nothing when digit d is inactive). Tile ids come from a
+// counter, so a tile's look-back only ever waits on tiles already taken by running workgroups; tiles
+// of carried buckets (whose whole look-back chain is skipped) are dropped.
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_pass(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
-                                                  const uint32_t* carried, const uint32_t* bucket_base,
-                                                  unsigned long long* status, const MsdCtl* mc, uint32_t eoff,
-                                                  int d, K* keys0, uint32_t* rows0, K* keys1, uint32_t* rows1) {
+struct LvlLds {
+  uint32_t wc[WAVES][WC_STRIDE];
+  uint32_t tile_total[WC_STRIDE];
+  uint32_t bin_start[RADIX];
+  uint32_t bin_base[WC_STRIDE];
+  uint32_t tmp[WAVES];
+  uint32_t tile;
+  uint8_t digit[BLOCK * ITEMS];
+  K stage[BLOCK * ITEMS];
+};
+
+template <typename K, int ITEMS>
+__device__ __forceinline__ void lvl_tile(LvlLds<K, ITEMS>& L, uint32_t tile, const Bucket* large,
+                                         const LvlTile* tiles, const uint32_t* carried, const uint32_t* bucket_base,
+                                         unsigned long long* status, uint32_t epoch, int d, K* keys0,
+                                         uint32_t* rows0, K* keys1, uint32_t* rows1) {
   constexpr int TILE = BLOCK * ITEMS;
-  __shared__ uint32_t wc[WAVES][WC_STRIDE];
-  __shared__ uint32_t tile_total[WC_STRIDE];
-  __shared__ uint32_t bin_start[RADIX];
-  __shared__ uint32_t bin_base[WC_STRIDE];
-  __shared__ uint32_t tmp[WAVES];
-  __shared__ uint32_t s_tile;
-  __shared__ uint8_t s_digit[TILE];
-  __shared__ K s_stage[TILE];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // Exactly the first LC_TILES workgroups take a tile each (in counter order); the rest exit at once.
-  if (blockIdx.x >= lc[LC_TILES] || !((mc->active_mask >> d) & 1u)) return;
-  const uint32_t epoch = mc->epoch + eoff;
-  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&wc[0][0])[i] = 0u;
-  if (threadIdx.x == 0) s_tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
   const LvlTile t = tiles[tile];
-  if (carried[t.bucket]) return;
+  if (carried[t.bucket]) return;   // uniform over the workgroup
   const bool from1 = large[t.bucket].buf != 0u;
   const K* in_keys = from1 ? keys1 : keys0;
   const uint32_t* in_rows = from1 ? rows1 : rows0;
@@ -268,19 +356,60 @@ __global__ __launch_bounds__(BLOCK) void lvl_pass(const Bucket* large, const Lvl
 #pragma unroll
   for (int j = 0; j < ITEMS; j++)
     bins[j] = base_e + uint32_t(j) * 64u < t.end ? uint32_t(keys[j] >> shift) & 0xFFu : uint32_t(BIN_NONE);
-  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, wc[w], lane);
-  else rank_tile<ITEMS, 9>(bins, wc[w], lane);
+  if (t.end - t.start == uint32_t(TILE)) rank_tile<ITEMS, 8>(bins, L.wc[w], lane);
+  else rank_tile<ITEMS, 9>(bins, L.wc[w], lane);
   __syncthreads();
-  const uint32_t vcount = tile_bin_layout<RADIX>(wc, tile_total, bin_start, tmp);
-  look_back<RADIX>(status, tile, epoch, tile_total, bin_base, t.head);
+  const uint32_t vcount = tile_bin_layout<RADIX>(L.wc, L.tile_total, L.bin_start, L.tmp);
+  look_back<RADIX>(status, tile, epoch, L.tile_total, L.bin_base, t.head);
   __syncthreads();
-  bin_base[threadIdx.x] += bucket_base[size_t(t.bucket) * RADIX + threadIdx.x];
+  L.bin_base[threadIdx.x] += bucket_base[size_t(t.bucket) * RADIX + threadIdx.x];
   __syncthreads();
-  scatter_values<K, ITEMS, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount, out_keys,
-                                 out_rows);
+  scatter_values<K, ITEMS, true>(keys, rows, bins, L.wc[w], L.bin_start, L.bin_base, L.digit, L.stage, vcount,
+                                 out_keys, out_rows);
 }
 
-constexpr uint32_t SUB_MAX = 32;   // largest sub-bucket ranked by counting
+// One tile per workgroup: exactly the first LC_TILES workgroups take a tile each; the rest exit.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(BLOCK, 3) void lvl_pass(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
+                                                  const uint32_t* carried, const uint32_t* bucket_base,
+                                                  unsigned long long* status, const MsdCtl* mc, uint32_t eoff,
+                                                  int d, K* keys0, uint32_t* rows0, K* keys1, uint32_t* rows1) {
+  __shared__ LvlLds<K, ITEMS> L;
+  if (blockIdx.x >= lc[LC_TILES] || !((mc->active_mask >> d) & 1u)) return;
+  for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&L.wc[0][0])[i] = 0u;
+  if (threadIdx.x == 0) L.tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
+  __syncthreads();
+  lvl_tile<K, ITEMS>(L, L.tile, large, tiles, carried, bucket_base, status, mc->epoch + eoff, d, keys0, rows0, keys1,
+                     rows1);
+}
+
+// Persistent form (blind levels, where the tile count is not known on the host): each workgroup takes
+// tiles until none is left.
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(BLOCK, 3) void lvl_pass_loop(const Bucket* large, const LvlTile* tiles, uint32_t* lc,
+                                                       const uint32_t* carried, const uint32_t* bucket_base,
+                                                       unsigned long long* status, const MsdCtl* mc, uint32_t eoff,
+                                                       int d, K* keys0, uint32_t* rows0, K* keys1, uint32_t* rows1) {
+  __shared__ LvlLds<K, ITEMS> L;
+  if (!((mc->active_mask >> d) & 1u)) return;   // uniform over the grid
+  const uint32_t ntiles = lc[LC_TILES];
+  const uint32_t epoch = mc->epoch + eoff;
+  while (true) {
+    for (int i = threadIdx.x; i < WAVES * WC_STRIDE; i += BLOCK) (&L.wc[0][0])[i] = 0u;
+    if (threadIdx.x == 0) L.tile = atomicAdd(&lc[LC_NEXT_TILE], 1u);
+    __syncthreads();
+    const uint32_t tile = L.tile;
+    if (tile >= ntiles) return;   // uniform: every workgroup ends here once the tiles run out
+    lvl_tile<K, ITEMS>(L, tile, large, tiles, carried, bucket_base, status, epoch, d, keys0, rows0, keys1, rows1);
+    __syncthreads();
+  }
+}
+
+
+// The bucket sort's LDS besides the bucket: the counters of its split.
+struct LocalLds {
+  uint32_t cnt[RADIX], start[RADIX], cursor[RADIX], tmp[WAVES];
+};
 
 // One stable LDS pass of a bucket by digit d: keys and their positions in the bucket (pos[j]) land
 // in s_keys / s_pos in (digit, input) order; on return bin_start / tile_total give each digit's run.
@@ -310,13 +439,16 @@ __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t 
   }
 }
 
-// Stable LSD passes over every varying digit below the top two, in LDS, for one bucket of at most
-// LI * BLOCK rows (rows in input order), then its rows written to their final slots.
+// One bucket of at most LI * BLOCK rows (rows in input order inside each digit run of the levels
+// before) in LDS, its rows written to their final slots. A bucket whose top varying digit is 7 is
+// first split stably by it; when no digit run is longer than run_max rows, a row's place inside its
+// run is the number of rows there with a smaller key, or an equal key earlier (LDS order is input
+// order inside a run). Otherwise stable LSD passes run over every varying digit.
 template <typename K, int LI>
 __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
-                                          uint32_t low_mask, uint32_t* out_idx, uint32_t (*wc)[WC_STRIDE],
-                                          uint32_t* tile_total, uint32_t* bin_start, uint32_t* tmp, K* s_keys,
-                                          uint16_t* s_pos) {
+                                          uint32_t low_mask, uint32_t run_max, uint32_t* out_idx,
+                                          uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
+                                          uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t base_e = uint32_t(w) * 64u * LI + uint32_t(lane);
   K keys[LI];
@@ -325,6 +457,38 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
   for (int j = 0; j < LI; j++) {
     pos[j] = base_e + uint32_t(j) * 64u;
     keys[j] = keys_in[off + min(pos[j], size - 1u)];
+  }
+  if ((low_mask >> 7) & 1u) {
+    local_split<K, LI>(keys, pos, base_e, size, 7, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+    __syncthreads();
+    if (!__syncthreads_or(tile_total[threadIdx.x] > run_max ? 1 : 0)) {
+      uint32_t dest[LI], src[LI];
+#pragma unroll
+      for (int j = 0; j < LI; j++) {
+        const uint32_t e = base_e + uint32_t(j) * 64u;
+        dest[j] = ~0u;
+        src[j] = off;
+        if (e < size) {
+          const K key = s_keys[e];
+          const uint32_t dd = uint32_t(key >> 56);
+          const uint32_t lo = bin_start[dd], hi = lo + tile_total[dd];
+          uint32_t below = 0;
+#pragma unroll 4
+          for (uint32_t q = lo; q < e; q++) below += s_keys[q] <= key ? 1u : 0u;
+#pragma unroll 4
+          for (uint32_t q = e + 1u; q < hi; q++) below += s_keys[q] < key ? 1u : 0u;
+          dest[j] = off + lo + below;
+          src[j] = off + s_pos[e];
+        }
+      }
+      uint32_t r[LI];
+#pragma unroll
+      for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];
+#pragma unroll
+      for (int j = 0; j < LI; j++)
+        if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
+      return;
+    }
   }
   for (int d = 0; d < 8; d++) {
     if (!((low_mask >> d) & 1u)) continue;   // uniform
@@ -348,67 +512,58 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
   }
 }
 
-// The bucket sort: one workgroup per entry of its list (a bucket of at most LI * BLOCK rows; the host
-// picks LI from the largest). The rows of a bucket arrive in input order (every partition before was
-// stable), so the stable order is the order of (key, position in the bucket) pairs, which are all
-// distinct. The bucket is split by its most significant varying digit with LDS counters (any order
-// inside a digit), as one 64-bit word per row: digit << 56 | (key bits below the digit) << POS_BITS |
-// position, so that inside one digit run words order as those pairs do; each row's place in its run
-// is the number of words there below its own. A bucket whose varying digits do not fit that word (a
-// top-digit bucket), or with a run longer than SUB_MAX rows (many equal keys), is queued for the
-// stable LSD passes of hyb_local_lsd instead. A bucket of equal keys (no varying digit left), of any
-// size, is copied.
-constexpr int POS_BITS = 12;   // positions in a bucket of at most LOCAL_MAX rows
-static_assert((1u << POS_BITS) >= LOCAL_MAX, "positions fit POS_BITS");
-template <typename K, int LI>
-__device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, uint32_t idx, const K* keys0,
-                                          const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
-                                          uint32_t* out_idx, uint32_t* queue, uint32_t* lc, uint32_t* cnt,
-                                          uint32_t* start, uint32_t* cursor, uint32_t* tmp, K* s_words) {
-  const Bucket bk = list[idx];
-  const K* keys_in = bk.buf ? keys1 : keys0;
-  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
-  const uint32_t size = bk.size, off = bk.start;
-  const uint32_t low_mask = mc->active_mask & ((1u << bk.level) - 1u);
-  uint32_t* out = out_idx + mc->value_base;
-  if (low_mask == 0u) {   // equal keys: the bucket is in order already
-    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
-    return;
-  }
-  if (size > uint32_t(LI * BLOCK)) return;   // never: LI covers every bucket to sort
-  const int top = 31 - __builtin_clz(low_mask);
-  if (top > 5) {   // 8 + 8 * top + POS_BITS bits would not fit one word
-    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
-    return;
-  }
-  const int shift = 8 * top;   // at most 40: 8 + 40 + POS_BITS bits per word
-  cnt[threadIdx.x] = 0u;
-  K keys[LI];
+// A group of small sibling buckets (at most GROUP_ROWS rows in all; their digits `top` in one aligned
+// window of 1 << span_log values; rows in input order inside each): split with LDS counters by (digit
+// top mod the window, digit `sub`, the next varying one), in that order, each row as one word, (key bits
+// below digit sub) << 12 | position, its bin beside it; a row's place inside its bin is the number of
+// words there below its own. Returns false, having written nothing, when a bin holds more than SUB_MAX
+// rows. g: RADIX << span_log words of LDS (counts, then starts, then, after the scatter, each bin's
+// end).
+template <typename K, int LI, uint32_t SPAN_LOG>
+__device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
+                                            int top, int sub, uint32_t* out, uint32_t* tmp, uint32_t* g, K* s_words,
+                                            uint16_t* s_bin) {
+  static_assert(RADIX == BLOCK, "a thread per digit value");
+  constexpr uint32_t per = 1u << SPAN_LOG;   // bins per thread
+  const int ts = 8 * top, ss = 8 * sub;
+  uint32_t* mine = g + threadIdx.x * per;
 #pragma unroll
-  for (int j = 0; j < LI; j++) keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
+  for (uint32_t k = 0; k < per; k++) mine[k] = 0u;
+  K keys[LI];
+  uint32_t bins[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
+    bins[j] = ((uint32_t(keys[j] >> ts) & (per - 1u)) << 8) | (uint32_t(keys[j] >> ss) & 0xFFu);
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < LI; j++)
-    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&cnt[uint32_t(keys[j] >> shift) & 0xFFu], 1u);
+    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&g[bins[j]], 1u);
   __syncthreads();
-  const uint32_t c = cnt[threadIdx.x];
-  uint32_t total;
-  const uint32_t st = block_excl_scan(c, tmp, total);
-  start[threadIdx.x] = st;
-  cursor[threadIdx.x] = st;
-  if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
-    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
-    return;
+  uint32_t c[per], sum = 0, big = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < per; k++) {
+    c[k] = mine[k];
+    sum += c[k];
+    big |= c[k] > SUB_MAX ? 1u : 0u;
   }
-  const unsigned long long low_bits = (1ull << shift) - 1ull;
+  uint32_t total;
+  uint32_t run = block_excl_scan(sum, tmp, total);
+#pragma unroll
+  for (uint32_t k = 0; k < per; k++) {
+    mine[k] = run;
+    run += c[k];
+  }
+  if (__syncthreads_or(int(big))) return false;
+  const unsigned long long low_bits = (1ull << ss) - 1ull;
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
     if (p < size) {
-      const uint32_t dd = uint32_t(keys[j] >> shift) & 0xFFu;
-      const uint32_t slot = atomicAdd(&cursor[dd], 1u);
-      s_words[slot] = K((static_cast<unsigned long long>(dd) << 56) |
-                        ((static_cast<unsigned long long>(keys[j]) & low_bits) << POS_BITS) | p);
+      const uint32_t slot = atomicAdd(&g[bins[j]], 1u);
+      s_words[slot] = K(((static_cast<unsigned long long>(keys[j]) & low_bits) << 12) | p);
+      s_bin[slot] = uint16_t(bins[j]);
     }
   }
   __syncthreads();
@@ -420,13 +575,100 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
     src[j] = off;
     if (p < size) {
       const K word = s_words[p];
-      const uint32_t dd = uint32_t(word >> 56);
-      const uint32_t lo = start[dd], hi = lo + cnt[dd];
+      const uint32_t bin = s_bin[p];
+      const uint32_t lo = bin ? g[bin - 1u] : 0u, hi = g[bin];
       uint32_t below = 0;
 #pragma unroll 4
       for (uint32_t q = lo; q < hi; q++) below += s_words[q] < word ? 1u : 0u;
       dest[j] = off + lo + below;
-      src[j] = off + uint32_t(word & ((1u << POS_BITS) - 1u));
+      src[j] = off + uint32_t(word & 0xFFFu);
+    }
+  }
+  uint32_t r[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (dest[j] != ~0u) out[dest[j]] = r[j];
+  return true;
+}
+
+// The bucket sort: one workgroup per entry of its list (a bucket of at most LI * BLOCK rows; the host
+// picks LI from the largest). The rows of a bucket arrive in input order inside each digit run of every
+// level (every partition before was stable), so the stable order is the order of (key, position)
+// pairs. The bucket is split by its most significant varying digit `top` with LDS counters (any order
+// inside a digit run), each row as one 64-bit word, (key bits below the digit) << 12 | position, its
+// digit in a byte beside it; inside a run, words order as the pairs do, and a row's place there is the
+// number of words below its own. A bucket whose words would not fit 64 bits (top = 7), or with a run
+// longer than SUB_MAX rows, is queued for hyb_local_lsd instead. A bucket of equal keys (no varying
+// digit left), of any size, is copied.
+template <typename K, int LI>
+__device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, uint32_t idx, const K* keys0,
+                                          const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
+                                          uint32_t* out_idx, uint32_t* queue, uint32_t* lc, LocalLds& L,
+                                          K* s_words, uint8_t* s_dig) {
+  const Bucket bk = list[idx];
+  const K* keys_in = bk.buf ? keys1 : keys0;
+  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+  const uint32_t size = bk.size, off = bk.start;
+  const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
+  uint32_t* out = out_idx + mc->value_base;
+  if (low_mask == 0u) {   // equal keys: the bucket is in order already
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
+    return;
+  }
+  if (size > uint32_t(LI * BLOCK)) return;   // never: LI covers every bucket to sort
+  const int top = 31 - __builtin_clz(low_mask);
+  if (top > 6) {   // the key bits below the digit and the position would not fit one word
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
+    return;
+  }
+  uint32_t dest[LI], src[LI];
+  const int shift = 8 * top;   // at most 48: 48 + 12 bits per word
+  L.cnt[threadIdx.x] = 0u;
+  K keys[LI];
+#pragma unroll
+  for (int j = 0; j < LI; j++) keys[j] = keys_in[off + min(threadIdx.x + uint32_t(j) * BLOCK, size - 1u)];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++)
+    if (threadIdx.x + uint32_t(j) * BLOCK < size) atomicAdd(&L.cnt[uint32_t(keys[j] >> shift) & 0xFFu], 1u);
+  __syncthreads();
+  const uint32_t c = L.cnt[threadIdx.x];
+  uint32_t total;
+  const uint32_t st = block_excl_scan(c, L.tmp, total);
+  L.start[threadIdx.x] = st;
+  L.cursor[threadIdx.x] = st;
+  if (__syncthreads_or(c > SUB_MAX ? 1 : 0)) {
+    if (threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
+    return;
+  }
+  const unsigned long long low_bits = (1ull << shift) - 1ull;
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    if (p < size) {
+      const uint32_t dd = uint32_t(keys[j] >> shift) & 0xFFu;
+      const uint32_t slot = atomicAdd(&L.cursor[dd], 1u);
+      s_words[slot] = K(((static_cast<unsigned long long>(keys[j]) & low_bits) << 12) | p);
+      s_dig[slot] = uint8_t(dd);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LI; j++) {
+    const uint32_t p = threadIdx.x + uint32_t(j) * BLOCK;
+    dest[j] = ~0u;
+    src[j] = off;
+    if (p < size) {
+      const K word = s_words[p];
+      const uint32_t dd = s_dig[p];
+      const uint32_t lo = L.start[dd], hi = lo + L.cnt[dd];
+      uint32_t below = 0;
+#pragma unroll 4
+      for (uint32_t q = lo; q < hi; q++) below += s_words[q] < word ? 1u : 0u;
+      dest[j] = off + lo + below;
+      src[j] = off + uint32_t(word & 0xFFFu);
     }
   }
   uint32_t r[LI];
@@ -437,27 +679,69 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
     if (dest[j] != ~0u) out[dest[j]] = r[j];
 }
 
-// The bucket sort over its list, an entry per workgroup in turn (LDS for the bucket is dynamic:
-// LI * BLOCK * sizeof(K) bytes).
+// The bucket sort over its list, an entry per workgroup in turn (nothing when the list ran out of room).
+// LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 1) bytes.
 template <typename K, int LI>
 __global__ __launch_bounds__(BLOCK) void hyb_local(const MsdCtl* mc, const Bucket* list, uint32_t local_cap,
                                                    const K* keys0, const uint32_t* rows0, const K* keys1,
                                                    const uint32_t* rows1, uint32_t* out_idx, uint32_t* queue,
                                                    uint32_t* lc) {
-  __shared__ uint32_t cnt[RADIX], start[RADIX], cursor[RADIX];
-  __shared__ uint32_t tmp[WAVES];
+  __shared__ LocalLds L;
   extern __shared__ __align__(16) unsigned char local_lds[];
-  K* s_words = reinterpret_cast<K*>(local_lds);   // LI * BLOCK words
-  const uint32_t nlocal = min(lc[LC_LOCAL], local_cap);
+  K* s_words = reinterpret_cast<K*>(local_lds);
+  uint8_t* s_dig = reinterpret_cast<uint8_t*>(s_words + LI * BLOCK);
+  if (list_overflow(lc, local_cap)) return;   // uniform
+  const uint32_t nlocal = lc[LC_LOCAL];
   for (uint32_t i = blockIdx.x; i < nlocal; i += gridDim.x) {
-    local_one<K, LI>(mc, list, i, keys0, rows0, keys1, rows1, out_idx, queue, lc, cnt, start, cursor, tmp,
-                     s_words);
+    local_one<K, LI>(mc, list, i, keys0, rows0, keys1, rows1, out_idx, queue, lc, L, s_words, s_dig);
     __syncthreads();
   }
 }
 
-// The queued buckets, each by stable LSD passes in LDS: persistent workgroups take them from a
-// counter until none is left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
+// The groups of small buckets (the far end of the list), a group per workgroup in turn; a group with a
+// bin longer than SUB_MAX rows is queued for hyb_local_lsd. LDS for the group is dynamic:
+// GROUP_ROWS * (sizeof(K) + 2) + GROUP_BINS * 4 bytes.
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucket* list, uint32_t local_cap,
+                                                   const K* keys0, const uint32_t* rows0, const K* keys1,
+                                                   const uint32_t* rows1, uint32_t* out_idx, uint32_t* queue,
+                                                   uint32_t* lc) {
+  constexpr int LI = int(GROUP_ROWS / BLOCK);
+  __shared__ uint32_t tmp[WAVES];
+  extern __shared__ __align__(16) unsigned char local_lds[];
+  K* s_words = reinterpret_cast<K*>(local_lds);
+  uint16_t* s_bin = reinterpret_cast<uint16_t*>(s_words + GROUP_ROWS);
+  uint32_t* g_bins = reinterpret_cast<uint32_t*>(s_bin + GROUP_ROWS);
+  if (list_overflow(lc, local_cap)) return;   // uniform
+  const uint32_t ngroups = lc[LC_GROUPS];
+  uint32_t* out = out_idx + mc->value_base;
+  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const uint32_t idx = local_cap - 1u - g;
+    const Bucket bk = list[idx];
+    const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
+    const int top = 31 - __builtin_clz(low_mask);   // a group's buckets differ in digit top (its level - 1)
+    const uint32_t below_top = low_mask & ((1u << top) - 1u);   // never 0: they are not of equal keys
+    const uint32_t span_log = (bk.level >> SPAN_SHIFT) & 0xFu;
+    const K* keys_in = bk.buf ? keys1 : keys0;
+    const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+    const int sub = 31 - __builtin_clz(below_top);
+    bool done = bk.size > GROUP_ROWS;   // never
+    if (!done) {
+      switch (span_log) {
+        case 0: done = local_group<K, LI, 0>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+        case 1: done = local_group<K, LI, 1>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+        case 2: done = local_group<K, LI, 2>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+        case 3: done = local_group<K, LI, 3>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+        default: done = local_group<K, LI, 4>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+      }
+    }
+    if (!done && threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
+    __syncthreads();
+  }
+}
+
+// The queued buckets, each by local_lsd: persistent workgroups take them from a counter until none is
+// left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
 template <typename K, int LI>
 __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const Bucket* list, const K* keys0,
                                                        const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
@@ -477,9 +761,10 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
     const uint32_t qi = s_q;
     if (qi >= queued) return;   // uniform: every workgroup ends here once the queue is empty
     const Bucket bk = list[queue[qi]];
-    const uint32_t low_mask = mc->active_mask & ((1u << bk.level) - 1u);
-    local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask,
-                     out_idx + mc->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+    const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
+    if (bk.size <= uint32_t(LI * BLOCK))   // always: the host sizes LI for every queued bucket
+      local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask, SUB_MAX,
+                       out_idx + mc->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
   }
 }
@@ -487,7 +772,9 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
 
 // ---------------------------------------------------------------- host side
 
-// The device buffers of the levels for up to m value rows, with `items` keys per thread in a pass.
+// The device buffers of the levels for up to m value rows, with `items` keys per thread in a pass
+// (the tile tables are sized for that many or more), and extra_local more entries in the bucket
+// sort's list (a batched plan's batches).
 // Capacities: buckets above LOCAL_MAX / 2 rows at one level (the least local_max used); entries of the
 // bucket sort's list (every entry has a row, and runs of many tiny buckets are what fills it); chunks
 // and tiles of a level.
@@ -504,11 +791,11 @@ struct MsdWork {
   MsdCtl* mc = nullptr;
 };
 
-inline MsdWork msd_work(uint32_t m, int items, DevBuf (*alloc)(size_t)) {
+inline MsdWork msd_work(uint32_t m, int items, DevBuf (*alloc)(size_t), uint32_t extra_local = 0) {
   MsdWork w;
   const uint32_t tile = uint32_t(BLOCK * items);
   w.large_cap = m / (LOCAL_MAX / 2) + 4096u;
-  w.local_cap = m / 32u + (1u << 17);
+  w.local_cap = m / 32u + (1u << 17) + extra_local;
   const uint32_t max_chunks = m / LVL_CHUNK + w.large_cap + 1u;
   const uint32_t max_tiles = m / tile + w.large_cap + 1u;
   w.lists = alloc((size_t(w.large_cap) * 2 + w.local_cap) * sizeof(Bucket));
@@ -549,7 +836,11 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
   constexpr int P = int(sizeof(K));
   constexpr uint32_t TILE = BLOCK * ITEMS;
   const uint32_t cus = uint32_t(device().cu_count);
-  uint32_t* hp = host_plan_words();
+  uint32_t* hp = blind ? nullptr : host_plan_words();   // nothing pinned is touched in a capture
+  // Sub-buckets packed into groups: up to 512 rows from host read-backs (larger ones sort faster one
+  // per workgroup, where the workgroup count follows the list), any bucket sort size when blind (the
+  // bucket sort's workgroups are persistent and take 8 keys per thread).
+  const uint32_t small_max = blind ? GROUP_ROWS : 512u;
   auto next_digit = [&](int d) {
     while (--d >= 0 && !blind && !((mask >> d) & 1u)) {}
     return d;
@@ -576,15 +867,19 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
     {
       ProfScope ps("sort_msd_plan");
       hipLaunchKernelGGL(lvl_plan, dim3(std::min(bound, cus * 4u)), dim3(BLOCK), 0, s, w.large[cur], cur,
-                         w.bucket_hist, w.bucket_or, w.bucket_and, w.mc, d, local_max, w.bucket_base, w.carried,
-                         w.large[cur ^ 1], w.local, w.local_cap, w.lc);
+                         w.bucket_hist, w.bucket_or, w.bucket_and, w.mc, d, local_max, small_max, GROUP_ROWS,
+                         w.bucket_base, w.carried, w.large[cur ^ 1], w.local, w.local_cap, w.lc);
       AM_HIP(hipGetLastError());
     }
     {
       ProfScope ps("sort_msd_scatter");
-      hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(rows / TILE + bound + 1u), dim3(BLOCK), 0, s, w.large[cur],
-                         w.tiles, w.lc, w.carried, w.bucket_base, status, w.mc, 8u + uint32_t(P - 1 - d), d, k0, r0,
-                         k1, r1);
+      if (blind)
+        hipLaunchKernelGGL((lvl_pass_loop<K, ITEMS>), dim3(cus * 3u), dim3(BLOCK), 0, s, w.large[cur], w.tiles, w.lc,
+                           w.carried, w.bucket_base, status, w.mc, 8u + uint32_t(P - 1 - d), d, k0, r0, k1, r1);
+      else
+        hipLaunchKernelGGL((lvl_pass<K, ITEMS>), dim3(rows / TILE + bound + 1u), dim3(BLOCK), 0, s, w.large[cur],
+                           w.tiles, w.lc, w.carried, w.bucket_base, status, w.mc, 8u + uint32_t(P - 1 - d), d, k0,
+                           r0, k1, r1);
       AM_HIP(hipGetLastError());
     }
     cur ^= 1;
@@ -597,24 +892,25 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
       if (bound > w.large_cap) fail("argsort: bucket list overflow");   // cannot happen: each holds > local_max rows
     }
   }
-  int li = 8;
-  uint32_t grid = cus * 8u;
+  int li = 8, li_lsd = 8;
+  uint32_t grid = cus * 8u, group_grid = cus * 4u;
   if (!blind) {
     if (!read) {   // no level ran (no active digit below the top one)
       AM_HIP(hipMemcpyAsync(hp, w.lc, LC_READ * 4, hipMemcpyDeviceToHost, s));
       AM_HIP(hipStreamSynchronize(s));
     }
-    if (hp[LC_OVERFLOW]) return false;
-    if (hp[LC_LOCAL] == 0u) return true;
+    if (hp[LC_OVERFLOW] || uint64_t(hp[LC_LOCAL]) + hp[LC_GROUPS] > w.local_cap) return false;
     const uint32_t biggest = hp[LC_MAXLOCAL];
     li = biggest <= 4u * BLOCK ? 4 : biggest <= 8u * BLOCK ? 8 : 16;
+    li_lsd = hp[LC_GROUPS] != 0u ? std::max(li, int(GROUP_ROWS / BLOCK)) : li;   // a queued group fits
     grid = hp[LC_LOCAL];
+    group_grid = std::min(hp[LC_GROUPS], cus * 4u);
   }
-  // The LSD kernel takes the buckets the counting sort queued (its workgroups exit at once when none
+  // The LSD kernel takes the buckets the counting sorts queued (its workgroups exit at once when none
   // was).
-  const dim3 lsd_grid(cus * 2u);
-  const size_t lds = size_t(li) * BLOCK * sizeof(K), lsd_lds = size_t(li) * BLOCK * (sizeof(K) + 2);
-  {
+  const dim3 lsd_grid(cus * 4u);
+  const size_t lds = size_t(li) * BLOCK * (sizeof(K) + 1), lsd_lds = size_t(li_lsd) * BLOCK * (sizeof(K) + 2);
+  if (grid != 0u) {
     ProfScope ps("sort_msd_local");
     if (li == 4)
       hipLaunchKernelGGL((hyb_local<K, 4>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
@@ -627,12 +923,19 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
                          r1, out_idx, w.queue, w.lc);
     AM_HIP(hipGetLastError());
   }
+  if (group_grid != 0u) {
+    ProfScope ps("sort_msd_group");
+    hipLaunchKernelGGL((hyb_group<K>), dim3(group_grid), dim3(BLOCK),
+                       GROUP_ROWS * (sizeof(K) + 2) + GROUP_BINS * 4, s, w.mc, w.local, w.local_cap, k0, r0, k1, r1,
+                       out_idx, w.queue, w.lc);
+    AM_HIP(hipGetLastError());
+  }
   {
     ProfScope ps("sort_msd_local_lsd");
-    if (li == 4)
+    if (li_lsd == 4)
       hipLaunchKernelGGL((hyb_local_lsd<K, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
                          out_idx, w.queue, w.lc);
-    else if (li == 8)
+    else if (li_lsd == 8)
       hipLaunchKernelGGL((hyb_local_lsd<K, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
                          out_idx, w.queue, w.lc);
     else

@@ -308,7 +308,8 @@ def test_msd_hybrid_and_fallback(case):
             bad = int(np.nonzero(got != want)[0][0])
             raise AssertionError(f"{case} desc={desc} place={place} order={order}: first difference at {bad}: "
                                  f"got {got[bad]} want {want[bad]}")
-        took = ("hybrid" if "sort_msd_local" in names else "fallback" if "sort_msd_first" in names else "lsd")
+        bucket_sort = "sort_msd_local" in names or "sort_msd_group" in names
+        took = ("hybrid" if bucket_sort else "fallback" if "sort_msd_first" in names else "lsd")
         assert took == path, f"{case}: took the {took} path, kernels {sorted(names)}"
 
 
@@ -326,4 +327,4 @@ def test_msd_hybrid_small_inputs(n, case):
         if not np.array_equal(got, want):
             bad = int(np.nonzero(got != want)[0][0])
             raise AssertionError(f"{case} n={n}: first difference at {bad}: got {got[bad]} want {want[bad]}")
-        assert "sort_msd_local" in names, f"{case} n={n}: kernels {sorted(names)}"
+        assert "sort_msd_local" in names or "sort_msd_group" in names, f"{case} n={n}: kernels {sorted(names)}"
