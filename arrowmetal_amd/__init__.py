@@ -490,10 +490,11 @@ class Batched:
 
     @property
     def config(self):
-        v = (ctypes.c_int64 * 6)()
-        _check(lib().am_batched_config(self._h, v, 6))
+        v = (ctypes.c_int64 * 7)()
+        _check(lib().am_batched_config(self._h, v, 7))
         return {"tile_items": v[0], "hash_capacity": v[1], "strategy": ("lds", "global")[v[2]],
-                "digit_bits": v[3], "regrouped_batches": v[4], "sort_tiles": v[5]}
+                "digit_bits": v[3], "regrouped_batches": v[4], "sort_tiles": v[5],
+                "sort_path": ("lsd", "msd")[v[6]]}
 
 
 class Server:

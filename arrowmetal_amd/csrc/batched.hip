@@ -1181,9 +1181,9 @@ void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, c
 }
 
 void batched_config(const Batched* b, int64_t* out, int count) {
-  const int64_t v[6] = {b->items, int64_t(b->cap), b->strategy, 8 /* digit bits */, int64_t(b->fallback_batches),
-                        int64_t(b->ntiles)};
-  for (int i = 0; i < count && i < 6; i++) out[i] = v[i];
+  const int64_t v[7] = {b->items, int64_t(b->cap), b->strategy, 8 /* digit bits */, int64_t(b->fallback_batches),
+                        int64_t(b->ntiles), (b->ran_sort && b->msd_used && b->use_msd) ? 1 : 0};
+  for (int i = 0; i < count && i < 7; i++) out[i] = v[i];
 }
 
 // Results of the last run: 0 argsort indices (uint32, local rows, concatenated by batch), 1 group keys

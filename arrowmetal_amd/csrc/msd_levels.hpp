@@ -72,23 +72,17 @@ __device__ __forceinline__ void push_local(Bucket* local, uint32_t local_cap, ui
   else atomicOr(&lc[LC_OVERFLOW], 1u);
   if (!sorted) atomicMax(&lc[LC_MAXLOCAL], b.size);
 }
-// Appends one group of small buckets to the list, from its far end (entry local_cap - 1 down). The
-// list ran out of room when LC_LOCAL + LC_GROUPS > local_cap (list_overflow).
-__device__ __forceinline__ void push_group(Bucket* local, uint32_t local_cap, uint32_t* lc, Bucket b) {
-  const uint32_t g = atomicAdd(&lc[LC_GROUPS], 1u);
-  if (g < local_cap) local[local_cap - 1u - g] = b;
-}
+// Groups of small buckets fill the list from its far end (entry local_cap - 1 down, counted by
+// LC_GROUPS). The list ran out of room when LC_LOCAL + LC_GROUPS > local_cap.
 __device__ __forceinline__ bool list_overflow(const uint32_t* lc, uint32_t local_cap) {
   return lc[LC_OVERFLOW] != 0u || uint64_t(lc[LC_LOCAL]) + lc[LC_GROUPS] > local_cap;
 }
 
 // One workgroup: where each large bucket's chunks and tiles start in a level's tables (list `cur`
-// order; bofs[2b], bofs[2b + 1]); the bucket histograms and OR / AND cleared; the other list's
-// counter, the tile counter and the row count reset. lvl_fill writes the tables.
+// order; bofs[2b], bofs[2b + 1]); the other list's counter, the tile counter and the row count reset.
+// lvl_fill writes the tables.
 template <int ITEMS>
-__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, uint32_t* bofs,
-                                                    uint32_t* bucket_hist, unsigned long long* bucket_or,
-                                                    unsigned long long* bucket_and, uint32_t* lc) {
+__global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur, uint32_t* bofs, uint32_t* lc) {
   __shared__ uint32_t tmp[WAVES];
   constexpr uint32_t TILE = BLOCK * ITEMS;
   const uint32_t nb = lc[LC_LIST + cur];
@@ -107,11 +101,6 @@ __global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur
     coff += tc;
     toff += tt;
   }
-  for (uint32_t i = threadIdx.x; i < nb * RADIX; i += BLOCK) bucket_hist[i] = 0u;
-  for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) {
-    bucket_or[i] = 0ull;
-    bucket_and[i] = ~0ull;
-  }
   if (threadIdx.x == 0) {
     lc[LC_CHUNKS] = coff;
     lc[LC_TILES] = toff;
@@ -121,14 +110,23 @@ __global__ __launch_bounds__(BLOCK) void lvl_layout(const Bucket* large, int cur
   }
 }
 
-// The chunks and tiles of each large bucket, a bucket per workgroup in turn, written by all threads.
+// The chunks and tiles of each large bucket, a bucket per workgroup in turn, written by all threads;
+// its histogram and OR / AND cleared.
 template <int ITEMS>
 __global__ __launch_bounds__(BLOCK) void lvl_fill(const Bucket* large, int cur, const uint32_t* lc,
-                                                  const uint32_t* bofs, LvlChunk* chunks, LvlTile* tiles) {
+                                                  const uint32_t* bofs, LvlChunk* chunks, LvlTile* tiles,
+                                                  uint32_t* bucket_hist, unsigned long long* bucket_or,
+                                                  unsigned long long* bucket_and) {
   constexpr uint32_t TILE = BLOCK * ITEMS;
+  static_assert(RADIX == BLOCK, "a thread per digit value");
   const uint32_t nb = lc[LC_LIST + cur];
   for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const Bucket bk = large[b];
+    bucket_hist[size_t(b) * RADIX + threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+      bucket_or[b] = 0ull;
+      bucket_and[b] = ~0ull;
+    }
     const uint32_t c0 = bofs[2 * b], t0 = bofs[2 * b + 1];
     const uint32_t nc = (bk.size + LVL_CHUNK - 1) / LVL_CHUNK, nt = (bk.size + TILE - 1) / TILE;
     for (uint32_t c = threadIdx.x; c < nc; c += BLOCK)
@@ -225,9 +223,10 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
                                                   uint32_t local_max, uint32_t small_max, uint32_t group_max,
                                                   uint32_t* bucket_base, uint32_t* carried, Bucket* next,
                                                   Bucket* local, uint32_t local_cap, uint32_t* lc) {
-  enum : uint32_t { SUB_EMPTY = 0, SUB_COPY = 1, SUB_SMALL = 2, SUB_OTHER = 3 };
+  enum : uint32_t { SUB_EMPTY = 0, SUB_COPY = 1, SUB_SMALL = 2, SUB_SINGLE = 3, SUB_LARGE = 4 };
   __shared__ uint32_t tmp[WAVES];
   __shared__ uint2 s_sub[RADIX];   // (rows, start) of each non-empty sub-bucket of equal keys
+  __shared__ uint32_t s_base[3];   // where this bucket's entries go in the three lists
   const uint32_t nb = lc[LC_LIST + cur];
   const uint32_t mask = mc->active_mask;
   const bool active = (mask >> d) & 1u;
@@ -254,21 +253,12 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
     if (v != 0u) {
       const unsigned long long diff = active ? (bucket_or[b] ^ bucket_and[b]) : ~0ull;
       const bool equal_below = (diff & ((1ull << (8 * d)) - 1ull)) == 0ull;
-      if (last || equal_below) {
-        kind = SUB_COPY;
-      } else if (v <= small_max) {
-        kind = SUB_SMALL;
-      } else if (v <= local_max) {
-        kind = SUB_OTHER;
-        push_local(local, local_cap, lc, Bucket{base, v, buf, uint32_t(d)}, false);
-      } else {
-        kind = SUB_OTHER;
-        next[atomicAdd(&lc[LC_LIST + (cur ^ 1)], 1u)] = Bucket{base, v, buf, uint32_t(d)};
-        atomicAdd(&lc[LC_ROWS], v);
-      }
+      kind = (last || equal_below) ? SUB_COPY : v <= small_max ? SUB_SMALL : v <= local_max ? SUB_SINGLE : SUB_LARGE;
     }
     // Groups: the rows of each aligned window (any other sub-bucket weighs more than a group holds);
-    // the first non-empty sub-bucket of the chosen window pushes it.
+    // the first non-empty sub-bucket of the chosen window stands for it.
+    bool group_head = false;
+    uint32_t group_rows = 0, group_span = 1;
     {
       const uint32_t w1 = kind == SUB_SMALL ? v : (v != 0u ? group_max + 1u : 0u);
       const uint32_t w2 = w1 + __shfl_xor(w1, 1, 64);
@@ -278,20 +268,20 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
       static_assert(GROUP_SPAN == 16 && GROUP_BINS == RADIX << 4, "window sums up to GROUP_SPAN digits");
       const unsigned long long nonempty = __ballot(v != 0u);
       if (kind == SUB_SMALL) {
-        uint32_t span = 16, rows = w16;
-        if (rows > group_max) { span = 8; rows = w8; }
-        if (rows > group_max) { span = 4; rows = w4; }
-        if (rows > group_max) { span = 2; rows = w2; }
-        if (rows > group_max) { span = 1; rows = w1; }
-        const uint32_t first = lane & ~(span - 1u);
-        const unsigned long long earlier = nonempty & ((1ull << lane) - 1ull) & ~((1ull << first) - 1ull);
-        if (earlier == 0ull)
-          push_group(local, local_cap, lc,
-                     Bucket{base, rows, buf,
-                            (uint32_t(d) + 1u) | GROUP_FLAG | (uint32_t(__builtin_ctz(span)) << SPAN_SHIFT)});
+        group_span = 16;
+        group_rows = w16;
+        if (group_rows > group_max) { group_span = 8; group_rows = w8; }
+        if (group_rows > group_max) { group_span = 4; group_rows = w4; }
+        if (group_rows > group_max) { group_span = 2; group_rows = w2; }
+        if (group_rows > group_max) { group_span = 1; group_rows = w1; }
+        const uint32_t first = lane & ~(group_span - 1u);
+        group_head = (nonempty & ((1ull << lane) - 1ull) & ~((1ull << first) - 1ull)) == 0ull;
       }
     }
-    // Copies: the non-empty sub-buckets compacted in digit order; a run of equal-key ones leaves as one.
+    // Copies: the non-empty sub-buckets compacted in digit order; a run of equal-key ones leaves as one
+    // (thread i stands for the run that ends at compacted sub-bucket i).
+    bool copy_end = false;
+    uint32_t copy_start = 0, copy_rows = 0;
     uint32_t nne;
     const uint32_t ci = block_excl_scan(v != 0u ? 1u : 0u, tmp, nne);
     if (v != 0u) s_sub[ci] = make_uint2(kind == SUB_COPY ? v : 0u, base);
@@ -304,11 +294,40 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
         if (i + 1 < nne) next_sub = s_sub[i + 1];
       }
       const bool copy = me.x != 0u;
-      const bool run_start = copy && prev.x == 0u, run_end = copy && next_sub.x == 0u;
-      const uint32_t run_base = block_incl_max(run_start ? me.y + 1u : 0u, tmp) - 1u;
-      // A copy keeps level 0: no digit is left to sort by.
-      if (run_end) push_local(local, local_cap, lc, Bucket{run_base, me.y + me.x - run_base, buf, 0u}, true);
+      const bool run_start = copy && prev.x == 0u;
+      copy_end = copy && next_sub.x == 0u;
+      copy_start = block_incl_max(run_start ? me.y + 1u : 0u, tmp) - 1u;
+      copy_rows = me.y + me.x - copy_start;
     }
+    // Every entry of this bucket into its list with one counter bump per list.
+    const bool single = kind == SUB_SINGLE, to_next = kind == SUB_LARGE;
+    uint32_t n_local, n_group, n_next, n_rows;
+    const uint32_t r_local = block_excl_scan((single ? 1u : 0u) + (copy_end ? 1u : 0u), tmp, n_local);
+    const uint32_t r_group = block_excl_scan(group_head ? 1u : 0u, tmp, n_group);
+    const uint32_t r_next = block_excl_scan(to_next ? 1u : 0u, tmp, n_next);
+    block_excl_scan(to_next ? v : 0u, tmp, n_rows);
+    const uint32_t biggest = block_incl_max(single ? v : 0u, tmp);   // thread BLOCK - 1 holds the maximum
+    if (threadIdx.x == 0) {
+      s_base[0] = n_local ? atomicAdd(&lc[LC_LOCAL], n_local) : 0u;
+      s_base[1] = n_group ? atomicAdd(&lc[LC_GROUPS], n_group) : 0u;
+      s_base[2] = n_next ? atomicAdd(&lc[LC_LIST + (cur ^ 1)], n_next) : 0u;
+      if (n_rows) atomicAdd(&lc[LC_ROWS], n_rows);
+    }
+    if (threadIdx.x == BLOCK - 1 && biggest) atomicMax(&lc[LC_MAXLOCAL], biggest);
+    __syncthreads();
+    uint32_t at = s_base[0] + r_local;
+    if (single) {
+      if (at < local_cap) local[at] = Bucket{base, v, buf, uint32_t(d)};
+      at++;
+    }
+    if (copy_end && at < local_cap) local[at] = Bucket{copy_start, copy_rows, buf, 0u};   // level 0: no digit left
+    if (group_head) {
+      const uint32_t g = s_base[1] + r_group;
+      if (g < local_cap)
+        local[local_cap - 1u - g] =
+            Bucket{base, group_rows, buf, (uint32_t(d) + 1u) | GROUP_FLAG | (uint32_t(__builtin_ctz(group_span)) << SPAN_SHIFT)};
+    }
+    if (to_next) next[s_base[2] + r_next] = Bucket{base, v, buf, uint32_t(d)};
     __syncthreads();
   }
 }
@@ -852,10 +871,9 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
   for (int d = next_digit(top); d >= 0 && bound > 0; d = next_digit(d)) {
     {
       ProfScope ps("sort_msd_layout");
-      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, w.large[cur], cur, w.bofs, w.bucket_hist,
-                         w.bucket_or, w.bucket_and, w.lc);
+      hipLaunchKernelGGL(lvl_layout<ITEMS>, dim3(1), dim3(BLOCK), 0, s, w.large[cur], cur, w.bofs, w.lc);
       hipLaunchKernelGGL(lvl_fill<ITEMS>, dim3(std::min(bound, cus * 4u)), dim3(BLOCK), 0, s, w.large[cur], cur,
-                         w.lc, w.bofs, w.chunks, w.tiles);
+                         w.lc, w.bofs, w.chunks, w.tiles, w.bucket_hist, w.bucket_or, w.bucket_and);
       AM_HIP(hipGetLastError());
     }
     {

@@ -147,7 +147,8 @@ int  am_unique_rows(am_array** columns, int64_t count, am_array** out);
  *                       3 counts (int64); 4 group offsets (int64, nbatches + 1: batch b's groups are
  *                       rows [o[b], o[b+1]) of results 1-3).
  *   am_batched_config   up to `count` of: tile items, hash capacity, group-by strategy, digit bits,
- *                       batches regrouped by the general path in the last run, sort tiles. */
+ *                       batches regrouped by the general path in the last run, sort tiles, and 1 when
+ *                       the last run sorted by the MSD radix sort (64-bit keys), 0 by the LSD passes. */
 int  am_batched_create(const int64_t* offsets, int64_t nbatches, int hash_capacity, int tile_items,
                        am_batched** out);
 int  am_batched_tune(am_batched* b, am_array* sort_keys, am_array* group_keys, am_array* values,

@@ -95,6 +95,39 @@ def test_batched_sort_key_types(dtype, forder):
         check_sort(idx, v, offs, desc, forder, f"{np.dtype(dtype).name} desc={desc}")
 
 
+def test_batched_msd_levels_large_batches():
+    """64-bit sort keys take the MSD radix sort: batches above 2048 rows are partitioned by digit 7, and
+    batches above ~524K rows a second time by digit 6, before the LDS bucket sorts (groups of small
+    sibling buckets, larger buckets one per workgroup, LSD passes for runs of equal keys). Each batch
+    must match the oracle, with full-range keys, keys of few distinct values, and float64 keys."""
+    lens = [1, 0, 2047, 2048, 2049, 5000, 31_000, 200_000, 700_000, 1_300_000, 40, 600_000, 3]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(offs[-1])
+    rng = np.random.default_rng(21)
+    sk = O.splitmix64(0xB16B00B5, n).view(np.int64).copy()
+    a, e = int(offs[8]), int(offs[9])   # the 700K-row batch: 3000 distinct keys
+    sk[a:e] = rng.integers(-(2**62), 2**62, 3000)[rng.integers(0, 3000, e - a)]
+    a, e = int(offs[11]), int(offs[12])   # the 600K-row batch: keys differing only in their top 3 bytes
+    sk[a:e] = (rng.integers(0, 1 << 24, e - a) << 40) - (1 << 62)
+    plan = am.Batched(offs)
+    mk = am.MetalArray.from_numpy(sk)
+    for desc in (False, True):
+        print(f"batched msd levels int64 n={n} desc={desc}", flush=True)
+        plan.run(mk, descending=desc, use_graph=True)
+        assert plan.config["sort_path"] == "msd"
+        idx, _ = plan.result("indices").to_numpy()
+        check_sort(idx, sk, offs, desc, 0, f"int64 desc={desc}")
+    v = np.round(rng.normal(0, 1e6, n), 1)
+    v[rng.random(n) < 0.002] = np.nan
+    v[rng.random(n) < 0.002] = -0.0
+    mv = am.MetalArray.from_numpy(v)
+    print(f"batched msd levels float64 n={n}", flush=True)
+    plan.run(mv, descending=True, float_order="total", use_graph=True)
+    assert plan.config["sort_path"] == "msd"
+    idx, _ = plan.result("indices").to_numpy()
+    check_sort(idx, v, offs, True, 1, "float64 total desc")
+
+
 def test_batched_overflow_regroups():
     """Batches with more distinct keys than the hash capacity are regrouped by the general path."""
     offs = np.array([0, 100, 5100, 5100, 9000, 40_000, 40_050], dtype=np.int64)
