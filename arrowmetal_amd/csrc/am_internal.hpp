@@ -103,6 +103,7 @@ struct DeviceInfo {
   int id = 0;
   int cu_count = 256;
   size_t lds_per_block = 65536;
+  size_t lds_per_cu = 163840;
   std::string name;
 };
 const DeviceInfo& device();

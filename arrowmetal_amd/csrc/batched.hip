@@ -433,16 +433,21 @@ __global__ __launch_bounds__(256) void bg_agg_global(const void* keys, int ktype
 
 // Group-by strategy 0 (LDS). Persistent: workgroups take chunks from the queue until it is empty (an
 // exit every wave reaches). Each chunk is aggregated in an LDS table of `cap` slots (+1 for the
-// EMPTY key) and flushed into its batch's table of the same shape. A batch with more distinct keys
-// than `cap` is flagged.
+// EMPTY key; counts in 32 bits, a chunk holding at most CHUNK_ROWS rows) and flushed into its batch's
+// table of the same shape. A batch with more distinct keys than `cap` is flagged. WIDE: 64-bit keys
+// and values, loaded directly with ROWS rows in flight per thread; otherwise one row at a time, of any
+// integer types.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const void* vals, int vtype,
                                               const SegChunk* chunks, uint32_t nchunks, SegCtl* ctl,
                                               unsigned long long* tkey, unsigned long long* tsum,
                                               unsigned long long* tcnt, uint32_t cap, uint32_t* overflow) {
+  constexpr int ROWS = WIDE ? 8 : 1;
+  static_assert(CHUNK_ROWS <= 0xFFFFFFFFu, "chunk counts fit 32 bits");
   extern __shared__ unsigned long long lds[];
-  unsigned long long* lkey = lds;                     // [cap + 1]
-  unsigned long long* lsum = lds + (cap + 1);         // [cap + 1]
-  unsigned long long* lcnt = lds + 2 * (cap + 1);     // [cap + 1]
+  unsigned long long* lkey = lds;                                    // [cap + 1]
+  unsigned long long* lsum = lds + (cap + 1);                        // [cap + 1]
+  uint32_t* lcnt = reinterpret_cast<uint32_t*>(lds + 2 * (cap + 1));   // [cap + 1]
   __shared__ uint32_t s_chunk, s_over;
   const uint32_t mask = cap - 1u;
   while (true) {
@@ -454,36 +459,51 @@ __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const
     for (uint32_t j = threadIdx.x; j <= cap; j += blockDim.x) {
       lkey[j] = BG_EMPTY;
       lsum[j] = 0ull;
-      lcnt[j] = 0ull;
+      lcnt[j] = 0u;
     }
     __syncthreads();
     const uint32_t ci = s_chunk;
     if (ci >= nchunks) break;
     const SegChunk c = chunks[ci];
     bool over = false;
-    for (uint32_t i = c.start + threadIdx.x; i < c.end; i += blockDim.x) {
-      unsigned long long k = bg_key(keys, i, ktype);
-      unsigned long long v = (unsigned long long)bg_value(vals, i, vtype);
-      uint32_t s;
-      if (k == BG_EMPTY) {
-        s = cap;
-      } else {
-        s = bg_hash(k, mask);
-        uint32_t probes = 0;
-        while (true) {
-          unsigned long long cur = lkey[s];
-          if (cur == k) break;
-          if (cur == BG_EMPTY) {
-            unsigned long long prev = atomicCAS(&lkey[s], BG_EMPTY, k);
-            if (prev == BG_EMPTY || prev == k) break;
-          }
-          if (++probes >= cap) { over = true; break; }
-          s = (s + 1u) & mask;
+    for (uint32_t i0 = c.start + threadIdx.x; i0 < c.end && !over; i0 += blockDim.x * ROWS) {
+      unsigned long long kk[ROWS], vv[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        const uint32_t i = min(i0 + uint32_t(r) * blockDim.x, c.end - 1u);
+        if (WIDE) {
+          kk[r] = static_cast<const unsigned long long*>(keys)[i];
+          vv[r] = static_cast<const unsigned long long*>(vals)[i];
+        } else {
+          kk[r] = bg_key(keys, i, ktype);
+          vv[r] = (unsigned long long)bg_value(vals, i, vtype);
         }
-        if (over) break;
       }
-      atomicAdd(&lsum[s], v);
-      atomicAdd(&lcnt[s], 1ull);
+#pragma unroll
+      for (int r = 0; r < ROWS; r++) {
+        if (over || i0 + uint32_t(r) * blockDim.x >= c.end) break;
+        const unsigned long long k = kk[r];
+        uint32_t s;
+        if (k == BG_EMPTY) {
+          s = cap;
+        } else {
+          s = bg_hash(k, mask);
+          uint32_t probes = 0;
+          while (true) {
+            unsigned long long cur = lkey[s];
+            if (cur == k) break;
+            if (cur == BG_EMPTY) {
+              unsigned long long prev = atomicCAS(&lkey[s], BG_EMPTY, k);
+              if (prev == BG_EMPTY || prev == k) break;
+            }
+            if (++probes >= cap) { over = true; break; }
+            s = (s + 1u) & mask;
+          }
+          if (over) break;
+        }
+        atomicAdd(&lsum[s], vv[r]);
+        atomicAdd(&lcnt[s], 1u);
+      }
     }
     if (over) atomicOr(&s_over, 1u);
     __syncthreads();
@@ -497,7 +517,7 @@ __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const
     unsigned long long* gc = tcnt + size_t(c.seg) * (cap + 1);
     bool gover = false;
     for (uint32_t j = threadIdx.x; j <= cap; j += blockDim.x) {
-      unsigned long long cnt = lcnt[j];
+      const uint32_t cnt = lcnt[j];
       if (!cnt) continue;
       unsigned long long k = lkey[j];
       uint32_t s;
@@ -519,7 +539,7 @@ __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const
         if (gover) break;
       }
       atomicAdd(&gs[s], lsum[j]);
-      atomicAdd(&gc[s], cnt);
+      atomicAdd(&gc[s], (unsigned long long)cnt);
     }
     if (gover) atomicOr(&overflow[c.seg], 1u);
   }
@@ -540,19 +560,28 @@ __global__ __launch_bounds__(256) void bg_sort(const unsigned long long* tkey, c
   __shared__ uint32_t s_count;
   if (threadIdx.x == 0) s_count = 0u;
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < pow2; j += blockDim.x) {
-    bool occ = j <= cap && tcnt[off + j] != 0;
-    unsigned long long k = j <= cap ? tkey[off + j] : 0ull;
-    if (j == cap) k = BG_EMPTY;
-    ok[j] = occ ? (signed_keys ? k ^ 0x8000000000000000ull : k) : ~0ull;
-    sl[j] = occ ? j : 0xFFFFFFFFu;
-    if (occ) atomicAdd(&s_count, 1u);
+  // The occupied slots, compacted in any order, then padded with all-ones keys (which sort last) up to
+  // the next power of two.
+  for (uint32_t j = threadIdx.x; j <= cap; j += blockDim.x) {
+    if (tcnt[off + j] == 0) continue;
+    const unsigned long long k = j == cap ? BG_EMPTY : tkey[off + j];
+    const uint32_t at = atomicAdd(&s_count, 1u);
+    ok[at] = signed_keys ? k ^ 0x8000000000000000ull : k;
+    sl[at] = j;
   }
   __syncthreads();
-  // Bitonic sort by (order key, slot); unoccupied entries carry all-ones keys and sort last.
-  for (uint32_t k = 2; k <= pow2; k <<= 1) {
+  const uint32_t D = s_count;
+  uint32_t p2 = 1;
+  while (p2 < D) p2 <<= 1;   // at most pow2: D <= cap + 1
+  for (uint32_t j = D + threadIdx.x; j < p2; j += blockDim.x) {
+    ok[j] = ~0ull;
+    sl[j] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  // Bitonic sort by (order key, slot) over the p2 entries.
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
     for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-      for (uint32_t i = threadIdx.x; i < pow2; i += blockDim.x) {
+      for (uint32_t i = threadIdx.x; i < p2; i += blockDim.x) {
         uint32_t l = i ^ jj;
         if (l > i) {
           bool up = (i & k) == 0;
@@ -570,7 +599,6 @@ __global__ __launch_bounds__(256) void bg_sort(const unsigned long long* tkey, c
       __syncthreads();
     }
   }
-  const uint32_t D = s_count;
   for (uint32_t r = threadIdx.x; r < D; r += blockDim.x) {
     uint32_t j = sl[r];
     skey[off + r] = j == cap ? BG_EMPTY : tkey[off + j];
@@ -752,7 +780,7 @@ uint32_t pow2_at_least(uint32_t v) {
 }
 
 // LDS bytes of the group-by kernels at capacity `cap`.
-size_t agg_lds(uint32_t cap) { return size_t(cap + 1) * 24; }
+size_t agg_lds(uint32_t cap) { return size_t(cap + 1) * 20; }
 size_t sort_lds(uint32_t cap) { return size_t(pow2_at_least(cap + 1)) * 12; }
 
 void group_launch(Batched& b, const void* keys, int ktype, const void* vals, int vtype, hipStream_t s) {
@@ -765,15 +793,18 @@ void group_launch(Batched& b, const void* keys, int ktype, const void* vals, int
                      slots, static_cast<uint32_t*>(b.overflow->ptr), b.nseg, ctl);
   if (b.ngb_chunks > 0) {
     ProfScope ps("bg_agg");
-    int grid = std::max(1, std::min<int>(int(b.ngb_chunks), device().cu_count * 2));
+    // As many persistent workgroups as the LDS tables let each CU hold (at most 4).
+    const int per_cu = int(std::clamp<size_t>(device().lds_per_cu / agg_lds(b.cap), 1, 4));
+    int grid = std::max(1, std::min<int>(int(b.ngb_chunks), device().cu_count * per_cu));
     if (b.strategy == 1) {
       hipLaunchKernelGGL(bg_agg_global, dim3(std::min<int>(int(b.ngb_chunks), device().cu_count * 8)), dim3(256), 0, s,
                          keys, ktype, vals, vtype, static_cast<const SegChunk*>(b.gb_chunks->ptr), b.ngb_chunks, ctl,
                          tk, ts, tc, b.cap, static_cast<uint32_t*>(b.overflow->ptr));
     } else {
-      hipLaunchKernelGGL(bg_agg, dim3(grid), dim3(256), agg_lds(b.cap), s, keys, ktype, vals, vtype,
-                         static_cast<const SegChunk*>(b.gb_chunks->ptr), b.ngb_chunks, ctl, tk, ts, tc, b.cap,
-                         static_cast<uint32_t*>(b.overflow->ptr));
+      const bool wide = type_width(ktype) == 8 && type_width(vtype) == 8;
+      hipLaunchKernelGGL(wide ? bg_agg<true> : bg_agg<false>, dim3(grid), dim3(256), agg_lds(b.cap), s, keys, ktype,
+                         vals, vtype, static_cast<const SegChunk*>(b.gb_chunks->ptr), b.ngb_chunks, ctl, tk, ts, tc,
+                         b.cap, static_cast<uint32_t*>(b.overflow->ptr));
     }
   }
   const uint32_t pow2 = pow2_at_least(b.cap + 1);
@@ -1129,7 +1160,7 @@ void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, c
     return best;
   };
   try {
-    if (sort_keys) {
+    if (sort_keys && !(type_width(sort_keys->type) == 8 && b->use_msd)) {   // the MSD sort has no tile size
       int best_items = b->items;
       float best = 1e30f;
       for (int it : {8, 16, 20}) {

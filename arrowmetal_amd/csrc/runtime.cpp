@@ -71,6 +71,7 @@ const DeviceInfo& device() {
     g_dev.id = id;
     g_dev.cu_count = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
     g_dev.lds_per_block = p.sharedMemPerBlock;
+    if (p.maxSharedMemoryPerMultiProcessor > 0) g_dev.lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
     g_dev.name = std::string(p.name) + " (" + p.gcnArchName + ", " + std::to_string(g_dev.cu_count) + " CUs)";
   });
   return g_dev;

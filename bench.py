@@ -172,7 +172,7 @@ def batched_leg(args, log):
         "per_batch_loop_rows_per_s": n / (loop_ms * 1e-3),
         "speedup_vs_per_batch_loop": loop_ms / gms,
         "step": "am_batched_run (graph replay): stable ascending argsort + group-by sum/count_all per batch",
-        "tuned": cfg,
+        "tuned": plan.config,
         "tune_s": round(tune_s, 3),
         "b_alg_bytes": b_alg,
         "hbm_frac_of_b_alg": b_alg / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS,
