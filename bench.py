@@ -367,10 +367,10 @@ def orderby_alg_bytes(n: int, m: int):
 ROCPROF_NAME = {
     "take": "take_kernel<u64, u32>",
     "sort_upsweep": "sort_upsweep<u64, 8>",
-    "sort_msd_first": "sort_first<u64, 8, 16>",
+    "sort_msd_first": "msd_part<u64, 8, 16>",
     "sort_msd_hist": "lvl_hist<u64>",
     "sort_msd_scatter": "lvl_pass<u64, 20>",
-    "sort_msd_local": "hyb_local<u64, 8, 4>",
+    "sort_msd_local": "hyb_local<u64, 8>",
     "sort_first": "sort_first<u64, 8, 16>",
     "sort_scatter": "sort_mid<u64, 20, 0>",
     "sort_scatter_last": "sort_mid<u64, 20, 1>",
