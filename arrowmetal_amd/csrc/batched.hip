@@ -252,13 +252,12 @@ __global__ __launch_bounds__(BLOCK) void seg_pass(int pass, const void* vals, in
 // whole sequence is fixed (a map pass, then one level per digit, then the bucket sort), so it can be
 // captured in the plan's graph, and every kernel reads from device memory what it has to do.
 
-// Maps the sort keys of every batch (the same map as the LSD pass) into k0, with rows counted from
-// each batch's start in r0, and the OR / AND of the mapped keys into the control block; a 64K-row
-// chunk of one batch per workgroup.
+// Maps the sort keys of every batch (the same map as the LSD pass) into k0, and the OR / AND of the
+// mapped keys into the control block; a 64K-row chunk of one batch per workgroup. No row buffer: the
+// batches enter the levels as buckets with implicit rows (a row is its index inside its batch).
 template <typename K, int W>
 __global__ __launch_bounds__(BLOCK) void seg_msd_map(const void* vals, int type, int forder, int desc,
-                                                     const SegChunk* chunks, const uint32_t* seg_off, SegCtl* ctl,
-                                                     K* k0, uint32_t* r0) {
+                                                     const SegChunk* chunks, SegCtl* ctl, K* k0) {
   __shared__ unsigned long long s_or, s_and;
   if (threadIdx.x == 0) {
     s_or = 0ull;
@@ -266,7 +265,6 @@ __global__ __launch_bounds__(BLOCK) void seg_msd_map(const void* vals, int type,
   }
   __syncthreads();
   const SegChunk c = chunks[blockIdx.x];
-  const uint32_t seg_start = seg_off[c.seg];
   unsigned long long o = 0ull, a = ~0ull;
   constexpr int U = 8;
   for (uint32_t base = c.start + threadIdx.x; base < c.end; base += BLOCK * U) {
@@ -279,10 +277,7 @@ __global__ __launch_bounds__(BLOCK) void seg_msd_map(const void* vals, int type,
       const K k = seg_key<K>(raw[u], type, forder, desc);
       o |= (unsigned long long)k;   // rows past the end repeat the chunk's last key
       a &= (unsigned long long)k;
-      if (i < c.end) {
-        k0[i] = k;
-        r0[i] = i - seg_start;
-      }
+      if (i < c.end) k0[i] = k;
     }
   }
 #pragma unroll
@@ -324,7 +319,7 @@ __global__ void seg_msd_lists(const uint32_t* seg_off, uint32_t nseg, const MsdC
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nseg; b += gridDim.x * blockDim.x) {
     const uint32_t start = seg_off[b], size = seg_off[b + 1] - start;
     if (size == 0u) continue;
-    const Bucket bk{start, size, 0u, uint32_t(P)};
+    const Bucket bk{start, size, BUF_IMPLICIT, uint32_t(P)};   // rows: index - start
     if (size > local_max && mask != 0u) large[atomicAdd(&lc[LC_LIST], 1u)] = bk;
     else push_local(local, local_cap, lc, bk, mask == 0u);
   }
@@ -740,8 +735,7 @@ void seg_sort_msd(Batched& b, const void* vals, int type, int desc, int forder, 
   if (b.nchunks > 0) {
     ProfScope ps("seg_msd_map");
     hipLaunchKernelGGL((seg_msd_map<K, 8>), dim3(b.nchunks), dim3(BLOCK), 0, s, vals, type, forder, desc,
-                       static_cast<const SegChunk*>(b.chunks->ptr), static_cast<const uint32_t*>(b.seg_off->ptr),
-                       ctl, kp0, rp0);
+                       static_cast<const SegChunk*>(b.chunks->ptr), ctl, kp0);
   }
   hipLaunchKernelGGL(seg_msd_ctl<P>, dim3(1), dim3(BLOCK), 0, s, ctl, b.msd.mc, b.msd.lc);
   hipLaunchKernelGGL(seg_msd_lists<P>, dim3(std::min<uint32_t>((b.nseg + 255) / 256, 1024)), dim3(256), 0, s,

@@ -33,6 +33,29 @@ constexpr uint32_t GROUP_ROWS = 2048;  // rows of a group at most (the bucket so
 struct Bucket {
   uint32_t start, size, buf, level;
 };
+// Bucket::buf: 0 = keys0 / rows0, 1 = keys1 / rows1, 2 = keys0 with implicit rows: a whole batch in
+// input order, whose row at index i is i - start (the batched plan's first level reads no row buffer).
+constexpr uint32_t BUF_IMPLICIT = 2;
+template <typename K>
+__device__ __forceinline__ const K* bucket_keys(const Bucket& b, const K* keys0, const K* keys1) {
+  return b.buf == 1u ? keys1 : keys0;
+}
+// Null for implicit rows.
+__device__ __forceinline__ const uint32_t* bucket_rows(const Bucket& b, const uint32_t* rows0, const uint32_t* rows1) {
+  return b.buf == BUF_IMPLICIT ? nullptr : b.buf == 1u ? rows1 : rows0;
+}
+// The rows at indices src[j] of a bucket starting at `start`: from `rows`, or implicit (rows null).
+template <int LI>
+__device__ __forceinline__ void load_rows(const uint32_t* rows, uint32_t start, const uint32_t (&src)[LI],
+                                          uint32_t (&r)[LI]) {
+  if (rows) {
+#pragma unroll
+    for (int j = 0; j < LI; j++) r[j] = rows[src[j]];
+  } else {
+#pragma unroll
+    for (int j = 0; j < LI; j++) r[j] = src[j] - start;
+  }
+}
 constexpr uint32_t LEVEL_BITS = 0xFFu;    // Bucket::level: the level ...
 constexpr uint32_t GROUP_FLAG = 0x100u;   // ... and, in the bucket sort's list, a group of small buckets
 constexpr int SPAN_SHIFT = 12;            // ... whose digits share an aligned window of 1 << (level >> 12)
@@ -149,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void lvl_hist(const K* keys0, const K* keys1
   const uint32_t nch = lc[LC_CHUNKS];
   for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
     const LvlChunk c = chunks[ci];
-    const K* keys = large[c.bucket].buf ? keys1 : keys0;
+    const K* keys = bucket_keys(large[c.bucket], keys0, keys1);
     h[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
       s_or = 0ull;
@@ -248,7 +271,7 @@ __global__ __launch_bounds__(BLOCK) void lvl_plan(const Bucket* large, int cur, 
       carry = true;
     }
     if (threadIdx.x == 0) carried[b] = carry ? 1u : 0u;
-    const uint32_t buf = carry ? bk.buf : (bk.buf ^ 1u);
+    const uint32_t buf = carry ? bk.buf : (bk.buf == 1u ? 0u : 1u);   // a pass writes the other buffer
     uint32_t kind = SUB_EMPTY;
     if (v != 0u) {
       const unsigned long long diff = active ? (bucket_or[b] ^ bucket_and[b]) : ~0ull;
@@ -357,7 +380,8 @@ __device__ __forceinline__ void lvl_tile(LvlLds<K, ITEMS>& L, uint32_t tile, con
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const LvlTile t = tiles[tile];
   if (carried[t.bucket]) return;   // uniform over the workgroup
-  const bool from1 = large[t.bucket].buf != 0u;
+  const Bucket bk = large[t.bucket];
+  const bool from1 = bk.buf == 1u;
   const K* in_keys = from1 ? keys1 : keys0;
   const uint32_t* in_rows = from1 ? rows1 : rows0;
   K* out_keys = from1 ? keys0 : keys1;
@@ -366,11 +390,20 @@ __device__ __forceinline__ void lvl_tile(LvlLds<K, ITEMS>& L, uint32_t tile, con
   K keys[ITEMS];
   uint32_t rows[ITEMS], bins[ITEMS];
   const uint32_t base_e = t.start + uint32_t(w) * 64u * ITEMS + uint32_t(lane);
+  if (bk.buf == BUF_IMPLICIT) {
 #pragma unroll
-  for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the tile read its last row)
-    const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
-    keys[j] = in_keys[e];
-    rows[j] = in_rows[e];
+    for (int j = 0; j < ITEMS; j++) {   // every load first (lanes past the tile read its last row)
+      const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
+      keys[j] = in_keys[e];
+      rows[j] = e - bk.start;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      const uint32_t e = min(base_e + uint32_t(j) * 64u, t.end - 1u);
+      keys[j] = in_keys[e];
+      rows[j] = in_rows[e];
+    }
   }
 #pragma unroll
   for (int j = 0; j < ITEMS; j++)
@@ -501,8 +534,7 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
         }
       }
       uint32_t r[LI];
-#pragma unroll
-      for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];
+      load_rows<LI>(rows_in, off, src, r);
 #pragma unroll
       for (int j = 0; j < LI; j++)
         if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
@@ -521,9 +553,10 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
     }
     __syncthreads();
   }
-  uint32_t r[LI];
+  uint32_t src[LI], r[LI];
 #pragma unroll
-  for (int j = 0; j < LI; j++) r[j] = rows_in[off + min(pos[j], size - 1u)];
+  for (int j = 0; j < LI; j++) src[j] = off + min(pos[j], size - 1u);
+  load_rows<LI>(rows_in, off, src, r);
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t e = base_e + uint32_t(j) * 64u;
@@ -604,8 +637,7 @@ __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* ro
     }
   }
   uint32_t r[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
+  load_rows<LI>(rows_in, off, src, r);   // every row load before the first store
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
@@ -627,13 +659,13 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
                                           uint32_t* out_idx, uint32_t* queue, uint32_t* lc, LocalLds& L,
                                           K* s_words, uint8_t* s_dig) {
   const Bucket bk = list[idx];
-  const K* keys_in = bk.buf ? keys1 : keys0;
-  const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+  const K* keys_in = bucket_keys(bk, keys0, keys1);
+  const uint32_t* rows_in = bucket_rows(bk, rows0, rows1);
   const uint32_t size = bk.size, off = bk.start;
   const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
   uint32_t* out = out_idx + mc->value_base;
   if (low_mask == 0u) {   // equal keys: the bucket is in order already
-    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in[off + p];
+    for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in ? rows_in[off + p] : p;
     return;
   }
   if (size > uint32_t(LI * BLOCK)) return;   // never: LI covers every bucket to sort
@@ -691,8 +723,7 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
     }
   }
   uint32_t r[LI];
-#pragma unroll
-  for (int j = 0; j < LI; j++) r[j] = rows_in[src[j]];   // every row load before the first store
+  load_rows<LI>(rows_in, off, src, r);   // every row load before the first store
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
@@ -741,8 +772,8 @@ __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucke
     const int top = 31 - __builtin_clz(low_mask);   // a group's buckets differ in digit top (its level - 1)
     const uint32_t below_top = low_mask & ((1u << top) - 1u);   // never 0: they are not of equal keys
     const uint32_t span_log = (bk.level >> SPAN_SHIFT) & 0xFu;
-    const K* keys_in = bk.buf ? keys1 : keys0;
-    const uint32_t* rows_in = bk.buf ? rows1 : rows0;
+    const K* keys_in = bucket_keys(bk, keys0, keys1);
+    const uint32_t* rows_in = bucket_rows(bk, rows0, rows1);
     const int sub = 31 - __builtin_clz(below_top);
     bool done = bk.size > GROUP_ROWS;   // never
     if (!done) {
@@ -782,7 +813,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
     const Bucket bk = list[queue[qi]];
     const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
     if (bk.size <= uint32_t(LI * BLOCK))   // always: the host sizes LI for every queued bucket
-      local_lsd<K, LI>(bk.buf ? keys1 : keys0, bk.buf ? rows1 : rows0, bk.start, bk.size, low_mask, SUB_MAX,
+      local_lsd<K, LI>(bucket_keys(bk, keys0, keys1), bucket_rows(bk, rows0, rows1), bk.start, bk.size, low_mask, SUB_MAX,
                        out_idx + mc->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
   }
