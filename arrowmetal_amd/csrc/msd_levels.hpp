@@ -23,7 +23,7 @@ namespace {
 constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
 constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes (local_max is at most this)
 static_assert(LOCAL_MAX <= 4096u, "positions in a bucket fit the 12 low bits of the bucket sort's words");
-constexpr uint32_t SUB_MAX = 32;    // longest digit run of one bucket the bucket sort ranks by counting
+constexpr uint32_t SUB_MAX = 64;    // longest digit run of one bucket the bucket sort ranks by counting
 constexpr uint32_t GROUP_SPAN = 16;    // a group's buckets have their digit in one window of this many values
 constexpr uint32_t GROUP_BINS = GROUP_SPAN * RADIX;
 constexpr uint32_t GROUP_ROWS = 2048;  // rows of a group at most (the bucket sort at 8 keys per thread)
@@ -805,6 +805,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
   K* s_keys = reinterpret_cast<K*>(local_lds);
   uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_keys + LI * BLOCK);
   const uint32_t queued = lc[LC_QLEN];
+  if (blockIdx.x >= queued) return;   // the first `queued` workgroups take every entry between them
   while (true) {
     if (threadIdx.x == 0) s_q = atomicAdd(&lc[LC_QNEXT], 1u);
     __syncthreads();
