@@ -847,6 +847,11 @@ inline MsdWork msd_work(uint32_t m, int items, DevBuf (*alloc)(size_t), uint32_t
   const uint32_t tile = uint32_t(BLOCK * items);
   w.large_cap = m / (LOCAL_MAX / 2) + 4096u;
   w.local_cap = m / 32u + (1u << 17) + extra_local;
+  // ARROWMETAL_MSD_LIST_CAP (tests only) caps the bucket-sort list, so that a test can send a sort down
+  // the overflow path: the plain argsort falls back to the LSD passes, a batched plan to its segmented
+  // LSD passes.
+  if (const char* e = getenv("ARROWMETAL_MSD_LIST_CAP"))
+    if (atoi(e) > 0) w.local_cap = std::min(w.local_cap, uint32_t(atoi(e)));
   const uint32_t max_chunks = m / LVL_CHUNK + w.large_cap + 1u;
   const uint32_t max_tiles = m / tile + w.large_cap + 1u;
   w.lists = alloc((size_t(w.large_cap) * 2 + w.local_cap) * sizeof(Bucket));

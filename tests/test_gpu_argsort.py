@@ -313,6 +313,20 @@ def test_msd_hybrid_and_fallback(case):
         assert took == path, f"{case}: took the {took} path, kernels {sorted(names)}"
 
 
+def test_msd_list_overflow_falls_back(monkeypatch):
+    """A hybrid sort whose bucket-sort list runs out of room (forced here by capping the list at 64
+    entries) drops the bucket sort and finishes with the LSD passes, whose order is the same."""
+    monkeypatch.setenv("ARROWMETAL_MSD_LIST_CAP", "64")
+    vals, valid, _ = msd_case("uniform", 4_500_000)
+    for desc, place, order in ((False, 0, 0), (True, 1, 0)):
+        print(f"msd list overflow desc={desc} place={place}", flush=True)
+        got, names = sort_with_kernels(vals, valid, desc, place, order)
+        want = O.argsort(vals, valid, desc, place, order)
+        assert np.array_equal(got, want), f"desc={desc}: first difference at {int(np.nonzero(got != want)[0][0])}"
+        assert "sort_msd_first" in names and "sort_msd_local" not in names and "sort_msd_group" not in names, \
+            sorted(names)
+
+
 @pytest.mark.parametrize("n", [1 << 18, 300_007, 1_000_000, (1 << 22) - 1])
 @pytest.mark.parametrize("case", ["uniform", "float_bits", "half_range", "duplicates"])
 def test_msd_hybrid_small_inputs(n, case):

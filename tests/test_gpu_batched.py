@@ -128,6 +128,25 @@ def test_batched_msd_levels_large_batches():
     check_sort(idx, v, offs, True, 1, "float64 total desc")
 
 
+def test_batched_msd_list_overflow_falls_back(monkeypatch):
+    """A batched run whose MSD bucket-sort list runs out of room (forced here by capping the list at 64
+    entries when the plan is made) is sorted again by the segmented LSD passes, which the plan keeps
+    from then on; every batch still matches the oracle."""
+    monkeypatch.setenv("ARROWMETAL_MSD_LIST_CAP", "64")
+    offs = lognormal_offsets(200, 20_000, 1.0, 200_000, seed=31)
+    plan = am.Batched(offs)
+    monkeypatch.delenv("ARROWMETAL_MSD_LIST_CAP")
+    n = int(offs[-1])
+    sk = O.splitmix64(0x0FF1CE, n).view(np.int64)
+    mk = am.MetalArray.from_numpy(sk)
+    for rep in range(2):
+        print(f"batched msd list overflow n={n} run {rep}", flush=True)
+        plan.run(mk, use_graph=True)
+        assert plan.config["sort_path"] == "lsd"
+        idx, _ = plan.result("indices").to_numpy()
+        check_sort(idx, sk, offs, False, 0, f"run {rep}")
+
+
 def test_batched_overflow_regroups():
     """Batches with more distinct keys than the hash capacity are regrouped by the general path."""
     offs = np.array([0, 100, 5100, 5100, 9000, 40_000, 40_050], dtype=np.int64)
