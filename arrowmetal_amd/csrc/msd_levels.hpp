@@ -1,13 +1,16 @@
 // The levels of the adaptive MSD radix argsort of 64-bit keys, shared by the plain argsort (sort.hip)
 // and the batched plan (batched.hip).
 //
-// Rows are in buckets (Bucket: a range of the value block, in one of two key / row buffers, whose keys
-// share their digits from the top down to `level`). Level by level down the digits, every bucket
-// larger than `local_max` rows is partitioned stably by the next digit (onesweep tiles whose
-// look-back stops at their bucket's first tile) into the other buffer at the same positions; a bucket
-// whose rows all share the digit is carried without moving; a bucket whose keys differ in no lower
-// bit leaves as a copy. Buckets of at most `local_max` rows leave for the bucket sort (hyb_local),
-// which sorts each in LDS by its remaining digits and writes its rows to their final slots.
+// Rows are in buckets (Bucket: a range of the value block, in one of two key / row buffers, or in the
+// first key buffer with implicit rows, whose keys share their digits from the top down to `level`).
+// Level by level down the digits, every bucket larger than `local_max` rows is partitioned stably by
+// the next digit (onesweep tiles whose look-back stops at their bucket's first tile) into the other
+// buffer at the same positions; a bucket whose rows all share the digit is carried without moving; a
+// run of sub-buckets whose keys differ in no lower bit leaves as one copy. Sub-buckets of at most
+// `local_max` rows leave for the bucket sorts, which sort them in LDS by their remaining digits and
+// write their rows to their final slots: one workgroup per bucket (hyb_local), or, for small sibling
+// buckets packed into groups, one workgroup per group (hyb_group); buckets with long runs of equal
+// keys are queued for LSD passes in LDS (hyb_local_lsd).
 //
 // The kernels take their counts from device memory (the `lc` counters and an MsdCtl), so a level can
 // run from host read-backs (the plain argsort sizes each launch from them) or blind, inside a graph
@@ -23,13 +26,13 @@ namespace {
 constexpr uint32_t LVL_CHUNK = 16384;   // rows per histogram chunk of a level
 constexpr uint32_t LOCAL_MAX = 4096;    // largest bucket the LDS sort takes (local_max is at most this)
 static_assert(LOCAL_MAX <= 4096u, "positions in a bucket fit the 12 low bits of the bucket sort's words");
-constexpr uint32_t SUB_MAX = 64;    // longest digit run of one bucket the bucket sort ranks by counting
+constexpr uint32_t SUB_MAX = 64;       // longest digit run the bucket sorts rank by counting
 constexpr uint32_t GROUP_SPAN = 16;    // a group's buckets have their digit in one window of this many values
 constexpr uint32_t GROUP_BINS = GROUP_SPAN * RADIX;
-constexpr uint32_t GROUP_ROWS = 2048;  // rows of a group at most (the bucket sort at 8 keys per thread)
+constexpr uint32_t GROUP_ROWS = 2048;  // rows of a group at most (the group sort at 8 keys per thread)
 
-// A bucket: rows [start, start + size) of the value block, in key / row buffer `buf` (0 or 1), all
-// sharing the digits from the top down to `level`.
+// A bucket: rows [start, start + size) of the value block, in buffer `buf`, all sharing the digits from
+// the top down to `level`.
 struct Bucket {
   uint32_t start, size, buf, level;
 };
