@@ -779,21 +779,22 @@ __device__ __forceinline__ unsigned long long identity(int op) { return op == W_
 
 
 // One word update in LDS or in this XCD's private global table; workgroup-scope atomics in both
-// (ds_* in LDS, L2-resident in global memory). MIN / MAX / OR read first and skip the atomic when it
-// cannot change anything, which keeps hot groups read-mostly: a stale read never lets a needed
-// update be skipped, because those words only ever move one way.
+// (ds_* in LDS, L2-resident in global memory). A contribution equal to the op's identity touches
+// nothing, not even a read. MIN / MAX / OR read first and skip the atomic when it cannot change
+// anything, which keeps hot groups read-mostly: a stale read never lets a needed update be skipped,
+// because those words only ever move one way.
 template <int OP>
 __device__ __forceinline__ void word_update(unsigned long long* p, unsigned long long x) {
   if (OP == W_ADD) {
     if (x) __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MIN) {
-    if (x < *reinterpret_cast<volatile unsigned long long*>(p))
+    if (x != ~0ull && x < *reinterpret_cast<volatile unsigned long long*>(p))
       __hip_atomic_fetch_min(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MAX) {
-    if (x > *reinterpret_cast<volatile unsigned long long*>(p))
+    if (x != 0ull && x > *reinterpret_cast<volatile unsigned long long*>(p))
       __hip_atomic_fetch_max(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_OR) {
-    if (x & ~*reinterpret_cast<volatile unsigned long long*>(p))
+    if (x != 0ull && (x & ~*reinterpret_cast<volatile unsigned long long*>(p)))
       __hip_atomic_fetch_or(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
