@@ -646,6 +646,7 @@ struct CountP {
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
   static constexpr bool USES_VALUE = false;
   static constexpr int FIXED_VW = 0;
+  static constexpr bool AUX = false;
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long, unsigned long long (&c)[1]) {
     c[0] = 1;
     return true;
@@ -659,6 +660,7 @@ struct SumCountP {
   __host__ __device__ static constexpr int op(int) { return W_ADD; }
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 0;
+  static constexpr bool AUX = false;
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     c[0] = x;
     c[1] = 1;
@@ -674,6 +676,7 @@ struct MinMaxP {
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MIN : W_MAX; }
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 0;
+  static constexpr bool AUX = false;
   __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     unsigned long long b = x, key;
     switch (a.v.type) {
@@ -705,6 +708,7 @@ struct F64AP {
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MAX : (w == 1 ? W_OR : (w == 2 ? W_ADD : W_MIN)); }
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 8;
+  static constexpr bool AUX = false;
   __device__ static bool contrib(const AggArgs&, int64_t i, long long, unsigned long long v, unsigned long long (&c)[4]) {
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     unsigned long long mag = v & 0x7FFFFFFFFFFFFFFFull;
@@ -755,10 +759,17 @@ struct F64BP {
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_ADD128 : (w == 1 ? W_HI : W_ADD); }
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 8;
-  __device__ static bool contrib(const AggArgs& a, int64_t, long long g, unsigned long long v, unsigned long long (&c)[3]) {
+  // AUX: the row's group words 0 (largest exponent) and 1 (special values) of the pass-A table, which
+  // the aggregation loop loads for all its rows in flight before the contributions.
+  static constexpr bool AUX = true;
+  __device__ static void aux_load(const AggArgs& a, long long g, unsigned long long (&st)[2]) {
+    st[0] = a.aux[size_t(g) * 4];
+    st[1] = a.aux[size_t(g) * 4 + 1];
+  }
+  __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long v,
+                                 const unsigned long long (&st)[2], unsigned long long (&c)[3]) {
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) return false;
-    const unsigned long long* st = a.aux + size_t(g) * 4;
     if (st[1] & FX_SPECIAL) return false;
     uint32_t U = fx_window(uint32_t(st[0]), uint32_t(a.extra));
     unsigned long long lo, hi;
@@ -887,8 +898,23 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
       if constexpr (IDW == 8) g = (long long)idr[r];
       else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(idr[r])) : (long long)uint32_t(idr[r]);
       if (i >= end || !iok[r] || g < 0 || g >= a.K || !vok[r]) g = -1;
-      if (g >= 0 && !P::contrib(a, i, g, val_of_raw(vr[r], a.v.type), cs[r])) g = -1;
       gs[r] = g;
+    }
+    if constexpr (P::AUX) {
+      unsigned long long st[AGG_ROWS][2];
+#pragma unroll
+      for (int r = 0; r < AGG_ROWS; r++) P::aux_load(a, gs[r] >= 0 ? gs[r] : 0, st[r]);   // all loads first
+#pragma unroll
+      for (int r = 0; r < AGG_ROWS; r++) {
+        const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(vr[r], a.v.type), st[r], cs[r])) gs[r] = -1;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < AGG_ROWS; r++) {
+        const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(vr[r], a.v.type), cs[r])) gs[r] = -1;
+      }
     }
 #pragma unroll
     for (int r = 0; r < AGG_ROWS; r++) {
