@@ -222,6 +222,29 @@ def test_exact_float_sum_random(K, spread):
     check_grouping([keys], [kv], vals, vv, f"K={K} spread={spread}")
 
 
+def test_float_specials_many_groups():
+    """Float64 sum, mean, min and max over 100k groups (most outside the LDS cache, in the global
+    tables) with NaN, +inf, -inf, -0.0 and nulls mixed into ordinary values: the special-value flags
+    and the lowest NaN row of each group, and groups left with only zeros or only specials."""
+    rng = np.random.default_rng(0x5BEC)
+    n, K = 600_000, 100_000
+    keys = rng.integers(0, K, n).astype(np.int64)
+    vals = rng.uniform(-1e3, 1e3, n) * np.exp2(rng.integers(-40, 41, n).astype(np.float64))
+    pick = rng.random(n)
+    vals[pick < 0.004] = np.nan
+    vals[(pick >= 0.004) & (pick < 0.008)] = np.inf
+    vals[(pick >= 0.008) & (pick < 0.012)] = -np.inf
+    vals[(pick >= 0.012) & (pick < 0.03)] = -0.0
+    vals.view(np.uint64)[(pick >= 0.03) & (pick < 0.032)] = np.uint64(0xFFF8000000000123)   # a NaN payload
+    vv = rng.random(n) > 0.05
+    g = am.group_by([am.MetalArray.from_numpy(keys)])
+    ids_w, K2, _ = O.group_keys([keys])
+    mv = am.MetalArray.from_numpy(vals, vv)
+    for op, name in ((0, "sum"), (3, "mean"), (4, "min"), (5, "max")):
+        print(f"float specials many groups {name}", flush=True)
+        assert_agg(getattr(g, name)(mv), O.group_agg(ids_w, K2, vals, op, vv), f"specials {name}")
+
+
 def test_cancellation_heavy():
     rng = np.random.default_rng(42)
     for K in (3, 2000):
