@@ -115,6 +115,7 @@ _SIGNATURES = {
                         ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "am_server_stats": ([_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
     "am_server_release": ([_P], None),
+    "am_set_max_row_index": ([ctypes.c_int64], ctypes.c_int64),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
     "am_prof_report": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int64),
@@ -557,6 +558,12 @@ class batch:
     def __exit__(self, exc_type, exc, tb):
         _check(lib().am_batch_end())
         return False
+
+
+def set_max_row_index(max_row):
+    """Test hook (the reference's Dispatch.maxRowIndex): the largest row number index-returning calls
+    may produce; None or an out-of-range value restores 2^32 - 1. Returns the previous limit."""
+    return lib().am_set_max_row_index(-1 if max_row is None else int(max_row))
 
 
 def prof_enable(on=True):

@@ -33,6 +33,11 @@ struct Error : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 [[noreturn]] void fail(const std::string& msg);
+// The reference's error texts (ArrowMetalError.description, Sources/ArrowMetal/MetalContext.swift:19-31):
+// "Invalid ArrowArray: <msg>", "Unsupported Arrow type: <msg>", "Array length mismatch: <a> vs <b>".
+[[noreturn]] void fail_invalid(const std::string& msg);
+[[noreturn]] void fail_unsupported(const std::string& msg);
+[[noreturn]] void fail_length_mismatch(int64_t a, int64_t b);
 void hip_check(hipError_t e, const char* what);
 #define AM_HIP(x) ::am::hip_check((x), #x)
 
@@ -165,9 +170,13 @@ int64_t count_nulls(const Array& a);
 // ---------------------------------------------------------------- limits
 
 // Row numbers are uint32 ("I"): every index-returning call refuses more than 2^32 rows
-// (Sources/ArrowMetal/Kernels/Dispatch.swift:186-203).
+// (Sources/ArrowMetal/Kernels/Dispatch.swift:186-203). The largest row number an index array may hold
+// is 2^32 - 1 unless a test lowers it (the reference's Dispatch.maxRowIndex, :190-196), so that the
+// refusal is reachable at small sizes.
 void check_length(int64_t n);
 void check_index_rows(int64_t rows, const char* op);
+int64_t max_row_index();
+int64_t set_max_row_index(int64_t v);   // returns the previous limit; out-of-range v restores 2^32 - 1
 
 // ---------------------------------------------------------------- operations
 

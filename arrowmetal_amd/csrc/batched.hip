@@ -1049,7 +1049,7 @@ void check_inputs(const Batched* b, const Array* sort_keys, const Array* group_k
   if ((group_keys != nullptr) != (values != nullptr)) fail("batched: group keys and values go together");
   if (group_keys && (!type_is_int(group_keys->type) || !type_is_int(values->type)))
     fail("batched: group-by takes integer keys and integer values");
-  if (forder < 0 || forder > 2) fail("unknown float_order " + std::to_string(forder));
+  if (forder < 0 || forder > 2) fail_invalid("unknown float_order " + std::to_string(forder));
 }
 
 // The kernel sequence of one run, on stream s.

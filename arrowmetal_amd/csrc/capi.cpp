@@ -144,7 +144,7 @@ int am_argsort_ex(am_array* a, int descending, int null_placement, am_array** ou
 int am_argsort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out) {
   if (!a || !out) return 2;
   return guarded([&] {
-    if (float_order < 0 || float_order > 2) am::fail("unknown float_order " + std::to_string(float_order));
+    if (float_order < 0 || float_order > 2) am::fail_invalid("unknown float_order " + std::to_string(float_order));
     *out = H(am::argsort(*A(a), descending != 0, null_placement == 1 ? 1 : 0, float_order));
   });
 }
@@ -284,6 +284,8 @@ int am_unique_rows(am_array** columns, int64_t count, am_array** out) {
     if (!columns[i]) return 2;
   return guarded([&] { *out = H(am::unique_rows(reinterpret_cast<Array* const*>(columns), count)); });
 }
+
+int64_t am_set_max_row_index(int64_t max_row) { return am::set_max_row_index(max_row); }
 
 int am_prof_enable(int on) {
   am::prof_enable(on != 0);

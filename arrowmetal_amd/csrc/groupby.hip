@@ -24,6 +24,7 @@
 //                workgroup when the groups fit, otherwise an LDS cache of the groups a workgroup sees
 //                (claimed by CAS) in front of global 64-bit atomics; flushed once per workgroup.
 #include "am_internal.hpp"
+#include "fsum.hpp"
 
 #include <algorithm>
 #include <map>
@@ -634,8 +635,7 @@ struct AggArgs {
   ValSource v;
   int64_t n;
   long long K;
-  int extra;                     // family-specific (float64 sums: the guard bits G)
-  const unsigned long long* aux; // family-specific (float64 sums: the pass-A table)
+  const void* aux;               // family-specific (exact float sums: the groups' grid words)
 };
 
 // count_all (values ignored) and count (non-null values).
@@ -699,90 +699,84 @@ struct MinMaxP {
   }
 };
 
-// Float64 exact sum, pass A: the largest biased exponent of the finite non-zero values, the special
-// values seen (bit 0 +inf, bit 1 -inf, bit 2 any NaN, bit 3 a +0.0), the count of non-null values and
-// the lowest row holding a NaN.
-struct F64AP {
+// ---- exact float sums (arithmetic in fsum.hpp)
+//
+// Pass 1, FsScanP: per group, word 0 the largest biased exponent of the finite non-zero values (1 for
+// subnormals), word 1 the value classes seen (FS_*), word 2 the count of non-null values, word 3 the
+// lowest row holding a NaN. fs_window then turns words 0-2 into the group's grid: the bit offset
+// `lsb` of the fixed-point unit above 2^-1074, or FS_SKIP when the group has no finite non-zero value
+// or has a NaN or an infinity (its result does not depend on the finite values). Pass 2, FsFixP: per
+// group, the 128-bit two's-complement sum of floor(value / unit) (words 0-1) and the number of values
+// that lost bits to the floor (word 2). VW is the width of the values: 8 for float64, 4 for float32,
+// which is widened exactly on load.
+constexpr unsigned long long FS_POS_INF = 1ull, FS_NEG_INF = 2ull, FS_NAN = 4ull, FS_POS_ZERO = 8ull;
+constexpr uint32_t FS_SKIP = 0xFFFFFFFFu;
+
+template <int VW>
+__device__ __forceinline__ unsigned long long fs_bits(unsigned long long raw) {
+  if constexpr (VW == 4) return fs::widen_f32(uint32_t(raw));
+  else return raw;
+}
+
+template <int VW>
+struct FsScanP {
   static constexpr int W = 4;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_MAX : (w == 1 ? W_OR : (w == 2 ? W_ADD : W_MIN)); }
   static constexpr bool USES_VALUE = true;
-  static constexpr int FIXED_VW = 8;
+  static constexpr int FIXED_VW = VW;
   static constexpr bool AUX = false;
-  __device__ static bool contrib(const AggArgs&, int64_t i, long long, unsigned long long v, unsigned long long (&c)[4]) {
-    uint32_t e = uint32_t((v >> 52) & 0x7FF);
-    unsigned long long mag = v & 0x7FFFFFFFFFFFFFFFull;
-    c[0] = 0;
-    c[1] = 0;
-    c[2] = 1;
+  __device__ static bool contrib(const AggArgs&, int64_t row, long long, unsigned long long raw,
+                                 unsigned long long (&c)[4]) {
+    const unsigned long long v = fs_bits<VW>(raw);
+    const uint32_t be = uint32_t(v >> 52) & 0x7FFu;
+    const bool neg = (v >> 63) != 0ull, zero_frac = (v & 0xFFFFFFFFFFFFFull) == 0ull;
+    c[0] = 0ull;               // identities of MAX / OR / MIN: no effect
+    c[1] = 0ull;
+    c[2] = 1ull;
     c[3] = ~0ull;
-    if (e == 0x7FF) {
-      if (mag == 0x7FF0000000000000ull) c[1] = (v >> 63) ? 2u : 1u;
-      else { c[1] = 4u; c[3] = (unsigned long long)i; }
-    } else if (mag == 0) {
-      if (v == 0) c[1] = 8u;
-    } else {
-      c[0] = e ? e : 1u;
+    switch (be) {
+      case 0x7FFu:
+        if (zero_frac) c[1] = neg ? FS_NEG_INF : FS_POS_INF;
+        else { c[1] = FS_NAN; c[3] = (unsigned long long)row; }
+        break;
+      case 0u:
+        if (zero_frac) { c[1] = neg ? 0ull : FS_POS_ZERO; break; }
+        c[0] = 1ull;           // a subnormal: its unit is that of exponent 1
+        break;
+      default:
+        c[0] = be;
+        break;
     }
     return true;
   }
 };
 
-constexpr uint32_t FX_SPECIAL = 7u;   // +inf, -inf or NaN: the result is not a rounded sum
-
-__device__ __forceinline__ uint32_t fx_window(uint32_t emax, uint32_t G) { return emax > G + 1u ? emax - G : 1u; }
-
-// The fixed-point integer of a finite non-zero value in a window whose unit has biased exponent U:
-// (lo, hi) magnitude; `dropped` when bits fell below the window.
-__device__ __forceinline__ void fx_fixed(unsigned long long v, uint32_t U, unsigned long long& lo,
-                                         unsigned long long& hi, bool& dropped) {
-  uint32_t e = uint32_t((v >> 52) & 0x7FF);
-  unsigned long long m = (v & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
-  int sh = int(e ? e : 1u) - int(U);
-  dropped = false;
-  if (sh >= 64) { lo = 0; hi = m << (sh - 64); }
-  else if (sh > 0) { lo = m << sh; hi = m >> (64 - sh); }
-  else if (sh == 0) { lo = m; hi = 0; }
-  else {
-    int r = -sh;
-    hi = 0;
-    if (r >= 64) { lo = 0; dropped = true; }
-    else { lo = m >> r; dropped = (m & ((1ull << r) - 1ull)) != 0ull; }
-  }
-}
-
-// Float64 exact sum, pass B: the two's-complement 128-bit fixed-point sum in the group's window, and
-// the count of values truncated by the window. aux = the pass-A table (4 words per group).
-struct F64BP {
+template <int VW>
+struct FsFixP {
   static constexpr int W = 3;
   static constexpr bool SKIP_NULL = true;
   __host__ __device__ static constexpr int op(int w) { return w == 0 ? W_ADD128 : (w == 1 ? W_HI : W_ADD); }
   static constexpr bool USES_VALUE = true;
-  static constexpr int FIXED_VW = 8;
-  // AUX: the row's group words 0 (largest exponent) and 1 (special values) of the pass-A table, which
-  // the aggregation loop loads for all its rows in flight before the contributions.
+  static constexpr int FIXED_VW = VW;
+  // The group's grid word from fs_window, loaded for all rows in flight before the contributions.
   static constexpr bool AUX = true;
-  __device__ static void aux_load(const AggArgs& a, long long g, unsigned long long (&st)[2]) {
-    st[0] = a.aux[size_t(g) * 4];
-    st[1] = a.aux[size_t(g) * 4 + 1];
+  using AuxT = uint32_t;
+  __device__ static AuxT aux_load(const AggArgs& a, long long g) {
+    return reinterpret_cast<const uint32_t*>(a.aux)[g];
   }
-  __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long v,
-                                 const unsigned long long (&st)[2], unsigned long long (&c)[3]) {
-    uint32_t e = uint32_t((v >> 52) & 0x7FF);
-    if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) return false;
-    if (st[1] & FX_SPECIAL) return false;
-    uint32_t U = fx_window(uint32_t(st[0]), uint32_t(a.extra));
-    unsigned long long lo, hi;
-    bool dropped;
-    fx_fixed(v, U, lo, hi, dropped);
-    if (v >> 63) {   // two's complement of the magnitude
-      lo = ~lo + 1ull;
-      hi = ~hi + (lo == 0 ? 1ull : 0ull);
-    }
-    c[0] = lo;
-    c[1] = hi;
-    c[2] = dropped ? 1u : 0u;
-    return (c[0] | c[1] | c[2]) != 0;
+  __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long raw, AuxT lsb,
+                                 unsigned long long (&c)[3]) {
+    if (lsb == FS_SKIP) return false;
+    const unsigned long long v = fs_bits<VW>(raw);
+    const uint32_t be = uint32_t(v >> 52) & 0x7FFu;
+    if (be == 0x7FFu || (v << 1) == 0ull) return false;   // specials and zeros add nothing
+    bool lost;
+    const fs::U128 x = fs::to_grid(v, int(lsb), lost);
+    c[0] = x.lo;
+    c[1] = x.hi;
+    c[2] = lost ? 1ull : 0ull;
+    return (c[0] | c[1] | c[2]) != 0ull;
   }
 };
 
@@ -901,9 +895,9 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
       gs[r] = g;
     }
     if constexpr (P::AUX) {
-      unsigned long long st[AGG_ROWS][2];
+      typename P::AuxT st[AGG_ROWS];
 #pragma unroll
-      for (int r = 0; r < AGG_ROWS; r++) P::aux_load(a, gs[r] >= 0 ? gs[r] : 0, st[r]);   // all loads first
+      for (int r = 0; r < AGG_ROWS; r++) st[r] = P::aux_load(a, gs[r] >= 0 ? gs[r] : 0);   // all loads first
 #pragma unroll
       for (int r = 0; r < AGG_ROWS; r++) {
         const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
@@ -1135,126 +1129,103 @@ __global__ void gb_finalize_simple(const unsigned long long* gtab, int W, long l
   }
 }
 
-// ---------------------------------------------------------------- exact float64 rounding
+// ---------------------------------------------------------------- exact float sums: grids and results
 
-// Round-to-nearest-even of (w[0..nw) + sticky) * 2^(lsb - 1074), w != 0, as binary64 bits.
-__device__ unsigned long long round_words(const unsigned long long* w, int nw, int lsb, bool sticky, bool neg) {
-  int top = -1;
-  for (int j = nw - 1; j >= 0; j--)
-    if (w[j]) { top = j * 64 + 63 - __clzll(w[j]); break; }
-  unsigned long long sgn = neg ? 0x8000000000000000ull : 0ull;
-  int P = top + lsb;
-  if (P >= 2098) return sgn | 0x7FF0000000000000ull;
-  int s = P >= 52 ? top - 52 : -lsb;
-  unsigned long long r;
-  if (s <= 0) {
-    r = w[0] << (-s);
-  } else {
-    int j = s >> 6, b = s & 63;
-    r = j < nw ? (w[j] >> b) : 0ull;
-    if (b && j + 1 < nw) r |= w[j + 1] << (64 - b);
-    int rbpos = s - 1;
-    bool rb = ((w[rbpos >> 6] >> (rbpos & 63)) & 1ull) != 0;
-    bool st = sticky;
-    for (int t = 0; t < (rbpos >> 6) && !st; t++) st = w[t] != 0;
-    int bb = rbpos & 63;
-    if (!st && bb) st = (w[rbpos >> 6] & ((1ull << bb) - 1ull)) != 0;
-    if (rb && (st || (r & 1ull))) r++;
-  }
-  unsigned long long bits = P >= 52 ? r + ((unsigned long long)(P - 52) << 52) : r;
-  if (bits >= 0x7FF0000000000000ull) bits = 0x7FF0000000000000ull;
-  return sgn | bits;
-}
-
-// The sum (cnt == 0) or the mean of the signed 128-bit W * 2^(lsb - 1074); zero gives +0.0.
-__device__ unsigned long long fx_value(unsigned long long lo, unsigned long long hi, int lsb, unsigned long long cnt) {
-  bool neg = (hi >> 63) != 0;
-  if (neg) {
-    lo = ~lo + 1ull;
-    hi = ~hi + (lo == 0 ? 1ull : 0ull);
-  }
-  if (lo == 0 && hi == 0) return 0ull;
-  if (cnt == 0) {
-    unsigned long long w[2] = {lo, hi};
-    return round_words(w, 2, lsb, false, neg);
-  }
-  // (W << 128) / cnt by long division over 32-bit digits; the remainder is sticky.
-  // cnt counts rows, so it is below 2^32 and every partial dividend fits 64 bits.
-  uint32_t d[8] = {0, 0, 0, 0, uint32_t(lo), uint32_t(lo >> 32), uint32_t(hi), uint32_t(hi >> 32)};
-  uint32_t q[8];
-  unsigned long long rem = 0;
-  for (int j = 7; j >= 0; j--) {
-    unsigned long long cur = (rem << 32) | d[j];
-    q[j] = uint32_t(cur / cnt);
-    rem = cur % cnt;
-  }
-  unsigned long long qw[4];
-  for (int j = 0; j < 4; j++) qw[j] = (unsigned long long)q[2 * j] | ((unsigned long long)q[2 * j + 1] << 32);
-  return round_words(qw, 4, lsb - 128, rem != 0, neg);
-}
-
-// One thread per group: the sum and the mean, their validity, and whether the window cannot settle
-// the rounding (then the group is summed again with the full-width accumulator below).
-__global__ void gb_f64_finalize(const unsigned long long* ta, const unsigned long long* tb, long long K, uint32_t G,
-                                const unsigned long long* vals, unsigned long long* out_sum,
-                                unsigned long long* out_mean, uint8_t* vb, uint32_t* redo, uint32_t* redo_count) {
-  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// One thread per group: the fixed-point grid of the group (see FsScanP / FsFixP).
+__global__ void fs_window(const unsigned long long* scan, long long K, uint32_t* grid) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
-  const unsigned long long* a = ta + k * 4;
-  const unsigned long long* b = tb + k * 3;
-  unsigned long long cnt = a[2];
-  uint32_t emax = uint32_t(a[0]);
-  unsigned long long flags = a[1];
-  redo[k] = 0xFFFFFFFFu;
-  if (cnt == 0) { out_sum[k] = 0; out_mean[k] = 0; vb[k] = 0; return; }
-  vb[k] = 1;
-  if (flags & 4u) {                     // any NaN: the first NaN in row order, quieted
-    unsigned long long r = vals[a[3]] | (1ull << 51);
-    out_sum[k] = r; out_mean[k] = r;
+  const unsigned long long* t = scan + k * 4;
+  const uint32_t emax = uint32_t(t[0]);
+  const unsigned long long cnt = t[2];
+  if (emax == 0u || (t[1] & (FS_POS_INF | FS_NEG_INF | FS_NAN))) {
+    grid[k] = FS_SKIP;
     return;
   }
-  if (flags & 3u) {
-    unsigned long long r = (flags & 3u) == 3u ? 0x7FF8000000000000ull
-                         : ((flags & 1u) ? 0x7FF0000000000000ull : 0xFFF0000000000000ull);
-    out_sum[k] = r; out_mean[k] = r;
-    return;
-  }
-  if (emax == 0) {                      // every value is a zero
-    unsigned long long z = (flags & 8u) ? 0ull : 0x8000000000000000ull;
-    out_sum[k] = z; out_mean[k] = z;
-    return;
-  }
-  unsigned long long lo = b[0], hi = b[1], drops = b[2];
-  int lsb = int(fx_window(emax, G)) - 1;
-  if (drops == 0) {
-    out_sum[k] = fx_value(lo, hi, lsb, 0);
-    out_mean[k] = fx_value(lo, hi, lsb, cnt);
-    return;
-  }
-  // The exact sum lies strictly between W - drops and W + drops units.
-  unsigned long long dlo = lo - drops, dhi = hi - (lo < drops ? 1ull : 0ull);
-  unsigned long long ulo = lo + drops, uhi = hi + (ulo < lo ? 1ull : 0ull);
-  unsigned long long s0 = fx_value(dlo, dhi, lsb, 0), s1 = fx_value(ulo, uhi, lsb, 0);
-  unsigned long long m0 = fx_value(dlo, dhi, lsb, cnt), m1 = fx_value(ulo, uhi, lsb, cnt);
-  bool z0 = dlo == 0 && dhi == 0, z1 = ulo == 0 && uhi == 0;
-  if (s0 == s1 && m0 == m1 && !z0 && !z1) { out_sum[k] = s0; out_mean[k] = m0; return; }
-  redo[k] = atomicAdd(redo_count, 1u);
+  grid[k] = uint32_t(fs::grid_lsb(emax, cnt));
 }
 
-// Full-width exact accumulation for the groups the window could not settle: 72 signed 64-bit digits
-// per group, digit j holding the sum of the value bits at [32 j, 32 j + 32) (bit 0 weighs 2^-1074).
-// Each value touches three digits and no digit can overflow for fewer than 2^31 rows.
+// A result pattern of the output format: binary64 as it is, binary32 widened to binary64 bits.
+template <int P>
+__device__ __forceinline__ unsigned long long fs_out(unsigned long long bits) {
+  if constexpr (P == 24) return fs::widen_f32(uint32_t(bits));
+  else return bits;
+}
+
+// The result of a group whose sum does not come from its finite values (FS_SKIP grid), by the
+// reference's special-value rules (GroupSumExact.swift:24-60): any NaN gives the first NaN in row
+// order, quieted; +inf and -inf together the default NaN; one infinity that infinity; only zeros
+// -0.0 when every zero is -0.0, else +0.0. Patterns in the output format P.
+template <int P, int VW>
+__device__ unsigned long long fs_special(unsigned long long classes, unsigned long long nan_row, const void* vals) {
+  constexpr int SH = P == 53 ? 52 : 23;
+  constexpr unsigned long long EXP = P == 53 ? 0x7FFull : 0xFFull;
+  constexpr unsigned long long SIGN = 1ull << (P == 53 ? 63 : 31);
+  if (classes & FS_NAN) {
+    unsigned long long b;
+    if constexpr (VW == 8) b = static_cast<const unsigned long long*>(vals)[nan_row];
+    else b = static_cast<const uint32_t*>(vals)[nan_row];
+    if constexpr (P == 53 && VW == 4) b = fs::widen_f32(uint32_t(b));
+    return b | (1ull << (SH - 1));            // the quiet bit
+  }
+  const unsigned long long inf = EXP << SH;
+  switch (classes & (FS_POS_INF | FS_NEG_INF)) {
+    case FS_POS_INF | FS_NEG_INF: return inf | (1ull << (SH - 1));
+    case FS_POS_INF: return inf;
+    case FS_NEG_INF: return SIGN | inf;
+    default: return (classes & FS_POS_ZERO) ? 0ull : SIGN;
+  }
+}
+
+// One thread per group: sum and mean (binary64 bits; a binary32 result is widened), their validity,
+// and a slot in the list of groups to sum again at full width when the grid cannot decide.
+template <int P, int EMIN, int VW>
+__global__ void fs_settle(const unsigned long long* scan, const unsigned long long* fix, const uint32_t* grid,
+                          long long K, const void* vals, unsigned long long* out_sum, unsigned long long* out_mean,
+                          uint8_t* vb, uint32_t* redo, uint32_t* redo_count) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const unsigned long long* t = scan + k * 4;
+  const unsigned long long cnt = t[2];
+  redo[k] = FS_SKIP;
+  vb[k] = cnt != 0ull;
+  if (cnt == 0ull) {
+    out_sum[k] = 0ull;
+    out_mean[k] = 0ull;
+    return;
+  }
+  const uint32_t lsb = grid[k];
+  if (lsb == FS_SKIP) {
+    const unsigned long long r = fs_out<P>(fs_special<P, VW>(t[1], t[3], vals));
+    out_sum[k] = r;
+    out_mean[k] = r;
+    return;
+  }
+  const unsigned long long* f = fix + k * 3;
+  unsigned long long s, m;
+  if (fs::settle<P, EMIN>(fs::U128{f[0], f[1]}, f[2], int(lsb), cnt, s, m)) {
+    out_sum[k] = fs_out<P>(s);
+    out_mean[k] = fs_out<P>(m);
+  } else {
+    redo[k] = atomicAdd(redo_count, 1u);
+  }
+}
+
+// Full-width accumulation for the groups the grid could not decide: 72 signed 64-bit digits per group,
+// digit j collecting the value bits at [32 j, 32 j + 32) above 2^-1074. Each value touches three
+// digits; no digit can overflow below 2^31 rows.
 constexpr int XD = 72;
 
-__global__ void gb_f64_redo_rows(AggArgs a, const uint32_t* redo, long long* digits) {
+template <int VW>
+__global__ void fs_redo_rows(AggArgs a, const uint32_t* redo, long long* digits) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (; i < a.n; i += stride) {
     long long g = load_gid(a.ids, i, a.K);
     if (g < 0) continue;
     uint32_t slot = redo[g];
-    if (slot == 0xFFFFFFFFu || !val_valid(a.v, i)) continue;
-    unsigned long long v = raw64(a.v, i);
+    if (slot == FS_SKIP || !val_valid(a.v, i)) continue;
+    const unsigned long long v = fs_bits<VW>(ld_raw<VW>(a.v.vals, i));
     uint32_t e = uint32_t((v >> 52) & 0x7FF);
     if (e == 0x7FF || (v & 0x7FFFFFFFFFFFFFFFull) == 0) continue;
     unsigned long long m = (v & 0xFFFFFFFFFFFFFull) | (e ? (1ull << 52) : 0ull);
@@ -1270,11 +1241,12 @@ __global__ void gb_f64_redo_rows(AggArgs a, const uint32_t* redo, long long* dig
   }
 }
 
-__global__ void gb_f64_redo_finalize(const uint32_t* redo, long long K, const long long* digits,
-                                     const unsigned long long* ta, unsigned long long* out_sum,
-                                     unsigned long long* out_mean) {
+template <int P, int EMIN>
+__global__ void fs_redo_settle(const uint32_t* redo, long long K, const long long* digits,
+                               const unsigned long long* scan, unsigned long long* out_sum,
+                               unsigned long long* out_mean) {
   long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K || redo[k] == 0xFFFFFFFFu) return;
+  if (k >= K || redo[k] == FS_SKIP) return;
   const long long* D = digits + size_t(redo[k]) * XD;
   // Normalise the carry-save digits into a two's-complement number of 36 64-bit words.
   constexpr int NW = XD / 2;
@@ -1288,7 +1260,7 @@ __global__ void gb_f64_redo_finalize(const uint32_t* redo, long long K, const lo
     if (j & 1) w[j >> 1] |= (unsigned long long)low << 32;
     else w[j >> 1] = low;
   }
-  bool neg = carry < 0;
+  const bool neg = carry < 0;
   if (neg) {
     unsigned long long c = 1;
     for (int j = 0; j < NW; j++) {
@@ -1297,23 +1269,21 @@ __global__ void gb_f64_redo_finalize(const uint32_t* redo, long long K, const lo
       w[j] = x;
     }
   }
-  bool zero = true;
-  for (int j = 0; j < NW; j++) zero = zero && w[j] == 0;
-  if (zero) { out_sum[k] = 0; out_mean[k] = 0; return; }
-  out_sum[k] = round_words(w, NW, 0, false, neg);
-  // Mean: (W << 128) / cnt over 32-bit digits, highest first.
-  unsigned long long cnt = ta[k * 4 + 2];
-  unsigned long long q[NW + 2];
-  for (int j = 0; j < NW + 2; j++) q[j] = 0;
-  unsigned long long rem = 0;
-  for (int j = 2 * (NW + 2) - 1; j >= 0; j--) {
-    uint32_t dj = (j < 4) ? 0u : uint32_t(w[(j - 4) >> 1] >> (((j - 4) & 1) * 32));
-    unsigned long long cur = (rem << 32) | dj;
-    unsigned long long qq = cur / cnt;
-    rem = cur % cnt;
-    q[j >> 1] |= (qq & 0xFFFFFFFFull) << ((j & 1) * 32);
+  int lead = -1;
+  for (int j = NW - 1; j >= 0 && lead < 0; j--)
+    if (w[j]) lead = j * 64 + 63 - __clzll(w[j]);
+  if (lead < 0) {   // an exact cancellation: +0.0
+    out_sum[k] = 0ull;
+    out_mean[k] = 0ull;
+    return;
   }
-  out_mean[k] = round_words(q, NW + 2, -128, rem != 0, neg);
+  int low;
+  bool tail;
+  const fs::U128 top = fs::top128(w, NW, lead, low, tail);
+  constexpr unsigned long long SIGN = 1ull << (P == 53 ? 63 : 31);
+  const unsigned long long s = neg ? SIGN : 0ull;
+  out_sum[k] = fs_out<P>(s | fs::rne<P, EMIN>(top, tail, 0, low - 1074));
+  out_mean[k] = fs_out<P>(s | fs::rne_div<P, EMIN>(top, tail, 0, scan[k * 4 + 2], low - 1074));
 }
 
 }  // namespace
@@ -1516,16 +1486,16 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
 }
 
 GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep) {
-  if (count < 1) fail("group-by needs at least one key column");
+  if (count < 1) fail_invalid("group-by needs at least one key column");
   int64_t n = columns[0]->length;
   for (int64_t c = 1; c < count; c++)
     if (columns[c]->length != n)
-      fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(columns[c]->length));
+      fail_length_mismatch(n, columns[c]->length);
   check_length(n);
   for (int64_t c = 0; c < count; c++) {
     int t = columns[c]->type;
     if (!type_is_primitive(t))
-      fail("group-by over a " + columns[c]->format + " key column is not supported by this build");
+      fail_unsupported("group-by over a " + columns[c]->format + " key column is not implemented");
   }
   auto* g = new GroupBy();
   try {
@@ -1567,7 +1537,7 @@ GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep)
       Mapped mc = map_column(ac->data(), ac->type, ac->has_nulls() ? ac->valid_bits() : nullptr, ac->offset, n, false);
       int64_t K2 = mc.K;
       if (K2 > 0 && K > INT64_MAX / K2)
-        fail("multi-column group-by overflowed the 64-bit composite key (" + std::to_string(K) + " x " +
+        fail_invalid("multi-column group-by overflowed the 64-bit composite key (" + std::to_string(K) + " x " +
              std::to_string(K2) + " distinct)");
       DevBuf comp = device_alloc(size_t(n) * 8);
       hipLaunchKernelGGL(gb_combine, dim3(grid_for(n)), dim3(256), 0, s, static_cast<const int32_t*>(g->ids->ptr),
@@ -1608,7 +1578,7 @@ Array* group_by_ids(GroupBy* g) {
 
 Array* group_by_keys_result(GroupBy* g, int64_t i) {
   if (i < 0 || i >= int64_t(g->columns.size()))
-    fail("key column " + std::to_string(i) + " of " + std::to_string(g->columns.size()));
+    fail_invalid("key column " + std::to_string(i) + " of " + std::to_string(g->columns.size()));
   if (g->key_results.empty()) g->key_results.assign(g->columns.size(), nullptr);
   if (!g->key_results[size_t(i)]) {
     const Array* col = g->columns[size_t(i)];
@@ -1719,54 +1689,61 @@ AggArgs make_args(const IdSource& ids, const Array* v, int64_t n, long long K) {
   return a;
 }
 
-int guard_bits(int64_t rows) {
-  int bits = rows <= 0 ? 1 : 64 - __builtin_clzll(uint64_t(rows));
-  return 74 - bits;
-}
-
-// Correctly rounded float64 sum and mean per group, with their validity bytes.
-void exact_f64(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
+// The correctly rounded sum and mean per group of a float64 (vw 8) or float32 (vw 4) column, as
+// binary64 bits (prec 53), or as binary32 results widened to binary64 bits (prec 24), with validity
+// bytes.
+template <int P, int EMIN, int VW>
+void exact_fsum_t(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
   hipStream_t s = stream();
-  long long K = base.K;
+  const long long K = base.K;
+  const size_t k1 = size_t(std::max<long long>(K, 1));
   AggArgs a = base;
-  a.extra = guard_bits(base.n);
-  DevBuf ta = run_agg<F64AP>(a, s, "gb_f64_a");
-  a.aux = static_cast<const unsigned long long*>(ta->ptr);
-  DevBuf tb = run_agg<F64BP>(a, s, "gb_f64_b");
-  size_t k1 = size_t(std::max<long long>(K, 1));
+  DevBuf scan = run_agg<FsScanP<VW>>(a, s, "fs_scan");
+  DevBuf grid = device_alloc(k1 * 4);
+  hipLaunchKernelGGL(fs_window, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                     static_cast<const unsigned long long*>(scan->ptr), K, static_cast<uint32_t*>(grid->ptr));
+  AM_HIP(hipGetLastError());
+  a.aux = grid->ptr;
+  DevBuf fix = run_agg<FsFixP<VW>>(a, s, "fs_fixed");
   sum = device_alloc(k1 * 8);
   mean = device_alloc(k1 * 8);
   vb = device_alloc(k1);
   DevBuf redo = device_alloc(k1 * 4), rc = device_alloc(8);
   hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, static_cast<unsigned long long*>(rc->ptr), 1);
-  if (K > 0) {
-    ProfScope ps("gb_f64_finalize");
-    hipLaunchKernelGGL(gb_f64_finalize, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const unsigned long long*>(ta->ptr), static_cast<const unsigned long long*>(tb->ptr),
-                       K, uint32_t(a.extra), static_cast<const unsigned long long*>(base.v.vals),
-                       static_cast<unsigned long long*>(sum->ptr), static_cast<unsigned long long*>(mean->ptr),
-                       static_cast<uint8_t*>(vb->ptr), static_cast<uint32_t*>(redo->ptr),
-                       static_cast<uint32_t*>(rc->ptr));
+  {
+    ProfScope ps("fs_settle");
+    hipLaunchKernelGGL((fs_settle<P, EMIN, VW>), dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const unsigned long long*>(scan->ptr), static_cast<const unsigned long long*>(fix->ptr),
+                       static_cast<const uint32_t*>(grid->ptr), K, base.v.vals, static_cast<unsigned long long*>(sum->ptr),
+                       static_cast<unsigned long long*>(mean->ptr), static_cast<uint8_t*>(vb->ptr),
+                       static_cast<uint32_t*>(redo->ptr), static_cast<uint32_t*>(rc->ptr));
     AM_HIP(hipGetLastError());
   }
   uint32_t nredo = 0;
   AM_HIP(hipMemcpyAsync(&nredo, rc->ptr, 4, hipMemcpyDeviceToHost, s));
   AM_HIP(hipStreamSynchronize(s));
-  if (nredo > 0) {
-    if (base.n >= (int64_t(1) << 31)) fail("float64 group sum: more than 2^31 rows in an unsettled group");
-    DevBuf digits = device_alloc(size_t(nredo) * XD * 8);
-    hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(grid_for(int64_t(nredo) * XD, 4)), dim3(256), 0, s,
-                       static_cast<unsigned long long*>(digits->ptr), (long long)nredo * XD);
-    ProfScope ps("gb_f64_redo");
-    hipLaunchKernelGGL(gb_f64_redo_rows, dim3(grid_for(base.n)), dim3(256), 0, s, a,
-                       static_cast<const uint32_t*>(redo->ptr), static_cast<long long*>(digits->ptr));
-    hipLaunchKernelGGL(gb_f64_redo_finalize, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const uint32_t*>(redo->ptr), K, static_cast<const long long*>(digits->ptr),
-                       static_cast<const unsigned long long*>(ta->ptr), static_cast<unsigned long long*>(sum->ptr),
-                       static_cast<unsigned long long*>(mean->ptr));
-    AM_HIP(hipGetLastError());
-    AM_HIP(hipStreamSynchronize(s));
-  }
+  if (nredo == 0) return;
+  if (base.n >= (int64_t(1) << 31)) fail("exact float sum: more than 2^31 rows in a group the grid cannot settle");
+  DevBuf digits = device_alloc(size_t(nredo) * XD * 8);
+  hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(grid_for(int64_t(nredo) * XD, 4)), dim3(256), 0, s,
+                     static_cast<unsigned long long*>(digits->ptr), (long long)nredo * XD);
+  ProfScope ps("fs_redo");
+  hipLaunchKernelGGL(fs_redo_rows<VW>, dim3(grid_for(base.n)), dim3(256), 0, s, a,
+                     static_cast<const uint32_t*>(redo->ptr), static_cast<long long*>(digits->ptr));
+  hipLaunchKernelGGL((fs_redo_settle<P, EMIN>), dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                     static_cast<const uint32_t*>(redo->ptr), K, static_cast<const long long*>(digits->ptr),
+                     static_cast<const unsigned long long*>(scan->ptr), static_cast<unsigned long long*>(sum->ptr),
+                     static_cast<unsigned long long*>(mean->ptr));
+  AM_HIP(hipGetLastError());
+  AM_HIP(hipStreamSynchronize(s));
+}
+
+// Float64 values to binary64 results; float32 values to binary64 results (am_group_agg_ex) or to
+// binary32 results (the dense am_group_by sum).
+void exact_fsum(const AggArgs& base, bool f32_result, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
+  if (base.v.type == T_F64) exact_fsum_t<53, -1022, 8>(base, sum, mean, vb);
+  else if (f32_result) exact_fsum_t<24, -126, 4>(base, sum, mean, vb);
+  else exact_fsum_t<53, -1022, 4>(base, sum, mean, vb);
 }
 
 Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::string& fmt, int type) {
@@ -1812,8 +1789,8 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     }
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
-  if (!v) fail("am_group_agg_ex op " + std::to_string(op) + " needs a value column");
-  if (v->length != n) fail("length mismatch: " + std::to_string(n) + " vs " + std::to_string(v->length));
+  if (!v) fail_invalid("am_group_agg_ex op " + std::to_string(op) + " needs a value column");
+  if (v->length != n) fail_length_mismatch(n, v->length);
   // hash_count needs only the validity, so it takes a column of any type.
   if (op == 2) {
     // An integer sum of the same column already counted its non-null values (word 1).
@@ -1825,20 +1802,21 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
   if (!type_is_primitive(v->type))
-    fail("grouped aggregate over a " + v->format + " column is not supported by this build");
+    fail_unsupported("grouped aggregate over a " + v->format + " column is not supported by this build");
   AggArgs a = make_args(ids, v, n, K);
   if (op == 0 || op == 3) {
-    if (v->type == T_F64) {
+    if (type_is_float(v->type)) {
+      // float64: correctly rounded (GroupSumExact.swift); float32: summed as float64 values, the
+      // exact sum rounded once (Segmented.swift:103-110 sums them in float64).
       DevBuf sum, mean, vb;
       if (cache && cache->fx_col == v->uid) {
         sum = cache->fx_sum; mean = cache->fx_mean; vb = cache->fx_valid;
       } else {
-        exact_f64(a, sum, mean, vb);
+        exact_fsum(a, false, sum, mean, vb);
         if (cache) { cache->fx_col = v->uid; cache->fx_sum = sum; cache->fx_mean = mean; cache->fx_valid = vb; }
       }
       return bits_array(op == 0 ? sum : mean, vb, K, "g", T_F64);
     }
-    if (v->type == T_F32) fail("float32 grouped sum / mean is not supported by this build");
     DevBuf t;
     if (cache && cache->cnt_col == v->uid && cache->cnt_tab) {
       t = cache->cnt_tab;
@@ -1877,7 +1855,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     st->child_names = {"min", "max"};
     return st;
   }
-  if (op < 0 || op > 25) fail("unknown am_group_agg_ex op " + std::to_string(op));
+  if (op < 0 || op > 25) fail_invalid("unknown am_group_agg_ex op " + std::to_string(op));
   fail("am_group_agg_ex op " + std::to_string(op) + " is not supported by this build");
 }
 
@@ -1886,7 +1864,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
 Array* group_agg(GroupBy* g, const Array* values, int op, double /*p1*/) {
   IdSource ids{g->ids->ptr, T_I32, nullptr, 0};
   if (op != 1 && values && values->length != g->n)
-    fail("length mismatch: " + std::to_string(g->n) + " vs " + std::to_string(values->length));
+    fail_length_mismatch(g->n, values->length);
   // GroupBy refuses a zero key count, so an empty mapping aggregates over one empty group and the
   // result is trimmed to zero rows (GroupByKeys.swift:59-76, 159-165).
   long long K = std::max<int64_t>(g->K, 1);
@@ -1900,23 +1878,49 @@ Array* group_agg(GroupBy* g, const Array* values, int op, double /*p1*/) {
   return r;
 }
 
+// The dense form's float32 sum (ArrowMetalC.swift:426, GroupBy.swift:94-106): a float32 total per
+// key, returned as float64. The reference adds in float32 in whatever order its atomics run; here the
+// exact sum is rounded once to float32, which is what any order gives whenever float32 addition is exact.
+Array* sum_f32_as_f64(const IdSource& ids, int64_t n, long long K, const Array& v) {
+  AggArgs a = make_args(ids, &v, n, K);
+  DevBuf sum, mean, vb;
+  exact_fsum(a, true, sum, mean, vb);
+  return bits_array(sum, vb, K, "g", T_F64);
+}
+
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values) {
-  if (key_count <= 0 || key_count > int64_t(UINT32_MAX) / 4) fail("keyCount out of range");
+  if (key_count <= 0 || key_count > int64_t(UINT32_MAX) / 4) fail_invalid("keyCount out of range");
   if (keys.type != T_I32 && keys.type != T_I64 && keys.type != T_U32)
     fail("am_group_by: keys must be int32, int64 or uint32, got " + keys.format);
   check_length(keys.length);
   IdSource ids{keys.data(), keys.type, keys.has_nulls() ? keys.valid_bits() : nullptr, keys.offset};
   // agg: 0 sum, 1 count rows, 2 min, 3 max, 4 mean, 5 count values (ArrowMetalC.swift:416-469).
   static const int op_of[6] = {0, 1, 4, 5, 3, 2};
-  if (agg < 0 || agg > 5) fail("am_group_by: unknown agg " + std::to_string(agg));
-  int op = op_of[agg];
-  if ((agg == 2 || agg == 3) && values && values->width() > 4)
-    fail("group-by min/max needs a 32-bit or narrower type (64-bit atomics are not available); cast first");
-  if (agg == 1) values = nullptr;
-  return aggregate(ids, keys.length, key_count, values, op, nullptr);
+  if (agg == 1) return aggregate(ids, keys.length, key_count, nullptr, 1, nullptr);
+  if (!values) fail_invalid("values required for this aggregation");
+  const int vt = values->type;
+  const bool int_sum = type_is_int(vt);
+  // The value types each agg takes there: sums and means of integers, the float32 sum (a float64
+  // result), min / max of 32-bit or narrower types, counts of anything. Every other pairing is the
+  // error "group-by agg N on <format>".
+  bool taken = agg == 5 || ((agg == 0 || agg == 4) && int_sum) || (agg == 0 && vt == T_F32) ||
+               ((agg == 2 || agg == 3) && type_is_primitive(vt) && values->width() <= 4);
+  if (agg < 0 || agg > 5 || !taken) {
+    if ((agg == 2 || agg == 3) && type_is_primitive(vt))
+      fail_unsupported("group-by min/max needs a 32-bit or narrower type");
+    fail_unsupported("group-by agg " + std::to_string(agg) + " on " + values->format);
+  }
+  if (values->length != keys.length) fail_length_mismatch(keys.length, values->length);
+  if (agg == 0 && vt == T_F32) return sum_f32_as_f64(ids, keys.length, key_count, *values);
+  return aggregate(ids, keys.length, key_count, values, op_of[agg], nullptr);
 }
 
 Array* unique_rows(Array* const* columns, int64_t count) {
+  if (count >= 1) {
+    check_length(columns[0]->length);
+    // The representative rows come from the row numbers 0 .. n-1 (GroupByKeys.swift:126-149, 436-438).
+    check_index_rows(columns[0]->length, "row indices");
+  }
   GroupBy* g = group_by_keys_impl(columns, count, true);
   Array* out = nullptr;
   try {
@@ -1956,6 +1960,8 @@ namespace am {
 // kept once at the position of the first null row. order 1: ascending, nulls dropped.
 Array* unique_values(const Array& a, int order) {
   if (order == 0) {
+    check_length(a.length);
+    if (a.length > 0) check_index_rows(a.length, "unique");   // Kernels/UniqueOrder.swift:97
     Array* col = const_cast<Array*>(&a);
     Array* rows = unique_rows(&col, 1);
     Array* out = nullptr;

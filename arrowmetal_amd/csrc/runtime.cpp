@@ -26,6 +26,11 @@ void set_last_error(const std::string& msg) { t_last_error = msg; }
 const char* last_error_cstr() { return t_last_error.c_str(); }
 
 [[noreturn]] void fail(const std::string& msg) { throw Error(msg); }
+[[noreturn]] void fail_invalid(const std::string& msg) { throw Error("Invalid ArrowArray: " + msg); }
+[[noreturn]] void fail_unsupported(const std::string& msg) { throw Error("Unsupported Arrow type: " + msg); }
+[[noreturn]] void fail_length_mismatch(int64_t a, int64_t b) {
+  throw Error("Array length mismatch: " + std::to_string(a) + " vs " + std::to_string(b));
+}
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
@@ -386,7 +391,7 @@ void release_array(Array* a) {
 
 // A zero-copy window onto `a` (ArrowMetalC.swift:384-389): the buffers are shared, the offset moves.
 Array* slice(const Array& a, int64_t offset, int64_t length) {
-  if (offset < 0 || length < 0 || offset + length > a.length) fail("slice out of range");
+  if (offset < 0 || length < 0 || offset + length > a.length) fail_invalid("slice out of range");
   if (a.type == T_STRUCT) fail("am_slice of a struct array is not supported by this build");
   Array* out = new_array(a.format, a.type, length);
   out->values = a.values;
@@ -406,13 +411,22 @@ Array* slice(const Array& a, int64_t offset, int64_t length) {
 }
 
 void check_length(int64_t n) {
-  if (n > int64_t(UINT32_MAX)) fail("arrays above 2^32 elements are not supported yet");
+  if (n > int64_t(UINT32_MAX)) fail_invalid("arrays above 2^32 elements are not supported yet");
+}
+
+namespace {
+std::atomic<int64_t> g_max_row_index{int64_t(UINT32_MAX)};
+}
+int64_t max_row_index() { return g_max_row_index.load(std::memory_order_relaxed); }
+int64_t set_max_row_index(int64_t v) {
+  if (v < 0 || v > int64_t(UINT32_MAX)) v = int64_t(UINT32_MAX);
+  return g_max_row_index.exchange(v);
 }
 void check_index_rows(int64_t rows, const char* op) {
-  int64_t last = rows - 1;
-  if (last > int64_t(UINT32_MAX))
-    fail(std::string(op) + ": row " + std::to_string(last) +
-         " does not fit the UInt32 index type (row numbers go up to " + std::to_string(UINT32_MAX) + ")");
+  const int64_t last = rows - 1, top = max_row_index();
+  if (last > top)
+    fail_invalid(std::string(op) + ": row " + std::to_string(last) +
+                 " does not fit the UInt32 index type (row numbers go up to " + std::to_string(top) + ")");
 }
 
 namespace {

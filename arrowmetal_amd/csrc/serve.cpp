@@ -191,7 +191,7 @@ int64_t server_submit(Server* s, int op, Array* const* cols, int64_t ncols, Arra
   if (ncols < 1) fail("server: a request needs at least one key column");
   if (op == OP_ARGSORT && ncols != 1) fail("server: an argsort request takes one column");
   if (op == OP_GROUP_SUM_COUNT && !values) fail("server: a group-by request needs a value column");
-  if (op == OP_ARGSORT && (forder < 0 || forder > 2)) fail("unknown float_order " + std::to_string(forder));
+  if (op == OP_ARGSORT && (forder < 0 || forder > 2)) fail_invalid("unknown float_order " + std::to_string(forder));
   auto r = std::make_shared<Request>();
   r->op = op;
   for (int64_t c = 0; c < ncols; c++) {
@@ -226,11 +226,13 @@ int64_t server_wait(Server* s, int64_t ticket, Array** out, int64_t cap, double*
   std::shared_ptr<Request> r;
   {
     std::unique_lock<std::mutex> lk(s->mu);
+    // The waiter takes the ticket out of the map before it sleeps: submits that insert (and rehash)
+    // meanwhile touch no iterator of ours, and a second wait on the same ticket finds it gone.
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end()) fail("server: unknown ticket " + std::to_string(ticket));
-    r = it->second;
-    s->cv_done.wait(lk, [&] { return r->state == 2; });
+    r = std::move(it->second);
     s->tickets.erase(it);
+    s->cv_done.wait(lk, [&] { return r->state == 2; });
   }
   if (times) {
     times[0] = 1e3 * (r->t_start - r->t_submit);

@@ -748,7 +748,7 @@ void argsort_device(const void* vals, int type, const uint8_t* validity, int64_t
 Array* argsort(const Array& a, bool descending, int null_placement, int float_order) {
   check_length(a.length);
   check_index_rows(a.length, "argsort");
-  if (!type_is_primitive(a.type)) fail("argsort: unsupported type " + a.format);
+  if (!type_is_primitive(a.type)) fail_unsupported("sort by " + a.format + " is not implemented");
   Array* out = new_array("I", T_U32, a.length);
   try {
     out->values = device_alloc(size_t(std::max<int64_t>(a.length, 1)) * 4);

@@ -106,9 +106,9 @@ Array* take(const Array& a, const Array& ix) {
   check_length(a.length);
   check_length(ix.length);
   if (ix.type != T_I32 && ix.type != T_I64 && ix.type != T_U32)
-    fail("take: indices must be int32, int64 or uint32, got " + ix.format);
+    fail_unsupported("take indices must be int32, int64 or uint32");
   int w = a.width();
-  if (w == 0) fail("take: unsupported value type " + a.format);
+  if (w == 0) fail_unsupported("take of " + a.format + " is not supported by this build");
   int64_t n = ix.length;
   Array* out = new_array(a.format, a.type, n);
   try {
@@ -137,7 +137,7 @@ Array* take(const Array& a, const Array& ix) {
         Array* a;
         ~Drop() { release_array(a); }
       } drop{res};
-      if (host->out_of_range) fail("take: index out of range (array length " + std::to_string(len) + ")");
+      if (host->out_of_range) fail_invalid("take: index out of range (array length " + std::to_string(len) + ")");
       res->null_count = need_validity ? int64_t(host->nulls) : 0;
       if (res->null_count == 0) res->validity.reset();
     });

@@ -70,7 +70,9 @@ int  am_argsort(am_array* a, int descending, am_array** out);
 
 /* reference include/arrowmetal.h:113 (ArrowMetalC.swift:416-469): dense keys int32/int64/uint32 in
  * [0, key_count); null and out-of-range keys are skipped. agg: 0 sum, 1 count(rows), 2 min, 3 max,
- * 4 mean, 5 count(values). min/max take 32-bit-or-narrower values. */
+ * 4 mean, 5 count(values). sum and mean take integer values (int64 / uint64 / float64 out); sum also
+ * takes float32 values (the float32 total per key, returned as float64); min/max take 32-bit or
+ * narrower values; any other pairing is the error "Unsupported Arrow type: group-by agg N on <fmt>". */
 int  am_group_by(am_array* keys, int64_t key_count, int agg, am_array* values, am_array** out);
 
 /* reference include/arrowmetal.h:184-185 (MetalContext.swift:245-312): per-thread batching. */
@@ -88,10 +90,10 @@ void am_group_by_release(am_groupby* gb);
 
 /* reference include/arrowmetal.h:920-954 (ArrowMetalC_GroupByExtra.swift:92-181). Ops taken here:
  *   0 hash_sum       integers wrap in 64 bits (int64 out; uint64 out for uint64 values);
- *                    float64 is correctly rounded (float64 out)
+ *                    float64 and float32 values: the exact sum rounded once (float64 out)
  *   1 hash_count_all rows per group (values may be NULL)
  *   2 hash_count     non-null values per group
- *   3 hash_mean      float64 out; correctly rounded for float64 values
+ *   3 hash_mean      float64 out; for float values the exact sum over the count, rounded once
  *   4 hash_min       values' type; float64 compares -0.0 < +0.0 and skips NaN
  *   5 hash_max
  *   6 hash_min_max   struct<min, max> of the two above, the struct itself never null
@@ -186,6 +188,14 @@ int  am_server_wait(am_server* s, int64_t ticket, am_array** out, int64_t cap, d
                     int64_t* count);
 int  am_server_stats(am_server* s, int64_t* out, int count);
 void am_server_release(am_server* s);
+
+/* Test hook, the counterpart of the reference's Dispatch.maxRowIndex (Sources/ArrowMetal/Kernels/
+ * Dispatch.swift:190-196, lowered in Tests/ArrowMetalTests/IndexTypeTests.swift:142-144): the largest
+ * row number an index-returning call (argsort, unique rows, unique) may produce, 2^32 - 1 by default.
+ * Past it such calls fail with "Invalid ArrowArray: <op>: row R does not fit the UInt32 index type (row
+ * numbers go up to M)". Returns the previous limit; a value outside [0, 2^32 - 1] restores the default.
+ * Process-wide. */
+int64_t am_set_max_row_index(int64_t max_row);
 
 /* Kernel timing of the library's own launches, measured with HIP events on the stream each kernel
  * is launched on. am_prof_report writes "name count total_ms\n" lines into buf (NUL-terminated) and

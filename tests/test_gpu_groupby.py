@@ -275,6 +275,23 @@ def test_zipf_keys_match_oracle():
         assert_agg(getattr(g, name)(mv), O.group_agg(ids_w, K2, fvals, op), f"zipf f64 {name}")
 
 
+def test_uniform_200_keys_match_oracle():
+    """The 200-uniform-key shape of the group-by workload (the LDS-private table path), at 4M rows:
+    keys, ids, int64 sum / count and float64 min / max / correctly rounded sum, all bit for bit."""
+    n, K = 4_000_000, 200
+    rng = np.random.default_rng(0x200)
+    table = O.splitmix64(0x200, K).view(np.int64)
+    keys = table[rng.integers(0, K, n)]
+    ivals = rng.integers(-10**9, 10**9 + 1, n).astype(np.int64)
+    fvals = rng.uniform(-1e6, 1e6, n)
+    g = check_grouping([keys], [None], ivals, None, "uniform200 int")
+    assert g.group_count == K
+    ids_w, K2, _ = O.group_keys([keys])
+    mv = am.MetalArray.from_numpy(fvals)
+    for op, name in ((0, "sum"), (4, "min"), (5, "max"), (3, "mean")):
+        assert_agg(getattr(g, name)(mv), O.group_agg(ids_w, K2, fvals, op), f"uniform200 f64 {name}")
+
+
 def test_dense_group_by_out_of_range_and_nulls():
     rng = np.random.default_rng(3)
     n, K = 300000, 1025

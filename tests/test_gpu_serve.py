@@ -132,3 +132,39 @@ def test_server_errors(resident):
     st = srv.stats
     srv.close()
     assert st["failed"] == 1 and st["completed"] == 2
+
+
+def test_submit_from_another_thread_while_waiting(resident):
+    """A submitter thread keeps adding requests (enough to rehash the ticket map many times) while the
+    main thread blocks in wait(); every result still equals the plain call's, and a ticket can be
+    waited for once only."""
+    import threading
+    host, dev = resident
+    srv = am.Server(slots=4, budget_bytes=8 << 30)
+    n = len(dev["gk"])
+    reqs = []
+    for i in range(96):
+        length = 1000 + 997 * i
+        off = (7919 * i) % (n - length)
+        reqs.append(("argsort", [dev["sk"].slice(off, length)], None, {"descending": bool(i & 1)}))
+    tickets = [None] * len(reqs)
+    first = srv.submit(*reqs[0][:2], values=reqs[0][2], **reqs[0][3])
+    tickets[0] = first
+
+    def submitter():
+        for i in range(1, len(reqs)):
+            kind, cols, values, opts = reqs[i]
+            tickets[i] = srv.submit(kind, cols, values=values, **opts)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    got0, _ = srv.wait(first)
+    th.join()
+    results = [got0] + [srv.wait(t)[0] for t in tickets[1:]]
+    with pytest.raises(am.ArrowMetalError, match="unknown ticket"):
+        srv.wait(first)
+    srv.close()
+    for (kind, cols, values, opts), got in zip(reqs, results):
+        want = plain(kind, cols, values, opts)
+        assert len(got) == len(want)
+        assert all(same(a, b) for a, b in zip(got, want))

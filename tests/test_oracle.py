@@ -407,6 +407,21 @@ def test_integer_mean():
     assert m.view(np.float64).tolist() == [5 / 3, -7.0]
 
 
+def test_integer_mean_rounds_sum_and_count_first():
+    """The reference's integer mean is d_div(d_from_long(sum), d_from_ulong(count)), i.e.
+    Double(sum) / Double(count) with each step rounded (GroupBySource.swift:28-40,
+    GroupBy.swift:108-121). Past 2^53 that differs from rounding the exact quotient once: a sum of
+    2^53 + 3 over 3 values gives (2^53 + 4) / 3 = 3002399751580332.0, not 3002399751580331.5. A
+    mistaken exact-quotient oracle fails here."""
+    from fractions import Fraction
+    keys = np.array([0, 0, 0, 1, 1], dtype=np.int32)
+    vals = np.array([2**53 + 1, 1, 1, -(2**62), -(2**62) + 1], dtype=np.int64)
+    m, ok = O.group_agg(keys, 2, vals, 3)
+    got = m.view(np.float64).tolist()
+    assert got[0] == 3002399751580332.0 and float(Fraction(2**53 + 3, 3)) == 3002399751580331.5
+    assert got[1] == float(np.float64(-(2**63) + 1) / 2.0) == -(2.0**62)
+
+
 # ---------------------------------------------------------------- unique
 
 
