@@ -658,7 +658,9 @@ struct Batched {
   DevBuf k0, k1, r0, r1, out_idx;
   MsdWork msd;                       // the levels of the MSD argsort of 64-bit keys
   bool msd_used = false;             // the last recorded sort was the MSD one
-  bool use_msd = true;               // 64-bit sort keys take the MSD sort (the LSD one after an overflow)
+  bool use_msd = true;               // 64-bit sort keys take the MSD sort (the tuner may pick the LSD one)
+  bool msd_overflowed = false;       // a run overflowed the MSD bucket list: the LSD passes from then on
+  float tuned_sort_ms = 0.f;         // the tuner's best sort time (diagnostics)
   DevBuf tkey, tsum, tcnt, overflow, skey, ssum, scnt, gcount, goff, okey, osum, ocnt;
   size_t out_slots = 0;              // capacity of okey / osum / ocnt
   // What the last run produced.
@@ -1106,6 +1108,7 @@ void batched_run(Batched* b, const Array* sort_keys, const Array* group_keys, co
     AM_HIP(hipStreamSynchronize(s));
     if (words[LC_OVERFLOW] || uint64_t(words[LC_LOCAL]) + words[LC_GROUPS] > b->msd.local_cap) {
       b->use_msd = false;
+      b->msd_overflowed = true;
       b->drop_graph();
       seg_sort(*b, sort_keys->data(), sort_keys->type, desc, forder, s);
     }
@@ -1154,15 +1157,27 @@ void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, c
     return best;
   };
   try {
-    if (sort_keys && !(type_width(sort_keys->type) == 8 && b->use_msd)) {   // the MSD sort has no tile size
+    if (sort_keys) {
+      // 64-bit keys: the MSD radix sort (no tile size) against the LSD passes at each tile size;
+      // narrower keys: the LSD tile sizes.
       int best_items = b->items;
       float best = 1e30f;
+      bool best_msd = false;
+      if (type_width(sort_keys->type) == 8 && !b->msd_overflowed) {
+        b->use_msd = true;
+        best = time_ms(sort_keys, nullptr, nullptr);
+        best_msd = true;
+      }
+      b->use_msd = false;
       for (int it : {8, 16, 20}) {
         build_tiles(*b, it);
         float t = time_ms(sort_keys, nullptr, nullptr);
-        if (t < best) { best = t; best_items = it; }
+        if (t < best) { best = t; best_items = it; best_msd = false; }
       }
       build_tiles(*b, best_items);
+      b->use_msd = best_msd;
+      b->msd_used = best_msd;
+      b->tuned_sort_ms = best;
     }
     if (group_keys) {
       // Smallest capacity without overflow (the flags read back after each trial).
@@ -1202,12 +1217,14 @@ void batched_tune(Batched* b, const Array* sort_keys, const Array* group_keys, c
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  b->drop_graph();   // a graph recorded before the choices above is re-recorded with them
   b->ran_sort = b->ran_group = false;
 }
 
 void batched_config(const Batched* b, int64_t* out, int count) {
+  // The sort path is the one the last recorded sort (a run, or the tuner's pick) used.
   const int64_t v[7] = {b->items, int64_t(b->cap), b->strategy, 8 /* digit bits */, int64_t(b->fallback_batches),
-                        int64_t(b->ntiles), (b->ran_sort && b->msd_used && b->use_msd) ? 1 : 0};
+                        int64_t(b->ntiles), (b->msd_used && b->use_msd) ? 1 : 0};
   for (int i = 0; i < count && i < 7; i++) out[i] = v[i];
 }
 

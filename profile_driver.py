@@ -8,7 +8,7 @@ FETCH_SIZE / WRITE_SIZE counters against a kernel of known traffic (read 1 GiB, 
 With --workload orderby it stops after the ORDER BY steps; with --workload batched it instead runs the batched plan (2,048 lognormal batches, ~191M rows: tune,
 then three graph replays of the per-batch argsort + group-by sum / count_all).
 
-Usage: python3 arrowmetal_amd/profile_driver.py [--rows N] [--workload orderby_groupby|batched]
+Usage: python3 profile_driver.py [--rows N] [--workload orderby_groupby|orderby|batched]
 """
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ import sys
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import arrowmetal_amd as am  # noqa: E402
