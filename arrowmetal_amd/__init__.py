@@ -90,6 +90,14 @@ _SIGNATURES = {
     "am_argsort_ex": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
     "am_argsort_ex2": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
     "am_group_by": ([_P, ctypes.c_int64, ctypes.c_int, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_sort": ([_P, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_sort_ex": ([_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_sort_ex2": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_top_k": ([_P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_top_k_ex": ([_P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_lexsort": ([ctypes.POINTER(_P), _P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_lexsort_ex": ([ctypes.POINTER(_P), _P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_lexsort_ex2": ([ctypes.POINTER(_P), _P, _P, _P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
     "am_batch_begin": ([], ctypes.c_int),
     "am_batch_end": ([], ctypes.c_int),
     "am_group_by_keys": ([ctypes.POINTER(_P), ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
@@ -330,6 +338,16 @@ class MetalArray:
         return _call(lib().am_argsort_ex2, self._h, 1 if descending else 0,
                      NULL_PLACEMENTS.index(null_placement), _float_order(float_order))
 
+    def sort(self, descending=False, null_placement="at_end", float_order="ieee"):
+        """The values in argsort's order (`am_sort_ex2`), same type; -0.0 and NaN bits kept."""
+        return _call(lib().am_sort_ex2, self._h, 1 if descending else 0, NULL_PLACEMENTS.index(null_placement),
+                     _float_order(float_order))
+
+    def top_k(self, k, largest=True, null_placement="at_end", float_order="ieee"):
+        """The first k indices argsort(descending=largest, ...) gives (`am_top_k_ex`)."""
+        return _call(lib().am_top_k_ex, self._h, int(k), 1 if largest else 0, NULL_PLACEMENTS.index(null_placement),
+                     _float_order(float_order))
+
     def group_by(self, key_count):
         """Dense-key group-by (`am_group_by`): this array holds keys in [0, key_count)."""
         return GroupBy(self, key_count)
@@ -439,6 +457,27 @@ def group_by(keys):
     if not isinstance(keys, (list, tuple)):
         keys = [keys]
     return GroupByKeys(keys)
+
+
+def _int_array(vals, count):
+    if vals is None:
+        return None
+    if len(vals) != count:
+        raise ValueError(f"{len(vals)} options for {count} columns")
+    return (ctypes.c_int * count)(*[int(v) for v in vals])
+
+
+def lexsort(columns, descending=None, null_placement=None, float_order=None):
+    """Stable multi-key argsort (`am_lexsort_ex2`): options per key, or None for the defaults."""
+    handles = (_P * len(columns))(*[c._h for c in columns])
+    n = len(columns)
+    place = None if null_placement is None else [NULL_PLACEMENTS.index(p) if isinstance(p, str) else p
+                                                  for p in null_placement]
+    orders = None if float_order is None else [_float_order(f) for f in float_order]
+    d, p, f = _int_array(descending, n), _int_array(place, n), _int_array(orders, n)
+    out = _P()
+    _check(lib().am_lexsort_ex2(handles, d, p, f, n, ctypes.byref(out)))
+    return MetalArray(out)
 
 
 def unique_rows(columns):

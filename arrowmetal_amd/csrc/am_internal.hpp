@@ -191,6 +191,11 @@ void argsort_device(const void* vals, int type, const uint8_t* validity, int64_t
 // take.hip
 Array* take(const Array& a, const Array& indices);
 
+// select.hip
+Array* top_k(const Array& a, int64_t k, bool largest, int placement, int forder);
+Array* sort_values(const Array& a, bool descending, int placement, int forder);
+Array* lexsort(Array* const* cols, int64_t count, const int* desc, const int* placement, const int* forder);
+
 // groupby.hip
 struct GroupBy;
 GroupBy* group_by_keys(Array* const* columns, int64_t count);

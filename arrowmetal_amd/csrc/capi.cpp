@@ -5,6 +5,7 @@
 // with no message; an operation error returns 1 and leaves its message in the thread-local
 // am_last_error(); success returns 0 and writes a new reference to `*out`.
 #include <cstring>
+#include <vector>
 
 #include "../../include/arrowmetal.h"
 #include "am_internal.hpp"
@@ -146,6 +147,76 @@ int am_argsort_ex2(am_array* a, int descending, int null_placement, int float_or
   return guarded([&] {
     if (float_order < 0 || float_order > 2) am::fail_invalid("unknown float_order " + std::to_string(float_order));
     *out = H(am::argsort(*A(a), descending != 0, null_placement == 1 ? 1 : 0, float_order));
+  });
+}
+
+namespace {
+void check_float_order(int f) {
+  if (f < 0 || f > 2) am::fail_invalid("unknown float_order " + std::to_string(f));
+}
+// Handles of `count` key columns, or nullptr when one is NULL.
+bool all_handles(am_array** columns, int64_t count) {
+  if (!columns || count <= 0) return false;
+  for (int64_t i = 0; i < count; i++)
+    if (!columns[i]) return false;
+  return true;
+}
+}  // namespace
+
+int am_sort(am_array* a, int descending, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::sort_values(*A(a), descending != 0, 0, 0)); });
+}
+
+int am_sort_ex(am_array* a, int descending, int null_placement, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::sort_values(*A(a), descending != 0, null_placement == 1 ? 1 : 0, 0)); });
+}
+
+int am_sort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] {
+    check_float_order(float_order);
+    *out = H(am::sort_values(*A(a), descending != 0, null_placement == 1 ? 1 : 0, float_order));
+  });
+}
+
+int am_top_k(am_array* a, int64_t k, int largest, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] { *out = H(am::top_k(*A(a), k, largest != 0, 0, 0)); });
+}
+
+int am_top_k_ex(am_array* a, int64_t k, int largest, int null_placement, int float_order, am_array** out) {
+  if (!a || !out) return 2;
+  return guarded([&] {
+    if (k < 0) am::fail_invalid("top_k needs k >= 0");
+    check_float_order(float_order);
+    *out = H(am::top_k(*A(a), k, largest != 0, null_placement == 1 ? 1 : 0, float_order));
+  });
+}
+
+int am_lexsort(am_array** columns, const int* descending, int64_t count, am_array** out) {
+  if (!all_handles(columns, count) || !out) return 2;
+  return guarded([&] {
+    *out = H(am::lexsort(reinterpret_cast<Array* const*>(columns), count, descending, nullptr, nullptr));
+  });
+}
+
+int am_lexsort_ex(am_array** columns, const int* descending, int64_t count, int null_placement, am_array** out) {
+  if (!all_handles(columns, count) || !out) return 2;
+  return guarded([&] {
+    std::vector<int> place(size_t(count), null_placement == 1 ? 1 : 0);
+    *out = H(am::lexsort(reinterpret_cast<Array* const*>(columns), count, descending, place.data(), nullptr));
+  });
+}
+
+int am_lexsort_ex2(am_array** columns, const int* descending, const int* null_placement, const int* float_order,
+                   int64_t count, am_array** out) {
+  if (!all_handles(columns, count) || !out) return 2;
+  return guarded([&] {
+    if (float_order)
+      for (int64_t i = 0; i < count; i++) check_float_order(float_order[i]);
+    *out = H(am::lexsort(reinterpret_cast<Array* const*>(columns), count, descending, null_placement, float_order));
   });
 }
 

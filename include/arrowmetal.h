@@ -68,6 +68,16 @@ int  am_slice(am_array* a, int64_t offset, int64_t length, am_array** out);
  * ascending or descending, nulls last, float order ieee. */
 int  am_argsort(am_array* a, int descending, am_array** out);
 
+/* reference include/arrowmetal.h:109-110 (ArrowMetalC.swift:400-415): am_sort is the values in
+ * am_argsort's order (same type, nulls last); am_top_k the first k indices of am_argsort with
+ * descending = largest (k < 0 is the error "top_k needs k >= 0"; k above the length gives them all). */
+int  am_sort(am_array* a, int descending, am_array** out);
+int  am_top_k(am_array* a, int64_t k, int largest, am_array** out);
+
+/* reference include/arrowmetal.h:458 (ArrowMetalC_Window.swift:106-118): stable multi-key argsort,
+ * descending per key (NULL: all ascending), nulls last, float order ieee. */
+int  am_lexsort(am_array** columns, const int* descending, int64_t count, am_array** out);
+
 /* reference include/arrowmetal.h:113 (ArrowMetalC.swift:416-469): dense keys int32/int64/uint32 in
  * [0, key_count); null and out-of-range keys are skipped. agg: 0 sum, 1 count(rows), 2 min, 3 max,
  * 4 mean, 5 count(values). sum and mean take integer values (int64 / uint64 / float64 out); sum also
@@ -108,6 +118,25 @@ int  am_group_agg_ex(am_groupby* gb, am_array* values, int op, double p1, am_arr
  * ieee, 1 total, 2 nan_largest, any other value the error "unknown float_order N" (:52-59). */
 int  am_argsort_ex(am_array* a, int descending, int null_placement, am_array** out);
 int  am_argsort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out);
+
+/* reference include/arrowmetal.h:1332-1333, 1349-1354 (ArrowMetalC_Options.swift:126-228):
+ *   am_sort_ex / am_sort_ex2   the values in am_argsort_ex / am_argsort_ex2's order, the input's type,
+ *                              the null block where null_placement puts it; -0.0 and NaN payload bits
+ *                              come out unchanged.
+ *   am_top_k_ex                the first k indices am_argsort_ex2 gives with the same options
+ *                              (largest = descending); k < 0 is the error "top_k needs k >= 0".
+ *   am_lexsort_ex              stable multi-key argsort, one null_placement for every key.
+ *   am_lexsort_ex2             each of descending / null_placement / float_order holds `count` entries,
+ *                              or is NULL for the defaults (ascending, at_end, ieee).
+ * Index results are uint32 ("I"); more than 2^32 - 1 rows is the error "<op>: row R does not fit the
+ * UInt32 index type (...)" with op "top_k" or "lexsort". */
+int  am_sort_ex(am_array* a, int descending, int null_placement, am_array** out);
+int  am_sort_ex2(am_array* a, int descending, int null_placement, int float_order, am_array** out);
+int  am_top_k_ex(am_array* a, int64_t k, int largest, int null_placement, int float_order, am_array** out);
+int  am_lexsort_ex(am_array** columns, const int* descending /* or NULL */, int64_t count, int null_placement,
+                   am_array** out);
+int  am_lexsort_ex2(am_array** columns, const int* descending /* or NULL */, const int* null_placement /* or NULL */,
+                    const int* float_order /* or NULL */, int64_t count, am_array** out);
 
 /* reference include/arrowmetal.h:1359 (ArrowMetalC_Options.swift:284-297,
  * Kernels/UniqueOrder.swift:117-125, Kernels/Unique.swift:46-51): Arrow `unique` of a numeric

@@ -12,6 +12,9 @@
  *                     Tests/ArrowMetalTests/SortOptionsTests.swift:25-38 (the reference permutation).
  *                     The sort itself is an 8-bit-digit stable LSD radix sort, the reference's own
  *                     algorithm (SortSource.swift:3-17), run on the CPU.
+ *   or_lexsort        Sources/ArrowMetal/Kernels/MultiSort.swift:28-65 (stable argsort of the least
+ *                     significant key first, each more significant key sorted stably in the order the
+ *                     earlier passes left; options per key).
  *   or_take           Sources/ArrowMetal/Kernels/Take.swift:41-80, KernelSource.swift:209-227.
  *   or_group_keys     Sources/ArrowMetal/Kernels/GroupByKeys.swift:49-58, 170-236, 387-413
  *                     (ascending key order, null group last), HashTable.swift:290-302 (key
@@ -175,6 +178,34 @@ int or_argsort(const void* vals, int t, const uint8_t* validity, int64_t n,
         for (int64_t i = 0; i < z; i++) out[o++] = null_rows[i];
     }
     free(v); free(nan_rows); free(null_rows);
+    return 0;
+}
+
+/*
+ * Multi-key stable argsort (MultiSort.swift:28-65). cols[c] / validity[c] / types[c] describe key c
+ * (validity bitmaps or NULL), desc / placement / forder hold one option per key. Output: n rows.
+ */
+int or_lexsort(int ncols, const int* types, const void* const* cols, const uint8_t* const* validity,
+               int64_t n, const int* desc, const int* placement, const int* forder, uint32_t* out) {
+    if (n == 0) return 0;
+    uint32_t* perm = (uint32_t*)malloc((size_t)n * 4);
+    uint32_t* idx = (uint32_t*)malloc((size_t)n * 4);
+    uint8_t* gathered = (uint8_t*)malloc((size_t)n * 8);
+    uint8_t* gvalid = (uint8_t*)malloc((size_t)(n + 7) / 8);
+    for (int64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+    for (int c = ncols - 1; c >= 0; c--) {
+        int w = or_width(types[c]);
+        /* the key column in the order the passes so far left (perm), with its validity */
+        memset(gvalid, 0, (size_t)(n + 7) / 8);
+        for (int64_t i = 0; i < n; i++) {
+            memcpy(gathered + i * w, (const char*)cols[c] + (int64_t)perm[i] * w, (size_t)w);
+            if (!validity[c] || bit_get(validity[c], perm[i])) gvalid[i >> 3] |= (uint8_t)(1u << (i & 7));
+        }
+        or_argsort(gathered, types[c], validity[c] ? gvalid : NULL, n, desc[c], placement[c], forder[c], idx);
+        for (int64_t i = 0; i < n; i++) out[i] = perm[idx[i]];
+        memcpy(perm, out, (size_t)n * 4);
+    }
+    free(perm); free(idx); free(gathered); free(gvalid);
     return 0;
 }
 

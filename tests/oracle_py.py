@@ -42,6 +42,7 @@ def lib():
         L.or_unique_values.restype = i64
         L.or_group_agg.argtypes = [P, i64, i64, P, c_int, P, c_int, P, P]
         L.or_group_by_dense.argtypes = [P, c_int, P, i64, i64, P, c_int, P, c_int, P, P]
+        L.or_lexsort.argtypes = [c_int, P, P, P, i64, P, P, P, P]
         L.or_unique_rows.argtypes = [c_int, P, P, P, i64, P]
         L.or_unique_rows.restype = i64
         _lib = L
@@ -72,6 +73,29 @@ def argsort(values: np.ndarray, valid=None, descending=False, null_placement=0, 
     lib().or_argsort(_ptr(values), TYPE_OF[values.dtype], _ptr(bm), len(values), int(descending),
                      int(null_placement), int(float_order), _ptr(out))
     return out
+
+
+def top_k(values, k, valid=None, largest=True, null_placement=0, float_order=0):
+    """The first k rows of the stable argsort with descending = largest (the reference's definition,
+    Kernels/Sort.swift:95-112)."""
+    return argsort(values, valid, largest, null_placement, float_order)[:max(k, 0)]
+
+
+def lexsort(columns, valids=None, descending=None, null_placement=None, float_order=None):
+    """Multi-key stable argsort with options per key (MultiSort.swift:28-65)."""
+    columns = [np.ascontiguousarray(c) for c in columns]
+    nc, n = len(columns), len(columns[0])
+    valids = valids or [None] * nc
+    bms = [pack_validity(v) for v in valids]
+    opt = lambda v: np.ascontiguousarray([0] * nc if v is None else [int(x) for x in v], dtype=np.int32)
+    types = np.array([TYPE_OF[c.dtype] for c in columns], dtype=np.int32)
+    vptrs = (ctypes.c_void_p * nc)(*[c.ctypes.data for c in columns])
+    bptrs = (ctypes.c_void_p * nc)(*[0 if b is None else b.ctypes.data for b in bms])
+    d, p, f = opt(descending), opt(null_placement), opt(float_order)
+    out = np.empty(max(n, 1), dtype=np.uint32)
+    lib().or_lexsort(nc, _ptr(types), ctypes.cast(vptrs, ctypes.c_void_p), ctypes.cast(bptrs, ctypes.c_void_p),
+                     n, _ptr(d), _ptr(p), _ptr(f), _ptr(out))
+    return out[:n].copy()
 
 
 def take(values: np.ndarray, indices: np.ndarray, valid=None, idx_valid=None):

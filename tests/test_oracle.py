@@ -479,3 +479,85 @@ def test_unique_values_matches_arrow_and_numpy(n, span, null_frac):
     rows1 = O.unique_values(vals, valid, 1)
     assert vals[rows1].tolist() == np.unique(vals[valid]).tolist()
     assert all(valid[rows1])
+
+
+# ---------------------------------------------------------------- lexsort and top-k
+
+
+def _dense_ranks(col, valid, desc, place, forder):
+    """Rank of every row in its column's own stable order, equal keys sharing a rank. Equality comes
+    from the (pinned) single-key oracle itself: in the sorted order, a row ties the one before it
+    exactly when sorting the pair the other way round keeps it first (nothing strictly smaller)."""
+    order = O.argsort(col, valid, desc, place, forder)
+    rank = np.empty(len(col), dtype=np.int64)
+    r = 0
+    for i, row in enumerate(order):
+        if i:
+            prev = order[i - 1]
+            pair = np.array([col[row], col[prev]], dtype=col.dtype)
+            pv = None if valid is None else np.array([valid[row], valid[prev]])
+            if O.argsort(pair, pv, desc, place, forder)[0] == 1:
+                r += 1
+        rank[row] = r
+    return rank
+
+
+def _lex_columns(rng, n):
+    a = rng.integers(-3, 4, n).astype(np.int64)
+    b = rng.choice(np.array([-0.0, 0.0, 1.5, -2.0, np.nan, np.inf, -np.inf], dtype=np.float64), n)
+    b[rng.random(n) < 0.05] = np.frombuffer(np.uint64(0xFFF8000000000123).tobytes(), dtype=np.float64)[0]
+    c = rng.integers(0, 5, n).astype(np.uint8)
+    d = rng.integers(-2, 3, n).astype(np.int32)
+    valids = [rng.random(n) > 0.15, rng.random(n) > 0.2, None, rng.random(n) > 0.1]
+    return [a, b, c, d], valids
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_lexsort_brute_force_per_key_options(seed):
+    """The definition of lexicographic order: rows ordered by (rank in key 0, rank in key 1, ...,
+    row), each rank from that key's own pinned single-key order with its own options."""
+    rng = np.random.default_rng(seed)
+    n = 400
+    cols, valids = _lex_columns(rng, n)
+    k = 2 + seed % 3
+    pick = rng.permutation(4)[:k]
+    cols = [cols[i] for i in pick]
+    valids = [valids[i] for i in pick]
+    desc = rng.integers(0, 2, k).tolist()
+    place = rng.integers(0, 2, k).tolist()
+    forder = rng.integers(0, 3, k).tolist()
+    ranks = [_dense_ranks(c, v, d, p, f) for c, v, d, p, f in zip(cols, valids, desc, place, forder)]
+    want = sorted(range(n), key=lambda i: tuple(r[i] for r in ranks) + (i,))
+    got = O.lexsort(cols, valids, desc, place, forder)
+    np.testing.assert_array_equal(got, np.array(want, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("placement", ["at_end", "at_start"])
+def test_lexsort_matches_arrow_multi_key_sort(placement):
+    """Arrow C++ sort_indices over several keys (one null_placement for every key, ieee floats) is
+    the reference's am_lexsort_ex order (MultiSort.swift:9-13)."""
+    pa = pytest.importorskip("pyarrow")
+    pc = pytest.importorskip("pyarrow.compute")
+    rng = np.random.default_rng(11)
+    n = 3000
+    cols, valids = _lex_columns(rng, n)
+    desc = [0, 1, 1, 0]
+    t = pa.table({f"c{i}": pa.array(c, mask=None if v is None else ~v) for i, (c, v) in enumerate(zip(cols, valids))})
+    keys = [(f"c{i}", "descending" if d else "ascending") for i, d in enumerate(desc)]
+    want = pc.sort_indices(t, sort_keys=keys, null_placement=placement).to_numpy()
+    p = 1 if placement == "at_start" else 0
+    got = O.lexsort(cols, valids, desc, [p] * 4, [0] * 4)
+    np.testing.assert_array_equal(got.astype(np.uint64), want)
+
+
+def test_lexsort_planted_mistake_is_caught():
+    """Sorting the keys most significant first (the wrong way round) fails the brute-force order."""
+    rng = np.random.default_rng(3)
+    n = 300
+    a = rng.integers(0, 3, n).astype(np.int64)
+    b = rng.integers(0, 3, n).astype(np.int64)
+    ra, rb = _dense_ranks(a, None, 0, 0, 0), _dense_ranks(b, None, 0, 0, 0)
+    want = np.array(sorted(range(n), key=lambda i: (ra[i], rb[i], i)), dtype=np.uint32)
+    wrong = O.lexsort([b, a])   # keys swapped: b most significant
+    assert not np.array_equal(wrong, want)
+    np.testing.assert_array_equal(O.lexsort([a, b]), want)
