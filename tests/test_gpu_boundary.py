@@ -134,8 +134,11 @@ def test_dense_group_by_float_value_rules():
     # min / max of float32 still answer (-0.0 below +0.0, NaN skipped).
     fm = np.array([-0.0, 0.0, np.nan, 1.5] * 10, dtype=np.float32)
     km = am.MetalArray.from_numpy((np.arange(40) % 2).astype(np.int32))
+    # Group 0 holds -0.0 and NaN, group 1 holds +0.0 and 1.5.
     lo, _ = km.group_by(2).min(am.MetalArray.from_numpy(fm)).to_numpy()
-    assert lo.view(np.uint32).tolist() == [0x80000000, 0x3FC00000]
+    assert lo.view(np.uint32).tolist() == [0x80000000, 0x00000000]
+    hi, _ = km.group_by(2).max(am.MetalArray.from_numpy(fm)).to_numpy()
+    assert hi.view(np.uint32).tolist() == [0x80000000, 0x3FC00000]
 
 
 def test_integer_mean_is_double_sum_over_double_count():
