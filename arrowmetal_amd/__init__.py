@@ -123,6 +123,17 @@ _SIGNATURES = {
                         ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "am_server_stats": ([_P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
     "am_server_release": ([_P], None),
+    "am_plan_source_create": ([ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int64,
+                               ctypes.POINTER(_P)], ctypes.c_int),
+    "am_plan_source_release": ([_P], None),
+    "am_plan_run": ([ctypes.c_char_p, ctypes.POINTER(_P), ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)],
+                    ctypes.c_int),
+    "am_plan_explain": ([ctypes.c_char_p, ctypes.POINTER(_P), ctypes.c_int64, ctypes.c_int], ctypes.c_char_p),
+    "am_plan_column_count": ([_P], ctypes.c_int64),
+    "am_plan_row_count": ([_P], ctypes.c_int64),
+    "am_plan_column_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
+    "am_plan_column": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_plan_result_release": ([_P], None),
     "am_set_max_row_index": ([ctypes.c_int64], ctypes.c_int64),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
@@ -486,6 +497,66 @@ def unique_rows(columns):
     out = _P()
     _check(lib().am_unique_rows(handles, len(columns), ctypes.byref(out)))
     return MetalArray(out)
+
+
+class PlanSource:
+    """A registered table the plan engine scans by name (`am_plan_source_create`; reference
+    python/arrowmetal/lazy.py:650-680). `columns` maps names to MetalArray (or NumPy arrays, imported
+    once)."""
+
+    def __init__(self, name, columns):
+        self._h = None
+        self.name = name
+        self._cols = {k: (v if isinstance(v, MetalArray) else MetalArray.from_numpy(v)) for k, v in columns.items()}
+        names = list(self._cols)
+        handles = (_P * max(len(names), 1))(*[self._cols[k]._h for k in names])
+        cnames = (ctypes.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+        out = _P()
+        _check(lib().am_plan_source_create(name.encode(), handles, cnames, len(names), ctypes.byref(out)))
+        self._h = out
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.am_plan_source_release(h)
+
+
+def _plan_text(plan):
+    import json
+    return plan if isinstance(plan, str) else json.dumps(plan)
+
+
+def _source_array(sources):
+    sources = list(sources)
+    return (_P * max(len(sources), 1))(*[s._h for s in sources]), len(sources)
+
+
+def plan_run(plan, sources, optimize=True):
+    """Runs a JSON plan (a dict or its text) with `am_plan_run`; returns the result as an ordered dict
+    of column name -> MetalArray."""
+    arr, n = _source_array(sources)
+    out = _P()
+    _check(lib().am_plan_run(_plan_text(plan).encode(), arr, n, 1 if optimize else 0, ctypes.byref(out)))
+    try:
+        L = lib()
+        cols = {}
+        for i in range(L.am_plan_column_count(out)):
+            name = L.am_plan_column_name(out, i).decode()
+            h = _P()
+            _check(L.am_plan_column(out, i, ctypes.byref(h)))
+            cols[name] = MetalArray(h)
+    finally:
+        lib().am_plan_result_release(out)
+    return cols
+
+
+def plan_explain(plan, sources=(), optimize=True):
+    """The physical plan `am_plan_explain` prints (raises with am_last_error() when it does not parse)."""
+    arr, n = _source_array(sources)
+    r = lib().am_plan_explain(_plan_text(plan).encode(), arr, n, 1 if optimize else 0)
+    if r is None:
+        raise ArrowMetalError((lib().am_last_error() or b"unknown error").decode())
+    return r.decode()
 
 
 class Batched:

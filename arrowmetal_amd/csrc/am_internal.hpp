@@ -7,6 +7,9 @@
 //                 kernel per 8-bit digit with a decoupled look-back) and its key maps
 //   take.hip      the gather behind am_take
 //   groupby.hip   the hash group-by: key table build, ascending relabel, grouped aggregates
+//   select.hip    sorted values, top-k by radix select, lexsort
+//   plan.cpp      the plan engine's scan / sort / group_by / unique / limit subset (am_plan_*)
+//   batched.hip   the batched plan (am_batched_*); serve.cpp the request server (am_server_*)
 //   capi.cpp      the extern "C" am_* entry points of include/arrowmetal.h
 #pragma once
 
@@ -188,6 +191,14 @@ void argsort_device(const void* vals, int type, const uint8_t* validity, int64_t
                     bool descending, int null_placement, int float_order, uint32_t* out_idx,
                     hipStream_t s);
 
+// ORDER BY of a whole table by one 64-bit key (int64 / uint64 / float64): every column of `cols`
+// (each of n rows) in the key's stable order with am_argsort_ex2's options, as new arrays in `out`.
+// The columns move with the hybrid sort's top-digit partition instead of one gather across the table.
+// Returns false, with nothing in `out`, when the key does not take the hybrid sort (fewer than 2^18
+// rows, a constant top digit, a full bucket list, a graph capture); the caller then sorts and takes.
+bool order_by_columns(const Array& key, bool descending, int null_placement, int float_order,
+                      const std::vector<const Array*>& cols, std::vector<Array*>& out);
+
 // take.hip
 Array* take(const Array& a, const Array& indices);
 
@@ -207,6 +218,19 @@ void group_by_release(GroupBy* g);
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values);
 Array* unique_rows(Array* const* columns, int64_t count);
 Array* unique_values(const Array& a, int order);
+
+// plan.cpp: the plan engine's scan / sort / group_by / unique / limit subset.
+struct PlanSource;
+struct PlanResult;
+PlanSource* plan_source_create(const char* name, Array* const* cols, const char* const* names, int64_t n);
+void plan_source_release(PlanSource* s);
+PlanResult* plan_run(const char* text, PlanSource* const* sources, int64_t n, int optimize);
+const char* plan_explain(const char* text, PlanSource* const* sources, int64_t n, int optimize);
+int64_t plan_column_count(const PlanResult* r);
+int64_t plan_row_count(const PlanResult* r);
+const char* plan_column_name(const PlanResult* r, int64_t i);
+Array* plan_column(const PlanResult* r, int64_t i);
+void plan_result_release(PlanResult* r);
 
 // Launch-shape parameters the autotuner chooses (runtime.cpp keeps the table).
 struct SortTune {

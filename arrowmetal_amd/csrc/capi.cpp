@@ -356,6 +356,69 @@ int am_unique_rows(am_array** columns, int64_t count, am_array** out) {
   return guarded([&] { *out = H(am::unique_rows(reinterpret_cast<Array* const*>(columns), count)); });
 }
 
+// ---- the plan engine (reference Sources/ArrowMetalC/ArrowMetalC_Plan.swift)
+
+int am_plan_source_create(const char* name, am_array** columns, const char** names, int64_t n_columns,
+                          am_plan_source** out) {
+  if (!name || !columns || !names || !out || n_columns < 0) return 2;
+  for (int64_t i = 0; i < n_columns; i++)
+    if (!columns[i] || !names[i]) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_plan_source*>(
+        am::plan_source_create(name, reinterpret_cast<Array* const*>(columns), names, n_columns));
+  });
+}
+
+void am_plan_source_release(am_plan_source* s) {
+  if (s) am::plan_source_release(reinterpret_cast<am::PlanSource*>(s));
+}
+
+int am_plan_run(const char* plan_json, am_plan_source** sources, int64_t n_sources, int optimize,
+                am_plan_result** out) {
+  if (!plan_json || !out || n_sources < 0 || (!sources && n_sources > 0)) return 2;
+  for (int64_t i = 0; i < n_sources; i++)
+    if (!sources[i]) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_plan_result*>(am::plan_run(
+        plan_json, reinterpret_cast<am::PlanSource* const*>(sources), n_sources, optimize));
+  });
+}
+
+const char* am_plan_explain(const char* plan_json, am_plan_source** sources, int64_t n_sources, int optimize) {
+  if (!plan_json || n_sources < 0 || (!sources && n_sources > 0)) return nullptr;
+  for (int64_t i = 0; i < n_sources; i++)
+    if (!sources[i]) return nullptr;
+  const char* text = nullptr;
+  if (guarded([&] {
+        text = am::plan_explain(plan_json, reinterpret_cast<am::PlanSource* const*>(sources), n_sources, optimize);
+      }) != 0)
+    return nullptr;
+  return text;
+}
+
+int64_t am_plan_column_count(am_plan_result* r) {
+  return r ? am::plan_column_count(reinterpret_cast<am::PlanResult*>(r)) : -1;
+}
+
+int64_t am_plan_row_count(am_plan_result* r) {
+  return r ? am::plan_row_count(reinterpret_cast<am::PlanResult*>(r)) : -1;
+}
+
+const char* am_plan_column_name(am_plan_result* r, int64_t i) {
+  if (!r || i < 0 || i >= am::plan_column_count(reinterpret_cast<am::PlanResult*>(r))) return nullptr;
+  return am::plan_column_name(reinterpret_cast<am::PlanResult*>(r), i);
+}
+
+int am_plan_column(am_plan_result* r, int64_t i, am_array** out) {
+  if (!r || !out || i < 0 || i >= am::plan_column_count(reinterpret_cast<am::PlanResult*>(r))) return 2;
+  *out = H(am::plan_column(reinterpret_cast<am::PlanResult*>(r), i));
+  return 0;
+}
+
+void am_plan_result_release(am_plan_result* r) {
+  if (r) am::plan_result_release(reinterpret_cast<am::PlanResult*>(r));
+}
+
 int64_t am_set_max_row_index(int64_t max_row) { return am::set_max_row_index(max_row); }
 
 int am_prof_enable(int on) {

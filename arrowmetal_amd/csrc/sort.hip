@@ -428,12 +428,15 @@ __global__ __launch_bounds__(BLOCK) void msd_unit_scan(const SortCtl* ctl, const
   }
 }
 
-template <typename K, int W, int ITEMS>
+// POS_ROWS: the rows written beside the keys are their own positions in this pass's output, and
+// pos_of_row[row] gets that position (all ones for NaN and null rows), for the ORDER BY of whole
+// tables (order_by_columns).
+template <typename K, int W, int ITEMS, bool POS_ROWS = false>
 __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uint8_t* valid, int64_t voff,
                                                      uint32_t n, int type, int forder, int desc, SortCtl* ctl,
                                                      const uint32_t* unit_base, uint32_t nunits,
                                                      uint32_t unit_shift, K* out_keys, uint32_t* out_rows,
-                                                     uint32_t* out_idx) {
+                                                     uint32_t* out_idx, uint32_t* pos_of_row = nullptr) {
   constexpr int P = int(sizeof(K));
   constexpr int TILE = BLOCK * ITEMS;
   static_assert((1u << UNIT_SHIFT_SMALL) % TILE == 0, "units hold whole tiles");
@@ -500,11 +503,13 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 #pragma unroll
       for (int j = 0; j < ITEMS; j++) {   // NaN and null rows go straight to their final slots
         const uint32_t b = bin_of(bins[j]);
-        if (b == BIN_NAN || b == BIN_NULL)
+        if (b == BIN_NAN || b == BIN_NULL) {
           out_idx[bin_base[b] + wc[w][b] + rank_of(bins[j])] = base_e + uint32_t(j) * 64u;
+          if constexpr (POS_ROWS) pos_of_row[base_e + uint32_t(j) * 64u] = 0xFFFFFFFFu;
+        }
       }
-      scatter_values<K, ITEMS, true, true>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage, vcount,
-                                           out_keys, out_rows, base_e);
+      scatter_values<K, ITEMS, true, true, POS_ROWS>(keys, rows, bins, wc[w], bin_start, bin_base, s_digit, s_stage,
+                                                     vcount, out_keys, out_rows, base_e, pos_of_row);
       __syncthreads();
     }
   }
@@ -553,7 +558,7 @@ template <typename K, int W, int FITEMS, int ITEMS>
 bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
                 SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, uint32_t mask, K* k0,
                 uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base,
-                uint32_t nunits, uint32_t unit_shift, hipStream_t s) {
+                uint32_t nunits, uint32_t unit_shift, hipStream_t s, uint32_t* pos_of_row) {
   constexpr int P = int(sizeof(K));
   const MsdWork w = msd_work(m, ITEMS, device_alloc);
   {
@@ -564,8 +569,12 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   {
     ProfScope ps("sort_msd_first");
     const uint32_t workers = std::min(nunits, uint32_t(device().cu_count) * 3u);
-    hipLaunchKernelGGL((msd_part<K, W, FITEMS>), dim3(workers), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
-                       dsc, ctl, unit_base, nunits, unit_shift, k0, r0, out_idx);
+    if (pos_of_row)
+      hipLaunchKernelGGL((msd_part<K, W, FITEMS, true>), dim3(workers), dim3(BLOCK), 0, s, vals, valid, voff, n, type,
+                         forder, dsc, ctl, unit_base, nunits, unit_shift, k0, r0, out_idx, pos_of_row);
+    else
+      hipLaunchKernelGGL((msd_part<K, W, FITEMS>), dim3(workers), dim3(BLOCK), 0, s, vals, valid, voff, n, type, forder,
+                         dsc, ctl, unit_base, nunits, unit_shift, k0, r0, out_idx, nullptr);
     AM_HIP(hipGetLastError());
   }
   {
@@ -577,10 +586,15 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   return msd_levels<K, ITEMS>(w, m, P - 1, mask, false, LOCAL_MAX, status, k0, r0, k1, r1, out_idx, s);
 }
 
+// With pos_of_row (the ORDER BY of whole tables): only the hybrid sort runs, with positions for rows
+// (msd_part's POS_ROWS form); when it cannot (too few rows, a constant top digit, a full bucket list)
+// the call returns false and out_idx holds nothing to use. *value_rows receives m.
 template <typename K, int W, int ITEMS>
-void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
-              int placement, int forder, uint32_t* out_idx, hipStream_t s) {
+bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
+              int placement, int forder, uint32_t* out_idx, hipStream_t s, uint32_t* pos_of_row = nullptr,
+              uint32_t* value_rows = nullptr) {
   constexpr int P = int(sizeof(K));
+  if (pos_of_row && !(P == 8 && n >= (1u << 18) && !capturing(s))) return false;
   constexpr int TILE = BLOCK * ITEMS;
   // The first pass keeps at most 16 keys per thread: its raw-column loads and 258-bin classification
   // need more registers, and 16 keeps it at three workgroups per CU.
@@ -654,12 +668,14 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   if constexpr (P == 8) {
     // The hybrid sort when the top digit varies; the LSD passes below run instead if its bucket list
     // runs out of room.
+    if (value_rows) *value_rows = plan.m;
     if (hybrid_ok && analyse && ((plan.mask >> (P - 1)) & 1u) &&
         run_hybrid<K, W, AM_MSD_FITEMS, ITEMS>(
             vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, plan.mask, kp0, rp0, kp1, rp1,
-            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, unit_shift, s))
-      return;
+            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, unit_shift, s, pos_of_row))
+      return true;
   }
+  if (pos_of_row) return false;
   if (hybrid_ok) {
     // The LSD passes need every digit's histogram: the analysis again, from a cleared control block
     // (the plan read back above stays the same).
@@ -701,6 +717,7 @@ void run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
       AM_HIP(hipGetLastError());
     }
   }
+  return true;
 }
 
 template <typename K, int W>
@@ -713,6 +730,159 @@ void dispatch_items(int items, const void* vals, int type, const uint8_t* valid,
     case 20: run_sort<K, W, 20>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
     default: run_sort<K, W, 16>(vals, type, valid, voff, n, desc, placement, forder, out_idx, s); break;
   }
+}
+
+// ---------------------------------------------------------------- ORDER BY of whole tables
+//
+// order_by_columns sorts by one 64-bit key with the hybrid sort in its POS_ROWS form: rows carried
+// through the levels are positions p in the top-digit partition's output (the value block), and
+// pos_of_row[r] = p for every value row r. A column then moves in two row passes:
+//   ob_partition  part[pos_of_row[r]] = col[r]: the column laid out as the partition laid out its keys
+//                 (reads in row order; writes in runs, as the partition's own)
+//   ob_gather     out[i] = part[out_idx[i]] in the value block, col[out_idx[i]] in the NaN and null
+//                 blocks (whose rows keep input order)
+// Output i of the value block and its position p lie in the same top-digit bucket, so the gather of an
+// output range reads one bucket's part of the column. Each XCD takes its own eighth of the outputs
+// (XCD-private chunk counters; a workgroup that finds its eighth done takes chunks of the others), so a
+// bucket's lines are fetched into one L2, not eight.
+
+constexpr uint32_t OB_ROWS = 4;          // rows per thread per step of the partition
+constexpr uint32_t OB_CHUNK = 4096;      // outputs per chunk of the gather (64-row words)
+constexpr uint32_t OB_GROUPS = OB_CHUNK / 64 / WAVES;   // 64-row groups per wave per chunk
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void ob_partition(const void* col, const uint8_t* valid, int64_t voff,
+                                                      uint32_t n, const uint32_t* pos_of_row, void* part,
+                                                      uint8_t* vpart) {
+  const uint32_t stride = gridDim.x * BLOCK * OB_ROWS;
+  for (uint32_t base = blockIdx.x * BLOCK * OB_ROWS + threadIdx.x; base < n; base += stride) {
+    uint64_t v[OB_ROWS];
+    uint32_t p[OB_ROWS], vb[OB_ROWS], bit[OB_ROWS];
+#pragma unroll
+    for (uint32_t u = 0; u < OB_ROWS; u++) {   // every load first (rows past the end read the last row)
+      const uint32_t r = min(base + u * BLOCK, n - 1u);
+      p[u] = pos_of_row[r];
+      v[u] = load_raw<W>(col, r);
+      vb[u] = load_valid_byte(valid, voff, r, bit[u]);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < OB_ROWS; u++) {
+      if (base + u * BLOCK >= n || p[u] == 0xFFFFFFFFu) continue;
+      if constexpr (W == 8) static_cast<uint64_t*>(part)[p[u]] = v[u];
+      else if constexpr (W == 4) static_cast<uint32_t*>(part)[p[u]] = uint32_t(v[u]);
+      else if constexpr (W == 2) static_cast<uint16_t*>(part)[p[u]] = uint16_t(v[u]);
+      else static_cast<uint8_t*>(part)[p[u]] = uint8_t(v[u]);
+      if (vpart) vpart[p[u]] = uint8_t((vb[u] >> bit[u]) & 1u);
+    }
+  }
+}
+
+// Counters of the gather: [x * 32] the next chunk of XCD x's eighth (each on its own 128-byte line).
+constexpr uint32_t OB_CTR_STRIDE = 32;
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void ob_gather(const uint32_t* out_idx, uint32_t n, uint32_t vbase, uint32_t m,
+                                                   const void* part, const uint8_t* vpart, const void* col,
+                                                   const uint8_t* valid, int64_t voff, void* out,
+                                                   unsigned long long* out_valid, uint32_t* ctr) {
+  __shared__ uint32_t s_chunk;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t words = (n + 63u) / 64u;
+  // Eighth x of the outputs: 64-row words [x * words / 8, (x + 1) * words / 8).
+  const uint32_t home = xcd_id();
+  for (uint32_t k = 0; k < uint32_t(NXCD); k++) {
+    const uint32_t x = (home + k) & uint32_t(NXCD - 1);
+    const uint32_t w0 = uint32_t(uint64_t(words) * x / NXCD), w1 = uint32_t(uint64_t(words) * (x + 1) / NXCD);
+    const uint32_t chunks = (w1 - w0 + OB_CHUNK / 64 - 1) / (OB_CHUNK / 64);
+    while (true) {
+      if (threadIdx.x == 0) s_chunk = atomicAdd(&ctr[x * OB_CTR_STRIDE], 1u);
+      __syncthreads();
+      const uint32_t c = s_chunk;
+      __syncthreads();
+      if (c >= chunks) break;   // uniform: this eighth has no chunk left
+      uint32_t idx[OB_GROUPS];
+      bool in_val[OB_GROUPS];
+#pragma unroll
+      for (uint32_t g = 0; g < OB_GROUPS; g++) {
+        const uint32_t word = w0 + c * (OB_CHUNK / 64) + uint32_t(w) * OB_GROUPS + g;
+        const uint32_t i = min(word * 64u + uint32_t(lane), n - 1u);
+        in_val[g] = i - vbase < m;
+        // Positions index the m-row partition, rows the n-row column (clamped: reads stay in bounds).
+        idx[g] = min(out_idx[i], (in_val[g] ? m : n) - 1u);
+      }
+      uint64_t v[OB_GROUPS];
+      uint32_t vb[OB_GROUPS], bit[OB_GROUPS];
+#pragma unroll
+      for (uint32_t g = 0; g < OB_GROUPS; g++) {
+        // A value-block output reads the partitioned column at its position, any other the column at
+        // its row.
+        const void* src = in_val[g] ? part : col;
+        v[g] = load_raw<W>(src, idx[g]);
+        if (in_val[g]) {
+          vb[g] = vpart ? vpart[idx[g]] : 1u;
+          bit[g] = 0u;
+        } else {
+          vb[g] = load_valid_byte(valid, voff, idx[g], bit[g]);
+        }
+      }
+#pragma unroll
+      for (uint32_t g = 0; g < OB_GROUPS; g++) {
+        const uint32_t word = w0 + c * (OB_CHUNK / 64) + uint32_t(w) * OB_GROUPS + g;
+        const uint32_t i = word * 64u + uint32_t(lane);
+        const bool live = word < w1 && i < n;
+        if (live) {
+          if constexpr (W == 8) static_cast<uint64_t*>(out)[i] = v[g];
+          else if constexpr (W == 4) static_cast<uint32_t*>(out)[i] = uint32_t(v[g]);
+          else if constexpr (W == 2) static_cast<uint16_t*>(out)[i] = uint16_t(v[g]);
+          else static_cast<uint8_t*>(out)[i] = uint8_t(v[g]);
+        }
+        if (out_valid) {
+          const unsigned long long bits = __ballot(live && ((vb[g] >> bit[g]) & 1u));
+          if (lane == 0 && word < w1) out_valid[word] = bits;
+        }
+      }
+    }
+  }
+}
+
+__global__ void ob_clear(uint32_t* ctr) {
+  if (threadIdx.x < uint32_t(NXCD * OB_CTR_STRIDE)) ctr[threadIdx.x] = 0u;
+}
+
+template <int W>
+void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, const uint32_t* out_idx, uint32_t n,
+                    uint32_t vbase, uint32_t m, Array* out, uint32_t* ctr, hipStream_t s) {
+  DevBuf part = device_alloc(size_t(std::max<uint32_t>(m, 1)) * W);
+  DevBuf vpart;
+  if (nulls) vpart = device_alloc(std::max<uint32_t>(m, 1));
+  retain(part);
+  if (vpart) retain(vpart);
+  const uint8_t* valid = nulls ? c.valid_bits() : nullptr;
+  {
+    ProfScope ps("order_by_partition");
+    const uint32_t blocks = std::max(1u, std::min(uint32_t(device().cu_count) * 8u, (n + BLOCK * OB_ROWS - 1) / (BLOCK * OB_ROWS)));
+    hipLaunchKernelGGL((ob_partition<W>), dim3(blocks), dim3(BLOCK), 0, s, c.data(), valid, c.offset, n, pos_of_row,
+                       part->ptr, nulls ? static_cast<uint8_t*>(vpart->ptr) : nullptr);
+    AM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(ob_clear, dim3(1), dim3(BLOCK), 0, s, ctr);
+  AM_HIP(hipGetLastError());
+  {
+    ProfScope ps("order_by_gather");
+    const uint32_t blocks = uint32_t(device().cu_count) * 4u;
+    hipLaunchKernelGGL((ob_gather<W>), dim3(blocks), dim3(BLOCK), 0, s, out_idx, n, vbase, m, part->ptr,
+                       nulls ? static_cast<const uint8_t*>(vpart->ptr) : nullptr, c.data(), valid, c.offset,
+                       out->values->ptr, nulls ? static_cast<unsigned long long*>(out->validity->ptr) : nullptr, ctr);
+    AM_HIP(hipGetLastError());
+  }
+}
+
+template <int ITEMS>
+bool ob_sort(const Array& key, bool desc, int placement, int forder, uint32_t* out_idx, uint32_t* pos_of_row,
+             uint32_t* m, hipStream_t s) {
+  return run_sort<unsigned long long, 8, ITEMS>(key.data(), key.type, key.has_nulls() ? key.valid_bits() : nullptr,
+                                                key.offset, uint32_t(key.length), desc, placement, forder, out_idx,
+                                                s, pos_of_row, m);
 }
 
 }  // namespace
@@ -763,6 +933,62 @@ Array* argsort(const Array& a, bool descending, int null_placement, int float_or
     throw;
   }
   return out;
+}
+
+bool order_by_columns(const Array& key, bool descending, int null_placement, int float_order,
+                      const std::vector<const Array*>& cols, std::vector<Array*>& out) {
+  out.clear();
+  const int64_t n = key.length;
+  if (key.width() != 8 || !type_is_primitive(key.type) || n < (int64_t(1) << 18) || n > int64_t(UINT32_MAX) ||
+      capturing(stream()))
+    return false;
+  for (const Array* c : cols)
+    if (c->length != n || !type_is_primitive(c->type)) return false;
+  check_index_rows(n, "argsort");
+  hipStream_t s = stream();
+  DevBuf idx = device_alloc(size_t(n) * 4), pos = device_alloc(size_t(n) * 4), ctr = device_alloc(NXCD * OB_CTR_STRIDE * 4);
+  retain(idx);
+  retain(pos);
+  retain(ctr);
+  auto* out_idx = static_cast<uint32_t*>(idx->ptr);
+  auto* pos_of_row = static_cast<uint32_t*>(pos->ptr);
+  uint32_t m = 0;
+  const int place = null_placement == 1 ? 1 : 0;
+  bool ok;
+  switch (sort_tune(n, 8).items) {
+    case 4: ok = ob_sort<4>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+    case 8: ok = ob_sort<8>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+    case 12: ok = ob_sort<12>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+    case 20: ok = ob_sort<20>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+    default: ok = ob_sort<16>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+  }
+  if (!ok) return false;
+  // The value block: [0, m) with nulls last, [n - m, n) with nulls first.
+  const uint32_t vbase = place == 1 ? uint32_t(n) - m : 0u;
+  try {
+    for (const Array* c : cols) {
+      Array* o = new_array(c->format, c->type, n);
+      out.push_back(o);
+      o->values = device_alloc(size_t(n) * c->width());
+      int64_t nulls = c->validity ? c->null_count : 0;
+      if (nulls < 0) nulls = count_nulls(*c);
+      o->null_count = nulls;   // a permutation keeps the null count
+      if (nulls > 0) o->validity = device_alloc(size_t((n + 63) / 64) * 8);
+      auto* ctrp = static_cast<uint32_t*>(ctr->ptr);
+      switch (c->width()) {
+        case 8: ob_move_column<8>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        case 4: ob_move_column<4>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        case 2: ob_move_column<2>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        default: ob_move_column<1>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+      }
+    }
+    sync();
+  } catch (...) {
+    for (Array* a : out) release_array(a);
+    out.clear();
+    throw;
+  }
+  return true;
 }
 
 }  // namespace am

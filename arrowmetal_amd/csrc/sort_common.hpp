@@ -191,12 +191,16 @@ __device__ __forceinline__ void rank_tile(uint32_t (&bins)[ITEMS], uint32_t* wc_
 // bin_base[d] + pos - bin_start[d].
 // ROWS_IOTA: the row of element j is row_base + 64 * j (a first pass over the raw column), so no
 // row registers are kept; otherwise rows[j].
-template <typename K, int ITEMS, bool WRITE_KEYS, bool ROWS_IOTA = false>
+// POS_ROWS (with ROWS_IOTA, the ORDER BY of whole tables): the row written beside each key is its own
+// destination, and pos_of_row[row] receives that destination, so that later passes carry positions
+// of this pass's output and the columns can be partitioned the same way from pos_of_row.
+template <typename K, int ITEMS, bool WRITE_KEYS, bool ROWS_IOTA = false, bool POS_ROWS = false>
 __device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uint32_t* rows,
                                                const uint32_t (&packed)[ITEMS], const uint32_t* wc_row,
                                                const uint32_t* bin_start, uint32_t* bin_base, uint8_t* s_digit,
                                                K* s_stage, uint32_t vcount, K* out_keys, uint32_t* out_rows,
-                                               uint32_t row_base = 0) {
+                                               uint32_t row_base = 0, uint32_t* pos_of_row = nullptr) {
+  static_assert(!POS_ROWS || (WRITE_KEYS && ROWS_IOTA), "positions as rows need the first pass's keys and rows");
   auto row = [&](int j) { return ROWS_IOTA ? row_base + uint32_t(j) * 64u : rows[j]; };
   uint32_t pos[ITEMS];
 #pragma unroll
@@ -230,6 +234,19 @@ __device__ __forceinline__ void scatter_values(const K (&keys)[ITEMS], const uin
     for (int k = 0; k < ITEMS; k++) {
       uint32_t p = threadIdx.x + uint32_t(k) * BLOCK;
       if (p < vcount) out_keys[dest[k]] = s_stage[p];
+    }
+    if constexpr (POS_ROWS) {
+      // Every delta is in place (the barrier after the key staging); rows in input order, so these
+      // writes are coalesced.
+#pragma unroll
+      for (int j = 0; j < ITEMS; j++)
+        if (pos[j] != 0xFFFFFFFFu) pos_of_row[row(j)] = bin_base[bin_of(packed[j])] + pos[j];
+#pragma unroll
+      for (int k = 0; k < ITEMS; k++) {
+        uint32_t p = threadIdx.x + uint32_t(k) * BLOCK;
+        if (p < vcount) out_rows[dest[k]] = dest[k];
+      }
+      return;
     }
     __syncthreads();
     uint32_t* s_rows = reinterpret_cast<uint32_t*>(s_stage);

@@ -71,6 +71,10 @@ def order_by_inputs(n: int):
     return keys, valid, payload
 
 
+def order_by_inputs_payload(n: int):
+    return unit(splitmix64(0xA11CE503, n)) * 2e6 - 1e6
+
+
 def zipf_ranks(n: int, support: int, s: float, seed: int) -> np.ndarray:
     w = 1.0 / np.arange(1, support + 1, dtype=np.float64) ** s
     cdf = np.cumsum(w)
@@ -134,7 +138,7 @@ def profiled(fn, reps: int = 1):
     return am.prof_report()
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_traffic_orderby_groupby_50M.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "orderby_groupby_50M_pmc_traffic.txt")
 
 # rocprofv3's kernel names (as written in the committed PMC summary) of the library's launch names.
 ROCPROF_NAME = {
@@ -383,6 +387,58 @@ def order_by_leg(args, log):
         log(f"order-by parity: oracle {res['parity_vs_oracle']}, arrow {res['parity_vs_arrow']}; "
             f"arrow {ns} rows in {s:.2f} s")
     return res, keys, valid, k, p
+
+
+def plan_order_by_alg_bytes(n: int, m: int):
+    d = sort_alg_bytes(n, m)
+    d["order_by_partition"] = (20 * n, "column into the partition's layout: read position + 8 B value, write 8 B; "
+                                       "20 B/row")
+    d["order_by_gather"] = (20 * n, "column out of the partition: read position + 8 B value (one bucket's "
+                                    "range), write 8 B; 20 B/row")
+    return d
+
+
+def plan_order_by_leg(args, log, keys, valid, payload, k, p):
+    """ORDER BY through the plan engine (am_plan_run), the route DataFusion / Spark / Polars use: the
+    same 50M-row table (key, payload) sorted by key descending, nulls last; both columns come back
+    sorted. The one-key sort moves every column with its top-digit partition (no table-wide gather)."""
+    n = len(keys)
+    m = int(valid.sum())
+    src = am.PlanSource("t", {"key": k, "payload": p})
+    plan = json.dumps({"op": "sort", "by": [["key", True]], "input": {"op": "scan", "source": "t"}})
+
+    def step():
+        out = am.plan_run(plan, [src])
+        del out
+
+    ts = timed(step, args.steps, args.warmup)
+    ms, best = 1e3 * float(np.median(ts)), 1e3 * float(np.min(ts))
+    raw = profiled(step, 3)
+    prof = merge_bucket_sorts(raw)
+    alg = plan_order_by_alg_bytes(n, m)
+    roof, rows = roofline_of(prof, alg)
+    b_step = sum(alg[kn][0] * prof[kn][0] // 3 for kn in rows)
+    res = {"workload": "plan_order_by_50M_int64_desc_nulls_last_table_of_key_and_f64_payload", "ms": ms,
+           "best": best, "rows_per_s": n / (ms * 1e-3), "roofline": roof, "kernels_roofline": rows,
+           "kernels_ms": kernels_ms(prof), "b_alg_step_bytes": b_step,
+           "step_hbm_frac_of_b_alg": b_step / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "outputs": "both columns sorted (the two-call form returns the index and the payload only)",
+           "parity_vs_oracle": None}
+    log(f"plan order-by: median {ms:.3f} ms, best {best:.3f} ms; kernels {json.dumps(raw)}")
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py as O   # the checker only
+        out = am.plan_run(plan, [src])
+        gk, gkv = out["key"].to_numpy()
+        gp, _ = out["payload"].to_numpy()
+        want = O.argsort(keys, valid, True, 0, 0)
+        wv = valid[want]
+        res["parity_vs_oracle"] = bool(np.array_equal(gkv if gkv is not None else np.ones(n, bool), wv)
+                                       and np.array_equal(gk[wv], keys[want][wv])
+                                       and np.array_equal(gp.view(np.uint64), payload[want].view(np.uint64)))
+        del want, out
+        log(f"plan order-by parity: {res['parity_vs_oracle']}")
+    return res
 
 
 def argsort_1m_leg(args, log):
@@ -864,6 +920,7 @@ def main():
     log("device:", am.device_name())
 
     ob, keys, valid, k, p = order_by_leg(args, log)
+    pob = plan_order_by_leg(args, log, keys, valid, order_by_inputs_payload(n), k, p)
     a1 = argsort_1m_leg(args, log)
     f64_res = None if args.skip_f64 else order_by_f64_leg(args, log, keys, valid, p)
     del keys, valid, k, p
@@ -899,6 +956,7 @@ def main():
         "parity_vs_arrow": ob["parity_vs_arrow"],
         "cpu_baseline": ob["cpu_baseline"],
         "host": host_info(),
+        "plan_order_by": pob,
         "argsort_1M": a1,
         "order_by_f64_total": f64_res,
         "group_by": gb_res,

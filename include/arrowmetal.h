@@ -144,6 +144,43 @@ int  am_lexsort_ex2(am_array** columns, const int* descending /* or NULL */, con
  * position of the first null row; order 1: ascending, nulls dropped. */
 int  am_unique_ex(am_array* a, int order, am_array** out);
 
+/* reference include/arrowmetal.h:1362-1433 (Sources/ArrowMetalC/ArrowMetalC_Plan.swift,
+ * Sources/ArrowMetal/Engine/PlanJSON.swift): the plan engine, one JSON plan per call over tables
+ * registered with am_plan_source_create (which retains the column handles it is given; columns of
+ * unequal length are the error "Array length mismatch: A vs B").
+ * This build runs the subset on the ORDER BY / group-by path:
+ *   plan  := {"op": "scan",     "source": NAME, "columns": [NAME, ...]?}
+ *          | {"op": "sort",     "input": plan, "by": [KEY, ...], "nulls": NULLS?, "float_order": FORDER?}
+ *          | {"op": "limit",    "input": plan, "count": INT, "offset": INT?}
+ *          | {"op": "unique",   "input": plan, "subset": [NAME, ...]?}
+ *          | {"op": "group_by", "input": plan, "keys": [[NAME, "(col \"c\")"], ...],
+ *                               "aggs": [[AGGOP, NAME, "(col \"c\")"?], ...]}
+ *   KEY, NULLS, FORDER and AGGOP as in the reference grammar; a sort under a limit of 1..1024 rows with
+ *   no offset and one numeric key is a top-k selection. Expressions are column references only. The
+ *   reference's other operators (filter, select, with_columns, aggregate, join, join_asof, concat,
+ *   window, explode) and expressions return 1 with "Unsupported Arrow type: plan: ... is not
+ *   supported by this build"; an unknown operator with "Invalid ArrowArray: plan: unknown operator".
+ * Results: group_by keys in ascending key order with the null group last, then one column per
+ * aggregate (sum: int64 for signed integers, uint64 for unsigned, float64 for float values rounded
+ * once from the exact sum; mean float64; count int64; min/max the values' type). `optimize` is
+ * accepted; it changes no result on this subset. am_plan_explain's text is valid until the next call
+ * on the thread (NULL plus am_last_error() when the plan does not parse). am_plan_column hands out a
+ * new reference the caller releases with am_release. */
+typedef struct am_plan_source_t am_plan_source;
+typedef struct am_plan_result_t am_plan_result;
+int  am_plan_source_create(const char* name, am_array** columns, const char** names,
+                           int64_t n_columns, am_plan_source** out);
+void am_plan_source_release(am_plan_source* s);
+int  am_plan_run(const char* plan_json, am_plan_source** sources, int64_t n_sources, int optimize,
+                 am_plan_result** out);
+const char* am_plan_explain(const char* plan_json, am_plan_source** sources, int64_t n_sources,
+                            int optimize);
+int64_t     am_plan_column_count(am_plan_result* r);
+int64_t     am_plan_row_count(am_plan_result* r);
+const char* am_plan_column_name(am_plan_result* r, int64_t i);
+int         am_plan_column(am_plan_result* r, int64_t i, am_array** out);
+void        am_plan_result_release(am_plan_result* r);
+
 /* ---------------------------------------------------------------------------------------------
  * Extensions of this build (no counterpart in the reference header). They serve the batched and
  * served forms of the same kernels, the keep-first unique of the plan engine's "unique" operator

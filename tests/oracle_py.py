@@ -182,3 +182,119 @@ def unique_values(values, valid=None, order=0):
     m = lib().or_unique_values(TYPE_OF[values.dtype], _ptr(values), None if bm is None else _ptr(bm), n,
                                order, _ptr(out))
     return out[:m].copy()
+
+
+# ---------------------------------------------------------------- the plan engine's subset
+
+_AGG_OP = {"sum": 0, "count": 2, "mean": 3, "min": 4, "max": 5}
+
+
+def _agg_dtype(op, values):
+    if op == "count":
+        return np.dtype(np.int64)
+    if op == "mean":
+        return np.dtype(np.float64)
+    if op == "sum":
+        if values.dtype.kind == "u":
+            return np.dtype(np.uint64)
+        return np.dtype(np.float64) if values.dtype.kind == "f" else np.dtype(np.int64)
+    return values.dtype
+
+
+def _from_bits(bits, dtype):
+    dtype = np.dtype(dtype)
+    if dtype.itemsize == 8:
+        return bits.view(dtype)
+    mask = np.uint64((1 << (8 * dtype.itemsize)) - 1)
+    return (bits & mask).astype(np.dtype(f"u{dtype.itemsize}")).view(dtype)
+
+
+def _col_name(sexpr):
+    s = sexpr.strip()
+    assert s.startswith("(col") and s.endswith(")"), sexpr
+    return s[4:-1].strip().strip('"')
+
+
+def _sort_keys(node):
+    """(column, descending, nulls_first, float_order) per key, with the sort-level defaults
+    (Engine/PlanJSON.swift:96-118)."""
+    dnf = node.get("nulls") == "first"
+    dfo = {"ieee": 0, "total": 1, "nan_largest": 2}[node.get("float_order") or "ieee"]
+    out = []
+    for k in node.get("by", []):
+        if isinstance(k, dict):
+            col, desc, opts = k["column"], bool(k.get("descending", False)), k
+        else:
+            col, desc = k[0], bool(k[1]) if len(k) > 1 else False
+            opts = k[2] if len(k) > 2 and isinstance(k[2], dict) else {}
+        nf = dnf if opts.get("nulls") is None else opts["nulls"] == "first"
+        fo = dfo if opts.get("float_order") is None else {"ieee": 0, "total": 1, "nan_largest": 2}[opts["float_order"]]
+        out.append((col, desc, nf, fo))
+    return out
+
+
+def _take_table(t, idx):
+    out = {}
+    for name, (v, valid) in t.items():
+        vals, ov = take(v, idx.astype(np.uint32), valid)
+        out[name] = (vals, ov if valid is not None else None)
+    return out
+
+
+def plan_run(plan, sources):
+    """CPU restatement of the plan engine's subset (Engine/Executor.swift:46-160, 243-350;
+    PhysicalPlan.swift:118-126): `sources` maps a source name to {column: (values, valid or None)};
+    returns {column: (values, valid or None)} in output order. Composed from the oracle's own argsort,
+    lexsort, take, group keys, aggregates and keep-first unique."""
+    op = plan["op"]
+    if op == "scan":
+        src = sources[plan["source"]]
+        cols = plan.get("columns")
+        return dict(src) if cols is None else {c: src[c] for c in cols}
+    if op == "sort":
+        t = plan_run(plan["input"], sources)
+        keys = _sort_keys(plan)
+        n = len(next(iter(t.values()))[0]) if t else 0
+        if not keys or n == 0:
+            return t
+        idx = lexsort([t[k[0]][0] for k in keys], [t[k[0]][1] for k in keys], [k[1] for k in keys],
+                      [1 if k[2] else 0 for k in keys], [k[3] for k in keys])
+        return _take_table(t, idx)
+    if op == "limit":
+        # A top-k selection is the first k rows of the stable sort, so the sort-then-slice form gives
+        # the same rows.
+        t = plan_run(plan["input"], sources)
+        n = len(next(iter(t.values()))[0]) if t else 0
+        lo = min(max(int(plan.get("offset", 0)), 0), n)
+        count = plan.get("count", 2**63 - 1)
+        ln = max(min(int(count), n - lo), 0)
+        return {k: (v[lo:lo + ln], None if va is None else va[lo:lo + ln]) for k, (v, va) in t.items()}
+    if op == "unique":
+        t = plan_run(plan["input"], sources)
+        names = plan.get("subset") or list(t)
+        n = len(next(iter(t.values()))[0]) if t else 0
+        if n == 0:
+            return t
+        keep = unique_rows([t[c][0] for c in names], [t[c][1] for c in names])
+        return _take_table(t, keep)
+    if op == "group_by":
+        t = plan_run(plan["input"], sources)
+        keys = [(nm, _col_name(e)) for nm, e in plan.get("keys", [])]
+        aggs = [(a[0], a[1], _col_name(a[2]) if len(a) > 2 and a[2] else None) for a in plan.get("aggs", [])]
+        kcols = [t[c][0] for _, c in keys]
+        kvalid = [t[c][1] for _, c in keys]
+        ids, K, rep = group_keys(kcols, kvalid)
+        out = {}
+        for (nm, c), v, va in zip(keys, kcols, kvalid):
+            kv, kva = take(v, rep.astype(np.uint32), va)
+            out[nm] = (kv, kva if va is not None else None)
+        for aop, nm, c in aggs:
+            if aop == "count" and c is None:
+                bits, ok = group_agg(ids, K, None, 1)
+                out[nm] = (bits.view(np.int64), None)
+                continue
+            v, va = t[c]
+            bits, ok = group_agg(ids, K, v, 2 if aop == "count" else _AGG_OP[aop], va)
+            out[nm] = (_from_bits(bits, _agg_dtype(aop, v)), None if ok.all() else ok)
+        return out
+    raise ValueError(f"oracle plan: operator {op} is outside the subset")

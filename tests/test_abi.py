@@ -57,3 +57,61 @@ def test_null_handles_return_2_without_a_gpu():
 def test_library_is_built_for_gfx950():
     data = open(am.library_path(), "rb").read()
     assert b"gfx950" in data
+
+
+def test_plan_null_handles_and_parse_errors_without_a_gpu():
+    """am_plan_*: NULL arguments return 2 (or -1 / NULL) with no message (ArrowMetalC_Plan.swift:52-160);
+    a plan that does not parse is refused before any device work, with PlanJSON.swift's texts."""
+    L = am.lib()
+    out = ctypes.c_void_p()
+    assert L.am_plan_run(None, None, 0, 1, ctypes.byref(out)) == 2
+    assert L.am_plan_run(b"{}", None, 0, 1, None) == 2
+    assert L.am_plan_run(b"{}", None, -1, 1, ctypes.byref(out)) == 2
+    assert L.am_plan_source_create(None, None, None, 0, ctypes.byref(out)) == 2
+    assert L.am_plan_column_count(None) == -1
+    assert L.am_plan_row_count(None) == -1
+    assert L.am_plan_column_name(None, 0) is None
+    assert L.am_plan_column(None, 0, ctypes.byref(out)) == 2
+    assert L.am_plan_explain(None, None, 0, 1) is None
+    L.am_plan_result_release(None)
+    L.am_plan_source_release(None)
+    cases = {
+        "{": "Invalid ArrowArray: plan: not a JSON object",
+        "[1, 2]": "Invalid ArrowArray: plan: not a JSON object",
+        '{"x": 1}': 'Invalid ArrowArray: plan: node has no "op"',
+        '{"op": "sort"}': 'Invalid ArrowArray: plan: sort has no "input"',
+        '{"op": "scan", "source": "t"}': 'Invalid ArrowArray: plan: unknown source "t"',
+        '{"op": "nope"}': 'Invalid ArrowArray: plan: unknown operator "nope"',
+        '{"op": "join", "left": {}, "right": {}}': 'Unsupported Arrow type: plan: operator "join" is not supported '
+                                                   'by this build',
+        '{"op": "limit", "count": 2, "input": {"op": "scan", "source": "s\\u00e9"}}':
+            'Invalid ArrowArray: plan: unknown source "sé"',
+    }
+    for text, msg in cases.items():
+        assert L.am_plan_run(text.encode(), None, 0, 1, ctypes.byref(out)) == 1, text
+        assert L.am_last_error().decode() == msg, text
+        with pytest.raises(am.ArrowMetalError) as e:
+            am.plan_explain(text)
+        assert str(e.value) == msg
+
+
+def test_plan_expressions_outside_the_subset_are_refused():
+    """Only column references are expressions here; the s-expression forms of ExprText.swift:14-45
+    ("(col "x")", "(col x)") both name a column. A table with no columns needs no device."""
+    L = am.lib()
+    src = ctypes.c_void_p()
+    assert L.am_plan_source_create(b"t", (ctypes.c_void_p * 1)(), (ctypes.c_char_p * 1)(), 0, ctypes.byref(src)) == 0
+    try:
+        scan = {"op": "scan", "source": "t"}
+        for expr in ['(add (col "a") (int 1))', '(cast (col "a") f64)', '(int 3)', '(col "a") extra', 'col a']:
+            plan = {"op": "group_by", "keys": [["k", expr]], "aggs": [], "input": scan}
+            with pytest.raises(am.ArrowMetalError) as e:
+                am.plan_explain(plan, [type("S", (), {"_h": src})()])
+            assert str(e.value) == (f"Unsupported Arrow type: plan: the expression {expr} is not supported by this "
+                                    "build (only (col \"name\"))"), expr
+        for expr in ['(col "a")', '(col a)', ' ( col  "a b" ) ']:
+            plan = {"op": "group_by", "keys": [["k", expr]], "aggs": [["count", "n", ""]], "input": scan}
+            text = am.plan_explain(plan, [type("S", (), {"_h": src})()])
+            assert "HASH_AGGREGATE keys [k]" in text and "SOURCE t" in text
+    finally:
+        L.am_plan_source_release(src)

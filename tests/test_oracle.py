@@ -561,3 +561,85 @@ def test_lexsort_planted_mistake_is_caught():
     wrong = O.lexsort([b, a])   # keys swapped: b most significant
     assert not np.array_equal(wrong, want)
     np.testing.assert_array_equal(O.lexsort([a, b]), want)
+
+
+# ---------------------------------------------------------------- the plan engine's subset
+
+
+def _plan_table(rng, n):
+    k = rng.integers(-20, 20, n).astype(np.int64)
+    kv = rng.random(n) > 0.1
+    g = rng.integers(0, 3, n).astype(np.int32)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    vv = rng.random(n) > 0.2
+    f = rng.standard_normal(n)
+    return {"k": (k, kv), "g": (g, None), "v": (v, vv), "f": (f, None)}
+
+
+def _arrow_table(pa, t):
+    return pa.table({c: pa.array(v, mask=None if va is None else ~va) for c, (v, va) in t.items()})
+
+
+@pytest.mark.parametrize("nulls", ["last", "first"])
+def test_plan_sort_matches_arrow_sort_by(nulls):
+    """A plan sort is the stable multi-key sort of every column (Kernels/MultiSort.swift:201-213);
+    Arrow's Table.sort_by gives the same rows (ieee order, one null placement)."""
+    pa = pytest.importorskip("pyarrow")
+    rng = np.random.default_rng(5)
+    t = _plan_table(rng, 2000)
+    plan = {"op": "sort", "nulls": nulls, "by": [["g", False], ["k", True]], "input": {"op": "scan", "source": "t"}}
+    got = O.plan_run(plan, {"t": t})
+    want = _arrow_table(pa, t).sort_by([("g", "ascending"), ("k", "descending")],
+                                       null_placement="at_end" if nulls == "last" else "at_start")
+    for c in t:
+        a = want.column(c)
+        ok = ~np.asarray(a.is_null())
+        np.testing.assert_array_equal(got[c][0][ok], a.to_numpy(zero_copy_only=False)[ok])
+        gv = got[c][1] if got[c][1] is not None else np.ones(len(ok), bool)
+        np.testing.assert_array_equal(gv, ok)
+
+
+def test_plan_group_by_matches_arrow():
+    """group_by: keys ascending with the null group last, then sum / count / count_all / mean / min /
+    max of integers, against Arrow's hash aggregates sorted by key."""
+    pa = pytest.importorskip("pyarrow")
+    rng = np.random.default_rng(6)
+    t = _plan_table(rng, 3000)
+    plan = {"op": "group_by", "keys": [["key", "(col \"k\")"]],
+            "aggs": [["sum", "s", "(col \"v\")"], ["count", "c", "(col \"v\")"], ["count", "n", ""],
+                     ["mean", "m", "(col \"v\")"], ["min", "lo", "(col \"v\")"], ["max", "hi", "(col \"v\")"]],
+            "input": {"op": "scan", "source": "t"}}
+    got = O.plan_run(plan, {"t": t})
+    at = _arrow_table(pa, t)
+    agg = at.group_by("k", use_threads=False).aggregate(
+        [("v", "sum"), ("v", "count"), ([], "count_all"), ("v", "mean"), ("v", "min"), ("v", "max")])
+    agg = agg.sort_by([("k", "ascending")], null_placement="at_end")
+    kv = got["key"]
+    want_k = agg.column("k")
+    np.testing.assert_array_equal(kv[1], ~np.asarray(want_k.is_null()))
+    np.testing.assert_array_equal(kv[0][kv[1]], want_k.drop_null().to_numpy())
+    for ours, theirs in (("s", "v_sum"), ("c", "v_count"), ("n", "count_all"), ("m", "v_mean"), ("lo", "v_min"),
+                         ("hi", "v_max")):
+        a = agg.column(theirs).to_numpy(zero_copy_only=False)
+        np.testing.assert_array_equal(got[ours][0], a.astype(got[ours][0].dtype), err_msg=ours)
+
+
+def test_plan_unique_limit_and_top_k_by_brute_force():
+    rng = np.random.default_rng(7)
+    t = _plan_table(rng, 500)
+    # unique over a subset keeps each combination's first row, in input order.
+    got = O.plan_run({"op": "unique", "subset": ["g", "k"], "input": {"op": "scan", "source": "t"}}, {"t": t})
+    seen, keep = set(), []
+    k, kv = t["k"]
+    for i in range(500):
+        key = (int(t["g"][0][i]), int(k[i]) if kv[i] else None)
+        if key not in seen:
+            seen.add(key)
+            keep.append(i)
+    np.testing.assert_array_equal(got["f"][0], t["f"][0][keep])
+    # limit with an offset is a slice; a sort under a limit is the sort's first rows.
+    got = O.plan_run({"op": "limit", "count": 7, "offset": 3, "input": {"op": "scan", "source": "t"}}, {"t": t})
+    np.testing.assert_array_equal(got["v"][0], t["v"][0][3:10])
+    plan = {"op": "limit", "count": 5, "input": {"op": "sort", "by": [["f", True]], "input": {"op": "scan", "source": "t"}}}
+    got = O.plan_run(plan, {"t": t})
+    np.testing.assert_array_equal(got["f"][0], np.sort(t["f"][0])[::-1][:5])
