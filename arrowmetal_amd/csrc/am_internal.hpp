@@ -74,6 +74,15 @@ __device__ __forceinline__ uint32_t xcd_id() {
   return uint32_t(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & (NXCD - 1);
 }
 
+// A load through a pointer known to address global memory (device buffers, __device__ variables).
+// A pointer chosen between two buffers (a column or a constant, a partition or a column) is a generic
+// pointer to the compiler, and a generic (flat) load counts against both the vector-memory and the
+// LDS counters, so waiting for it also waits for every LDS operation in flight and the reverse.
+template <typename T>
+__device__ __forceinline__ T load_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 __host__ __device__ inline bool type_is_float(int t) { return t == T_F32 || t == T_F64; }
 __host__ __device__ inline bool type_is_signed(int t) { return t >= T_I8 && t <= T_I64; }
 __host__ __device__ inline bool type_is_int(int t) { return t >= T_I8 && t <= T_U64; }

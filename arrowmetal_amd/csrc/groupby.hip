@@ -75,7 +75,7 @@ __device__ __forceinline__ unsigned long long ld_raw(const void* p, int64_t i) {
 __device__ __forceinline__ bool ld_valid(const uint8_t* valid, int64_t voff, int64_t i) {
   const int64_t b = voff + i;
   const uint8_t* p = valid ? valid + (b >> 3) : &kAllValid;
-  return (*p >> (valid ? (b & 7) : 0)) & 1;
+  return (load_global(p) >> (valid ? (b & 7) : 0)) & 1;
 }
 
 // The group key of raw element bits: integers widened with their sign, float NaN made canonical and
@@ -636,7 +636,14 @@ struct AggArgs {
   int64_t n;
   long long K;
   const void* aux;               // family-specific (exact float sums: the groups' grid words)
+  const long long* hot;          // HOT_GROUPS ids (-1: none) summed in registers, or null
 };
+
+// Up to this many of a column's most frequent groups (from a sample of its ids) are summed in each
+// lane's registers by the families whose words are all additions (sums and counts), so that the rows
+// of a hot group take no atomic at all: with Zipf(1.1) keys the top group alone is ~13% of the rows,
+// about nine lanes of every wave, whose LDS atomics on one word would otherwise serialize.
+constexpr int HOT_GROUPS = 8;
 
 // count_all (values ignored) and count (non-null values).
 template <bool VALID_ONLY>
@@ -785,22 +792,22 @@ __device__ __forceinline__ unsigned long long identity(int op) { return op == W_
 
 // One word update in LDS or in this XCD's private global table; workgroup-scope atomics in both
 // (ds_* in LDS, L2-resident in global memory). A contribution equal to the op's identity touches
-// nothing, not even a read. MIN / MAX / OR read first and skip the atomic when it cannot change
-// anything, which keeps hot groups read-mostly: a stale read never lets a needed update be skipped,
-// because those words only ever move one way.
+// nothing, not even a read. MIN / MAX / OR read the word first (an atomic relaxed load of the same
+// address: one 8-byte access, never torn) and skip the atomic when it cannot change anything, which
+// keeps hot groups read-mostly. Those words only ever move one way (MIN down, MAX up, OR gaining
+// bits), so any value the read returns, however stale, lies on the far side of the current value:
+// x >= read >= current makes the MIN a no-op, and likewise for MAX and OR.
 template <int OP>
 __device__ __forceinline__ void word_update(unsigned long long* p, unsigned long long x) {
+  auto seen = [p] { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
   if (OP == W_ADD) {
     if (x) __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MIN) {
-    if (x != ~0ull && x < *reinterpret_cast<volatile unsigned long long*>(p))
-      __hip_atomic_fetch_min(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (x != ~0ull && x < seen()) __hip_atomic_fetch_min(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_MAX) {
-    if (x != 0ull && x > *reinterpret_cast<volatile unsigned long long*>(p))
-      __hip_atomic_fetch_max(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (x != 0ull && x > seen()) __hip_atomic_fetch_max(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else if (OP == W_OR) {
-    if (x != 0ull && (x & ~*reinterpret_cast<volatile unsigned long long*>(p)))
-      __hip_atomic_fetch_or(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (x != 0ull && (x & ~seen())) __hip_atomic_fetch_or(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
@@ -845,53 +852,91 @@ __device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsign
 
 constexpr int AGG_ROWS = 4;   // rows per thread whose loads are issued together
 
-// Mode 0: every group has a private LDS accumulator (K <= cap). Mode 1: an LDS cache of `cap`
-// groups, claimed by CAS on a key word, in front of this XCD's table in global memory (L2); a row
-// whose group cannot be cached within 8 probes goes to that table directly. The tables of the
-// eight XCDs are merged by gb_agg_merge.
 // 1024 threads per workgroup and one workgroup per CU: the whole LDS budget goes to one table.
 #ifndef AM_AGG_THREADS
 #define AM_AGG_THREADS 1024
 #endif
+#ifndef AM_AGG_PROBES
+#define AM_AGG_PROBES 8   // LDS cache slots a row probes before its group goes to the XCD's table
+#endif
+#ifndef AM_AGG_HOT
+#define AM_AGG_HOT 1      // sampled hot groups summed in registers
+#endif
+
+template <class P>
+__host__ __device__ constexpr bool all_add() {
+  for (int w = 0; w < P::W; w++)
+    if (P::op(w) != W_ADD) return false;
+  return true;
+}
+
+// The rows one thread has in flight: group ids, values and validity as loaded.
+struct AggRows {
+  unsigned long long idr[AGG_ROWS], vr[AGG_ROWS];
+  bool iok[AGG_ROWS], vok[AGG_ROWS];
+};
+
+template <class P, int IDW, int VW>
+__device__ __forceinline__ void agg_load(const AggArgs& a, int64_t base, int64_t end, AggRows& r) {
+#pragma unroll
+  for (int k = 0; k < AGG_ROWS; k++) {   // rows past the end read the last row
+    const int64_t i = base + int64_t(k) * blockDim.x + threadIdx.x;
+    const int64_t ii = i < end ? i : end - 1;
+    r.idr[k] = ld_raw<IDW>(a.ids.ids, ii);
+    r.iok[k] = ld_valid(a.ids.valid, a.ids.voff, ii);
+    r.vr[k] = 0;
+    if constexpr (VW != 0) r.vr[k] = ld_raw<VW>(a.v.vals, ii);
+    r.vok[k] = P::SKIP_NULL ? ld_valid(a.v.valid, a.v.voff, ii) : true;
+  }
+}
+
 // IDW: byte width of the group ids (4: int32 / uint32, 8: int64); VW: byte width of the values (0 when
 // the family reads none).
+// Mode 0: every group has a private LDS accumulator (K <= cap). Mode 1: an LDS cache of `cap`
+// groups, claimed by CAS on a key word, in front of this XCD's table in global memory (L2); a row
+// whose group cannot be cached within AM_AGG_PROBES probes goes to that table directly. The tables of
+// the eight XCDs are merged by gb_agg_merge.
+// The next rows' loads are issued before the current rows' updates, so that waiting for them never
+// waits for the updates' atomics (vector memory operations complete in issue order).
 template <class P, int IDW, int VW>
-__global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rows_per_block,
+__global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rpb,
                                               unsigned long long* gtab8) {
   extern __shared__ unsigned long long lds[];
   unsigned long long* lacc = lds;                                     // [cap][W]
   unsigned int* lkey = reinterpret_cast<unsigned int*>(lds + size_t(cap) * P::W);   // [cap] (mode 1)
   unsigned long long* gtab = gtab8 + size_t(xcd_id()) * size_t(a.K) * P::W;
+  constexpr bool HOTREG = AM_AGG_HOT && all_add<P>();
+  long long hot[HOT_GROUPS];
+  unsigned long long hacc[HOT_GROUPS][P::W];
+#pragma unroll
+  for (int j = 0; j < HOT_GROUPS; j++) {
+    hot[j] = HOTREG && a.hot ? a.hot[j] : -1;
+#pragma unroll
+    for (int w = 0; w < P::W; w++) hacc[j][w] = 0ull;
+  }
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
 #pragma unroll
     for (int w = 0; w < P::W; w++) lacc[size_t(s) * P::W + w] = identity(P::op(w));
     if (mode == 1) lkey[s] = 0xFFFFFFFFu;
   }
   __syncthreads();
-  const int64_t begin = int64_t(blockIdx.x) * rows_per_block;
-  const int64_t end = std::min<int64_t>(a.n, begin + rows_per_block);
-  for (int64_t base = begin; base < end; base += int64_t(blockDim.x) * AGG_ROWS) {
+  const int64_t begin = int64_t(blockIdx.x) * rpb;
+  const int64_t end = std::min<int64_t>(a.n, begin + rpb);
+  const int64_t step = int64_t(blockDim.x) * AGG_ROWS;
+  AggRows cur;
+  if (begin < end) agg_load<P, IDW, VW>(a, begin, end, cur);
+  for (int64_t base = begin; base < end; base += step) {
+    AggRows nxt;
+    if (base + step < end) agg_load<P, IDW, VW>(a, base + step, end, nxt);   // uniform
     long long gs[AGG_ROWS];
     unsigned long long cs[AGG_ROWS][P::W];
-    unsigned long long idr[AGG_ROWS], vr[AGG_ROWS];
-    bool iok[AGG_ROWS], vok[AGG_ROWS];
-#pragma unroll
-    for (int r = 0; r < AGG_ROWS; r++) {   // every load first (rows past the end read the last row)
-      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-      const int64_t ii = i < end ? i : end - 1;
-      idr[r] = ld_raw<IDW>(a.ids.ids, ii);
-      iok[r] = ld_valid(a.ids.valid, a.ids.voff, ii);
-      vr[r] = 0;
-      if constexpr (VW != 0) vr[r] = ld_raw<VW>(a.v.vals, ii);
-      vok[r] = P::SKIP_NULL ? ld_valid(a.v.valid, a.v.voff, ii) : true;
-    }
 #pragma unroll
     for (int r = 0; r < AGG_ROWS; r++) {
       const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
       long long g;
-      if constexpr (IDW == 8) g = (long long)idr[r];
-      else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(idr[r])) : (long long)uint32_t(idr[r]);
-      if (i >= end || !iok[r] || g < 0 || g >= a.K || !vok[r]) g = -1;
+      if constexpr (IDW == 8) g = (long long)cur.idr[r];
+      else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(cur.idr[r])) : (long long)uint32_t(cur.idr[r]);
+      if (i >= end || !cur.iok[r] || g < 0 || g >= a.K || !cur.vok[r]) g = -1;
       gs[r] = g;
     }
     if constexpr (P::AUX) {
@@ -901,13 +946,28 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
 #pragma unroll
       for (int r = 0; r < AGG_ROWS; r++) {
         const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(vr[r], a.v.type), st[r], cs[r])) gs[r] = -1;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(cur.vr[r], a.v.type), st[r], cs[r])) gs[r] = -1;
       }
     } else {
 #pragma unroll
       for (int r = 0; r < AGG_ROWS; r++) {
         const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(vr[r], a.v.type), cs[r])) gs[r] = -1;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(cur.vr[r], a.v.type), cs[r])) gs[r] = -1;
+      }
+    }
+    if (HOTREG && hot[0] >= 0) {   // uniform; the sampler fills hot[] most frequent first
+      // A hot group's contribution goes to this lane's registers.
+#pragma unroll
+      for (int r = 0; r < AGG_ROWS; r++) {
+        bool taken = false;
+#pragma unroll
+        for (int j = 0; j < HOT_GROUPS; j++) {
+          const bool m = !taken && gs[r] >= 0 && gs[r] == hot[j];
+#pragma unroll
+          for (int w = 0; w < P::W; w++) hacc[j][w] += m ? cs[r][w] : 0ull;
+          taken = taken || m;
+        }
+        if (taken) gs[r] = -1;
       }
     }
 #pragma unroll
@@ -921,13 +981,13 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
       uint32_t h = uint32_t(g) * 2654435761u;
       uint32_t sl = __umulhi(h ^ (h >> 15), cap);   // [0, cap) for any cap
       bool done = false;
-      for (int probe = 0; probe < 8; probe++) {
-        unsigned int cur = lkey[sl];
-        if (cur == 0xFFFFFFFFu) {
+      for (int probe = 0; probe < AM_AGG_PROBES; probe++) {
+        unsigned int c = lkey[sl];
+        if (c == 0xFFFFFFFFu) {
           unsigned int prev = atomicCAS(&lkey[sl], 0xFFFFFFFFu, uint32_t(g));
-          cur = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
+          c = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
         }
-        if (cur == uint32_t(g)) {
+        if (c == uint32_t(g)) {
           apply<P>(&lacc[size_t(sl) * P::W], cs[r]);
           done = true;
           break;
@@ -936,6 +996,22 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
       }
       if (!done) apply<P>(&gtab[size_t(g) * P::W], cs[r]);
     }
+    cur = nxt;
+  }
+  if constexpr (HOTREG) {
+    // Each wave's sum of its lanes' hot accumulators, added once into this XCD's table.
+#pragma unroll
+    for (int j = 0; j < HOT_GROUPS; j++) {
+      if (hot[j] < 0) continue;   // uniform
+#pragma unroll
+      for (int w = 0; w < P::W; w++) {
+        unsigned long long v = hacc[j][w];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        hacc[j][w] = v;
+      }
+      if ((threadIdx.x & 63) == 0) apply<P>(&gtab[size_t(hot[j]) * P::W], hacc[j]);
+    }
   }
   __syncthreads();
   for (uint32_t sl = threadIdx.x; sl < cap; sl += blockDim.x) {
@@ -943,6 +1019,76 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
     if (mode == 0) g = sl < uint32_t(a.K) ? (long long)sl : -1;
     else g = lkey[sl] == 0xFFFFFFFFu ? -1 : (long long)lkey[sl];
     if (g >= 0) flush_one<P>(&gtab[size_t(g) * P::W], &lacc[size_t(sl) * P::W]);
+  }
+}
+
+// The most frequent group ids of a sample of the rows: S rows at even strides counted in an LDS
+// table, then the HOT_GROUPS largest counts of at least S / 64 (about one lane per wave). One
+// workgroup of 1024 threads (four rows each); hot[j] = -1 where there is no such group.
+constexpr uint32_t HOT_SAMPLE = 4096;   // a group of >= 1/64 of the rows shows ~64 times
+constexpr uint32_t HOT_SLOTS = 8192;    // twice the sample: probe chains stay short
+__global__ __launch_bounds__(1024) void gb_hot_sample(IdSource ids, int64_t n, long long K, long long* hot) {
+  __shared__ uint32_t skey[HOT_SLOTS];
+  __shared__ uint32_t scnt[HOT_SLOTS];
+  __shared__ unsigned long long best[32];
+  for (uint32_t i = threadIdx.x; i < HOT_SLOTS; i += blockDim.x) {
+    skey[i] = 0xFFFFFFFFu;
+    scnt[i] = 0u;
+  }
+  __syncthreads();
+  const uint32_t S = uint32_t(std::min<int64_t>(n, HOT_SAMPLE));
+  constexpr int IN_FLIGHT = 4;
+  for (uint32_t b = threadIdx.x; b < S; b += blockDim.x * IN_FLIGHT) {
+    long long g[IN_FLIGHT];
+#pragma unroll
+    for (int k = 0; k < IN_FLIGHT; k++) {
+      const uint32_t t = b + uint32_t(k) * blockDim.x;
+      const int64_t row = t < S ? int64_t(uint64_t(t) * uint64_t(n) / S) : 0;   // n < 2^33, S <= 2^16
+      g[k] = t < S ? load_gid(ids, row, K) : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < IN_FLIGHT; k++) {
+      if (g[k] < 0) continue;
+      uint32_t h = uint32_t(g[k]) * 2654435761u;
+      uint32_t sl = (h ^ (h >> 15)) & (HOT_SLOTS - 1);
+      for (uint32_t probe = 0; probe < 64; probe++) {   // a full table drops the row from the sample
+        uint32_t c = skey[sl];
+        if (c == 0xFFFFFFFFu) {
+          const uint32_t prev = atomicCAS(&skey[sl], 0xFFFFFFFFu, uint32_t(g[k]));
+          c = prev == 0xFFFFFFFFu ? uint32_t(g[k]) : prev;
+        }
+        if (c == uint32_t(g[k])) {
+          atomicAdd(&scnt[sl], 1u);
+          break;
+        }
+        sl = (sl + 1u) & (HOT_SLOTS - 1);
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t floor = std::max<uint32_t>(S / 64u, 2u);
+  for (int j = 0; j < HOT_GROUPS; j++) {
+    // (count << 32 | ~slot): the largest count, lowest slot on ties.
+    unsigned long long mine = 0ull;
+    for (uint32_t i = threadIdx.x; i < HOT_SLOTS; i += blockDim.x) {
+      const unsigned long long c = (unsigned long long)scnt[i] << 32 | (0xFFFFFFFFull - i);
+      mine = c > mine ? c : mine;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mine, off, 64);
+      mine = o > mine ? o : mine;
+    }
+    if ((threadIdx.x & 63) == 0) best[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long m = 0ull;
+      for (uint32_t w = 0; w < blockDim.x / 64; w++) m = best[w] > m ? best[w] : m;
+      const uint32_t slot = 0xFFFFFFFFu - uint32_t(m);
+      const uint32_t cnt = uint32_t(m >> 32);
+      hot[j] = cnt >= floor ? (long long)skey[slot] : -1;
+      if (cnt > 0u) scnt[slot] = 0u;
+    }
+    __syncthreads();
   }
 }
 
@@ -1313,6 +1459,7 @@ struct GroupBy {
   DevBuf cnt_tab;          // K x 2 integer sum and count of non-null values of column cnt_col
   bool cnt_all_rows = false;   // column cnt_col had no nulls: its counts are the rows per group
   DevBuf count_all_tab;    // K x 1 rows per group
+  DevBuf hot;              // HOT_GROUPS sampled frequent ids (sums and counts in registers)
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
     for (Array* a : key_results) release_array(a);
@@ -1773,8 +1920,25 @@ Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::
 }
 
 // The shared op table of am_group_agg_ex and am_group_by over an id source.
+// The sampled hot groups of the ids (gb_hot_sample), once per mapping; none for small inputs.
+const long long* hot_groups(const IdSource& ids, int64_t n, long long K, GroupBy* cache, DevBuf& holder) {
+  if (n < (int64_t(1) << 20)) return nullptr;
+  if (cache && cache->hot) return static_cast<const long long*>(cache->hot->ptr);
+  holder = device_alloc(HOT_GROUPS * sizeof(long long));
+  retain(holder);
+  {
+    ProfScope ps("gb_hot_sample");
+    hipLaunchKernelGGL(gb_hot_sample, dim3(1), dim3(1024), 0, stream(), ids, n, K,
+                       static_cast<long long*>(holder->ptr));
+    AM_HIP(hipGetLastError());
+  }
+  if (cache) cache->hot = holder;
+  return static_cast<const long long*>(holder->ptr);
+}
+
 Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, int op, GroupBy* cache) {
   hipStream_t s = stream();
+  DevBuf hot_holder;
   if (op == 1) {
     DevBuf t;
     // An integer sum of a column without nulls already counted every row of each group (word 1).
@@ -1784,6 +1948,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
       t = cache->count_all_tab;
     } else {
       AggArgs a = make_args(ids, nullptr, n, K);
+      a.hot = hot_groups(ids, n, K, cache, hot_holder);
       t = run_agg<CountP<false>>(a, s, "gb_count");
       if (cache) cache->count_all_tab = t;
     }
@@ -1798,6 +1963,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     // A column without nulls has as many non-null values per group as rows.
     if (cache && cache->count_all_tab && !v->has_nulls()) return finalize_simple(cache->count_all_tab, 1, K, 3, T_I64, "l", T_I64);
     AggArgs a = make_args(ids, v, n, K);
+    a.hot = hot_groups(ids, n, K, cache, hot_holder);
     DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
@@ -1821,6 +1987,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     if (cache && cache->cnt_col == v->uid && cache->cnt_tab) {
       t = cache->cnt_tab;
     } else {
+      a.hot = hot_groups(ids, n, K, cache, hot_holder);
       t = run_agg<SumCountP>(a, s, "gb_sum");
       if (cache) { cache->cnt_col = v->uid; cache->cnt_tab = t; cache->cnt_all_rows = !v->has_nulls(); }
     }

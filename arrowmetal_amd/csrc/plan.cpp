@@ -669,7 +669,14 @@ Table run_group_by(const Node& n) {
       else if (op == "mean") code = 3;
       else if (op == "min") code = 4;
       else code = 5;
-      o.cols.emplace_back(group_agg(g, vals[i], code, 0.0));
+      Array* r = group_agg(g, vals[i], code, 0.0);
+      // An unsigned sum is uint64 on both of the reference's plan paths (Executor.swift:308-311, and
+      // ExprGroupBy.swift's sumU); the wrapped 64-bit total has the same bits either way.
+      if (code == 0 && r->type == T_I64 && (vals[i]->type == T_U8 || vals[i]->type == T_U16 || vals[i]->type == T_U32)) {
+        r->format = "L";
+        r->type = T_U64;
+      }
+      o.cols.emplace_back(r);
     }
   } catch (...) {
     group_by_release(g);

@@ -592,7 +592,7 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
 template <typename K, int W, int ITEMS>
 bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
               int placement, int forder, uint32_t* out_idx, hipStream_t s, uint32_t* pos_of_row = nullptr,
-              uint32_t* value_rows = nullptr) {
+              uint32_t* value_rows = nullptr, uint32_t* bucket_base = nullptr) {
   constexpr int P = int(sizeof(K));
   if (pos_of_row && !(P == 8 && n >= (1u << 18) && !capturing(s))) return false;
   constexpr int TILE = BLOCK * ITEMS;
@@ -650,6 +650,9 @@ bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     AM_HIP(hipGetLastError());
   };
   analyse_column(hybrid_ok);
+  if (bucket_base) {   // where each top-digit bucket starts in the value block (the partition's layout)
+    AM_HIP(hipMemcpyAsync(bucket_base, &ctl->digit_base[P - 1][0], RADIX * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  }
   // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
   // that only active passes are launched, over the value rows only.
   struct Plan { uint32_t mask, first, last, m; } plan{0xFFu, 0, uint32_t(P - 1), n};
@@ -746,34 +749,88 @@ void dispatch_items(int items, const void* vals, int type, const uint8_t* valid,
 // (XCD-private chunk counters; a workgroup that finds its eighth done takes chunks of the others), so a
 // bucket's lines are fetched into one L2, not eight.
 
-constexpr uint32_t OB_ROWS = 4;          // rows per thread per step of the partition
-constexpr uint32_t OB_CHUNK = 4096;      // outputs per chunk of the gather (64-row words)
+#ifndef AM_OB_CHUNK
+#define AM_OB_CHUNK 4096
+#endif
+#ifndef AM_OB_GATHER_WG
+#define AM_OB_GATHER_WG 4
+#endif
+constexpr uint32_t OB_CHUNK = AM_OB_CHUNK;   // outputs per chunk of the gather (64-row words)
 constexpr uint32_t OB_GROUPS = OB_CHUNK / 64 / WAVES;   // 64-row groups per wave per chunk
 
+// Each workgroup takes one contiguous range of rows, tile by tile. The partition is stable, so the
+// rows of one tile that belong to one top-digit bucket have consecutive positions p: the tile is
+// staged in LDS in position order (bucket by bucket, found by a binary search of p in the buckets'
+// starts) and written out as runs of consecutive addresses, from this XCD's L2 only.
+constexpr uint32_t OBP_ITEMS = 8;
+constexpr uint32_t OBP_TILE = BLOCK * OBP_ITEMS;
 template <int W>
 __global__ __launch_bounds__(BLOCK) void ob_partition(const void* col, const uint8_t* valid, int64_t voff,
-                                                      uint32_t n, const uint32_t* pos_of_row, void* part,
-                                                      uint8_t* vpart) {
-  const uint32_t stride = gridDim.x * BLOCK * OB_ROWS;
-  for (uint32_t base = blockIdx.x * BLOCK * OB_ROWS + threadIdx.x; base < n; base += stride) {
-    uint64_t v[OB_ROWS];
-    uint32_t p[OB_ROWS], vb[OB_ROWS], bit[OB_ROWS];
+                                                      uint32_t n, uint32_t rows_per_block, const uint32_t* pos_of_row,
+                                                      const uint32_t* bucket_base, void* part, uint8_t* vpart) {
+  __shared__ uint32_t s_base[RADIX];
+  __shared__ uint32_t s_minp[RADIX], s_cnt[RADIX], s_off[RADIX];
+  __shared__ uint32_t tmp[WAVES];
+  __shared__ uint64_t s_val[OBP_TILE];
+  __shared__ uint8_t s_bkt[OBP_TILE];
+  __shared__ uint8_t s_vb[OBP_TILE];
+  s_base[threadIdx.x] = bucket_base[threadIdx.x];
+  s_minp[threadIdx.x] = 0xFFFFFFFFu;
+  s_cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t first = blockIdx.x * rows_per_block;
+  const uint32_t last = min(n, first + rows_per_block);
+  for (uint32_t t0 = first; t0 < last; t0 += OBP_TILE) {
+    uint32_t p[OBP_ITEMS], bk[OBP_ITEMS], vb[OBP_ITEMS], bit[OBP_ITEMS];
+    uint64_t v[OBP_ITEMS];
 #pragma unroll
-    for (uint32_t u = 0; u < OB_ROWS; u++) {   // every load first (rows past the end read the last row)
-      const uint32_t r = min(base + u * BLOCK, n - 1u);
-      p[u] = pos_of_row[r];
-      v[u] = load_raw<W>(col, r);
-      vb[u] = load_valid_byte(valid, voff, r, bit[u]);
+    for (uint32_t j = 0; j < OBP_ITEMS; j++) {   // every load first (rows past the range read its last row)
+      const uint32_t r = min(t0 + j * BLOCK + threadIdx.x, last - 1u);
+      p[j] = pos_of_row[r];
+      v[j] = load_raw<W>(col, r);
+      vb[j] = load_valid_byte(valid, voff, r, bit[j]);
     }
 #pragma unroll
-    for (uint32_t u = 0; u < OB_ROWS; u++) {
-      if (base + u * BLOCK >= n || p[u] == 0xFFFFFFFFu) continue;
-      if constexpr (W == 8) static_cast<uint64_t*>(part)[p[u]] = v[u];
-      else if constexpr (W == 4) static_cast<uint32_t*>(part)[p[u]] = uint32_t(v[u]);
-      else if constexpr (W == 2) static_cast<uint16_t*>(part)[p[u]] = uint16_t(v[u]);
-      else static_cast<uint8_t*>(part)[p[u]] = uint8_t(v[u]);
-      if (vpart) vpart[p[u]] = uint8_t((vb[u] >> bit[u]) & 1u);
+    for (uint32_t j = 0; j < OBP_ITEMS; j++) {
+      bk[j] = 0xFFFFFFFFu;
+      if (t0 + j * BLOCK + threadIdx.x >= last || p[j] == 0xFFFFFFFFu) continue;
+      uint32_t lo = 0, hi = RADIX;   // the last bucket whose start is <= p
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_base[mid] <= p[j]) lo = mid;
+        else hi = mid;
+      }
+      bk[j] = lo;
+      atomicMin(&s_minp[lo], p[j]);
+      atomicAdd(&s_cnt[lo], 1u);
     }
+    __syncthreads();
+    uint32_t total;
+    s_off[threadIdx.x] = block_excl_scan(s_cnt[threadIdx.x], tmp, total);
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < OBP_ITEMS; j++) {
+      if (bk[j] == 0xFFFFFFFFu) continue;
+      const uint32_t k = s_off[bk[j]] + (p[j] - s_minp[bk[j]]);
+      if (k >= OBP_TILE) continue;   // cannot happen for a stable partition; keeps LDS writes in bounds
+      s_val[k] = v[j];
+      s_bkt[k] = uint8_t(bk[j]);
+      s_vb[k] = uint8_t((vb[j] >> bit[j]) & 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < total; k += BLOCK) {
+      const uint32_t b = s_bkt[k];
+      const uint32_t dest = s_minp[b] + (k - s_off[b]);
+      if constexpr (W == 8) static_cast<uint64_t*>(part)[dest] = s_val[k];
+      else if constexpr (W == 4) static_cast<uint32_t*>(part)[dest] = uint32_t(s_val[k]);
+      else if constexpr (W == 2) static_cast<uint16_t*>(part)[dest] = uint16_t(s_val[k]);
+      else static_cast<uint8_t*>(part)[dest] = uint8_t(s_val[k]);
+      if (vpart) vpart[dest] = s_vb[k];
+    }
+    __syncthreads();
+    s_minp[threadIdx.x] = 0xFFFFFFFFu;
+    s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
   }
 }
 
@@ -819,7 +876,7 @@ __global__ __launch_bounds__(BLOCK) void ob_gather(const uint32_t* out_idx, uint
         const void* src = in_val[g] ? part : col;
         v[g] = load_raw<W>(src, idx[g]);
         if (in_val[g]) {
-          vb[g] = vpart ? vpart[idx[g]] : 1u;
+          vb[g] = vpart ? load_global(vpart + idx[g]) : 1u;
           bit[g] = 0u;
         } else {
           vb[g] = load_valid_byte(valid, voff, idx[g], bit[g]);
@@ -850,8 +907,9 @@ __global__ void ob_clear(uint32_t* ctr) {
 }
 
 template <int W>
-void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, const uint32_t* out_idx, uint32_t n,
-                    uint32_t vbase, uint32_t m, Array* out, uint32_t* ctr, hipStream_t s) {
+void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, const uint32_t* bucket_base,
+                    const uint32_t* out_idx, uint32_t n, uint32_t vbase, uint32_t m, Array* out, uint32_t* ctr,
+                    hipStream_t s) {
   DevBuf part = device_alloc(size_t(std::max<uint32_t>(m, 1)) * W);
   DevBuf vpart;
   if (nulls) vpart = device_alloc(std::max<uint32_t>(m, 1));
@@ -860,16 +918,18 @@ void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, cons
   const uint8_t* valid = nulls ? c.valid_bits() : nullptr;
   {
     ProfScope ps("order_by_partition");
-    const uint32_t blocks = std::max(1u, std::min(uint32_t(device().cu_count) * 8u, (n + BLOCK * OB_ROWS - 1) / (BLOCK * OB_ROWS)));
-    hipLaunchKernelGGL((ob_partition<W>), dim3(blocks), dim3(BLOCK), 0, s, c.data(), valid, c.offset, n, pos_of_row,
-                       part->ptr, nulls ? static_cast<uint8_t*>(vpart->ptr) : nullptr);
+    const uint32_t tiles = (n + OBP_TILE - 1) / OBP_TILE;
+    const uint32_t blocks = std::max(1u, std::min(uint32_t(device().cu_count) * 4u, tiles));
+    const uint32_t rpb = (tiles + blocks - 1) / blocks * OBP_TILE;   // whole tiles per workgroup
+    hipLaunchKernelGGL((ob_partition<W>), dim3(blocks), dim3(BLOCK), 0, s, c.data(), valid, c.offset, n, rpb,
+                       pos_of_row, bucket_base, part->ptr, nulls ? static_cast<uint8_t*>(vpart->ptr) : nullptr);
     AM_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(ob_clear, dim3(1), dim3(BLOCK), 0, s, ctr);
   AM_HIP(hipGetLastError());
   {
     ProfScope ps("order_by_gather");
-    const uint32_t blocks = uint32_t(device().cu_count) * 4u;
+    const uint32_t blocks = uint32_t(device().cu_count) * AM_OB_GATHER_WG;   // workgroups per CU
     hipLaunchKernelGGL((ob_gather<W>), dim3(blocks), dim3(BLOCK), 0, s, out_idx, n, vbase, m, part->ptr,
                        nulls ? static_cast<const uint8_t*>(vpart->ptr) : nullptr, c.data(), valid, c.offset,
                        out->values->ptr, nulls ? static_cast<unsigned long long*>(out->validity->ptr) : nullptr, ctr);
@@ -879,10 +939,10 @@ void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, cons
 
 template <int ITEMS>
 bool ob_sort(const Array& key, bool desc, int placement, int forder, uint32_t* out_idx, uint32_t* pos_of_row,
-             uint32_t* m, hipStream_t s) {
+             uint32_t* m, uint32_t* bucket_base, hipStream_t s) {
   return run_sort<unsigned long long, 8, ITEMS>(key.data(), key.type, key.has_nulls() ? key.valid_bits() : nullptr,
                                                 key.offset, uint32_t(key.length), desc, placement, forder, out_idx,
-                                                s, pos_of_row, m);
+                                                s, pos_of_row, m, bucket_base);
 }
 
 }  // namespace
@@ -947,20 +1007,23 @@ bool order_by_columns(const Array& key, bool descending, int null_placement, int
   check_index_rows(n, "argsort");
   hipStream_t s = stream();
   DevBuf idx = device_alloc(size_t(n) * 4), pos = device_alloc(size_t(n) * 4), ctr = device_alloc(NXCD * OB_CTR_STRIDE * 4);
+  DevBuf bb = device_alloc(RADIX * 4);
   retain(idx);
   retain(pos);
   retain(ctr);
+  retain(bb);
   auto* out_idx = static_cast<uint32_t*>(idx->ptr);
   auto* pos_of_row = static_cast<uint32_t*>(pos->ptr);
+  auto* bbase = static_cast<uint32_t*>(bb->ptr);
   uint32_t m = 0;
   const int place = null_placement == 1 ? 1 : 0;
   bool ok;
   switch (sort_tune(n, 8).items) {
-    case 4: ok = ob_sort<4>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
-    case 8: ok = ob_sort<8>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
-    case 12: ok = ob_sort<12>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
-    case 20: ok = ob_sort<20>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
-    default: ok = ob_sort<16>(key, descending, place, float_order, out_idx, pos_of_row, &m, s); break;
+    case 4: ok = ob_sort<4>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
+    case 8: ok = ob_sort<8>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
+    case 12: ok = ob_sort<12>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
+    case 20: ok = ob_sort<20>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
+    default: ok = ob_sort<16>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
   }
   if (!ok) return false;
   // The value block: [0, m) with nulls last, [n - m, n) with nulls first.
@@ -976,10 +1039,10 @@ bool order_by_columns(const Array& key, bool descending, int null_placement, int
       if (nulls > 0) o->validity = device_alloc(size_t((n + 63) / 64) * 8);
       auto* ctrp = static_cast<uint32_t*>(ctr->ptr);
       switch (c->width()) {
-        case 8: ob_move_column<8>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
-        case 4: ob_move_column<4>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
-        case 2: ob_move_column<2>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
-        default: ob_move_column<1>(*c, nulls > 0, pos_of_row, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        case 8: ob_move_column<8>(*c, nulls > 0, pos_of_row, bbase, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        case 4: ob_move_column<4>(*c, nulls > 0, pos_of_row, bbase, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        case 2: ob_move_column<2>(*c, nulls > 0, pos_of_row, bbase, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
+        default: ob_move_column<1>(*c, nulls > 0, pos_of_row, bbase, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
       }
     }
     sync();

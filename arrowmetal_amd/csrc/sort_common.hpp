@@ -42,10 +42,10 @@ __device__ const uint8_t kAllValid = 0xFFu;
 
 template <int W>
 __device__ __forceinline__ uint64_t load_raw(const void* vals, uint32_t i) {
-  if constexpr (W == 8) return static_cast<const uint64_t*>(vals)[i];
-  else if constexpr (W == 4) return static_cast<const uint32_t*>(vals)[i];
-  else if constexpr (W == 2) return static_cast<const uint16_t*>(vals)[i];
-  else return static_cast<const uint8_t*>(vals)[i];
+  if constexpr (W == 8) return load_global(static_cast<const uint64_t*>(vals) + i);
+  else if constexpr (W == 4) return load_global(static_cast<const uint32_t*>(vals) + i);
+  else if constexpr (W == 2) return load_global(static_cast<const uint16_t*>(vals) + i);
+  else return load_global(static_cast<const uint8_t*>(vals) + i);
 }
 
 // The validity byte of row i and the bit of row i in it.
@@ -53,7 +53,7 @@ __device__ __forceinline__ uint32_t load_valid_byte(const uint8_t* valid, int64_
   const int64_t b = voff + int64_t(i);
   const uint8_t* p = valid ? valid + (b >> 3) : &kAllValid;
   bit = valid ? uint32_t(b & 7) : 0u;
-  return *p;
+  return load_global(p);
 }
 
 // Category of a row (0 value, 1 NaN in its own block, 2 null) from its raw bits and validity and,
