@@ -134,6 +134,7 @@ _SIGNATURES = {
     "am_plan_column_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
     "am_plan_column": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
     "am_plan_result_release": ([_P], None),
+    "am_group_by_noalloc": ([_P, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P], ctypes.c_int),
     "am_set_max_row_index": ([ctypes.c_int64], ctypes.c_int64),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),

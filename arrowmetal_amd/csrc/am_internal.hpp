@@ -225,6 +225,10 @@ Array* group_by_ids(GroupBy* g);
 Array* group_agg(GroupBy* g, const Array* values, int op, double p1);
 void group_by_release(GroupBy* g);
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values);
+// group_by_dense's integer aggregates into caller device buffers, allocating nothing (a fixed static
+// device workspace; key_count <= 65536).
+void group_by_noalloc(const Array& keys, int64_t key_count, int agg, const Array* values, void* out_values,
+                      uint64_t* out_validity, int64_t* out_null_count);
 Array* unique_rows(Array* const* columns, int64_t count);
 Array* unique_values(const Array& a, int order);
 

@@ -419,6 +419,14 @@ void am_plan_result_release(am_plan_result* r) {
   if (r) am::plan_result_release(reinterpret_cast<am::PlanResult*>(r));
 }
 
+int am_group_by_noalloc(am_array* keys, int64_t key_count, int agg, am_array* values, void* out_values,
+                        uint64_t* out_validity, int64_t* out_null_count) {
+  if (!keys || !out_values || !out_validity || !out_null_count) return 2;
+  return guarded([&] {
+    am::group_by_noalloc(*A(keys), key_count, agg, OA(values), out_values, out_validity, out_null_count);
+  });
+}
+
 int64_t am_set_max_row_index(int64_t max_row) { return am::set_max_row_index(max_row); }
 
 int am_prof_enable(int on) {

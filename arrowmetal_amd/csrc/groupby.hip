@@ -1155,19 +1155,16 @@ void launch_agg_vw(int vw, int blocks, size_t shmem, hipStream_t s, const AggArg
   }
 }
 
-// Runs family P over the rows into a fresh global table of K x W words.
+// Runs family P over the rows into `o` (K x W words), with `g8` (NXCD x K x W words) for the XCDs'
+// partial tables; allocates nothing.
 template <class P>
-DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
+void run_agg_into(const AggArgs& a, hipStream_t s, const char* name, unsigned long long* g8, unsigned long long* o) {
   const long long K = std::max<long long>(a.K, 1);
-  DevBuf out = device_alloc(size_t(K) * P::W * 8);
-  auto* o = static_cast<unsigned long long*>(out->ptr);
   if (a.n == 0) {
     hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(K * P::W, 4)), dim3(256), 0, s, o, K);
     AM_HIP(hipGetLastError());
-    return out;
+    return;
   }
-  DevBuf tabs = device_alloc(size_t(NXCD) * size_t(K) * P::W * 8);
-  auto* g8 = static_cast<unsigned long long*>(tabs->ptr);
   hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(NXCD * K * P::W, 4)), dim3(256), 0, s, g8, NXCD * K);
   AM_HIP(hipGetLastError());
   // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
@@ -1200,6 +1197,17 @@ DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
   }
   hipLaunchKernelGGL(gb_agg_merge<P>, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s, g8, K, o);
   AM_HIP(hipGetLastError());
+}
+
+// Runs family P over the rows into a fresh global table of K x W words.
+template <class P>
+DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
+  const long long K = std::max<long long>(a.K, 1);
+  DevBuf out = device_alloc(size_t(K) * P::W * 8);
+  DevBuf tabs;
+  if (a.n > 0) tabs = device_alloc(size_t(NXCD) * size_t(K) * P::W * 8);
+  run_agg_into<P>(a, s, name, tabs ? static_cast<unsigned long long*>(tabs->ptr) : nullptr,
+                  static_cast<unsigned long long*>(out->ptr));
   return out;
 }
 
@@ -2080,6 +2088,100 @@ Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array
   if (values->length != keys.length) fail_length_mismatch(keys.length, values->length);
   if (agg == 0 && vt == T_F32) return sum_f32_as_f64(ids, keys.length, key_count, *values);
   return aggregate(ids, keys.length, key_count, values, op_of[agg], nullptr);
+}
+
+// ---------------------------------------------------------------- dense group-by without allocation
+//
+// am_group_by_noalloc: the integer aggregates of am_group_by written into caller buffers, for callers
+// that cannot allocate device memory. Its only memory besides the inputs and outputs is `scratch`, a
+// fixed array in every GPU's global memory that holds the XCDs' partial tables [8][K][2], the merged
+// table [K][2] and K validity bytes for at most NOALLOC_MAX_KEYS groups (about 9 MiB, held by each GPU
+// the library runs on until the process ends). Every call on a GPU shares that GPU's array: calls that
+// overlap on one GPU race; each call initializes what it reads (fill kernels, no memset).
+constexpr long long NOALLOC_MAX_KEYS = 65536;
+constexpr size_t NOALLOC_G8_WORDS = size_t(NXCD) * NOALLOC_MAX_KEYS * 2;
+constexpr size_t NOALLOC_TAB_WORDS = size_t(NOALLOC_MAX_KEYS) * 2;
+constexpr size_t NOALLOC_VB_WORDS = size_t(NOALLOC_MAX_KEYS) / 8;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_G8_WORDS + NOALLOC_TAB_WORDS + NOALLOC_VB_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// The device address of this GPU's `scratch` (the host-side name is only a stand-in for it), looked up
+// once per device with that device current.
+unsigned long long* scratch_words() {
+  int dev = 0;
+  AM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) fail("am_group_by_noalloc: device " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    AM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail("am_group_by_noalloc: no device address for the workspace");
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<unsigned long long*>(g_scratch_ptr[dev]);
+}
+
+template <class P>
+void noalloc_run(const AggArgs& a, int kind, int vtype, void* out_values, unsigned long long* out_valid,
+                 unsigned long long* out_nulls, unsigned long long* ws, hipStream_t s) {
+  unsigned long long* g8 = ws;
+  unsigned long long* tab = ws + NOALLOC_G8_WORDS;
+  uint8_t* vb = reinterpret_cast<uint8_t*>(ws + NOALLOC_G8_WORDS + NOALLOC_TAB_WORDS);
+  run_agg_into<P>(a, s, "gb_noalloc", g8, tab);
+  const long long K = a.K;
+  hipLaunchKernelGGL(gb_finalize_simple, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s, tab, P::W, K, kind, vtype,
+                     out_values, vb);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(gb_agg_init<CountP<false>>, dim3(1), dim3(64), 0, s, out_nulls, 1);
+  AM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(pack_validity, dim3(grid_for(K, 4)), dim3(256), 0, s, vb, K, out_valid, out_nulls);
+  AM_HIP(hipGetLastError());
+}
+}  // namespace
+
+void group_by_noalloc(const Array& keys, int64_t key_count, int agg, const Array* values, void* out_values,
+                      uint64_t* out_validity, int64_t* out_null_count) {
+  if (key_count <= 0 || key_count > int64_t(UINT32_MAX) / 4) fail_invalid("keyCount out of range");
+  if (key_count > NOALLOC_MAX_KEYS)
+    fail_invalid("am_group_by_noalloc: key_count " + std::to_string(key_count) + " exceeds the " +
+                 std::to_string(NOALLOC_MAX_KEYS) + " groups of its fixed workspace");
+  if (keys.type != T_I32 && keys.type != T_I64 && keys.type != T_U32)
+    fail("am_group_by: keys must be int32, int64 or uint32, got " + keys.format);
+  check_length(keys.length);
+  if (agg < 0 || agg > 5) fail_unsupported("group-by agg " + std::to_string(agg));
+  if (agg != 1) {
+    if (!values) fail_invalid("values required for this aggregation");
+    const int vt = values->type;
+    if (agg == 0 && vt == T_F32)
+      fail_unsupported("am_group_by_noalloc does not take the float32 sum (its exact sum needs workspace "
+                       "that grows with the groups); use am_group_by");
+    const bool taken = agg == 5 || ((agg == 0 || agg == 4) && type_is_int(vt)) ||
+                       ((agg == 2 || agg == 3) && type_is_primitive(vt) && values->width() <= 4);
+    if (!taken) {
+      if ((agg == 2 || agg == 3) && type_is_primitive(vt))
+        fail_unsupported("group-by min/max needs a 32-bit or narrower type");
+      fail_unsupported("group-by agg " + std::to_string(agg) + " on " + values->format);
+    }
+    if (values->length != keys.length) fail_length_mismatch(keys.length, values->length);
+  }
+  if (!out_values || !out_validity || !out_null_count) fail_invalid("am_group_by_noalloc: NULL output buffer");
+  unsigned long long* ws = scratch_words();
+  hipStream_t s = stream();
+  IdSource ids{keys.data(), keys.type, keys.has_nulls() ? keys.valid_bits() : nullptr, keys.offset};
+  AggArgs a = make_args(ids, agg == 1 ? nullptr : values, keys.length, key_count);
+  auto* ov = reinterpret_cast<unsigned long long*>(out_validity);
+  auto* on = reinterpret_cast<unsigned long long*>(out_null_count);
+  switch (agg) {
+    case 1: noalloc_run<CountP<false>>(a, 3, T_I64, out_values, ov, on, ws, s); break;
+    case 5: noalloc_run<CountP<true>>(a, 3, T_I64, out_values, ov, on, ws, s); break;
+    case 0: noalloc_run<SumCountP>(a, 0, values->type, out_values, ov, on, ws, s); break;
+    case 4: noalloc_run<SumCountP>(a, type_is_signed(values->type) ? 1 : 2, values->type, out_values, ov, on, ws, s); break;
+    default: noalloc_run<MinMaxP>(a, agg == 2 ? 4 : 5, values->type, out_values, ov, on, ws, s); break;
+  }
+  AM_HIP(hipStreamSynchronize(s));
 }
 
 Array* unique_rows(Array* const* columns, int64_t count) {

@@ -187,6 +187,20 @@ void        am_plan_result_release(am_plan_result* r);
  * (reference Sources/ArrowMetal/Engine/Executor.swift:134-148), and measurement.
  * ------------------------------------------------------------------------------------------- */
 
+/* No counterpart in the reference: am_group_by for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The integer
+ * aggregates of am_group_by (agg 0 sum, 1 count rows, 2 min, 3 max of 32-bit or narrower values,
+ * 4 mean, 5 count values; not the float32 sum) over key_count <= 65536 groups, written into caller
+ * device buffers: out_values holds key_count elements (int64 for sums and counts, uint64 for sums of
+ * uint64, float64 for means, the values' type for min / max), out_validity ceil(key_count / 64)
+ * little-endian bitmap words (bit k: group k has a value), out_null_count one int64. The only other
+ * memory it touches is a fixed workspace in each GPU's global memory (about 9 MiB, held until the
+ * process ends); a larger key_count returns 1 with "Invalid ArrowArray: am_group_by_noalloc: key_count
+ * N exceeds the 65536 groups of its fixed workspace". Calls that overlap on one GPU share that
+ * workspace and race. Results are ready when the call returns. */
+int  am_group_by_noalloc(am_array* keys, int64_t key_count, int agg, am_array* values, void* out_values,
+                         uint64_t* out_validity, int64_t* out_null_count);
+
 /* Keep-first unique rows over `count` key columns: the lowest row of every distinct key combination,
  * ascending, as uint32 row numbers (take them with am_take). */
 int  am_unique_rows(am_array** columns, int64_t count, am_array** out);
