@@ -135,6 +135,19 @@ _SIGNATURES = {
     "am_plan_column": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
     "am_plan_result_release": ([_P], None),
     "am_group_by_noalloc": ([_P, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P], ctypes.c_int),
+    "am_import_chunks": ([_P, _P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_import_chunks_supported": ([_P], ctypes.c_int),
+    "am_query": ([ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int64, ctypes.c_char_p,
+                  ctypes.POINTER(_P)], ctypes.c_int),
+    "am_query_column_count": ([_P], ctypes.c_int64),
+    "am_query_column_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
+    "am_query_column": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_query_scalar_count": ([_P], ctypes.c_int64),
+    "am_query_scalar_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
+    "am_query_scalar": ([_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "am_query_result_release": ([_P], None),
+    "am_query_canonical": ([ctypes.c_char_p], ctypes.c_char_p),
     "am_set_max_row_index": ([ctypes.c_int64], ctypes.c_int64),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
@@ -263,6 +276,53 @@ class MetalArray:
             arr.release(ctypes.byref(arr))
         _check(rc)
         return cls(out)
+
+    @classmethod
+    def from_chunks(cls, chunks, fmt=None):
+        """One array from several chunks of one type (`am_import_chunks`): a list of (values, valid,
+        offset, length) tuples of NumPy buffers (valid: a bool mask, an Arrow bitmap with null_count -1,
+        or None), or a pyarrow ChunkedArray. Returns None when the library does not take the type (3)."""
+        try:
+            import pyarrow as pa
+            is_chunked = isinstance(chunks, pa.ChunkedArray)
+        except ImportError:
+            is_chunked = False
+        n = len(chunks.chunks) if is_chunked else len(chunks)
+        arrs = (_ArrowArray * max(n, 1))()
+        schema = _ArrowSchema()
+        extra = []
+        try:
+            if is_chunked:
+                for i, c in enumerate(chunks.chunks):
+                    sc = schema if i == 0 else _ArrowSchema()
+                    c._export_to_c(ctypes.addressof(arrs[i]), ctypes.addressof(sc))
+                    if i:
+                        extra.append(sc)
+                if n == 0:
+                    chunks.type._export_to_c(ctypes.addressof(schema))
+            else:
+                first = np.ascontiguousarray(chunks[0][0]) if n else np.zeros(0, dtype=np.int64)
+                schema.format = (fmt or _NP_FORMAT[first.dtype]).encode()
+                schema.name = b""
+                schema.flags = 2
+                for i, (v, valid, off, ln) in enumerate(chunks):
+                    a = _c_array(v, valid, off, ln)
+                    extra.append(a)   # its token keys the buffers in _keep until the release runs
+                    ctypes.memmove(ctypes.addressof(arrs[i]), ctypes.addressof(a), ctypes.sizeof(_ArrowArray))
+            out = _P()
+            rc = lib().am_import_chunks(ctypes.addressof(schema), ctypes.addressof(arrs), n, ctypes.byref(out))
+            if rc == 3:
+                return None
+            _check(rc)
+            return cls(out)
+        finally:
+            for i in range(n):
+                if arrs[i].release:
+                    arrs[i].release(ctypes.byref(arrs[i]))
+            if is_chunked:
+                for sc in [schema] + extra:
+                    if sc.release:
+                        sc.release(ctypes.byref(sc))
 
     @classmethod
     def from_arrow(cls, obj):
@@ -520,6 +580,33 @@ class PlanSource:
         h, self._h = getattr(self, "_h", None), None
         if h and _lib is not None:
             _lib.am_plan_source_release(h)
+
+
+def query(columns, text):
+    """am_query over named columns ({name: MetalArray or NumPy array}); returns {name: MetalArray}."""
+    cols = {k: (v if isinstance(v, MetalArray) else MetalArray.from_numpy(v)) for k, v in columns.items()}
+    names = list(cols)
+    handles = (_P * max(len(names), 1))(*[cols[k]._h for k in names])
+    cnames = (ctypes.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+    out = _P()
+    L = lib()
+    _check(L.am_query(handles, cnames, len(names), text.encode(), ctypes.byref(out)))
+    try:
+        res = {}
+        for i in range(L.am_query_column_count(out)):
+            h = _P()
+            _check(L.am_query_column(out, i, ctypes.byref(h)))
+            res[L.am_query_column_name(out, i).decode()] = MetalArray(h)
+        return res
+    finally:
+        L.am_query_result_release(out)
+
+
+def query_canonical(text):
+    r = lib().am_query_canonical(text.encode())
+    if r is None:
+        raise ArrowMetalError((lib().am_last_error() or b"unknown error").decode())
+    return r.decode()
 
 
 def _plan_text(plan):

@@ -9,6 +9,7 @@
 //   groupby.hip   the hash group-by: key table build, ascending relabel, grouped aggregates
 //   select.hip    sorted values, top-k by radix select, lexsort
 //   plan.cpp      the plan engine's scan / sort / group_by / unique / limit subset (am_plan_*)
+//   query.cpp     am_query's fused dense group-by subset
 //   batched.hip   the batched plan (am_batched_*); serve.cpp the request server (am_server_*)
 //   capi.cpp      the extern "C" am_* entry points of include/arrowmetal.h
 #pragma once
@@ -225,6 +226,11 @@ Array* group_by_ids(GroupBy* g);
 Array* group_agg(GroupBy* g, const Array* values, int op, double p1);
 void group_by_release(GroupBy* g);
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values);
+// group_by_dense with every key offset by `bias` before the range check, and the integer values
+// transformed before they are aggregated: xop 0 none, 1 shift right by xarg (arithmetic for signed
+// columns), 2 bitwise and with xarg.
+Array* group_by_dense_ex(const Array& keys, int64_t key_count, long long bias, int agg, const Array* values, int xop,
+                         long long xarg);
 // group_by_dense's integer aggregates into caller device buffers, allocating nothing (a fixed static
 // device workspace; key_count <= 65536).
 void group_by_noalloc(const Array& keys, int64_t key_count, int agg, const Array* values, void* out_values,
@@ -244,6 +250,15 @@ int64_t plan_row_count(const PlanResult* r);
 const char* plan_column_name(const PlanResult* r, int64_t i);
 Array* plan_column(const PlanResult* r, int64_t i);
 void plan_result_release(PlanResult* r);
+
+// query.cpp: am_query's fused dense group-by subset.
+struct QueryResult;
+QueryResult* query_run(Array* const* cols, const char* const* names, int64_t n, const char* text);
+const char* query_canonical(const char* text);
+int64_t query_column_count(const QueryResult* r);
+const char* query_column_name(const QueryResult* r, int64_t i);
+Array* query_column(const QueryResult* r, int64_t i);
+void query_result_release(QueryResult* r);
 
 // Launch-shape parameters the autotuner chooses (runtime.cpp keeps the table).
 struct SortTune {

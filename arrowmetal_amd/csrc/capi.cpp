@@ -14,6 +14,8 @@ namespace am {
 const char* last_error_cstr();
 Array* import_host(const ArrowSchema* schema, ArrowArray* array);
 Array* import_device(const ArrowSchema* schema, ArrowDeviceArray* array);
+bool chunks_supported(const ArrowSchema* schema);
+Array* import_chunks(const ArrowSchema* schema, ArrowArray* arrays, int64_t n);
 void export_host(Array* a, ArrowSchema* schema, ArrowArray* array);
 void export_device(Array* a, ArrowSchema* schema, ArrowDeviceArray* array);
 void batch_begin();
@@ -87,6 +89,18 @@ int am_import(const struct ArrowSchema* schema, struct ArrowArray* array, am_arr
   if (!schema || !array || !out) return 2;
   return guarded([&] { *out = H(am::import_host(schema, array)); });
 }
+
+int am_import_chunks(const struct ArrowSchema* schema, struct ArrowArray* arrays, int64_t n, am_array** out) {
+  if (!schema || n < 0 || (n > 0 && !arrays) || !out) return 2;
+  if (!am::chunks_supported(schema)) {
+    am::set_last_error(std::string("Unsupported Arrow type: am_import_chunks does not take ") +
+                       (schema->format ? schema->format : "(null format)") + " arrays; import their concatenation");
+    return 3;
+  }
+  return guarded([&] { *out = H(am::import_chunks(schema, arrays, n)); });
+}
+
+int am_import_chunks_supported(const struct ArrowSchema* schema) { return am::chunks_supported(schema) ? 1 : 0; }
 
 int am_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array, am_array** out) {
   if (!schema || !array || !out) return 2;
@@ -354,6 +368,50 @@ int am_unique_rows(am_array** columns, int64_t count, am_array** out) {
   for (int64_t i = 0; i < count; i++)
     if (!columns[i]) return 2;
   return guarded([&] { *out = H(am::unique_rows(reinterpret_cast<Array* const*>(columns), count)); });
+}
+
+// ---- am_query (reference Sources/ArrowMetalC/ArrowMetalC_Expr.swift:36-123)
+
+int am_query(am_array** columns, const char** names, int64_t n_columns, const char* expr_text,
+             am_query_result** out) {
+  if (!columns || !names || !expr_text || !out || n_columns < 0) return 2;
+  for (int64_t i = 0; i < n_columns; i++)
+    if (!columns[i] || !names[i]) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_query_result*>(
+        am::query_run(reinterpret_cast<Array* const*>(columns), names, n_columns, expr_text));
+  });
+}
+
+int64_t am_query_column_count(am_query_result* r) {
+  return r ? am::query_column_count(reinterpret_cast<am::QueryResult*>(r)) : -1;
+}
+
+const char* am_query_column_name(am_query_result* r, int64_t i) {
+  if (!r || i < 0 || i >= am::query_column_count(reinterpret_cast<am::QueryResult*>(r))) return nullptr;
+  return am::query_column_name(reinterpret_cast<am::QueryResult*>(r), i);
+}
+
+int am_query_column(am_query_result* r, int64_t i, am_array** out) {
+  if (!r || !out || i < 0 || i >= am::query_column_count(reinterpret_cast<am::QueryResult*>(r))) return 2;
+  *out = H(am::query_column(reinterpret_cast<am::QueryResult*>(r), i));
+  return 0;
+}
+
+// The queries taken here end in a group_by terminal: no scalars.
+int64_t am_query_scalar_count(am_query_result* r) { return r ? 0 : -1; }
+const char* am_query_scalar_name(am_query_result*, int64_t) { return nullptr; }
+int am_query_scalar(am_query_result*, int64_t, int64_t*, double*, int*, int*) { return 2; }
+
+void am_query_result_release(am_query_result* r) {
+  if (r) am::query_result_release(reinterpret_cast<am::QueryResult*>(r));
+}
+
+const char* am_query_canonical(const char* expr_text) {
+  if (!expr_text) return nullptr;
+  const char* text = nullptr;
+  if (guarded([&] { text = am::query_canonical(expr_text); }) != 0) return nullptr;
+  return text;
 }
 
 // ---- the plan engine (reference Sources/ArrowMetalC/ArrowMetalC_Plan.swift)

@@ -583,6 +583,7 @@ struct IdSource {
   int type;               // T_I32, T_I64 or T_U32
   const uint8_t* valid;   // key validity, dense form only
   int64_t voff;
+  long long bias = 0;     // added to every key before the range check (am_query's (sub key N))
 };
 
 __device__ __forceinline__ long long load_gid(const IdSource& s, int64_t i, long long K) {
@@ -591,14 +592,20 @@ __device__ __forceinline__ long long load_gid(const IdSource& s, int64_t i, long
   else if (s.type == T_U32) g = ld_stream<uint32_t>(s.ids, i);
   else g = ld_stream<long long>(s.ids, i);
   if (s.valid && !bit_get(s.valid, s.voff + i)) return -1;
+  g += s.bias;
   return (g < 0 || g >= K) ? -1 : g;
 }
+
+// Integer transforms of a value column before it is aggregated (am_query's (shr c N), (bit_and c N)).
+enum ValOp : int { VX_NONE = 0, VX_SHR = 1, VX_AND = 2 };
 
 struct ValSource {
   const void* vals;
   int type;
   const uint8_t* valid;
   int64_t voff;
+  int xop = VX_NONE;      // applied to the widened 64-bit value
+  long long xarg = 0;
 };
 
 __device__ __forceinline__ unsigned long long raw64(const ValSource& v, int64_t i) {
@@ -626,6 +633,19 @@ __device__ __forceinline__ unsigned long long val_of_raw(unsigned long long r, i
     case T_I8: return (unsigned long long)(long long)int8_t(uint8_t(r));
     default: return r;
   }
+}
+
+// The value a row contributes: its widened word, transformed when the column asks for it. A shift
+// right of a signed value is arithmetic (Arrow's shift_right); the transforms take integer columns.
+__device__ __forceinline__ unsigned long long val_in(const ValSource& v, unsigned long long r) {
+  unsigned long long x = val_of_raw(r, v.type);
+  if (v.xop == VX_SHR) {
+    const int sh = int(v.xarg & 63);
+    x = type_is_signed(v.type) ? (unsigned long long)((long long)x >> sh) : x >> sh;
+  } else if (v.xop == VX_AND) {
+    x &= (unsigned long long)v.xarg;
+  }
+  return x;
 }
 
 // Aggregate families. Each has W 64-bit words per group with an op per word, an identity per word,
@@ -936,6 +956,7 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
       long long g;
       if constexpr (IDW == 8) g = (long long)cur.idr[r];
       else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(cur.idr[r])) : (long long)uint32_t(cur.idr[r]);
+      g += a.ids.bias;
       if (i >= end || !cur.iok[r] || g < 0 || g >= a.K || !cur.vok[r]) g = -1;
       gs[r] = g;
     }
@@ -946,13 +967,13 @@ __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, ui
 #pragma unroll
       for (int r = 0; r < AGG_ROWS; r++) {
         const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(cur.vr[r], a.v.type), st[r], cs[r])) gs[r] = -1;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), st[r], cs[r])) gs[r] = -1;
       }
     } else {
 #pragma unroll
       for (int r = 0; r < AGG_ROWS; r++) {
         const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_of_raw(cur.vr[r], a.v.type), cs[r])) gs[r] = -1;
+        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), cs[r])) gs[r] = -1;
       }
     }
     if (HOTREG && hot[0] >= 0) {   // uniform; the sampler fills hot[] most frequent first
@@ -1944,7 +1965,13 @@ const long long* hot_groups(const IdSource& ids, int64_t n, long long K, GroupBy
   return static_cast<const long long*>(holder->ptr);
 }
 
-Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, int op, GroupBy* cache) {
+// A transform of the value column (am_query's (shr c N) / (bit_and c N)).
+struct ValX {
+  int op = VX_NONE;
+  long long arg = 0;
+};
+
+Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, int op, GroupBy* cache, ValX vx = {}) {
   hipStream_t s = stream();
   DevBuf hot_holder;
   if (op == 1) {
@@ -1978,6 +2005,10 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
   if (!type_is_primitive(v->type))
     fail_unsupported("grouped aggregate over a " + v->format + " column is not supported by this build");
   AggArgs a = make_args(ids, v, n, K);
+  a.v.xop = vx.op;
+  a.v.xarg = vx.arg;
+  if (vx.op != VX_NONE && (cache || !type_is_int(v->type)))
+    fail("a value transform takes the dense group-by of an integer column");
   if (op == 0 || op == 3) {
     if (type_is_float(v->type)) {
       // float64: correctly rounded (GroupSumExact.swift); float32: summed as float64 values, the
@@ -2064,11 +2095,21 @@ Array* sum_f32_as_f64(const IdSource& ids, int64_t n, long long K, const Array& 
 }
 
 Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array* values) {
+  return group_by_dense_ex(keys, key_count, 0, agg, values, VX_NONE, 0);
+}
+
+// The dense group-by with the key offset by `bias` before the range check and the values transformed
+// by `xop` / `xarg` (am_query's (sub key N), (shr c N), (bit_and c N)).
+Array* group_by_dense_ex(const Array& keys, int64_t key_count, long long bias, int agg, const Array* values, int xop,
+                         long long xarg) {
   if (key_count <= 0 || key_count > int64_t(UINT32_MAX) / 4) fail_invalid("keyCount out of range");
   if (keys.type != T_I32 && keys.type != T_I64 && keys.type != T_U32)
     fail("am_group_by: keys must be int32, int64 or uint32, got " + keys.format);
   check_length(keys.length);
-  IdSource ids{keys.data(), keys.type, keys.has_nulls() ? keys.valid_bits() : nullptr, keys.offset};
+  IdSource ids{keys.data(), keys.type, keys.has_nulls() ? keys.valid_bits() : nullptr, keys.offset, bias};
+  ValX vx;
+  vx.op = xop;
+  vx.arg = xarg;
   // agg: 0 sum, 1 count rows, 2 min, 3 max, 4 mean, 5 count values (ArrowMetalC.swift:416-469).
   static const int op_of[6] = {0, 1, 4, 5, 3, 2};
   if (agg == 1) return aggregate(ids, keys.length, key_count, nullptr, 1, nullptr);
@@ -2086,8 +2127,8 @@ Array* group_by_dense(const Array& keys, int64_t key_count, int agg, const Array
     fail_unsupported("group-by agg " + std::to_string(agg) + " on " + values->format);
   }
   if (values->length != keys.length) fail_length_mismatch(keys.length, values->length);
-  if (agg == 0 && vt == T_F32) return sum_f32_as_f64(ids, keys.length, key_count, *values);
-  return aggregate(ids, keys.length, key_count, values, op_of[agg], nullptr);
+  if (agg == 0 && vt == T_F32 && xop == VX_NONE) return sum_f32_as_f64(ids, keys.length, key_count, *values);
+  return aggregate(ids, keys.length, key_count, values, op_of[agg], nullptr, vx);
 }
 
 // ---------------------------------------------------------------- dense group-by without allocation

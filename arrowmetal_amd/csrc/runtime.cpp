@@ -543,6 +543,100 @@ Array* import_host(const ArrowSchema* schema, ArrowArray* array) {
   return out;   // holder goes out of scope: the producer's release runs now, the data is copied
 }
 
+// ---- chunked import (reference include/arrowmetal.h:22-34, Sources/ArrowMetal/ChunkedImport.swift)
+
+bool chunks_supported(const ArrowSchema* schema) {
+  return schema && schema->format && type_is_primitive(type_of_format(schema->format));
+}
+
+namespace {
+// ORs `len` bits of an aligned bitmap (bit 0 first, trailing bits clear) into `dst` from bit `pos`.
+void or_bits_at(std::vector<uint8_t>& dst, int64_t pos, const std::vector<uint8_t>& src, int64_t len) {
+  const int sh = int(pos & 7);
+  const size_t at = size_t(pos >> 3), nbytes = size_t((len + 7) / 8);
+  for (size_t i = 0; i < nbytes; i++) {
+    const unsigned v = unsigned(src[i]) << sh;
+    dst[at + i] |= uint8_t(v);
+    if (v >> 8) dst[at + i + 1] |= uint8_t(v >> 8);
+  }
+}
+}  // namespace
+
+// n chunks of one primitive type as one array of their total length: each chunk's values (from its
+// offset) are copied host to device into their place in one buffer, and the validity bitmap is built
+// on the host at the chunks' bit positions (a chunk without a bitmap, or whose null_count is 0, is all
+// valid; a null_count of -1 is counted) and copied once. Everything is checked before anything is
+// moved; on success every chunk's release has run and is NULL.
+Array* import_chunks(const ArrowSchema* schema, ArrowArray* arrays, int64_t n) {
+  if (n == 1) return import_host(schema, &arrays[0]);
+  const std::string fmt = schema->format;
+  const int t = type_of_format(fmt);
+  const int w = type_width(t);
+  int64_t total = 0;
+  bool any_nulls = false;
+  for (int64_t c = 0; c < n; c++) {
+    const ArrowArray& A = arrays[c];
+    if (!A.release) fail_invalid("am_import_chunks: chunk " + std::to_string(c) + " has already been released");
+    if (A.n_buffers != 2) fail_invalid("am_import_chunks: a primitive array needs 2 buffers");
+    if (A.length < 0 || A.offset < 0) fail_invalid("am_import_chunks: negative length or offset");
+    if (A.length > 0 && !A.buffers[1]) fail_invalid("am_import_chunks: NULL values buffer");
+    total += A.length;
+    any_nulls = any_nulls || (A.buffers[0] && A.null_count != 0 && A.length > 0);
+  }
+  check_length(total);
+  Array* out = new_array(fmt, t, total);
+  try {
+    hipStream_t s = stream();
+    out->values = device_alloc(size_t(std::max<int64_t>(total, 1)) * w);
+    int64_t pos = 0;
+    for (int64_t c = 0; c < n; c++) {
+      const ArrowArray& A = arrays[c];
+      if (A.length > 0)
+        AM_HIP(hipMemcpyAsync(static_cast<uint8_t*>(out->values->ptr) + pos * w,
+                              static_cast<const uint8_t*>(A.buffers[1]) + A.offset * w, size_t(A.length) * w,
+                              hipMemcpyHostToDevice, s));
+      pos += A.length;
+    }
+    out->null_count = 0;
+    std::vector<uint8_t> bm;
+    if (any_nulls) {
+      bm.assign(size_t((total + 7) / 8) + 1, 0);
+      int64_t zeros = 0;
+      pos = 0;
+      for (int64_t c = 0; c < n; c++) {
+        const ArrowArray& A = arrays[c];
+        if (A.length == 0) continue;
+        std::vector<uint8_t> bits;
+        if (A.buffers[0] && A.null_count != 0) {
+          int64_t z = 0;
+          bits = shift_bitmap(static_cast<const uint8_t*>(A.buffers[0]), A.offset, A.length, &z);
+          zeros += z;
+        } else {
+          bits.assign(size_t((A.length + 7) / 8), 0xFF);
+          if (A.length % 8) bits.back() = uint8_t((1u << (A.length % 8)) - 1u);
+        }
+        or_bits_at(bm, pos, bits, A.length);
+        pos += A.length;
+      }
+      out->null_count = zeros;
+      if (zeros > 0) {
+        out->validity = device_alloc(size_t((total + 7) / 8));
+        AM_HIP(hipMemcpyAsync(out->validity->ptr, bm.data(), size_t((total + 7) / 8), hipMemcpyHostToDevice, s));
+      }
+    }
+    AM_HIP(hipStreamSynchronize(s));   // the chunks' buffers and bm are read until here
+  } catch (...) {
+    release_array(out);
+    throw;
+  }
+  for (int64_t c = 0; c < n; c++) {   // moved: the data is copied, so the producers' buffers go now
+    ArrowArray& A = arrays[c];
+    A.release(&A);
+    A.release = nullptr;
+  }
+  return out;
+}
+
 Array* import_device(const ArrowSchema* schema, ArrowDeviceArray* darr) {
   if (!schema || !darr || !schema->format) fail("am_import_device: NULL schema or array");
   if (darr->device_type == ARROW_DEVICE_CPU || darr->device_type == ARROW_DEVICE_ROCM_HOST)

@@ -45,6 +45,16 @@ const char* am_last_error(void);
  * producer when the last handle dies. ARROW_DEVICE_CPU goes through am_import. */
 int  am_import(const struct ArrowSchema* schema, struct ArrowArray* array, am_array** out);
 int  am_import_device(const struct ArrowSchema* schema, struct ArrowDeviceArray* array, am_array** out);
+/* reference include/arrowmetal.h:22-34 (ArrowMetalC.swift:52-67, ChunkedImport.swift): `n` arrays of the
+ * one type `schema` describes (`arrays` points at n contiguous structs), imported as one array of their
+ * total length; each chunk's offset, length, validity bitmap (or its absence) and null count (-1
+ * included) are honoured. On success every chunk is moved (its release is NULL on return); on failure
+ * a chunk whose release is still set remains the caller's. n == 1 is am_import. Taken here: the
+ * integer, float and temporal types. Returns 3, reading and moving nothing, with "Unsupported Arrow
+ * type: am_import_chunks does not take F arrays; import their concatenation" for any other type.
+ * am_import_chunks_supported returns 1 when the type is taken, 0 otherwise (and for NULL). */
+int  am_import_chunks(const struct ArrowSchema* schema, struct ArrowArray* arrays, int64_t n, am_array** out);
+int  am_import_chunks_supported(const struct ArrowSchema* schema);
 /* reference include/arrowmetal.h:48-58 (CInterop.swift:338-412): am_export fills host buffers
  * (copied from HBM once per handle); am_export_device hands out the device buffers themselves. */
 int  am_export(am_array* a, struct ArrowSchema* schema, struct ArrowArray* array);
@@ -143,6 +153,32 @@ int  am_lexsort_ex2(am_array** columns, const int* descending /* or NULL */, con
  * column. order 0 (and any value other than 1): first appearance, with the null kept once at the
  * position of the first null row; order 1: ascending, nulls dropped. */
 int  am_unique_ex(am_array* a, int order, am_array** out);
+
+/* reference include/arrowmetal.h:1178-1236 (Sources/ArrowMetalC/ArrowMetalC_Expr.swift:36-123): am_query
+ * over equal-length named columns. This build runs its fused dense group-by, the form the DuckDB
+ * extension sends for small key spans (duckdb-extension/src/arrowmetal_rewrite.cpp:1319-1350):
+ *   (query (group_by K "name" KEY) (aggregate AGG+))
+ *   KEY := (col "c") | (cast KEY INT-TYPE) | (sub KEY LIT) | (add KEY LIT)   -- keys outside [0,K) drop
+ *   AGG := (count "n") | (count "n" V) | (sum "n" V) | (mean "n" V) | (min "n" V) | (max "n" V)
+ *   V   := (col "c") | (shr (col "c") LIT) | (bit_and (col "c") LIT);  LIT := (int N) | (i64 N) | ...
+ * Result: column 0 "name" = int32 0..K-1, then one column per aggregate: counts int64 (never null);
+ * sums int64 (uint64 for unsigned columns, float64 for float32 values, rounded once from the exact
+ * sum); means float64; min / max the values' type; null where no value contributed. As in the
+ * reference: a float64 sum or mean, and min / max wider than 32 bits, are refused. Filters, projections,
+ * whole-input aggregates and other expressions return 1 with "Unsupported Arrow type: am_query: ... is
+ * not supported by this build". No scalars are produced (am_query_scalar_count 0). */
+typedef struct am_query_result_t am_query_result;
+int  am_query(am_array** columns, const char** names, int64_t n_columns, const char* expr_text,
+              am_query_result** out);
+int64_t     am_query_column_count(am_query_result* r);
+const char* am_query_column_name(am_query_result* r, int64_t i);
+int         am_query_column(am_query_result* r, int64_t i, am_array** out);
+int64_t     am_query_scalar_count(am_query_result* r);
+const char* am_query_scalar_name(am_query_result* r, int64_t i);
+int         am_query_scalar(am_query_result* r, int64_t i, int64_t* out_i64, double* out_f64,
+                            int* out_kind, int* is_null);
+void        am_query_result_release(am_query_result* r);
+const char* am_query_canonical(const char* expr_text);
 
 /* reference include/arrowmetal.h:1362-1433 (Sources/ArrowMetalC/ArrowMetalC_Plan.swift,
  * Sources/ArrowMetal/Engine/PlanJSON.swift): the plan engine, one JSON plan per call over tables

@@ -115,3 +115,22 @@ def test_plan_expressions_outside_the_subset_are_refused():
             assert "HASH_AGGREGATE keys [k]" in text and "SOURCE t" in text
     finally:
         L.am_plan_source_release(src)
+
+
+def test_import_chunks_refuses_without_a_gpu():
+    """am_import_chunks: NULL arguments return 2; a type it does not take returns 3 with the
+    reference's text and moves nothing (ArrowMetalC.swift:52-67); am_import_chunks_supported."""
+    L = am.lib()
+    sch = am._ArrowSchema()
+    sch.format = b"u"
+    out = ctypes.c_void_p()
+    assert L.am_import_chunks(None, None, 0, ctypes.byref(out)) == 2
+    assert L.am_import_chunks(ctypes.addressof(sch), None, 2, ctypes.byref(out)) == 2
+    assert L.am_import_chunks(ctypes.addressof(sch), None, 0, ctypes.byref(out)) == 3
+    assert L.am_last_error().decode() == ("Unsupported Arrow type: am_import_chunks does not take u arrays; "
+                                          "import their concatenation")
+    assert L.am_import_chunks_supported(ctypes.addressof(sch)) == 0
+    assert L.am_import_chunks_supported(None) == 0
+    for fmt in (b"l", b"g", b"i", b"C", b"tsu:UTC", b"tdD"):
+        sch.format = fmt
+        assert L.am_import_chunks_supported(ctypes.addressof(sch)) == 1, fmt
