@@ -148,6 +148,23 @@ _SIGNATURES = {
                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "am_query_result_release": ([_P], None),
     "am_query_canonical": ([ctypes.c_char_p], ctypes.c_char_p),
+    "am_stream_last_error": ([], ctypes.c_char_p),
+    "am_stream_from_c_stream": ([_P, ctypes.c_int], _P),
+    "am_stream_release": ([_P], None),
+    "am_stream_group_by": ([_P, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p), ctypes.c_int64,
+                            ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_stream_result_column_count": ([_P], ctypes.c_int64),
+    "am_stream_result_column_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
+    "am_stream_result_column": ([_P, ctypes.c_int64, ctypes.POINTER(_P)], ctypes.c_int),
+    "am_stream_result_scalar_count": ([_P], ctypes.c_int64),
+    "am_stream_result_scalar_name": ([_P, ctypes.c_int64], ctypes.c_char_p),
+    "am_stream_result_scalar": ([_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "am_stream_result_rows_out": ([_P], ctypes.c_int64),
+    "am_stream_result_stats": ([_P] + [ctypes.POINTER(ctypes.c_int64)] * 3 + [ctypes.POINTER(ctypes.c_double)] * 5,
+                               ctypes.c_int),
+    "am_stream_result_release": ([_P], None),
     "am_set_max_row_index": ([ctypes.c_int64], ctypes.c_int64),
     "am_prof_enable": ([ctypes.c_int], ctypes.c_int),
     "am_prof_reset": ([], ctypes.c_int),
@@ -607,6 +624,55 @@ def query_canonical(text):
     if r is None:
         raise ArrowMetalError((lib().am_last_error() or b"unknown error").decode())
     return r.decode()
+
+
+class _ArrowArrayStream(ctypes.Structure):
+    _fields_ = [("get_schema", ctypes.c_void_p), ("get_next", ctypes.c_void_p), ("get_last_error", ctypes.c_void_p),
+                ("release", ctypes.c_void_p), ("private_data", ctypes.c_void_p)]
+
+
+STREAM_OPS = {"sum": 0, "count": 1, "min": 2, "max": 3, "mean": 4, "variance": 5, "stddev": 6,
+              "count_distinct_approx": 7}
+
+
+def stream_group_by(reader, keys, aggs, dense_key_count=0):
+    """Streaming group-by (`am_stream_from_c_stream` + `am_stream_group_by`) over a pyarrow
+    RecordBatchReader (or anything with `_export_to_c`). `aggs` is a list of (op, column or None, name).
+    Returns ({name: MetalArray}, stats dict)."""
+    cs = _ArrowArrayStream()
+    reader._export_to_c(ctypes.addressof(cs))
+    L = lib()
+    s = L.am_stream_from_c_stream(ctypes.addressof(cs), 0)
+    if not s:
+        raise ArrowMetalError((L.am_stream_last_error() or b"").decode())
+    try:
+        k = (ctypes.c_char_p * len(keys))(*[x.encode() for x in keys])
+        ops = (ctypes.c_int * len(aggs))(*[STREAM_OPS[a[0]] if isinstance(a[0], str) else a[0] for a in aggs])
+        cols = (ctypes.c_char_p * len(aggs))(*[None if a[1] is None else a[1].encode() for a in aggs])
+        names = (ctypes.c_char_p * len(aggs))(*[a[2].encode() for a in aggs])
+        out = _P()
+        rc = L.am_stream_group_by(s, k, len(keys), ops, cols, names, len(aggs), dense_key_count, 0, ctypes.byref(out))
+        if rc == 2:
+            raise ArrowMetalError("bad argument (NULL handle)")
+        if rc != 0:
+            raise ArrowMetalError((L.am_stream_last_error() or b"").decode())
+        try:
+            res = {}
+            for i in range(L.am_stream_result_column_count(out)):
+                h = _P()
+                _check(L.am_stream_result_column(out, i, ctypes.byref(h)))
+                res[L.am_stream_result_column_name(out, i).decode()] = MetalArray(h)
+            ints = [ctypes.c_int64() for _ in range(3)]
+            dbl = [ctypes.c_double() for _ in range(5)]
+            _check(L.am_stream_result_stats(out, *[ctypes.byref(x) for x in ints], *[ctypes.byref(x) for x in dbl]))
+            stats = dict(zip(("batches", "rows", "bytes_read"), [x.value for x in ints]))
+            stats.update(zip(("wall_s", "read_s", "gpu_s", "merge_s", "overlap"), [x.value for x in dbl]))
+            stats["rows_out"] = L.am_stream_result_rows_out(out)
+            return res, stats
+        finally:
+            L.am_stream_result_release(out)
+    finally:
+        L.am_stream_release(s)
 
 
 def _plan_text(plan):

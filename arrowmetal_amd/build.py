@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libArrowMetalC.so")
-SOURCES = ["runtime.cpp", "sort.hip", "take.hip", "select.hip", "groupby.hip", "batched.hip", "serve.cpp", "plan.cpp", "query.cpp", "capi.cpp"]
+SOURCES = ["runtime.cpp", "sort.hip", "take.hip", "select.hip", "groupby.hip", "batched.hip", "serve.cpp", "plan.cpp", "query.cpp", "stream.cpp", "capi.cpp"]
 ARCH = os.environ.get("ARROWMETAL_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

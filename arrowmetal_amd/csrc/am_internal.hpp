@@ -10,6 +10,7 @@
 //   select.hip    sorted values, top-k by radix select, lexsort
 //   plan.cpp      the plan engine's scan / sort / group_by / unique / limit subset (am_plan_*)
 //   query.cpp     am_query's fused dense group-by subset
+//   stream.cpp    the streaming group-by (am_stream_from_c_stream, am_stream_group_by)
 //   batched.hip   the batched plan (am_batched_*); serve.cpp the request server (am_server_*)
 //   capi.cpp      the extern "C" am_* entry points of include/arrowmetal.h
 #pragma once
@@ -259,6 +260,24 @@ int64_t query_column_count(const QueryResult* r);
 const char* query_column_name(const QueryResult* r, int64_t i);
 Array* query_column(const QueryResult* r, int64_t i);
 void query_result_release(QueryResult* r);
+
+// stream.cpp: the streaming group-by over a producer of record batches.
+struct Stream;
+struct StreamResult;
+Stream* stream_from_c(ArrowArrayStream* in);
+void stream_release(Stream* s);
+StreamResult* stream_group_by(Stream* s, const char* const* keys, int64_t n_keys, const int* ops,
+                              const char* const* columns, const char* const* names, int64_t n_aggs,
+                              int64_t dense_key_count);
+int64_t stream_result_column_count(const StreamResult* r);
+const char* stream_result_column_name(const StreamResult* r, int64_t i);
+Array* stream_result_column(const StreamResult* r, int64_t i);
+int64_t stream_result_rows_out(const StreamResult* r);
+void stream_result_stats(const StreamResult* r, int64_t* batches, int64_t* rows, int64_t* bytes, double* wall,
+                         double* read, double* gpu, double* merge, double* overlap);
+void stream_result_release(StreamResult* r);
+// Imports a host array (moving it), as am_import does.
+Array* import_host(const ArrowSchema* schema, ArrowArray* array);
 
 // Launch-shape parameters the autotuner chooses (runtime.cpp keeps the table).
 struct SortTune {

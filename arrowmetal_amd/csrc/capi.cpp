@@ -370,6 +370,72 @@ int am_unique_rows(am_array** columns, int64_t count, am_array** out) {
   return guarded([&] { *out = H(am::unique_rows(reinterpret_cast<Array* const*>(columns), count)); });
 }
 
+// ---- the streaming group-by (reference Sources/ArrowMetalC/ArrowMetalC_Stream.swift:53-130, 259-281,
+// 419-496). Errors go to the same thread-local text as am_last_error().
+
+am_stream* am_stream_from_c_stream(struct ArrowArrayStream* in, int /*prefetch_depth*/) {
+  if (!in) return nullptr;
+  am_stream* s = nullptr;
+  if (guarded([&] { s = reinterpret_cast<am_stream*>(am::stream_from_c(in)); }) != 0) return nullptr;
+  return s;
+}
+
+void am_stream_release(am_stream* s) {
+  if (s) am::stream_release(reinterpret_cast<am::Stream*>(s));
+}
+
+const char* am_stream_last_error(void) { return am::last_error_cstr(); }
+
+int am_stream_group_by(am_stream* s, const char** keys, int64_t n_keys, const int* ops, const char** columns,
+                       const char** names, int64_t n_aggs, int64_t dense_key_count, int /*ddof*/,
+                       am_stream_result** out) {
+  if (!s || !ops || !names || !out || n_keys <= 0 || n_aggs <= 0 || !keys) return 2;
+  for (int64_t i = 0; i < n_keys; i++)
+    if (!keys[i]) return 2;
+  for (int64_t i = 0; i < n_aggs; i++)
+    if (!names[i]) return 2;
+  return guarded([&] {
+    *out = reinterpret_cast<am_stream_result*>(am::stream_group_by(
+        reinterpret_cast<am::Stream*>(s), keys, n_keys, ops, columns, names, n_aggs, dense_key_count));
+  });
+}
+
+int64_t am_stream_result_column_count(am_stream_result* r) {
+  return r ? am::stream_result_column_count(reinterpret_cast<am::StreamResult*>(r)) : 0;
+}
+
+const char* am_stream_result_column_name(am_stream_result* r, int64_t i) {
+  if (!r || i < 0 || i >= am::stream_result_column_count(reinterpret_cast<am::StreamResult*>(r))) return nullptr;
+  return am::stream_result_column_name(reinterpret_cast<am::StreamResult*>(r), i);
+}
+
+int am_stream_result_column(am_stream_result* r, int64_t i, am_array** out) {
+  if (!r || !out || i < 0 || i >= am::stream_result_column_count(reinterpret_cast<am::StreamResult*>(r))) return 2;
+  *out = H(am::stream_result_column(reinterpret_cast<am::StreamResult*>(r), i));
+  return 0;
+}
+
+// The group-by terminal returns columns only.
+int64_t am_stream_result_scalar_count(am_stream_result*) { return 0; }
+const char* am_stream_result_scalar_name(am_stream_result*, int64_t) { return nullptr; }
+int am_stream_result_scalar(am_stream_result*, int64_t, int64_t*, double*, int*, int*) { return 2; }
+
+int64_t am_stream_result_rows_out(am_stream_result* r) {
+  return r ? am::stream_result_rows_out(reinterpret_cast<am::StreamResult*>(r)) : 0;
+}
+
+int am_stream_result_stats(am_stream_result* r, int64_t* batches, int64_t* rows, int64_t* bytes_read, double* wall_s,
+                           double* read_s, double* gpu_s, double* merge_s, double* overlap) {
+  if (!r) return 2;
+  am::stream_result_stats(reinterpret_cast<am::StreamResult*>(r), batches, rows, bytes_read, wall_s, read_s, gpu_s,
+                          merge_s, overlap);
+  return 0;
+}
+
+void am_stream_result_release(am_stream_result* r) {
+  if (r) am::stream_result_release(reinterpret_cast<am::StreamResult*>(r));
+}
+
 // ---- am_query (reference Sources/ArrowMetalC/ArrowMetalC_Expr.swift:36-123)
 
 int am_query(am_array** columns, const char** names, int64_t n_columns, const char* expr_text,

@@ -154,6 +154,40 @@ int  am_lexsort_ex2(am_array** columns, const int* descending /* or NULL */, con
  * position of the first null row; order 1: ascending, nulls dropped. */
 int  am_unique_ex(am_array* a, int order, am_array** out);
 
+/* reference include/arrowmetal.h:1515-1600 (Sources/ArrowMetalC/ArrowMetalC_Stream.swift): streaming
+ * group-by over a producer of record batches (any Arrow C Stream: pyarrow's RecordBatchReader, Polars,
+ * DuckDB). am_stream_from_c_stream moves `in` (NULL plus am_stream_last_error() on failure); a stream is
+ * consumed by one terminal. am_stream_group_by: `n_keys` key columns; per aggregate an op (0 sum,
+ * 1 count, 2 min, 3 max, 4 mean; 5 variance, 6 stddev and 7 count_distinct_approx return 1 with
+ * "... is not supported by this build"), a value column (NULL only for count, which then counts rows)
+ * and an output name. dense_key_count > 0: one integer key column inside [0, dense_key_count), returned
+ * as int32, keys no row landed on (and keys outside the range) dropped. Result: the key columns, then
+ * one column per aggregate, groups in ascending key order with the null key last; sum null for a group
+ * without a value (int64 for signed, uint64 for unsigned columns, float64 for float values: the
+ * correctly rounded sum of the batches' correctly rounded sums), count int64, min / max the values'
+ * type, mean float64. Memory is one batch plus the batches' partial group states, folded back to the
+ * running answer as they grow. ddof and prefetch_depth are accepted and unused here. */
+typedef struct am_stream am_stream;
+typedef struct am_stream_result am_stream_result;
+const char* am_stream_last_error(void);
+am_stream*  am_stream_from_c_stream(struct ArrowArrayStream* in, int prefetch_depth);
+void        am_stream_release(am_stream* s);
+int am_stream_group_by(am_stream* s, const char** keys, int64_t n_keys, const int* ops, const char** columns,
+                       const char** names, int64_t n_aggs, int64_t dense_key_count, int ddof,
+                       am_stream_result** out);
+int64_t     am_stream_result_column_count(am_stream_result* r);
+const char* am_stream_result_column_name(am_stream_result* r, int64_t i);
+int         am_stream_result_column(am_stream_result* r, int64_t i, am_array** out);
+int64_t     am_stream_result_scalar_count(am_stream_result* r);
+const char* am_stream_result_scalar_name(am_stream_result* r, int64_t i);
+int         am_stream_result_scalar(am_stream_result* r, int64_t i, int64_t* out_i64, double* out_f64,
+                                    int* out_kind, int* is_null);
+int64_t     am_stream_result_rows_out(am_stream_result* r);
+int         am_stream_result_stats(am_stream_result* r, int64_t* batches, int64_t* rows, int64_t* bytes_read,
+                                   double* wall_s, double* read_s, double* gpu_s, double* merge_s,
+                                   double* overlap);
+void        am_stream_result_release(am_stream_result* r);
+
 /* reference include/arrowmetal.h:1178-1236 (Sources/ArrowMetalC/ArrowMetalC_Expr.swift:36-123): am_query
  * over equal-length named columns. This build runs its fused dense group-by, the form the DuckDB
  * extension sends for small key spans (duckdb-extension/src/arrowmetal_rewrite.cpp:1319-1350):
