@@ -1504,6 +1504,333 @@ struct Mapped {
   DevBuf ids, rep, okeys, perm;
 };
 
+// ---------------------------------------------------------------- the range path
+
+// ARROWMETAL_GROUPBY_RANGE=0 turns the range path off (the hash path then maps every key column), so
+// that tests can compare the two on the same keys.
+bool range_path_enabled() {
+  const char* e = getenv("ARROWMETAL_GROUPBY_RANGE");
+  return !(e && e[0] == '0');
+}
+//
+// Integer keys whose values span a small range (GroupByKeys.swift:268-337: span <= 2^24 and
+// <= max(2^16, 4n)) are mapped without hashing: one pass finds the smallest and largest order key, one
+// marks every present key in an occupancy array over the span (and its lowest row when representative
+// rows are tracked), a scan of the occupancy turns each present key into its rank, which is its id
+// in ascending key order, and one pass writes every row's id (the null group, last, is D).
+struct RangeStat {
+  unsigned long long kmin, kmax, nulls;
+  uint32_t null_first, pad;
+};
+
+__global__ void gb_range_init(RangeStat* st) {
+  st->kmin = ~0ull;
+  st->kmax = 0ull;
+  st->nulls = 0ull;
+  st->null_first = 0xFFFFFFFFu;
+}
+
+constexpr int RANGE_ROWS = 8;   // rows per thread per step, loaded together
+
+template <int W>
+__global__ __launch_bounds__(256) void gb_range_minmax(const void* vals, int type, const uint8_t* valid, int64_t voff,
+                                                       uint32_t n, RangeStat* st) {
+  __shared__ unsigned long long s_lo[4], s_hi[4], s_nz[4];
+  __shared__ uint32_t s_nf[4];
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  unsigned long long lo = ~0ull, hi = 0ull, nz = 0ull;
+  uint32_t nf = 0xFFFFFFFFu;
+  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
+    unsigned long long raw[RANGE_ROWS];
+    bool ok[RANGE_ROWS];
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      const uint64_t ii = i < n ? i : n - 1;
+      raw[u] = ld_raw<W>(vals, int64_t(ii));
+      ok[u] = ld_valid(valid, voff, int64_t(ii));
+    }
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      if (i >= n) continue;
+      if (!ok[u]) {
+        nz++;
+        nf = min(nf, uint32_t(i));
+        continue;
+      }
+      const unsigned long long k = order_key(key_of_raw(raw[u], type), type);
+      lo = k < lo ? k : lo;
+      hi = k > hi ? k : hi;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+    nz += __shfl_xor(nz, off, 64);
+    nf = min(nf, uint32_t(__shfl_xor(int(nf), off, 64)));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[w] = lo;
+    s_hi[w] = hi;
+    s_nz[w] = nz;
+    s_nf[w] = nf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      lo = s_lo[k] < lo ? s_lo[k] : lo;
+      hi = s_hi[k] > hi ? s_hi[k] : hi;
+      nz += s_nz[k];
+      nf = min(nf, s_nf[k]);
+    }
+    if (lo != ~0ull) atomicMin(&st->kmin, lo);
+    if (hi != 0ull) atomicMax(&st->kmax, hi);
+    if (nz) {
+      atomicAdd(&st->nulls, nz);
+      atomicMin(&st->null_first, nf);
+    }
+  }
+}
+
+// occ[k - kmin] = 1 for every present key; first[k - kmin] = its lowest row when tracked. Both only
+// ever move one way, so a read that already shows the value skips the write.
+template <int W, bool TRACK>
+__global__ __launch_bounds__(256) void gb_range_occ(const void* vals, int type, const uint8_t* valid, int64_t voff,
+                                                    uint32_t n, unsigned long long kmin, uint32_t* occ, uint32_t* first) {
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
+    unsigned long long raw[RANGE_ROWS];
+    bool ok[RANGE_ROWS];
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      const uint64_t ii = i < n ? i : n - 1;
+      raw[u] = ld_raw<W>(vals, int64_t(ii));
+      ok[u] = ld_valid(valid, voff, int64_t(ii));
+    }
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      if (i >= n || !ok[u]) continue;
+      const uint32_t x = uint32_t(order_key(key_of_raw(raw[u], type), type) - kmin);
+      // Plain accesses: a stale 0 costs one more store of the same 1; a stale row is larger than the
+      // current lowest one, so it never skips a needed atomicMin.
+      if (occ[x] == 0u) occ[x] = 1u;
+      if (TRACK && uint32_t(i) < first[x]) atomicMin(&first[x], uint32_t(i));
+    }
+  }
+}
+
+constexpr uint32_t RANGE_TILE = 4096;   // occupancy entries per workgroup of the scan (16 per thread)
+
+// Per tile of the occupancy: its count of present keys.
+__global__ __launch_bounds__(256) void gb_range_count(const uint32_t* occ, uint32_t span, uint32_t* tile_sum) {
+  __shared__ uint32_t s[4];
+  const uint32_t t0 = blockIdx.x * RANGE_TILE;
+  uint32_t c = 0;
+  for (uint32_t x = t0 + threadIdx.x; x < min(span, t0 + RANGE_TILE); x += 256) c += occ[x] ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+// One workgroup: exclusive prefix of the tile counts in place; the total is the distinct-key count.
+__global__ __launch_bounds__(1024) void gb_range_scan_tiles(uint32_t* tile_sum, uint32_t tiles, uint32_t* distinct) {
+  __shared__ uint32_t s[16];
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < tiles; t0 += 1024) {
+    const uint32_t t = t0 + threadIdx.x;
+    const uint32_t v = t < tiles ? tile_sum[t] : 0u;
+    uint32_t x = v;   // inclusive scan in the wave, then across the 16 waves
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) s[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < 16; w++) {
+      before += w < int(threadIdx.x >> 6) ? s[w] : 0u;
+      total += s[w];
+    }
+    if (t < tiles) tile_sum[t] = carry + before + x - v;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *distinct = carry;
+}
+
+// Each present key's rank (its id) in place of its occupancy mark, the ascending order keys, and the
+// lowest row of each group when tracked.
+__global__ __launch_bounds__(256) void gb_range_rank(uint32_t* occ, uint32_t span, const uint32_t* tile_base,
+                                                     unsigned long long kmin, unsigned long long* okeys,
+                                                     const uint32_t* first, uint32_t* rep) {
+  __shared__ uint32_t s[4];
+  const uint32_t t0 = blockIdx.x * RANGE_TILE;
+  uint32_t run = tile_base[blockIdx.x];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t x0 = t0; x0 < min(span, t0 + RANGE_TILE); x0 += 256) {
+    const uint32_t x = x0 + threadIdx.x;
+    const bool present = x < span && occ[x] != 0u;
+    const unsigned long long bal = __ballot(present);
+    if (lane == 0) s[w] = uint32_t(__popcll(bal));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < 4; k++) {
+      before += k < w ? s[k] : 0u;
+      total += s[k];
+    }
+    if (present) {
+      const uint32_t r = run + before + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
+      occ[x] = r;
+      okeys[r] = kmin + x;
+      if (rep) rep[r] = first[x];
+    }
+    run += total;
+    __syncthreads();
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void gb_range_ids(const void* vals, int type, const uint8_t* valid, int64_t voff,
+                                                    uint32_t n, unsigned long long kmin, const uint32_t* rank,
+                                                    uint32_t null_gid, uint32_t* ids) {
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
+    unsigned long long raw[RANGE_ROWS];
+    bool ok[RANGE_ROWS];
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      const uint64_t ii = i < n ? i : n - 1;
+      raw[u] = ld_raw<W>(vals, int64_t(ii));
+      ok[u] = ld_valid(valid, voff, int64_t(ii));
+    }
+    uint32_t g[RANGE_ROWS];
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++)
+      g[u] = ok[u] ? rank[uint32_t(order_key(key_of_raw(raw[u], type), type) - kmin)] : null_gid;
+#pragma unroll
+    for (int u = 0; u < RANGE_ROWS; u++) {
+      const uint64_t i = base + uint64_t(u) * gsize;
+      if (i < n) __builtin_nontemporal_store(g[u], ids + i);
+    }
+  }
+}
+
+__global__ void gb_iota(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+__global__ void gb_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <int W>
+void range_launch_minmax(int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
+                         uint32_t n, RangeStat* st) {
+  hipLaunchKernelGGL(gb_range_minmax<W>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, st);
+}
+
+// The range path's mapping, or false (after one read-back) when the keys' span is too wide for it.
+bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track, Mapped& out) {
+  hipStream_t s = stream();
+  const uint32_t un = uint32_t(n);
+  const int w = type_width(type);
+  DevBuf stb = device_alloc(sizeof(RangeStat));
+  auto* st = static_cast<RangeStat*>(stb->ptr);
+  const int grid = grid_for((n + RANGE_ROWS - 1) / RANGE_ROWS, 8);
+  hipLaunchKernelGGL(gb_range_init, dim3(1), dim3(1), 0, s, st);
+  {
+    ProfScope ps("gb_range_minmax");
+    switch (w) {
+      case 8: range_launch_minmax<8>(grid, s, vals, type, valid, voff, un, st); break;
+      case 4: range_launch_minmax<4>(grid, s, vals, type, valid, voff, un, st); break;
+      case 2: range_launch_minmax<2>(grid, s, vals, type, valid, voff, un, st); break;
+      default: range_launch_minmax<1>(grid, s, vals, type, valid, voff, un, st); break;
+    }
+    AM_HIP(hipGetLastError());
+  }
+  RangeStat host{};
+  AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  const bool any = host.kmin <= host.kmax;
+  const unsigned long long span64 = any ? host.kmax - host.kmin + 1ull : 0ull;
+  const unsigned long long limit = std::min<unsigned long long>(1ull << 24, std::max<unsigned long long>(1ull << 16, 4ull * uint64_t(n)));
+  if (any && (span64 == 0ull || span64 > limit)) return false;   // span64 == 0: the full 2^64 range
+  const uint32_t span = uint32_t(std::max<unsigned long long>(span64, 1ull));
+  DevBuf occ = device_alloc(size_t(span) * 4);
+  DevBuf first;
+  if (track) first = device_alloc(size_t(span) * 4);
+  hipLaunchKernelGGL(gb_fill_u32, dim3(grid_for(span, 4)), dim3(256), 0, s, static_cast<uint32_t*>(occ->ptr), size_t(span), 0u);
+  if (track)
+    hipLaunchKernelGGL(gb_fill_u32, dim3(grid_for(span, 4)), dim3(256), 0, s, static_cast<uint32_t*>(first->ptr),
+                       size_t(span), 0xFFFFFFFFu);
+  AM_HIP(hipGetLastError());
+  auto* o = static_cast<uint32_t*>(occ->ptr);
+  auto* f = track ? static_cast<uint32_t*>(first->ptr) : nullptr;
+  if (any) {
+    ProfScope ps("gb_range_occ");
+#define AM_RANGE_OCC(WW)                                                                                       \
+  if (track) hipLaunchKernelGGL((gb_range_occ<WW, true>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, f); \
+  else hipLaunchKernelGGL((gb_range_occ<WW, false>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, f)
+    switch (w) {
+      case 8: AM_RANGE_OCC(8); break;
+      case 4: AM_RANGE_OCC(4); break;
+      case 2: AM_RANGE_OCC(2); break;
+      default: AM_RANGE_OCC(1); break;
+    }
+#undef AM_RANGE_OCC
+    AM_HIP(hipGetLastError());
+  }
+  const uint32_t tiles = (span + RANGE_TILE - 1) / RANGE_TILE;
+  DevBuf tsum = device_alloc(size_t(tiles) * 4 + 4);
+  auto* ts = static_cast<uint32_t*>(tsum->ptr);
+  out.okeys = device_alloc(size_t(span) * 8);
+  out.rep = device_alloc(size_t(span + 1) * 4);
+  {
+    ProfScope ps("gb_range_scan");
+    hipLaunchKernelGGL(gb_range_count, dim3(tiles), dim3(256), 0, s, o, span, ts);
+    hipLaunchKernelGGL(gb_range_scan_tiles, dim3(1), dim3(1024), 0, s, ts, tiles, ts + tiles);
+    hipLaunchKernelGGL(gb_range_rank, dim3(tiles), dim3(256), 0, s, o, span, ts, host.kmin,
+                       static_cast<unsigned long long*>(out.okeys->ptr), f,
+                       track ? static_cast<uint32_t*>(out.rep->ptr) : nullptr);
+    AM_HIP(hipGetLastError());
+  }
+  uint32_t D = 0;
+  AM_HIP(hipMemcpyAsync(&D, ts + tiles, 4, hipMemcpyDeviceToHost, s));
+  AM_HIP(hipStreamSynchronize(s));
+  out.D = any ? D : 0u;
+  out.has_null = host.nulls > 0;
+  out.K = int64_t(out.D) + (out.has_null ? 1 : 0);
+  if (track && out.has_null) AM_HIP(hipMemcpyAsync(static_cast<uint32_t*>(out.rep->ptr) + out.D, &host.null_first, 4,
+                                                   hipMemcpyHostToDevice, s));
+  out.perm = device_alloc(size_t(std::max<uint32_t>(out.D, 1)) * 4);
+  hipLaunchKernelGGL(gb_iota, dim3((out.D + 255) / 256 + 1), dim3(256), 0, s, static_cast<uint32_t*>(out.perm->ptr), out.D);
+  out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
+  {
+    ProfScope ps("gb_ids");
+    auto* ids = static_cast<uint32_t*>(out.ids->ptr);
+    switch (w) {
+      case 8: hipLaunchKernelGGL(gb_range_ids<8>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
+      case 4: hipLaunchKernelGGL(gb_range_ids<4>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
+      case 2: hipLaunchKernelGGL(gb_range_ids<2>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
+      default: hipLaunchKernelGGL(gb_range_ids<1>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
+    }
+    AM_HIP(hipGetLastError());
+  }
+  AM_HIP(hipStreamSynchronize(s));   // host.null_first is read by the copy above
+  return true;
+}
+
 // gb_build for the key column's element width.
 template <bool TRACK>
 void launch_build(int w, int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
@@ -1523,6 +1850,8 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   hipStream_t s = stream();
   uint32_t un = uint32_t(n);
   Mapped out;
+  if (type_is_int(type) && n >= (int64_t(1) << 16) && range_path_enabled() && map_range(vals, type, valid, voff, n, track, out))
+    return out;
   out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
   DevBuf stb = device_alloc(sizeof(BuildStatus));
   auto* st = static_cast<BuildStatus*>(stb->ptr);

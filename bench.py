@@ -617,7 +617,29 @@ def group_by_leg(args, log):
     u.update(parity_and_cpu(k200, gk200, "200 keys"))
     res["uniform_200_keys"] = u
     log(f"group-by, 200 uniform keys: {ms200:.3f} ms")
-    del gk200, gi
+    del gk200
+    # Variant R: the same Zipf(1.1) rows with the key = its rank 1..100,000 (a small span: the range path,
+    # occupancy + scan + rank, no hashing).
+    kr = (zipf_ranks(n, 100_000, 1.1, 0x21F0).astype(np.int64) + 1)
+    gkr = am.MetalArray.from_numpy(kr)
+    stepr = int_step(gkr)
+    tr = timed(stepr, args.steps, args.warmup)
+    msr = 1e3 * float(np.median(tr))
+    rawr = profiled(stepr, 3)
+    algr = dict(alg)
+    algr["gb_range_minmax"] = (8 * n, "range path: smallest and largest key; read key; 8 B/row")
+    algr["gb_range_occ"] = (8 * n, "range path: occupancy of the span; read key; 8 B/row")
+    algr["gb_ids"] = (12 * n, "row to ascending id: read key, write id; 12 B/row")
+    roofr, rowsr = roofline_of(rawr, algr)
+    vr = {"workload": "group_by_50M_int64_zipf1.1_keys_are_ranks_1_to_100k", "ms_per_step": round(msr, 4),
+          "rows_per_s": n / (msr * 1e-3),
+          "step": "the same step; keys = Zipf rank (span 100,000: the range path)",
+          "step_hbm_frac_of_b_alg": (16 * n) / (msr * 1e-3) / 1e9 / HBM_PEAK_GBS,
+          "roofline": roofr, "kernels_roofline": rowsr, "kernels_ms": kernels_ms(per_step(rawr, 3))}
+    vr.update(parity_and_cpu(kr, gkr, "variant R"))
+    res["variant_r_rank_keys"] = vr
+    log(f"group-by, variant R (rank keys): {msr:.3f} ms")
+    del gkr, gi
     return res
 
 

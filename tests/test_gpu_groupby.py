@@ -409,3 +409,60 @@ def test_count_reuse_in_any_op_order(with_nulls):
             name = {0: "sum", 1: "count_all", 2: "count"}[op]
             got = g.count_all() if op == 1 else getattr(g, name)(mv)
             assert_agg(got, O.group_agg(ids_w, K, vals, op, vv), f"order {order} {name} nulls={with_nulls}")
+
+
+@pytest.mark.parametrize("case", ["small_span", "negative", "nulls", "all_null", "u64_top", "int8", "two_columns",
+                                  "unique_rows"])
+def test_range_path_equals_hash_path(case, monkeypatch):
+    """Integer keys over a small span take the range path (occupancy, scan, rank; GroupByKeys.swift:268-337);
+    the same keys through the hash path (ARROWMETAL_GROUPBY_RANGE=0) give the same ids, keys, counts and
+    representative rows, and both match the oracle."""
+    rng = np.random.default_rng(len(case))
+    n = 300_001
+    valid = None
+    if case == "small_span":
+        cols = [rng.integers(0, 5000, n).astype(np.int64)]
+    elif case == "negative":
+        cols = [rng.integers(-70_000, -10_000, n).astype(np.int32)]
+    elif case == "nulls":
+        cols = [rng.integers(100, 300, n).astype(np.uint32)]
+        valid = [rng.random(n) > 0.3]
+    elif case == "all_null":
+        cols = [rng.integers(0, 9, n).astype(np.int64)]
+        valid = [np.zeros(n, bool)]
+    elif case == "u64_top":
+        cols = [(np.uint64(2**64 - 1000) + rng.integers(0, 1000, n).astype(np.uint64))]
+    elif case == "int8":
+        cols = [rng.integers(-128, 128, n).astype(np.int8)]
+    else:
+        cols = [rng.integers(0, 50, n).astype(np.int64), rng.integers(-3, 3, n).astype(np.int16)]
+        valid = [rng.random(n) > 0.05, None]
+    arrays = [am.MetalArray.from_numpy(c, None if valid is None else valid[i]) for i, c in enumerate(cols)]
+    want_ids, K, want_rep = O.group_keys(cols, valid)
+    results = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("ARROWMETAL_GROUPBY_RANGE", mode)
+        if case == "unique_rows":
+            results[mode] = am.unique_rows(arrays).to_numpy()[0]
+            continue
+        am.prof_reset()
+        am.prof_enable(True)
+        g = am.group_by(arrays)
+        ids = g.ids().to_numpy()[0]
+        am.prof_enable(False)
+        names = set(am.prof_report())
+        assert ("gb_range_minmax" in names) == (mode == "1"), names
+        assert g.group_count == K
+        np.testing.assert_array_equal(ids, want_ids)
+        keys = [k.to_numpy() for k in g.key_arrays()]
+        cnt, _ = g.count_all().to_numpy()
+        results[mode] = (ids, keys, cnt)
+    if case == "unique_rows":
+        np.testing.assert_array_equal(results["1"], results["0"])
+        np.testing.assert_array_equal(results["1"], O.unique_rows(cols, valid))
+        return
+    for (a, av), (b, bv) in zip(results["1"][1], results["0"][1]):
+        ok = np.ones(len(a), bool) if av is None else av
+        np.testing.assert_array_equal(ok, np.ones(len(b), bool) if bv is None else bv)
+        np.testing.assert_array_equal(a[ok], b[ok])
+    np.testing.assert_array_equal(results["1"][2], results["0"][2])
