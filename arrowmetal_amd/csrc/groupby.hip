@@ -27,6 +27,7 @@
 #include "fsum.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <map>
 
 namespace am {
@@ -663,7 +664,10 @@ struct AggArgs {
 // lane's registers by the families whose words are all additions (sums and counts), so that the rows
 // of a hot group take no atomic at all: with Zipf(1.1) keys the top group alone is ~13% of the rows,
 // about nine lanes of every wave, whose LDS atomics on one word would otherwise serialize.
-constexpr int HOT_GROUPS = 8;
+#ifndef AM_HOT_GROUPS
+#define AM_HOT_GROUPS 8
+#endif
+constexpr int HOT_GROUPS = AM_HOT_GROUPS;
 
 // count_all (values ignored) and count (non-null values).
 template <bool VALID_ONLY>
@@ -817,8 +821,17 @@ __device__ __forceinline__ unsigned long long identity(int op) { return op == W_
 // keeps hot groups read-mostly. Those words only ever move one way (MIN down, MAX up, OR gaining
 // bits), so any value the read returns, however stale, lies on the far side of the current value:
 // x >= read >= current makes the MIN a no-op, and likewise for MAX and OR.
-template <int OP>
-__device__ __forceinline__ void word_update(unsigned long long* p, unsigned long long x) {
+// Accumulator words with their address space spelt out: an atomic through a generic pointer is a
+// flat instruction, and a flat operation in flight makes every later wait a full drain of both the
+// vector-memory and the LDS counters.
+typedef __attribute__((address_space(1))) unsigned long long gword;   // global memory (this XCD's table)
+typedef __attribute__((address_space(3))) unsigned long long lword;   // LDS
+
+__device__ __forceinline__ gword* as_global(unsigned long long* p) { return (gword*)p; }
+__device__ __forceinline__ lword* as_lds(unsigned long long* p) { return (lword*)p; }
+
+template <int OP, typename Word>
+__device__ __forceinline__ void word_update(Word* p, unsigned long long x) {
   auto seen = [p] { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
   if (OP == W_ADD) {
     if (x) __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -833,7 +846,8 @@ __device__ __forceinline__ void word_update(unsigned long long* p, unsigned long
 
 // 128-bit two's-complement add at p[0] (lo), p[1] (hi): exact modulo 2^128 under any interleaving,
 // because each carry comes from the lo word's own read-modify-write.
-__device__ __forceinline__ void add128(unsigned long long* p, unsigned long long lo, unsigned long long hi) {
+template <typename Word>
+__device__ __forceinline__ void add128(Word* p, unsigned long long lo, unsigned long long hi) {
   unsigned long long carry = 0;
   if (lo) {
     unsigned long long old = __hip_atomic_fetch_add(&p[0], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -843,8 +857,8 @@ __device__ __forceinline__ void add128(unsigned long long* p, unsigned long long
   if (h) __hip_atomic_fetch_add(&p[1], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <class P>
-__device__ __forceinline__ void apply(unsigned long long* acc, const unsigned long long (&c)[P::W]) {
+template <class P, typename Word>
+__device__ __forceinline__ void apply(Word* acc, const unsigned long long (&c)[P::W]) {
 #pragma unroll
   for (int w = 0; w < P::W; w++) {
     const int op = P::op(w);
@@ -859,7 +873,7 @@ __device__ __forceinline__ void apply(unsigned long long* acc, const unsigned lo
 
 // Merges one LDS accumulator into this XCD's table (skipping identities).
 template <class P>
-__device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsigned long long* lacc) {
+__device__ __forceinline__ void flush_one(gword* gacc, const lword* lacc) {
   unsigned long long c[P::W];
   bool any = false;
 #pragma unroll
@@ -870,7 +884,10 @@ __device__ __forceinline__ void flush_one(unsigned long long* gacc, const unsign
   if (any) apply<P>(gacc, c);
 }
 
-constexpr int AGG_ROWS = 4;   // rows per thread whose loads are issued together
+#ifndef AM_AGG_ROWS
+#define AM_AGG_ROWS 2
+#endif
+constexpr int AGG_ROWS = AM_AGG_ROWS;   // rows per thread per step, whose loads are issued together
 
 // 1024 threads per workgroup and one workgroup per CU: the whole LDS budget goes to one table.
 #ifndef AM_AGG_THREADS
@@ -880,7 +897,7 @@ constexpr int AGG_ROWS = 4;   // rows per thread whose loads are issued together
 #define AM_AGG_PROBES 8   // LDS cache slots a row probes before its group goes to the XCD's table
 #endif
 #ifndef AM_AGG_HOT
-#define AM_AGG_HOT 1      // sampled hot groups summed in registers
+#define AM_AGG_HOT 0      // sampled hot groups summed in registers (1), or not (0)
 #endif
 
 template <class P>
@@ -890,23 +907,150 @@ __host__ __device__ constexpr bool all_add() {
   return true;
 }
 
-// The rows one thread has in flight: group ids, values and validity as loaded.
+// The rows one thread has in flight: group ids, values and validity bytes exactly as loaded. The bits
+// are taken out only when the rows are used, so that nothing that is loaded is waited on before the
+// whole step's loads are issued (a bool made from a loaded byte is a lane mask, and making it waits).
+template <int IDW>
 struct AggRows {
-  unsigned long long idr[AGG_ROWS], vr[AGG_ROWS];
-  bool iok[AGG_ROWS], vok[AGG_ROWS];
+  typename std::conditional<IDW == 8, unsigned long long, uint32_t>::type idr[AGG_ROWS];
+  unsigned long long vr[AGG_ROWS];
+  uint32_t ib[AGG_ROWS], vb[AGG_ROWS];
 };
 
+__device__ __forceinline__ uint32_t ld_vbyte(const uint8_t* valid, int64_t voff, int64_t i) {
+  return load_global(valid ? valid + ((voff + i) >> 3) : &kAllValid);
+}
+
+// Row i's bit of its loaded validity byte (a column without a bitmap loads 0xFF: every bit set). No
+// branch: the byte is used on every path, so that its register is never left with a load in flight.
+__device__ __forceinline__ uint32_t vbit(int64_t voff, int64_t i, uint32_t byte) {
+  return (byte >> uint32_t((voff + i) & 7)) & 1u;
+}
+
+// Issues the loads of the step at `base` (rows past the end read the last row; end > 0).
 template <class P, int IDW, int VW>
-__device__ __forceinline__ void agg_load(const AggArgs& a, int64_t base, int64_t end, AggRows& r) {
+__device__ __forceinline__ void agg_load(const AggArgs& a, int64_t base, int64_t end, AggRows<IDW>& r) {
 #pragma unroll
-  for (int k = 0; k < AGG_ROWS; k++) {   // rows past the end read the last row
+  for (int k = 0; k < AGG_ROWS; k++) {
     const int64_t i = base + int64_t(k) * blockDim.x + threadIdx.x;
     const int64_t ii = i < end ? i : end - 1;
     r.idr[k] = ld_raw<IDW>(a.ids.ids, ii);
-    r.iok[k] = ld_valid(a.ids.valid, a.ids.voff, ii);
+    r.ib[k] = ld_vbyte(a.ids.valid, a.ids.voff, ii);
     r.vr[k] = 0;
     if constexpr (VW != 0) r.vr[k] = ld_raw<VW>(a.v.vals, ii);
-    r.vok[k] = P::SKIP_NULL ? ld_valid(a.v.valid, a.v.voff, ii) : true;
+    r.vb[k] = P::SKIP_NULL ? ld_vbyte(a.v.valid, a.v.voff, ii) : 0xFFu;
+  }
+}
+
+// What a workgroup of gb_agg holds while it runs over its rows.
+template <class P>
+struct AggState {
+  unsigned long long* lacc;   // LDS accumulators [cap][W]
+  unsigned int* lkey;         // LDS cache keys [cap] (mode 1)
+  unsigned long long* gtab;   // this XCD's table [K][W]
+  int mode;
+  uint32_t cap;
+  long long hot[HOT_GROUPS];
+  unsigned long long hacc[HOT_GROUPS][P::W];
+};
+
+// The updates of one step's rows (loaded by agg_load at `base`).
+template <class P, int IDW, int VW>
+__device__ __forceinline__ void agg_rows(const AggArgs& a, int64_t base, int64_t end, const AggRows<IDW>& cur,
+                                         AggState<P>& st) {
+  constexpr bool HOTREG = AM_AGG_HOT && all_add<P>();
+  // Every loaded register is taken here, on every path: a load whose register is only read on some
+  // paths is still in flight, as far as the compiler knows, when that register is next written, and
+  // that write then waits for all memory operations.
+#pragma unroll
+  for (int r = 0; r < AGG_ROWS; r++) asm volatile("" ::"v"(cur.idr[r]), "v"(cur.vr[r]), "v"(cur.ib[r]), "v"(cur.vb[r]));
+  long long gs[AGG_ROWS];
+  unsigned long long cs[AGG_ROWS][P::W];
+#pragma unroll
+  for (int r = 0; r < AGG_ROWS; r++) {
+    const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+    long long g;
+    if constexpr (IDW == 8) g = (long long)cur.idr[r];
+    else g = a.ids.type == T_I32 ? (long long)int32_t(cur.idr[r]) : (long long)cur.idr[r];
+    g += a.ids.bias;
+    const uint32_t ok = uint32_t(i < end) & vbit(a.ids.voff, i, cur.ib[r]) &
+                        (P::SKIP_NULL ? vbit(a.v.voff, i, cur.vb[r]) : 1u);
+    gs[r] = (ok && g >= 0 && g < a.K) ? g : -1;
+  }
+  // Every row's contribution is computed (registers only) and then dropped where the row has none,
+  // again so that each loaded value is used on every path.
+  if constexpr (P::AUX) {
+    typename P::AuxT ax[AGG_ROWS];
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) ax[r] = P::aux_load(a, gs[r] >= 0 ? gs[r] : 0);   // all loads first
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {
+      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+      if (!P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), ax[r], cs[r])) gs[r] = -1;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {
+      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
+      if (!P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), cs[r])) gs[r] = -1;
+    }
+  }
+  if (HOTREG && st.hot[0] >= 0) {   // uniform; the sampler fills hot[] most frequent first
+    // A hot group's contribution goes to this lane's registers (the hot ids are distinct, and an
+    // unused entry's accumulator is never flushed).
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++) {
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < HOT_GROUPS; j++) {
+        const bool m = gs[r] == st.hot[j];
+#pragma unroll
+        for (int w = 0; w < P::W; w++) st.hacc[j][w] += m ? cs[r][w] : 0ull;
+        any = any || m;
+      }
+      if (any) gs[r] = -1;
+    }
+  }
+  if (st.mode == 0) {
+#pragma unroll
+    for (int r = 0; r < AGG_ROWS; r++)
+      if (gs[r] >= 0) apply<P>(as_lds(&st.lacc[size_t(gs[r]) * P::W]), cs[r]);
+    return;
+  }
+  // Mode 1: every row's home slot is read at once; a row whose group holds it updates LDS at once,
+  // the others probe on (claiming a free slot by CAS) and fall back to the XCD's table.
+  uint32_t sl[AGG_ROWS], seen[AGG_ROWS];
+#pragma unroll
+  for (int r = 0; r < AGG_ROWS; r++) {
+    const uint32_t h = uint32_t(gs[r]) * 2654435761u;
+    sl[r] = __umulhi(h ^ (h >> 15), st.cap);   // [0, cap) for any cap
+    seen[r] = gs[r] >= 0 ? st.lkey[sl[r]] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < AGG_ROWS; r++) {
+    const long long g = gs[r];
+    if (g < 0) continue;
+    if (seen[r] == uint32_t(g)) {
+      apply<P>(as_lds(&st.lacc[size_t(sl[r]) * P::W]), cs[r]);
+      continue;
+    }
+    uint32_t s = sl[r];
+    uint32_t c = seen[r];
+    bool done = false;
+    for (int probe = 0; probe < AM_AGG_PROBES; probe++) {
+      if (probe > 0) c = st.lkey[s];
+      if (c == 0xFFFFFFFFu) {
+        const unsigned int prev = atomicCAS(&st.lkey[s], 0xFFFFFFFFu, uint32_t(g));
+        c = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
+      }
+      if (c == uint32_t(g)) {
+        apply<P>(as_lds(&st.lacc[size_t(s) * P::W]), cs[r]);
+        done = true;
+        break;
+      }
+      s = s + 1u == st.cap ? 0u : s + 1u;
+    }
+    if (!done) apply<P>(as_global(&st.gtab[size_t(g) * P::W]), cs[r]);
   }
 }
 
@@ -916,130 +1060,71 @@ __device__ __forceinline__ void agg_load(const AggArgs& a, int64_t base, int64_t
 // groups, claimed by CAS on a key word, in front of this XCD's table in global memory (L2); a row
 // whose group cannot be cached within AM_AGG_PROBES probes goes to that table directly. The tables of
 // the eight XCDs are merged by gb_agg_merge.
-// The next rows' loads are issued before the current rows' updates, so that waiting for them never
-// waits for the updates' atomics (vector memory operations complete in issue order).
+// Two row buffers take turns: the next step's loads are issued before the current step's updates, and
+// the loop is unrolled over the pair so that no loaded register is copied (a copy would wait for it).
 template <class P, int IDW, int VW>
 __global__ __launch_bounds__(AM_AGG_THREADS) void gb_agg(AggArgs a, int mode, uint32_t cap, int64_t rpb,
                                               unsigned long long* gtab8) {
   extern __shared__ unsigned long long lds[];
-  unsigned long long* lacc = lds;                                     // [cap][W]
-  unsigned int* lkey = reinterpret_cast<unsigned int*>(lds + size_t(cap) * P::W);   // [cap] (mode 1)
-  unsigned long long* gtab = gtab8 + size_t(xcd_id()) * size_t(a.K) * P::W;
   constexpr bool HOTREG = AM_AGG_HOT && all_add<P>();
-  long long hot[HOT_GROUPS];
-  unsigned long long hacc[HOT_GROUPS][P::W];
+  AggState<P> st;
+  st.lacc = lds;
+  st.lkey = reinterpret_cast<unsigned int*>(lds + size_t(cap) * P::W);
+  st.gtab = gtab8 + size_t(xcd_id()) * size_t(a.K) * P::W;
+  st.mode = mode;
+  st.cap = cap;
 #pragma unroll
   for (int j = 0; j < HOT_GROUPS; j++) {
-    hot[j] = HOTREG && a.hot ? a.hot[j] : -1;
+    const long long h = HOTREG && a.hot ? a.hot[j] : -1;   // the same for every lane
+    st.hot[j] = (long long)(((unsigned long long)uint32_t(__builtin_amdgcn_readfirstlane(int(h >> 32))) << 32) |
+                            uint32_t(__builtin_amdgcn_readfirstlane(int(h))));
 #pragma unroll
-    for (int w = 0; w < P::W; w++) hacc[j][w] = 0ull;
+    for (int w = 0; w < P::W; w++) st.hacc[j][w] = 0ull;
   }
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
 #pragma unroll
-    for (int w = 0; w < P::W; w++) lacc[size_t(s) * P::W + w] = identity(P::op(w));
-    if (mode == 1) lkey[s] = 0xFFFFFFFFu;
+    for (int w = 0; w < P::W; w++) st.lacc[size_t(s) * P::W + w] = identity(P::op(w));
+    if (mode == 1) st.lkey[s] = 0xFFFFFFFFu;
   }
   __syncthreads();
   const int64_t begin = int64_t(blockIdx.x) * rpb;
   const int64_t end = std::min<int64_t>(a.n, begin + rpb);
   const int64_t step = int64_t(blockDim.x) * AGG_ROWS;
-  AggRows cur;
-  if (begin < end) agg_load<P, IDW, VW>(a, begin, end, cur);
-  for (int64_t base = begin; base < end; base += step) {
-    AggRows nxt;
-    if (base + step < end) agg_load<P, IDW, VW>(a, base + step, end, nxt);   // uniform
-    long long gs[AGG_ROWS];
-    unsigned long long cs[AGG_ROWS][P::W];
-#pragma unroll
-    for (int r = 0; r < AGG_ROWS; r++) {
-      const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-      long long g;
-      if constexpr (IDW == 8) g = (long long)cur.idr[r];
-      else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(cur.idr[r])) : (long long)uint32_t(cur.idr[r]);
-      g += a.ids.bias;
-      if (i >= end || !cur.iok[r] || g < 0 || g >= a.K || !cur.vok[r]) g = -1;
-      gs[r] = g;
+  // The step count, read from the first lane so that the compiler keeps the loop uniform.
+  const int steps = __builtin_amdgcn_readfirstlane(int(begin < end ? (end - begin + step - 1) / step : 0));
+  if (steps > 0) {
+    AggRows<IDW> ra, rb;
+    agg_load<P, IDW, VW>(a, begin, end, ra);
+    for (int t = 0; t < steps; t += 2) {
+      const int64_t base = begin + int64_t(t) * step;
+      agg_load<P, IDW, VW>(a, base + step, end, rb);
+      agg_rows<P, IDW, VW>(a, base, end, ra, st);
+      if (t + 1 >= steps) break;
+      agg_load<P, IDW, VW>(a, base + 2 * step, end, ra);
+      agg_rows<P, IDW, VW>(a, base + step, end, rb, st);
     }
-    if constexpr (P::AUX) {
-      typename P::AuxT st[AGG_ROWS];
-#pragma unroll
-      for (int r = 0; r < AGG_ROWS; r++) st[r] = P::aux_load(a, gs[r] >= 0 ? gs[r] : 0);   // all loads first
-#pragma unroll
-      for (int r = 0; r < AGG_ROWS; r++) {
-        const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), st[r], cs[r])) gs[r] = -1;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < AGG_ROWS; r++) {
-        const int64_t i = base + int64_t(r) * blockDim.x + threadIdx.x;
-        if (gs[r] >= 0 && !P::contrib(a, i, gs[r], val_in(a.v, cur.vr[r]), cs[r])) gs[r] = -1;
-      }
-    }
-    if (HOTREG && hot[0] >= 0) {   // uniform; the sampler fills hot[] most frequent first
-      // A hot group's contribution goes to this lane's registers.
-#pragma unroll
-      for (int r = 0; r < AGG_ROWS; r++) {
-        bool taken = false;
-#pragma unroll
-        for (int j = 0; j < HOT_GROUPS; j++) {
-          const bool m = !taken && gs[r] >= 0 && gs[r] == hot[j];
-#pragma unroll
-          for (int w = 0; w < P::W; w++) hacc[j][w] += m ? cs[r][w] : 0ull;
-          taken = taken || m;
-        }
-        if (taken) gs[r] = -1;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < AGG_ROWS; r++) {
-      const long long g = gs[r];
-      if (g < 0) continue;
-      if (mode == 0) {
-        apply<P>(&lacc[size_t(g) * P::W], cs[r]);
-        continue;
-      }
-      uint32_t h = uint32_t(g) * 2654435761u;
-      uint32_t sl = __umulhi(h ^ (h >> 15), cap);   // [0, cap) for any cap
-      bool done = false;
-      for (int probe = 0; probe < AM_AGG_PROBES; probe++) {
-        unsigned int c = lkey[sl];
-        if (c == 0xFFFFFFFFu) {
-          unsigned int prev = atomicCAS(&lkey[sl], 0xFFFFFFFFu, uint32_t(g));
-          c = prev == 0xFFFFFFFFu ? uint32_t(g) : prev;
-        }
-        if (c == uint32_t(g)) {
-          apply<P>(&lacc[size_t(sl) * P::W], cs[r]);
-          done = true;
-          break;
-        }
-        sl = sl + 1u == cap ? 0u : sl + 1u;
-      }
-      if (!done) apply<P>(&gtab[size_t(g) * P::W], cs[r]);
-    }
-    cur = nxt;
   }
   if constexpr (HOTREG) {
     // Each wave's sum of its lanes' hot accumulators, added once into this XCD's table.
 #pragma unroll
     for (int j = 0; j < HOT_GROUPS; j++) {
-      if (hot[j] < 0) continue;   // uniform
+      if (st.hot[j] < 0) continue;   // uniform
 #pragma unroll
       for (int w = 0; w < P::W; w++) {
-        unsigned long long v = hacc[j][w];
+        unsigned long long v = st.hacc[j][w];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-        hacc[j][w] = v;
+        st.hacc[j][w] = v;
       }
-      if ((threadIdx.x & 63) == 0) apply<P>(&gtab[size_t(hot[j]) * P::W], hacc[j]);
+      if ((threadIdx.x & 63) == 0) apply<P>(as_global(&st.gtab[size_t(st.hot[j]) * P::W]), st.hacc[j]);
     }
   }
   __syncthreads();
   for (uint32_t sl = threadIdx.x; sl < cap; sl += blockDim.x) {
     long long g;
     if (mode == 0) g = sl < uint32_t(a.K) ? (long long)sl : -1;
-    else g = lkey[sl] == 0xFFFFFFFFu ? -1 : (long long)lkey[sl];
-    if (g >= 0) flush_one<P>(&gtab[size_t(g) * P::W], &lacc[size_t(sl) * P::W]);
+    else g = st.lkey[sl] == 0xFFFFFFFFu ? -1 : (long long)st.lkey[sl];
+    if (g >= 0) flush_one<P>(as_global(&st.gtab[size_t(g) * P::W]), as_lds(&st.lacc[size_t(sl) * P::W]));
   }
 }
 
@@ -1532,9 +1617,11 @@ __global__ void gb_range_init(RangeStat* st) {
 
 constexpr int RANGE_ROWS = 8;   // rows per thread per step, loaded together
 
+// Over rows 0, stride, 2 * stride, ... (n of them): the smallest and largest order key, the nulls and
+// the lowest null row. A stride above 1 is the sample that rules out a wide span before the full pass.
 template <int W>
 __global__ __launch_bounds__(256) void gb_range_minmax(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                                       uint32_t n, RangeStat* st) {
+                                                       uint32_t n, uint32_t stride, RangeStat* st) {
   __shared__ unsigned long long s_lo[4], s_hi[4], s_nz[4];
   __shared__ uint32_t s_nf[4];
   const uint32_t gsize = gridDim.x * blockDim.x;
@@ -1546,7 +1633,7 @@ __global__ __launch_bounds__(256) void gb_range_minmax(const void* vals, int typ
 #pragma unroll
     for (int u = 0; u < RANGE_ROWS; u++) {
       const uint64_t i = base + uint64_t(u) * gsize;
-      const uint64_t ii = i < n ? i : n - 1;
+      const uint64_t ii = (i < n ? i : n - 1) * stride;
       raw[u] = ld_raw<W>(vals, int64_t(ii));
       ok[u] = ld_valid(valid, voff, int64_t(ii));
     }
@@ -1556,7 +1643,7 @@ __global__ __launch_bounds__(256) void gb_range_minmax(const void* vals, int typ
       if (i >= n) continue;
       if (!ok[u]) {
         nz++;
-        nf = min(nf, uint32_t(i));
+        nf = min(nf, uint32_t(i * stride));
         continue;
       }
       const unsigned long long k = order_key(key_of_raw(raw[u], type), type);
@@ -1736,9 +1823,11 @@ __global__ void gb_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 
 template <int W>
 void range_launch_minmax(int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
-                         uint32_t n, RangeStat* st) {
-  hipLaunchKernelGGL(gb_range_minmax<W>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, st);
+                         uint32_t n, uint32_t stride, RangeStat* st) {
+  hipLaunchKernelGGL(gb_range_minmax<W>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, stride, st);
 }
+
+constexpr uint32_t RANGE_SAMPLE = 4096;   // rows read to rule out a wide span
 
 // The range path's mapping, or false (after one read-back) when the keys' span is too wide for it.
 bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track, Mapped& out) {
@@ -1747,25 +1836,35 @@ bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, i
   const int w = type_width(type);
   DevBuf stb = device_alloc(sizeof(RangeStat));
   auto* st = static_cast<RangeStat*>(stb->ptr);
-  const int grid = grid_for((n + RANGE_ROWS - 1) / RANGE_ROWS, 8);
-  hipLaunchKernelGGL(gb_range_init, dim3(1), dim3(1), 0, s, st);
-  {
-    ProfScope ps("gb_range_minmax");
-    switch (w) {
-      case 8: range_launch_minmax<8>(grid, s, vals, type, valid, voff, un, st); break;
-      case 4: range_launch_minmax<4>(grid, s, vals, type, valid, voff, un, st); break;
-      case 2: range_launch_minmax<2>(grid, s, vals, type, valid, voff, un, st); break;
-      default: range_launch_minmax<1>(grid, s, vals, type, valid, voff, un, st); break;
-    }
-    AM_HIP(hipGetLastError());
-  }
+  const unsigned long long limit = std::min<unsigned long long>(1ull << 24, std::max<unsigned long long>(1ull << 16, 4ull * uint64_t(n)));
+  // Pass 1 (sample, then all rows): the keys' smallest and largest values. A sample whose own span is
+  // already too wide sends the keys to the hash path after reading RANGE_SAMPLE rows.
   RangeStat host{};
-  AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
+  for (int full = 0; full < 2; full++) {
+    const uint32_t count = full ? un : std::min<uint32_t>(un, RANGE_SAMPLE);
+    const uint32_t stride = full ? 1u : un / count;
+    const int grid = grid_for((int64_t(count) + RANGE_ROWS - 1) / RANGE_ROWS, 8);
+    hipLaunchKernelGGL(gb_range_init, dim3(1), dim3(1), 0, s, st);
+    {
+      ProfScope ps(full ? "gb_range_minmax" : "gb_range_sample");
+      switch (w) {
+        case 8: range_launch_minmax<8>(grid, s, vals, type, valid, voff, count, stride, st); break;
+        case 4: range_launch_minmax<4>(grid, s, vals, type, valid, voff, count, stride, st); break;
+        case 2: range_launch_minmax<2>(grid, s, vals, type, valid, voff, count, stride, st); break;
+        default: range_launch_minmax<1>(grid, s, vals, type, valid, voff, count, stride, st); break;
+      }
+      AM_HIP(hipGetLastError());
+    }
+    AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
+    AM_HIP(hipStreamSynchronize(s));
+    const bool any = host.kmin <= host.kmax;
+    const unsigned long long span64 = any ? host.kmax - host.kmin + 1ull : 0ull;
+    if (any && (span64 == 0ull || span64 > limit)) return false;   // span64 == 0: the full 2^64 range
+    if (count == un) break;   // the sample was every row
+  }
   const bool any = host.kmin <= host.kmax;
   const unsigned long long span64 = any ? host.kmax - host.kmin + 1ull : 0ull;
-  const unsigned long long limit = std::min<unsigned long long>(1ull << 24, std::max<unsigned long long>(1ull << 16, 4ull * uint64_t(n)));
-  if (any && (span64 == 0ull || span64 > limit)) return false;   // span64 == 0: the full 2^64 range
+  const int grid = grid_for((n + RANGE_ROWS - 1) / RANGE_ROWS, 8);
   const uint32_t span = uint32_t(std::max<unsigned long long>(span64, 1ull));
   DevBuf occ = device_alloc(size_t(span) * 4);
   DevBuf first;
@@ -2280,7 +2379,7 @@ Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::
 // The shared op table of am_group_agg_ex and am_group_by over an id source.
 // The sampled hot groups of the ids (gb_hot_sample), once per mapping; none for small inputs.
 const long long* hot_groups(const IdSource& ids, int64_t n, long long K, GroupBy* cache, DevBuf& holder) {
-  if (n < (int64_t(1) << 20)) return nullptr;
+  if (!AM_AGG_HOT || n < (int64_t(1) << 20)) return nullptr;
   if (cache && cache->hot) return static_cast<const long long*>(cache->hot->ptr);
   holder = device_alloc(HOT_GROUPS * sizeof(long long));
   retain(holder);
