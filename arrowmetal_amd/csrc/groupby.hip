@@ -500,11 +500,57 @@ __global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t
   }
 }
 
-// Rows' slots to group ids, in place. Each workgroup caches slot -> id pairs in LDS (one 64-bit word
-// per entry, (slot + 1) | id << 32, direct mapped; any word a reader finds is a whole pair), so rows
-// of hot groups do not all read the same few lines of the id table.
 constexpr uint32_t IDS_CACHE = 2048;   // (slot * 2654435761) >> 21 indexes it
 static_assert(IDS_CACHE == (1u << 11), "the entry hash gives 11 bits");
+
+// The slots of one step's rows (BUILD_ROWS per thread, grid-strided), loaded together.
+struct IdsStep {
+  uint32_t sl[BUILD_ROWS];
+};
+
+__device__ __forceinline__ void ids_load(const uint32_t* slot_of, uint32_t n, uint64_t base, uint32_t gsize,
+                                         uint32_t tid, IdsStep& x) {
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {
+    const uint64_t i = base + uint64_t(r) * gsize + tid;
+    x.sl[r] = __builtin_nontemporal_load(slot_of + (i < n ? i : uint64_t(n) - 1u));
+  }
+}
+
+__device__ __forceinline__ void ids_step(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot, uint32_t null_gid,
+                                         uint64_t base, uint32_t gsize, uint32_t tid, const IdsStep& x,
+                                         unsigned long long* cache) {
+  auto entry = [](uint32_t sl) { return (sl * 2654435761u) >> 21; };   // [0, IDS_CACHE)
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) asm volatile("" ::"v"(x.sl[r]));   // every loaded register taken here
+  uint32_t g[BUILD_ROWS];
+  bool miss[BUILD_ROWS];
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {
+    // A row past the end holds the last row's word, which may already be an id: it is ignored.
+    const uint32_t sl = base + uint64_t(r) * gsize + tid < n ? x.sl[r] : NULL_SLOT;
+    miss[r] = false;
+    if (sl == NULL_SLOT) {
+      g[r] = null_gid;
+      continue;
+    }
+    const unsigned long long e = cache[entry(sl)];
+    miss[r] = uint32_t(e) != sl + 1u;
+    g[r] = miss[r] ? load_global(gid_of_slot + sl) : uint32_t(e >> 32);
+  }
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {
+    const uint64_t i = base + uint64_t(r) * gsize + tid;
+    if (miss[r]) cache[entry(x.sl[r])] = (unsigned long long)(x.sl[r] + 1u) | ((unsigned long long)g[r] << 32);   // i < n
+    if (i < n) __builtin_nontemporal_store(g[r], slot_of + i);
+  }
+}
+
+// Rows' slots to group ids, in place. Each workgroup caches slot -> id pairs in LDS (one 64-bit word
+// per entry, (slot + 1) | id << 32, direct mapped; any word a reader finds is a whole pair), so rows
+// of hot groups do not all read the same few lines of the id table. Two steps' slots take turns: the
+// next step's are loaded before this step's ids are looked up and stored. A step only reads slots that
+// no earlier step rewrites (each row is its own thread's), so the prefetch never sees an id.
 __global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot,
                                               uint32_t null_gid) {
   __shared__ unsigned long long cache[IDS_CACHE];
@@ -512,32 +558,18 @@ __global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, con
   __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  auto entry = [](uint32_t sl) { return (sl * 2654435761u) >> 21; };   // [0, IDS_CACHE)
-  for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
-    uint32_t sl[BUILD_ROWS], g[BUILD_ROWS];
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {
-      uint64_t i = base + uint64_t(r) * gsize + tid;
-      sl[r] = i < n ? __builtin_nontemporal_load(slot_of + i) : NULL_SLOT;
-    }
-    bool miss[BUILD_ROWS];
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {
-      miss[r] = false;
-      if (sl[r] == NULL_SLOT) {
-        g[r] = null_gid;
-        continue;
-      }
-      const unsigned long long e = cache[entry(sl[r])];
-      miss[r] = uint32_t(e) != sl[r] + 1u;
-      g[r] = miss[r] ? gid_of_slot[sl[r]] : uint32_t(e >> 32);
-    }
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {
-      uint64_t i = base + uint64_t(r) * gsize + tid;
-      if (miss[r]) cache[entry(sl[r])] = (unsigned long long)(sl[r] + 1u) | ((unsigned long long)g[r] << 32);
-      if (i < n) __builtin_nontemporal_store(g[r], slot_of + i);
-    }
+  const uint64_t stride = uint64_t(gsize) * BUILD_ROWS;
+  const int steps = int((uint64_t(n) + stride - 1) / stride);   // the same for every thread
+  if (steps == 0) return;
+  IdsStep xa, xb;
+  ids_load(slot_of, n, 0, gsize, tid, xa);
+  for (int t = 0; t < steps; t += 2) {
+    const uint64_t base = uint64_t(t) * stride;
+    ids_load(slot_of, n, base + stride, gsize, tid, xb);
+    ids_step(slot_of, n, gid_of_slot, null_gid, base, gsize, tid, xa, cache);
+    if (t + 1 >= steps) break;
+    ids_load(slot_of, n, base + 2 * stride, gsize, tid, xa);
+    ids_step(slot_of, n, gid_of_slot, null_gid, base + stride, gsize, tid, xb, cache);
   }
 }
 
@@ -678,6 +710,7 @@ struct CountP {
   static constexpr bool USES_VALUE = false;
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
+  static constexpr bool PART = true;   // may run through the partitioned passes (rows not needed)
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long, unsigned long long (&c)[1]) {
     c[0] = 1;
     return true;
@@ -692,6 +725,7 @@ struct SumCountP {
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
+  static constexpr bool PART = true;
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     c[0] = x;
     c[1] = 1;
@@ -708,6 +742,7 @@ struct MinMaxP {
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
+  static constexpr bool PART = true;
   __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     unsigned long long b = x, key;
     switch (a.v.type) {
@@ -757,6 +792,7 @@ struct FsScanP {
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = VW;
   static constexpr bool AUX = false;
+  static constexpr bool PART = false;   // word 3 is a row number
   __device__ static bool contrib(const AggArgs&, int64_t row, long long, unsigned long long raw,
                                  unsigned long long (&c)[4]) {
     const unsigned long long v = fs_bits<VW>(raw);
@@ -792,6 +828,7 @@ struct FsFixP {
   static constexpr int FIXED_VW = VW;
   // The group's grid word from fs_window, loaded for all rows in flight before the contributions.
   static constexpr bool AUX = true;
+  static constexpr bool PART = false;
   using AuxT = uint32_t;
   __device__ static AuxT aux_load(const AggArgs& a, long long g) {
     return reinterpret_cast<const uint32_t*>(a.aux)[g];
@@ -1261,6 +1298,15 @@ void launch_agg_vw(int vw, int blocks, size_t shmem, hipStream_t s, const AggArg
   }
 }
 
+#ifndef AM_AGG_LDS
+#define AM_AGG_LDS (144 * 1024)
+#endif
+// The groups whose accumulators all fit in one workgroup's LDS (the private-table mode of gb_agg).
+template <class P>
+uint32_t agg_dense_cap() {
+  return uint32_t(std::min<size_t>(AM_AGG_LDS, device().lds_per_block) / (P::W * 8));
+}
+
 // Runs family P over the rows into `o` (K x W words), with `g8` (NXCD x K x W words) for the XCDs'
 // partial tables; allocates nothing.
 template <class P>
@@ -1274,12 +1320,9 @@ void run_agg_into(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
   hipLaunchKernelGGL(gb_agg_init<P>, dim3(grid_for(NXCD * K * P::W, 4)), dim3(256), 0, s, g8, NXCD * K);
   AM_HIP(hipGetLastError());
   // LDS budget: a private table when the groups fit, otherwise a cache of that many slots.
-#ifndef AM_AGG_LDS
-#define AM_AGG_LDS (144 * 1024)
-#endif
   const size_t budget = std::min<size_t>(AM_AGG_LDS, device().lds_per_block);
   const uint32_t per_slot = P::W * 8;
-  uint32_t dense_cap = uint32_t(budget / per_slot);
+  const uint32_t dense_cap = agg_dense_cap<P>();
   int mode;
   uint32_t cap;
   if (a.K <= (long long)dense_cap) {
@@ -1305,11 +1348,440 @@ void run_agg_into(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
   AM_HIP(hipGetLastError());
 }
 
+// ---- partitioned aggregation: many groups and no global atomics
+//
+// A global atomic is carried out at the memory side here, not in L2, at about 23 G operations/s over the
+// whole chip, so an LDS cache in front of a global table spends most of its time on the rows it cannot
+// cache. Instead, with more groups than LDS holds, the rows are partitioned by group-id bucket (PART_G
+// consecutive ids) and each bucket is reduced in LDS:
+//   gb_part_hist     rows that contribute, per tile and bucket (bucket-major counts)
+//   gb_part_scan     per bucket, those counts -> each tile's offset in the bucket, and its total
+//   gb_part_bases    the buckets' bases and each bucket's first reduce workgroup
+//   gb_part_scatter  every contributing row's record (group id, raw value) to its bucket's run, staged
+//                    in LDS by bucket so that each run is written in consecutive addresses
+//   gb_part_reduce   one chunk of one bucket's records -> a partial table of PART_G groups (plain stores)
+//   gb_part_merge    each group's partials over its bucket's chunks -> the result
+// Every pass streams; the only atomics are LDS ones.
+constexpr int PART_LOG_G = 12;
+constexpr uint32_t PART_G = 1u << PART_LOG_G;      // groups per bucket (x W words of LDS in the reduce)
+constexpr int PART_ROWS = 16;                      // rows per thread of a 256-thread tile
+constexpr uint32_t PART_TILE = 256u * PART_ROWS;   // 4096 rows
+constexpr uint32_t PART_MAX_NB = 512;              // buckets, so at most 2^21 groups
+constexpr uint32_t PART_CHUNK = 65536;             // records per reduce workgroup
+
+// The id of row i when it contributes a record (a valid id in [0, K), and a non-null value for the
+// families that skip nulls), else -1. The same test in the count and the scatter.
+template <class P, int IDW>
+__device__ __forceinline__ long long part_gid(const AggArgs& a, int64_t i, int64_t n, unsigned long long idr,
+                                              uint32_t ib, uint32_t vb) {
+  long long g;
+  if constexpr (IDW == 8) g = (long long)idr;
+  else g = a.ids.type == T_I32 ? (long long)int32_t(uint32_t(idr)) : (long long)uint32_t(idr);
+  g += a.ids.bias;
+  const uint32_t ok = uint32_t(i < n) & vbit(a.ids.voff, i, ib) & (P::SKIP_NULL ? vbit(a.v.voff, i, vb) : 1u);
+  return (ok && g >= 0 && g < a.K) ? g : -1;
+}
+
+// Counts row `valid` in bucket b of s_cnt and returns its rank there. The lanes of the wave whose bucket is
+// the first lane's (most of them, where one bucket is hot) take one LDS atomic together; the others
+// take one each.
+__device__ __forceinline__ uint32_t part_rank(uint32_t* s_cnt, bool valid, uint32_t b) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t lead = __builtin_amdgcn_readfirstlane(valid ? b : 0xFFFFFFFFu);
+  const unsigned long long m = __ballot(valid && b == lead);
+  uint32_t r0 = 0;
+  if (m) {   // uniform
+    const int leader = __builtin_ctzll(m);
+    if (lane == leader) r0 = atomicAdd(&s_cnt[lead], uint32_t(__popcll(m)));
+    r0 = __shfl(r0, leader, 64);
+  }
+  if (valid && b == lead) return r0 + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+  return valid ? atomicAdd(&s_cnt[b], 1u) : 0u;
+}
+
+template <class P, int IDW>
+__global__ __launch_bounds__(256) void gb_part_hist(AggArgs a, uint32_t nb, uint32_t ntiles, uint32_t* hist) {
+  __shared__ uint32_t s_cnt[PART_MAX_NB];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) s_cnt[b] = 0u;
+  __syncthreads();
+  const int64_t t0 = int64_t(blockIdx.x) * PART_TILE;
+  unsigned long long idr[PART_ROWS];
+  uint32_t ib[PART_ROWS], vb[PART_ROWS];
+#pragma unroll
+  for (int k = 0; k < PART_ROWS; k++) {   // every load first; rows past the end read the last row
+    const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
+    const int64_t ii = i < a.n ? i : a.n - 1;
+    idr[k] = ld_raw<IDW>(a.ids.ids, ii);
+    ib[k] = ld_vbyte(a.ids.valid, a.ids.voff, ii);
+    vb[k] = P::SKIP_NULL ? ld_vbyte(a.v.valid, a.v.voff, ii) : 0xFFu;
+  }
+#pragma unroll
+  for (int k = 0; k < PART_ROWS; k++) {
+    const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
+    const long long g = part_gid<P, IDW>(a, i, a.n, idr[k], ib[k], vb[k]);
+    (void)part_rank(s_cnt, g >= 0, uint32_t(g) >> PART_LOG_G);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[size_t(b) * ntiles + blockIdx.x] = s_cnt[b];
+}
+
+// The exclusive scan of 1024 values held one per thread (s: 1024 words of LDS); returns this thread's
+// exclusive prefix and sets `total`.
+__device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* s, uint32_t& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) s[w] = incl;
+  __syncthreads();
+  uint32_t wb = 0, all = 0;
+  for (int j = 0; j < 16; j++) {
+    const uint32_t v = s[j];
+    wb += j < w ? v : 0u;
+    all += v;
+  }
+  __syncthreads();
+  total = all;
+  return wb + incl - x;
+}
+
+// Workgroup b: the exclusive scan of bucket b's tile counts in place (hist[b][t] becomes the first
+// record of tile t within bucket b) and the bucket's total. Four consecutive tiles per thread per step.
+__global__ __launch_bounds__(1024) void gb_part_scan(uint32_t* hist, uint32_t ntiles, uint32_t* total) {
+  __shared__ uint32_t s[16];
+  uint32_t* h = hist + size_t(blockIdx.x) * ntiles;
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < ntiles; t0 += 4u * blockDim.x) {
+    uint32_t v[4], x = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t t = t0 + 4u * threadIdx.x + uint32_t(k);
+      v[k] = t < ntiles ? h[t] : 0u;
+      x += v[k];
+    }
+    uint32_t step_total;
+    uint32_t run = carry + block_scan_1024(x, s, step_total);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t t = t0 + 4u * threadIdx.x + uint32_t(k);
+      if (t < ntiles) h[t] = run;
+      run += v[k];
+    }
+    carry += step_total;
+  }
+  if (threadIdx.x == 0) total[blockIdx.x] = carry;
+}
+
+// One workgroup over the buckets (nb <= 512 < 1024): base[b] = the records of the buckets before b
+// (base[nb] = all records), first[b] = the reduce workgroups (chunks) of the buckets before b.
+__global__ __launch_bounds__(1024) void gb_part_bases(const uint32_t* total, uint32_t nb, uint32_t* base, uint32_t* first) {
+  __shared__ uint32_t s[16];
+  const uint32_t b = threadIdx.x;
+  const uint32_t cnt = b < nb ? total[b] : 0u;
+  uint32_t all;
+  const uint32_t rb = block_scan_1024(cnt, s, all);
+  const uint32_t chunks = (cnt + PART_CHUNK - 1) / PART_CHUNK;
+  uint32_t all_chunks;
+  const uint32_t cb = block_scan_1024(chunks, s, all_chunks);
+  if (b <= nb) {
+    base[b] = b < nb ? rb : all;
+    first[b] = b < nb ? cb : all_chunks;
+  }
+}
+
+// Each contributing row's record: rec_gid[r] = its group id, rec_val[r] = its raw value bits (none for
+// families that read no value). Within a tile the records are ranked per bucket in LDS (their order
+// does not matter to any family here) and staged in bucket order, then written out in order: the run
+// of bucket b starts at base[b] + off[b][tile].
+template <class P, int IDW, int VW>
+__global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, uint32_t ntiles, const uint32_t* off,
+                                                       const uint32_t* base, uint32_t* rec_gid,
+                                                       unsigned long long* rec_val) {
+  __shared__ uint32_t s_cnt[PART_MAX_NB], s_off[PART_MAX_NB], s_dst[PART_MAX_NB];
+  __shared__ uint32_t s_gid[PART_TILE];
+  __shared__ unsigned long long s_val[VW ? PART_TILE : 1];
+  __shared__ uint32_t s_wsum[4];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    s_cnt[b] = 0u;
+    s_dst[b] = base[b] + off[size_t(b) * ntiles + blockIdx.x];
+  }
+  __syncthreads();
+  const int64_t t0 = int64_t(blockIdx.x) * PART_TILE;
+  unsigned long long idr[PART_ROWS], vr[PART_ROWS];
+  uint32_t ib[PART_ROWS], vb[PART_ROWS];
+#pragma unroll
+  for (int k = 0; k < PART_ROWS; k++) {
+    const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
+    const int64_t ii = i < a.n ? i : a.n - 1;
+    idr[k] = ld_raw<IDW>(a.ids.ids, ii);
+    ib[k] = ld_vbyte(a.ids.valid, a.ids.voff, ii);
+    vr[k] = 0;
+    if constexpr (VW != 0) vr[k] = ld_raw<VW>(a.v.vals, ii);
+    vb[k] = P::SKIP_NULL ? ld_vbyte(a.v.valid, a.v.voff, ii) : 0xFFu;
+  }
+  long long gs[PART_ROWS];
+  uint32_t rank[PART_ROWS];
+#pragma unroll
+  for (int k = 0; k < PART_ROWS; k++) {
+    const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
+    gs[k] = part_gid<P, IDW>(a, i, a.n, idr[k], ib[k], vb[k]);
+    rank[k] = part_rank(s_cnt, gs[k] >= 0, uint32_t(gs[k]) >> PART_LOG_G);
+  }
+  __syncthreads();
+  // Exclusive scan of the tile's bucket counts (nb <= 512: two per thread, then the four waves).
+  const uint32_t b0 = 2u * threadIdx.x, b1 = b0 + 1u;
+  const uint32_t c0 = b0 < nb ? s_cnt[b0] : 0u, c1 = b1 < nb ? s_cnt[b1] : 0u;
+  uint32_t x = c0 + c1, incl = x;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) s_wsum[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) wbase += s_wsum[w];
+  const uint32_t excl = wbase + incl - x;
+  if (b0 < nb) s_off[b0] = excl;
+  if (b1 < nb) s_off[b1] = excl + c0;
+  const uint32_t tile_count = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PART_ROWS; k++) {
+    if (gs[k] < 0) continue;
+    const uint32_t p = s_off[uint32_t(gs[k]) >> PART_LOG_G] + rank[k];
+    s_gid[p] = uint32_t(gs[k]);
+    if constexpr (VW != 0) s_val[p] = vr[k];
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < tile_count; p += blockDim.x) {
+    const uint32_t g = s_gid[p];
+    const uint32_t b = g >> PART_LOG_G;
+    const uint32_t dst = s_dst[b] + (p - s_off[b]);
+    rec_gid[dst] = g;
+    if constexpr (VW != 0) rec_val[dst] = s_val[p];
+  }
+}
+
+constexpr int PART_RED_ROWS = 4;   // records per thread per step of the reduce
+constexpr int PART_PEEL = 16;      // lanes on one group above which a wave combines them before the LDS atomic
+
+template <int VW>
+struct PartRecs {
+  uint32_t g[PART_RED_ROWS];
+  unsigned long long raw[PART_RED_ROWS];
+};
+
+// Loads the records of the step at r (records past r1 read the last one; r1 > 0).
+template <int VW>
+__device__ __forceinline__ void part_load(const uint32_t* rec_gid, const unsigned long long* rec_val, int64_t r,
+                                          int64_t r1, PartRecs<VW>& x) {
+#pragma unroll
+  for (int k = 0; k < PART_RED_ROWS; k++) {
+    const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
+    const int64_t ii = i < r1 ? i : r1 - 1;
+    x.g[k] = load_global(rec_gid + ii);
+    x.raw[k] = VW ? load_global(rec_val + ii) : 0ull;
+  }
+}
+
+// Applies the step's records at r to the LDS table. A hot group (many lanes of the wave on the first
+// lane's group) is combined across the wave first and applied once, instead of one LDS atomic per
+// lane on one address.
+template <class P, int VW>
+__device__ __forceinline__ void part_apply(const AggArgs& a, int64_t r, int64_t r1, const PartRecs<VW>& x,
+                                           unsigned long long* lds) {
+#pragma unroll
+  for (int k = 0; k < PART_RED_ROWS; k++) asm volatile("" ::"v"(x.g[k]), "v"(x.raw[k]));   // every loaded register taken here
+#pragma unroll
+  for (int k = 0; k < PART_RED_ROWS; k++) {
+    const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
+    const uint32_t g = x.g[k];
+    unsigned long long cs[P::W];
+    bool act = P::contrib(a, 0, (long long)g, val_in(a.v, x.raw[k]), cs) && i < r1;
+    const uint32_t lead = __builtin_amdgcn_readfirstlane(act ? g : 0xFFFFFFFFu);
+    const unsigned long long m = __ballot(act && g == lead);
+    if (__popcll(m) >= PART_PEEL) {   // uniform
+      unsigned long long v[P::W];
+#pragma unroll
+      for (int w = 0; w < P::W; w++) {
+        const int op = P::op(w);
+        unsigned long long y0 = (act && g == lead) ? cs[w] : identity(op);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const unsigned long long y = __shfl_xor(y0, off, 64);
+          y0 = op == W_ADD ? y0 + y : (op == W_MIN ? (y < y0 ? y : y0) : (op == W_MAX ? (y > y0 ? y : y0) : (y0 | y)));
+        }
+        v[w] = y0;
+      }
+      if ((threadIdx.x & 63) == __builtin_ctzll(m)) apply<P>(as_lds(&lds[size_t(lead & (PART_G - 1)) * P::W]), v);
+      act = act && g != lead;
+    }
+    if (act) apply<P>(as_lds(&lds[size_t(g & (PART_G - 1)) * P::W]), cs);
+  }
+}
+
+// Workgroup c reduces chunk c - first[b] of bucket b into part[c] (PART_G x W words, identities where
+// no record fell). Workgroups past the last chunk do nothing.
+template <class P, int VW>
+__global__ __launch_bounds__(512) void gb_part_reduce(AggArgs a, uint32_t nb, const uint32_t* base, const uint32_t* first,
+                                                      const uint32_t* rec_gid, const unsigned long long* rec_val,
+                                                      unsigned long long* part) {
+  extern __shared__ unsigned long long lds[];
+  const uint32_t c = blockIdx.x;
+  if (c >= first[nb]) return;   // uniform
+  uint32_t lo = 0, hi = nb;     // the bucket b with first[b] <= c < first[b + 1]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (first[mid] <= c) lo = mid; else hi = mid;
+  }
+  const uint32_t b = lo;
+  const int64_t r0 = int64_t(base[b]) + int64_t(c - first[b]) * PART_CHUNK;
+  const int64_t r1 = std::min<int64_t>(int64_t(base[b + 1]), r0 + PART_CHUNK);
+  for (uint32_t j = threadIdx.x; j < PART_G; j += blockDim.x)
+#pragma unroll
+    for (int w = 0; w < P::W; w++) lds[size_t(j) * P::W + w] = identity(P::op(w));
+  __syncthreads();
+  const int64_t step = int64_t(blockDim.x) * PART_RED_ROWS;
+  // Two record buffers take turns, as in gb_agg: the next step's loads are in flight while this step's
+  // records are applied (LDS only, so nothing here waits on a global write).
+  const int steps = __builtin_amdgcn_readfirstlane(int((r1 - r0 + step - 1) / step));
+  PartRecs<VW> ra, rb;
+  part_load<VW>(rec_gid, rec_val, r0, r1, ra);
+  for (int t = 0; t < steps; t += 2) {
+    const int64_t r = r0 + int64_t(t) * step;
+    part_load<VW>(rec_gid, rec_val, r + step, r1, rb);
+    part_apply<P, VW>(a, r, r1, ra, lds);
+    if (t + 1 >= steps) break;
+    part_load<VW>(rec_gid, rec_val, r + 2 * step, r1, ra);
+    part_apply<P, VW>(a, r + step, r1, rb, lds);
+  }
+  __syncthreads();
+  unsigned long long* dst = part + size_t(c) * PART_G * P::W;
+  for (uint32_t j = threadIdx.x; j < PART_G * P::W; j += blockDim.x) dst[j] = lds[j];
+}
+
+// out[g] = the combination of part[c][g mod PART_G] over the chunks c of g's bucket.
+template <class P>
+__global__ void gb_part_merge(const unsigned long long* part, const uint32_t* first, long long K, unsigned long long* out) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= K) return;
+  const uint32_t b = uint32_t(g >> PART_LOG_G), j = uint32_t(g & (PART_G - 1));
+  const uint32_t c0 = first[b], c1 = first[b + 1];
+#pragma unroll
+  for (int w = 0; w < P::W; w++) {
+    const int op = P::op(w);
+    unsigned long long acc = identity(op);
+    for (uint32_t c = c0; c < c1; c++) {
+      const unsigned long long v = part[(size_t(c) * PART_G + j) * P::W + w];
+      if (op == W_ADD) acc += v;
+      else if (op == W_MIN) acc = v < acc ? v : acc;
+      else if (op == W_MAX) acc = v > acc ? v : acc;
+      else if (op == W_OR) acc |= v;
+    }
+    out[size_t(g) * P::W + w] = acc;
+  }
+}
+
+// Family P over many groups through the partitioned passes into `o` (K x W words); false, with
+// nothing launched, where they do not apply (a family that needs its rows or its own grid words, too
+// many groups, too few rows).
+template <class P>
+bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned long long* o) {
+  if constexpr (!P::PART) {
+    return false;
+  } else {
+    const long long K = a.K;
+    const uint32_t nb = uint32_t((K + PART_G - 1) / PART_G);
+    if (a.n < int64_t(PART_TILE) * 16 || nb == 0 || nb > PART_MAX_NB || a.n >= (int64_t(1) << 32)) return false;
+    const uint32_t ntiles = uint32_t((a.n + PART_TILE - 1) / PART_TILE);
+    const int idw = a.ids.type == T_I64 ? 8 : 4;
+    const int vw = !P::USES_VALUE ? 0 : type_width(a.v.type);
+    DevBuf hist = device_alloc(size_t(nb) * ntiles * 4);
+    DevBuf meta = device_alloc(size_t(3) * (nb + 1) * 4);
+    uint32_t* base = static_cast<uint32_t*>(meta->ptr);
+    uint32_t* first = base + nb + 1;
+    uint32_t* total = first + nb + 1;
+    DevBuf gid = device_alloc(size_t(a.n) * 4);
+    DevBuf val = device_alloc(vw ? size_t(a.n) * 8 : 8);
+    const uint32_t chunk_bound = uint32_t((a.n + PART_CHUNK - 1) / PART_CHUNK) + nb;
+    DevBuf part = device_alloc(size_t(chunk_bound) * PART_G * P::W * 8);
+    uint32_t* h = static_cast<uint32_t*>(hist->ptr);
+    uint32_t* rg = static_cast<uint32_t*>(gid->ptr);
+    unsigned long long* rv = static_cast<unsigned long long*>(val->ptr);
+    {
+      ProfScope ps("gb_part_hist");
+      if (idw == 8) hipLaunchKernelGGL((gb_part_hist<P, 8>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
+      else hipLaunchKernelGGL((gb_part_hist<P, 4>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("gb_part_scan");
+      hipLaunchKernelGGL(gb_part_scan, dim3(nb), dim3(1024), 0, s, h, ntiles, total);
+      AM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(gb_part_bases, dim3(1), dim3(1024), 0, s, total, nb, base, first);
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("gb_part_scatter");
+#define AM_PART_SCATTER(IW, VV) \
+  hipLaunchKernelGGL((gb_part_scatter<P, IW, VV>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h, base, rg, rv)
+      if (idw == 8) {
+        switch (vw) {
+          case 0: AM_PART_SCATTER(8, 0); break;
+          case 8: AM_PART_SCATTER(8, 8); break;
+          case 4: AM_PART_SCATTER(8, 4); break;
+          case 2: AM_PART_SCATTER(8, 2); break;
+          default: AM_PART_SCATTER(8, 1); break;
+        }
+      } else {
+        switch (vw) {
+          case 0: AM_PART_SCATTER(4, 0); break;
+          case 8: AM_PART_SCATTER(4, 8); break;
+          case 4: AM_PART_SCATTER(4, 4); break;
+          case 2: AM_PART_SCATTER(4, 2); break;
+          default: AM_PART_SCATTER(4, 1); break;
+        }
+      }
+#undef AM_PART_SCATTER
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps(name);
+      const size_t shmem = size_t(PART_G) * P::W * 8;
+      if (vw) {
+        if (shmem > 65536)
+          AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_part_reduce<P, 8>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+        hipLaunchKernelGGL((gb_part_reduce<P, 8>), dim3(chunk_bound), dim3(512), shmem, s, a, nb, base, first, rg, rv,
+                           static_cast<unsigned long long*>(part->ptr));
+      } else {
+        if (shmem > 65536)
+          AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_part_reduce<P, 0>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
+        hipLaunchKernelGGL((gb_part_reduce<P, 0>), dim3(chunk_bound), dim3(512), shmem, s, a, nb, base, first, rg, rv,
+                           static_cast<unsigned long long*>(part->ptr));
+      }
+      AM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(gb_part_merge<P>, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const unsigned long long*>(part->ptr), first, K, o);
+    AM_HIP(hipGetLastError());
+    return true;
+  }
+}
+
 // Runs family P over the rows into a fresh global table of K x W words.
 template <class P>
 DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
   const long long K = std::max<long long>(a.K, 1);
   DevBuf out = device_alloc(size_t(K) * P::W * 8);
+  if (a.K > (long long)agg_dense_cap<P>() && run_agg_part<P>(a, s, name, static_cast<unsigned long long*>(out->ptr)))
+    return out;
   DevBuf tabs;
   if (a.n > 0) tabs = device_alloc(size_t(NXCD) * size_t(K) * P::W * 8);
   run_agg_into<P>(a, s, name, tabs ? static_cast<unsigned long long*>(tabs->ptr) : nullptr,
