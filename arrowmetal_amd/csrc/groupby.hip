@@ -1568,7 +1568,33 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
 }
 
 constexpr int PART_RED_ROWS = 4;   // records per thread per step of the reduce
-constexpr int PART_PEEL = 16;      // lanes on one group above which a wave combines them before the LDS atomic
+#ifndef AM_PART_PEEL
+#define AM_PART_PEEL 16
+#endif
+constexpr int PART_PEEL = AM_PART_PEEL;   // lanes on one group from which a wave combines them before the LDS atomic
+
+// One step of wave_combine64: x combined with its DPP-moved copy (lanes without a source see the
+// identity), each 64-bit word moved as two halves.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_combine(unsigned long long x, unsigned long long id, int op) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_update_dpp(int(uint32_t(id)), int(uint32_t(x)), CTRL, ROW_MASK, 0xF, false));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(int(uint32_t(id >> 32)), int(uint32_t(x >> 32)), CTRL, ROW_MASK, 0xF, false));
+  const unsigned long long y = (unsigned long long)hi << 32 | lo;
+  return op == W_ADD ? x + y : (op == W_MIN ? (y < x ? y : x) : (op == W_MAX ? (y > x ? y : x) : (x | y)));
+}
+
+// The combination under `op` of x over the wave, in lane 63: an inclusive scan by DPP row shifts and
+// row broadcasts (VALU only; no LDS traffic).
+__device__ __forceinline__ unsigned long long wave_combine64(unsigned long long x, int op) {
+  const unsigned long long id = identity(op);
+  x = dpp_combine<0x111, 0xF>(x, id, op);   // row_shr:1
+  x = dpp_combine<0x112, 0xF>(x, id, op);   // row_shr:2
+  x = dpp_combine<0x114, 0xF>(x, id, op);   // row_shr:4
+  x = dpp_combine<0x118, 0xF>(x, id, op);   // row_shr:8
+  x = dpp_combine<0x142, 0xA>(x, id, op);   // row_bcast:15 into rows 1 and 3
+  x = dpp_combine<0x143, 0xC>(x, id, op);   // row_bcast:31 into rows 2 and 3
+  return x;
+}
 
 template <int VW>
 struct PartRecs {
@@ -1610,15 +1636,9 @@ __device__ __forceinline__ void part_apply(const AggArgs& a, int64_t r, int64_t 
 #pragma unroll
       for (int w = 0; w < P::W; w++) {
         const int op = P::op(w);
-        unsigned long long y0 = (act && g == lead) ? cs[w] : identity(op);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const unsigned long long y = __shfl_xor(y0, off, 64);
-          y0 = op == W_ADD ? y0 + y : (op == W_MIN ? (y < y0 ? y : y0) : (op == W_MAX ? (y > y0 ? y : y0) : (y0 | y)));
-        }
-        v[w] = y0;
+        v[w] = wave_combine64((act && g == lead) ? cs[w] : identity(op), op);
       }
-      if ((threadIdx.x & 63) == __builtin_ctzll(m)) apply<P>(as_lds(&lds[size_t(lead & (PART_G - 1)) * P::W]), v);
+      if ((threadIdx.x & 63) == 63) apply<P>(as_lds(&lds[size_t(lead & (PART_G - 1)) * P::W]), v);
       act = act && g != lead;
     }
     if (act) apply<P>(as_lds(&lds[size_t(g & (PART_G - 1)) * P::W]), cs);
