@@ -1685,25 +1685,63 @@ __global__ __launch_bounds__(512) void gb_part_reduce(AggArgs a, uint32_t nb, co
   for (uint32_t j = threadIdx.x; j < PART_G * P::W; j += blockDim.x) dst[j] = lds[j];
 }
 
-// out[g] = the combination of part[c][g mod PART_G] over the chunks c of g's bucket.
+// out[g] = the combination of part[c][g mod PART_G] over the chunks c of g's bucket. A hot bucket has
+// hundreds of chunks, so a workgroup of 16 waves covers 64 groups: wave w combines the chunks
+// c0 + w, c0 + w + 16, ... (each load of a wave reads 64 consecutive groups of one chunk, several
+// chunks in flight), then the 16 wave results of each group are combined through LDS.
+constexpr int PART_MERGE_LOADS = 4;
+constexpr int PART_MERGE_WAVES = 16;
 template <class P>
-__global__ void gb_part_merge(const unsigned long long* part, const uint32_t* first, long long K, unsigned long long* out) {
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= K) return;
-  const uint32_t b = uint32_t(g >> PART_LOG_G), j = uint32_t(g & (PART_G - 1));
-  const uint32_t c0 = first[b], c1 = first[b + 1];
+__global__ __launch_bounds__(1024) void gb_part_merge(const unsigned long long* part, const uint32_t* first, long long K,
+                                                      unsigned long long* out) {
+  __shared__ unsigned long long s_acc[PART_MERGE_WAVES][64][P::W];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long g = (long long)blockIdx.x * 64 + lane;
+  const long long gg = g < K ? g : K - 1;
+  const uint32_t b = uint32_t(gg >> PART_LOG_G), j = uint32_t(gg & (PART_G - 1));
+  const uint32_t c0 = first[b], c1 = first[b + 1];   // the same for the 64 groups unless they straddle buckets
+  unsigned long long acc[P::W];
 #pragma unroll
-  for (int w = 0; w < P::W; w++) {
-    const int op = P::op(w);
-    unsigned long long acc = identity(op);
-    for (uint32_t c = c0; c < c1; c++) {
-      const unsigned long long v = part[(size_t(c) * PART_G + j) * P::W + w];
-      if (op == W_ADD) acc += v;
-      else if (op == W_MIN) acc = v < acc ? v : acc;
-      else if (op == W_MAX) acc = v > acc ? v : acc;
-      else if (op == W_OR) acc |= v;
+  for (int w = 0; w < P::W; w++) acc[w] = identity(P::op(w));
+  for (uint32_t c = c0 + uint32_t(wave); c < c1; c += PART_MERGE_WAVES * PART_MERGE_LOADS) {
+    unsigned long long v[PART_MERGE_LOADS][P::W];
+#pragma unroll
+    for (int u = 0; u < PART_MERGE_LOADS; u++) {   // every load first; chunks past c1 read chunk c
+      const uint32_t cu = c + uint32_t(u * PART_MERGE_WAVES);
+      const uint32_t cc = cu < c1 ? cu : c;
+#pragma unroll
+      for (int w = 0; w < P::W; w++) v[u][w] = part[(size_t(cc) * PART_G + j) * P::W + w];
     }
-    out[size_t(g) * P::W + w] = acc;
+#pragma unroll
+    for (int u = 0; u < PART_MERGE_LOADS; u++) {
+      if (c + uint32_t(u * PART_MERGE_WAVES) >= c1) break;
+#pragma unroll
+      for (int w = 0; w < P::W; w++) {
+        const int op = P::op(w);
+        if (op == W_ADD) acc[w] += v[u][w];
+        else if (op == W_MIN) acc[w] = v[u][w] < acc[w] ? v[u][w] : acc[w];
+        else if (op == W_MAX) acc[w] = v[u][w] > acc[w] ? v[u][w] : acc[w];
+        else if (op == W_OR) acc[w] |= v[u][w];
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < P::W; w++) s_acc[wave][lane][w] = acc[w];
+  __syncthreads();
+  if (wave == 0 && g < K) {
+    for (int x = 1; x < PART_MERGE_WAVES; x++) {
+#pragma unroll
+      for (int w = 0; w < P::W; w++) {
+        const int op = P::op(w);
+        const unsigned long long y = s_acc[x][lane][w];
+        if (op == W_ADD) acc[w] += y;
+        else if (op == W_MIN) acc[w] = y < acc[w] ? y : acc[w];
+        else if (op == W_MAX) acc[w] = y > acc[w] ? y : acc[w];
+        else if (op == W_OR) acc[w] |= y;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < P::W; w++) out[size_t(g) * P::W + w] = acc[w];
   }
 }
 
@@ -1788,9 +1826,12 @@ bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
       }
       AM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(gb_part_merge<P>, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
-                       static_cast<const unsigned long long*>(part->ptr), first, K, o);
-    AM_HIP(hipGetLastError());
+    {
+      ProfScope ps("gb_part_merge");
+      hipLaunchKernelGGL(gb_part_merge<P>, dim3(uint32_t((K + 63) / 64)), dim3(64 * PART_MERGE_WAVES), 0, s,
+                         static_cast<const unsigned long long*>(part->ptr), first, K, o);
+      AM_HIP(hipGetLastError());
+    }
     return true;
   }
 }

@@ -144,7 +144,7 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "orderby_groupby_50M_pmc_traffic.tx
 ROCPROF_NAME = {
     "take": "take_kernel<u64, u32>",
     "sort_upsweep": "sort_upsweep<u64, 8>",
-    "sort_msd_first": "msd_part<u64, 8, 16>",
+    "sort_msd_first": "msd_part<u64, 8, 16, false>",
     "sort_msd_hist": "lvl_hist<u64>",
     "sort_msd_scatter": "lvl_pass<u64, 20>",
     "sort_msd_local": "hyb_local<u64, 8>",
@@ -154,7 +154,11 @@ ROCPROF_NAME = {
     "gb_hll": "gb_hll<8>",
     "gb_build": "gb_build<false, 8>",
     "gb_ids": "gb_ids",
-    "gb_sum": "gb_agg<SumCountP, 4, 8>",
+    "gb_part_hist": "gb_part_hist<SumCountP, 4>",
+    "gb_part_scatter": "gb_part_scatter<SumCountP, 4, 8>",
+    "gb_sum": "gb_part_reduce<SumCountP, 8>",
+    "order_by_partition": "ob_partition<8>",
+    "order_by_gather": "ob_gather<8>",
 }
 
 
@@ -203,7 +207,10 @@ def group_by_alg_bytes(n: int):
         "gb_hll": (8 * n, "distinct-key estimate: read key; 8 B/row"),
         "gb_build": (12 * n, "hash insert: read key, write slot; 12 B/row"),
         "gb_ids": (8 * n, "slot to ascending id: read slot, write id; 8 B/row"),
-        "gb_sum": (12 * n, "int64 sum + count: read id + value; 12 B/row"),
+        "gb_part_hist": (4 * n, "partitioned aggregate, bucket counts per tile: read id; 4 B/row"),
+        "gb_part_scatter": (24 * n, "partitioned aggregate, records to their bucket: read id + value, write id + value; "
+                                    "24 B/row"),
+        "gb_sum": (12 * n, "int64 sum + count: read id + value (or the partitioned record); 12 B/row"),
         "gb_minmax": (12 * n, "min / max: read id + value; 12 B/row"),
         "fs_scan": (12 * n, "exact float sum, scan pass: read id + value; 12 B/row"),
         "fs_fixed": (12 * n, "exact float sum, fixed-point pass: read id + value; 12 B/row"),
