@@ -79,6 +79,17 @@ __device__ __forceinline__ bool ld_valid(const uint8_t* valid, int64_t voff, int
   return (load_global(p) >> (valid ? (b & 7) : 0)) & 1;
 }
 
+// A row's validity byte, loaded (a column without a bitmap loads the all-valid constant).
+__device__ __forceinline__ uint32_t ld_vbyte(const uint8_t* valid, int64_t voff, int64_t i) {
+  return load_global(valid ? valid + ((voff + i) >> 3) : &kAllValid);
+}
+
+// Row i's bit of its loaded validity byte (a column without a bitmap loads 0xFF: every bit set). No
+// branch: the byte is used on every path, so that its register is never left with a load in flight.
+__device__ __forceinline__ uint32_t vbit(int64_t voff, int64_t i, uint32_t byte) {
+  return (byte >> uint32_t((voff + i) & 7)) & 1u;
+}
+
 // The group key of raw element bits: integers widened with their sign, float NaN made canonical and
 // -0.0 made +0.0 (registers only).
 __device__ __forceinline__ unsigned long long key_of_raw(unsigned long long r, int type) {
@@ -274,7 +285,10 @@ __global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, uns
 // cached never reads the table: with skewed keys, the few hottest slot lines would otherwise take a
 // large share of all table reads, on one L2 channel each.
 constexpr int BUILD_ROWS = 8;
-constexpr uint32_t BUILD_CACHE = 2048;
+#ifndef AM_BUILD_CACHE
+#define AM_BUILD_CACHE 2048
+#endif
+constexpr uint32_t BUILD_CACHE = AM_BUILD_CACHE;   // a power of two
 
 // Caches key -> slot in the workgroup's LDS if the key's entry is still free. The slot is written after
 // the key; a reader that finds the key with no slot yet reads the table instead.
@@ -284,21 +298,139 @@ __device__ __forceinline__ void cache_slot(unsigned long long* ckey, uint32_t* c
   if (ckey[c] == EMPTY && atomicCAS(&ckey[c], EMPTY, k) == EMPTY) cslot[c] = slot;
 }
 
+// The keys and validity bytes of one build step's rows, as loaded.
+struct BuildRows {
+  unsigned long long k[BUILD_ROWS];
+  uint32_t vb[BUILD_ROWS];
+};
+
+template <int W>
+__device__ __forceinline__ void build_load(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+                                           uint64_t base, uint32_t gsize, uint32_t tid, BuildRows& x) {
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {   // rows past the end read the last row
+    const uint64_t iu = base + uint64_t(r) * gsize + tid;
+    const int64_t ii = int64_t(iu < uint64_t(n) ? iu : uint64_t(n) - 1u);
+    x.k[r] = ld_raw<W>(vals, ii);
+    x.vb[r] = ld_vbyte(valid, voff, ii);
+  }
+}
+
+// What a thread of gb_build keeps across its steps.
+struct BuildState {
+  unsigned long long* tkey;
+  uint32_t* tfirst;
+  uint32_t* special_flag;
+  uint32_t tbase, special, cap_mask, budget;
+  unsigned long long* ckey;
+  uint32_t* cslot;
+  unsigned long long nulls;
+  bool overflow;
+};
+
+// One step: the rows at `base` (loaded in `cur`). Their first table slots are read first, then the
+// next step's rows are loaded (into `nxt`, at `nbase`), so that those loads are in flight while this
+// step waits for its slots, and waiting for the slots never waits for them (memory operations
+// complete in issue order).
+template <bool TRACK_FIRST, int W>
+__device__ __forceinline__ void build_step(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n,
+                                           uint64_t base, uint64_t nbase, uint32_t gsize, uint32_t tid,
+                                           const BuildRows& cur, BuildRows& nxt, uint32_t* slot_of, BuildStatus* st,
+                                           BuildState& b) {
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) asm volatile("" ::"v"(cur.k[r]), "v"(cur.vb[r]));   // every loaded register taken here
+  unsigned long long k[BUILD_ROWS], cv[BUILD_ROWS];
+  uint32_t s[BUILD_ROWS];
+  int cat[BUILD_ROWS];   // 0 key, 1 null, 2 the EMPTY key, 3 past the end, 4 cached
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {
+    const uint64_t i = base + uint64_t(r) * gsize + tid;
+    const bool vok = vbit(voff, int64_t(i < n ? i : uint64_t(n) - 1u), cur.vb[r]) != 0u;
+    cat[r] = 3;
+    k[r] = 0;
+    if (i < n) {
+      if (!vok) {
+        cat[r] = 1;
+      } else {
+        k[r] = key_of_raw(cur.k[r], type);
+        cat[r] = k[r] == EMPTY ? 2 : 0;
+      }
+    }
+    const unsigned long long h = mix64(k[r]);
+    s[r] = uint32_t(h) & b.cap_mask;
+    // A cached key takes its slot from LDS (cat 4).
+    if (cat[r] == 0) {
+      const uint32_t c = uint32_t(h >> 40) & (BUILD_CACHE - 1u);
+      const uint32_t cs = b.cslot[c];
+      if (b.ckey[c] == k[r] && cs != NULL_SLOT) {
+        cat[r] = 4;
+        s[r] = cs;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) cv[r] = cat[r] == 0 ? probe_load(&b.tkey[s[r]]) : 0ull;
+  build_load<W>(vals, valid, voff, n, nbase, gsize, tid, nxt);
+  // Rows whose key already sits in its home slot (most rows of a repeated key) finish here; the
+  // others (a first insert or a collision) go through the probe loop one at a time below, with the
+  // key read again, so that the loop is not inlined per row.
+  uint32_t slow = 0;
+#pragma unroll
+  for (int r = 0; r < BUILD_ROWS; r++) {
+    const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
+    if (cat[r] == 3) continue;
+    uint32_t slot;
+    if (cat[r] == 1) {
+      slot = NULL_SLOT;
+      b.nulls++;
+      if (TRACK_FIRST) lower_first(&st->null_first, i);
+    } else if (cat[r] == 2) {
+      slot = b.special;
+      if (!*b.special_flag) atomicExch(b.special_flag, 1u);
+    } else if (cat[r] == 4) {
+      slot = s[r];
+    } else if (cv[r] == k[r]) {
+      slot = s[r];
+      cache_slot(b.ckey, b.cslot, k[r], slot);
+    } else {
+      slow |= 1u << r;
+      continue;
+    }
+    __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : b.tbase + slot, slot_of + i);
+    if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&b.tfirst[slot], i);
+  }
+  while (slow) {
+    const int r = __builtin_ctz(slow);
+    slow &= slow - 1u;
+    const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
+    const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), type);
+    const uint32_t ss = uint32_t(mix64(kk)) & b.cap_mask;
+    const uint32_t slot = probe_insert(b.tkey, kk, ss, probe_load(&b.tkey[ss]), b.cap_mask, b.budget);
+    if (slot == NULL_SLOT) b.overflow = true;
+    else cache_slot(b.ckey, b.cslot, kk, slot);
+    __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : b.tbase + slot, slot_of + i);
+    if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&b.tfirst[slot], i);
+  }
+}
+
 template <bool TRACK_FIRST, int W>
 __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st, int private_tables) {
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t special = cap_mask + 1u;
   // With private tables each XCD inserts into its own copy (cap + 1 entries each), so a slot line in
   // an XCD's L2 is only ever changed by that XCD's own atomics; rows store the slot's index over all
   // the copies.
   const uint32_t tbl = private_tables ? xcd_id() : 0u;
-  const uint32_t tbase = tbl * (cap_mask + 2u);
-  unsigned long long* tkey = table.key + tbase;
-  uint32_t* tfirst = TRACK_FIRST ? table.first + tbase : nullptr;
-  uint32_t* special_flag = private_tables ? &st->xspecial[tbl] : &st->special_used;
+  BuildState b;
+  b.tbase = tbl * (cap_mask + 2u);
+  b.tkey = table.key + b.tbase;
+  b.tfirst = TRACK_FIRST ? table.first + b.tbase : nullptr;
+  b.special_flag = private_tables ? &st->xspecial[tbl] : &st->special_used;
+  b.special = cap_mask + 1u;
+  b.cap_mask = cap_mask;
+  b.budget = budget;
   __shared__ unsigned long long ckey[BUILD_CACHE];
   __shared__ uint32_t cslot[BUILD_CACHE];
   for (uint32_t j = threadIdx.x; j < BUILD_CACHE; j += blockDim.x) {
@@ -306,91 +438,26 @@ __global__ __launch_bounds__(256) void gb_build(const void* vals, int type, cons
     cslot[j] = NULL_SLOT;
   }
   __syncthreads();
-  unsigned long long nulls = 0;
-  bool overflow = false;
-  for (uint64_t base = 0; base < n; base += uint64_t(gsize) * BUILD_ROWS) {
-    unsigned long long k[BUILD_ROWS], cur[BUILD_ROWS];
-    uint32_t s[BUILD_ROWS];
-    int cat[BUILD_ROWS];   // 0 key, 1 null, 2 the EMPTY key, 3 past the end
-    bool vok[BUILD_ROWS];
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {   // every load first (rows past the end read the last row)
-      const uint64_t iu = base + uint64_t(r) * gsize + tid;
-      const int64_t ii = int64_t(iu < uint64_t(n) ? iu : uint64_t(n) - 1u);
-      k[r] = ld_raw<W>(vals, ii);
-      vok[r] = ld_valid(valid, voff, ii);
-    }
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {
-      uint64_t i = base + uint64_t(r) * gsize + tid;
-      cat[r] = 3;
-      if (i < n) {
-        if (!vok[r]) {
-          cat[r] = 1;
-          k[r] = 0;
-        } else {
-          k[r] = key_of_raw(k[r], type);
-          cat[r] = k[r] == EMPTY ? 2 : 0;
-        }
-      } else {
-        k[r] = 0;
-      }
-      const unsigned long long h = mix64(k[r]);
-      s[r] = uint32_t(h) & cap_mask;
-      // A cached key takes its slot from LDS (cat 4).
-      if (cat[r] == 0) {
-        const uint32_t c = uint32_t(h >> 40) & (BUILD_CACHE - 1u);
-        const uint32_t cs = cslot[c];
-        if (ckey[c] == k[r] && cs != NULL_SLOT) {
-          cat[r] = 4;
-          s[r] = cs;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) cur[r] = cat[r] == 0 ? probe_load(&tkey[s[r]]) : 0ull;
-    // Rows whose key already sits in its home slot (most rows of a repeated key) finish here; the
-    // others (a first insert or a collision) go through the probe loop one at a time below, with the
-    // key read again, so that the loop is not inlined per row.
-    uint32_t slow = 0;
-#pragma unroll
-    for (int r = 0; r < BUILD_ROWS; r++) {
-      uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
-      if (cat[r] == 3) continue;
-      uint32_t slot;
-      if (cat[r] == 1) {
-        slot = NULL_SLOT;
-        nulls++;
-        if (TRACK_FIRST) lower_first(&st->null_first, i);
-      } else if (cat[r] == 2) {
-        slot = special;
-        if (!*special_flag) atomicExch(special_flag, 1u);
-      } else if (cat[r] == 4) {
-        slot = s[r];
-      } else if (cur[r] == k[r]) {
-        slot = s[r];
-        cache_slot(ckey, cslot, k[r], slot);
-      } else {
-        slow |= 1u << r;
-        continue;
-      }
-      __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : tbase + slot, slot_of + i);
-      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&tfirst[slot], i);
-    }
-    while (slow) {
-      const int r = __builtin_ctz(slow);
-      slow &= slow - 1u;
-      const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
-      const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), type);
-      const uint32_t ss = uint32_t(mix64(kk)) & cap_mask;
-      const uint32_t slot = probe_insert(tkey, kk, ss, probe_load(&tkey[ss]), cap_mask, budget);
-      if (slot == NULL_SLOT) overflow = true;
-      else cache_slot(ckey, cslot, kk, slot);
-      __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : tbase + slot, slot_of + i);
-      if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&tfirst[slot], i);
+  b.ckey = ckey;
+  b.cslot = cslot;
+  b.nulls = 0;
+  b.overflow = false;
+  const uint64_t stride = uint64_t(gsize) * BUILD_ROWS;
+  const int steps = int((uint64_t(n) + stride - 1u) / stride);   // the same for every thread
+  if (steps > 0) {
+    // Two row buffers take turns, so that no loaded register is copied (a copy would wait for it).
+    BuildRows xa, xb;
+    build_load<W>(vals, valid, voff, n, 0, gsize, tid, xa);
+    for (int t = 0; t < steps; t += 2) {
+      const uint64_t base = uint64_t(t) * stride;
+      build_step<TRACK_FIRST, W>(vals, type, valid, voff, n, base, base + stride, gsize, tid, xa, xb, slot_of, st, b);
+      if (t + 1 >= steps) break;
+      build_step<TRACK_FIRST, W>(vals, type, valid, voff, n, base + stride, base + 2 * stride, gsize, tid, xb, xa,
+                                 slot_of, st, b);
     }
   }
-  if (overflow) atomicOr(&st->overflow, 1u);
+  if (b.overflow) atomicOr(&st->overflow, 1u);
+  unsigned long long nulls = b.nulls;
   for (int off = 32; off > 0; off >>= 1) nulls += __shfl_down(nulls, off, 64);
   if ((threadIdx.x & 63) == 0 && nulls) atomicAdd(&st->nulls, nulls);
 }
@@ -500,8 +567,11 @@ __global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t
   }
 }
 
-constexpr uint32_t IDS_CACHE = 2048;   // (slot * 2654435761) >> 21 indexes it
-static_assert(IDS_CACHE == (1u << 11), "the entry hash gives 11 bits");
+#ifndef AM_IDS_CACHE_BITS
+#define AM_IDS_CACHE_BITS 11
+#endif
+constexpr int IDS_CACHE_BITS = AM_IDS_CACHE_BITS;
+constexpr uint32_t IDS_CACHE = 1u << IDS_CACHE_BITS;   // (slot * 2654435761) >> (32 - bits) indexes it
 
 // The slots of one step's rows (BUILD_ROWS per thread, grid-strided), loaded together.
 struct IdsStep {
@@ -520,7 +590,7 @@ __device__ __forceinline__ void ids_load(const uint32_t* slot_of, uint32_t n, ui
 __device__ __forceinline__ void ids_step(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot, uint32_t null_gid,
                                          uint64_t base, uint32_t gsize, uint32_t tid, const IdsStep& x,
                                          unsigned long long* cache) {
-  auto entry = [](uint32_t sl) { return (sl * 2654435761u) >> 21; };   // [0, IDS_CACHE)
+  auto entry = [](uint32_t sl) { return (sl * 2654435761u) >> (32 - IDS_CACHE_BITS); };   // [0, IDS_CACHE)
 #pragma unroll
   for (int r = 0; r < BUILD_ROWS; r++) asm volatile("" ::"v"(x.sl[r]));   // every loaded register taken here
   uint32_t g[BUILD_ROWS];
@@ -954,15 +1024,6 @@ struct AggRows {
   uint32_t ib[AGG_ROWS], vb[AGG_ROWS];
 };
 
-__device__ __forceinline__ uint32_t ld_vbyte(const uint8_t* valid, int64_t voff, int64_t i) {
-  return load_global(valid ? valid + ((voff + i) >> 3) : &kAllValid);
-}
-
-// Row i's bit of its loaded validity byte (a column without a bitmap loads 0xFF: every bit set). No
-// branch: the byte is used on every path, so that its register is never left with a load in flight.
-__device__ __forceinline__ uint32_t vbit(int64_t voff, int64_t i, uint32_t byte) {
-  return (byte >> uint32_t((voff + i) & 7)) & 1u;
-}
 
 // Issues the loads of the step at `base` (rows past the end read the last row; end > 0).
 template <class P, int IDW, int VW>
