@@ -72,7 +72,45 @@ struct LvlTile {
 // the value keys), where the value block starts in the output, and the run's look-back epoch.
 struct MsdCtl {
   uint32_t active_mask, value_base, epoch, pad;
+  // The ORDER BY of whole tables by a 64-bit integer key also takes the sorted key column from the
+  // bucket sorts: key_out[value_base + i] = the mapped key of output i XOR key_xor (its original
+  // bits). Null when not asked for.
+  unsigned long long* key_out;
+  unsigned long long key_xor;
 };
+
+// The sorted key column's output as one bucket sort sees it (already offset to the value block).
+struct KeyOut {
+  unsigned long long* p;
+  unsigned long long x;
+};
+
+__device__ __forceinline__ KeyOut key_out_of(const MsdCtl* mc) {
+  return KeyOut{mc->key_out ? mc->key_out + mc->value_base : nullptr, mc->key_xor};
+}
+
+// Beside the rows a bucket sort writes (dest[j] != ~0u), their keys, when the key column is asked for:
+// load_keys reads keys_in[src[j]] (src[j] is always a position of the bucket, so every load stays in
+// bounds; issued with the row loads, before any store), write_keys stores them.
+template <typename K, int LI>
+__device__ __forceinline__ void load_keys(const KeyOut& ko, const K* keys_in, const uint32_t (&src)[LI],
+                                          unsigned long long (&v)[LI]) {
+  if constexpr (sizeof(K) == 8) {
+    if (!ko.p) return;   // uniform
+#pragma unroll
+    for (int j = 0; j < LI; j++) v[j] = keys_in[src[j]];
+  }
+}
+
+template <typename K, int LI>
+__device__ __forceinline__ void write_keys(const KeyOut& ko, const uint32_t (&dest)[LI], const unsigned long long (&v)[LI]) {
+  if constexpr (sizeof(K) == 8) {
+    if (!ko.p) return;   // uniform
+#pragma unroll
+    for (int j = 0; j < LI; j++)
+      if (dest[j] != ~0u) ko.p[dest[j]] = v[j] ^ ko.x;
+  }
+}
 
 // Device counters of the levels (one small array). The host reads back the first LC_READ words. The
 // two counters that every workgroup of a kernel bumps each sit on a cache line of their own: sharing a
@@ -501,7 +539,7 @@ __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t 
 // order inside a run). Otherwise stable LSD passes run over every varying digit.
 template <typename K, int LI>
 __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
-                                          uint32_t low_mask, uint32_t run_max, uint32_t* out_idx,
+                                          uint32_t low_mask, uint32_t run_max, uint32_t* out_idx, const KeyOut& ko,
                                           uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
                                           uint32_t* tmp, K* s_keys, uint16_t* s_pos) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -537,10 +575,13 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
         }
       }
       uint32_t r[LI];
+      unsigned long long kv[LI];
       load_rows<LI>(rows_in, off, src, r);
+      load_keys<K, LI>(ko, keys_in, src, kv);
 #pragma unroll
       for (int j = 0; j < LI; j++)
         if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
+      write_keys<K, LI>(ko, dest, kv);
       return;
     }
   }
@@ -556,15 +597,19 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
     }
     __syncthreads();
   }
-  uint32_t src[LI], r[LI];
+  uint32_t src[LI], r[LI], dest[LI];
+  unsigned long long kv[LI];
 #pragma unroll
   for (int j = 0; j < LI; j++) src[j] = off + min(pos[j], size - 1u);
   load_rows<LI>(rows_in, off, src, r);
+  load_keys<K, LI>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t e = base_e + uint32_t(j) * 64u;
+    dest[j] = e < size ? off + e : ~0u;
     if (e < size) out_idx[off + e] = r[j];
   }
+  write_keys<K, LI>(ko, dest, kv);
 }
 
 // A group of small sibling buckets (at most GROUP_ROWS rows in all; their digits `top` in one aligned
@@ -576,8 +621,8 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
 // end).
 template <typename K, int LI, uint32_t SPAN_LOG>
 __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
-                                            int top, int sub, uint32_t* out, uint32_t* tmp, uint32_t* g, K* s_words,
-                                            uint16_t* s_bin) {
+                                            int top, int sub, uint32_t* out, const KeyOut& ko, uint32_t* tmp, uint32_t* g,
+                                            K* s_words, uint16_t* s_bin) {
   static_assert(RADIX == BLOCK, "a thread per digit value");
   constexpr uint32_t per = 1u << SPAN_LOG;   // bins per thread
   const int ts = 8 * top, ss = 8 * sub;
@@ -640,10 +685,13 @@ __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* ro
     }
   }
   uint32_t r[LI];
-  load_rows<LI>(rows_in, off, src, r);   // every row load before the first store
+  unsigned long long kv[LI];
+  load_rows<LI>(rows_in, off, src, r);   // every row and key load before the first store
+  load_keys<K, LI>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
+  write_keys<K, LI>(ko, dest, kv);
   return true;
 }
 
@@ -667,8 +715,13 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
   const uint32_t size = bk.size, off = bk.start;
   const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
   uint32_t* out = out_idx + mc->value_base;
+  const KeyOut ko = key_out_of(mc);
   if (low_mask == 0u) {   // equal keys: the bucket is in order already
     for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in ? rows_in[off + p] : p;
+    if constexpr (sizeof(K) == 8) {
+      if (ko.p)
+        for (uint32_t p = threadIdx.x; p < size; p += BLOCK) ko.p[off + p] = keys_in[off + p] ^ ko.x;
+    }
     return;
   }
   if (size > uint32_t(LI * BLOCK)) return;   // never: LI covers every bucket to sort
@@ -726,10 +779,13 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
     }
   }
   uint32_t r[LI];
-  load_rows<LI>(rows_in, off, src, r);   // every row load before the first store
+  unsigned long long kv[LI];
+  load_rows<LI>(rows_in, off, src, r);   // every row and key load before the first store
+  load_keys<K, LI>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
+  write_keys<K, LI>(ko, dest, kv);
 }
 
 // The bucket sort over its list, an entry per workgroup in turn (nothing when the list ran out of room).
@@ -768,6 +824,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucke
   if (list_overflow(lc, local_cap)) return;   // uniform
   const uint32_t ngroups = lc[LC_GROUPS];
   uint32_t* out = out_idx + mc->value_base;
+  const KeyOut ko = key_out_of(mc);
   for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const uint32_t idx = local_cap - 1u - g;
     const Bucket bk = list[idx];
@@ -781,11 +838,11 @@ __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucke
     bool done = bk.size > GROUP_ROWS;   // never
     if (!done) {
       switch (span_log) {
-        case 0: done = local_group<K, LI, 0>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
-        case 1: done = local_group<K, LI, 1>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
-        case 2: done = local_group<K, LI, 2>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
-        case 3: done = local_group<K, LI, 3>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
-        default: done = local_group<K, LI, 4>(keys_in, rows_in, bk.start, bk.size, top, sub, out, tmp, g_bins, s_words, s_bin); break;
+        case 0: done = local_group<K, LI, 0>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 1: done = local_group<K, LI, 1>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 2: done = local_group<K, LI, 2>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 3: done = local_group<K, LI, 3>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        default: done = local_group<K, LI, 4>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
       }
     }
     if (!done && threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
@@ -818,7 +875,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
     const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
     if (bk.size <= uint32_t(LI * BLOCK))   // always: the host sizes LI for every queued bucket
       local_lsd<K, LI>(bucket_keys(bk, keys0, keys1), bucket_rows(bk, rows0, rows1), bk.start, bk.size, low_mask, SUB_MAX,
-                       out_idx + mc->value_base, wc, tile_total, bin_start, tmp, s_keys, s_pos);
+                       out_idx + mc->value_base, key_out_of(mc), wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
   }
 }

@@ -16,7 +16,8 @@
 // Execution is this build's own. A sort by one 64-bit key of a table of at least 2^18 rows moves
 // every column with the sort (order_by_columns in sort.hip): the columns are partitioned along the
 // sort's top-digit partition, and the final gather reads from the partition of its own output range,
-// so no column is gathered across the whole table. Other sorts are a lexsort and a take per column,
+// so no column is gathered across the whole table; an int64 / uint64 key column comes out of the
+// sort's bucket sorts directly. Other sorts are a lexsort and a take per column,
 // as the reference does.
 #include <algorithm>
 #include <cmath>

@@ -536,9 +536,10 @@ __global__ __launch_bounds__(BLOCK, 3) void msd_part(const void* vals, const uin
 template <int P>
 __global__ __launch_bounds__(BLOCK) void msd_init_lists(const SortCtl* ctl, MsdCtl* mc, uint32_t epoch,
                                                         Bucket* large, Bucket* local, uint32_t local_cap,
-                                                        uint32_t* lc) {
+                                                        uint32_t* lc, unsigned long long* key_out,
+                                                        unsigned long long key_xor) {
   if (threadIdx.x < LC_WORDS) lc[threadIdx.x] = 0u;
-  if (threadIdx.x == 0) *mc = {ctl->active_mask, ctl->value_base, epoch, 0u};
+  if (threadIdx.x == 0) *mc = {ctl->active_mask, ctl->value_base, epoch, 0u, key_out, key_xor};
   __syncthreads();
   const uint32_t b = threadIdx.x;
   const uint32_t size = ctl->hist[P - 1][b];
@@ -558,7 +559,8 @@ template <typename K, int W, int FITEMS, int ITEMS>
 bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, int dsc, int forder,
                 SortCtl* ctl, unsigned long long* status, uint32_t epoch, uint32_t m, uint32_t mask, K* k0,
                 uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx, const uint32_t* unit_hist, uint32_t* unit_base,
-                uint32_t nunits, uint32_t unit_shift, hipStream_t s, uint32_t* pos_of_row) {
+                uint32_t nunits, uint32_t unit_shift, hipStream_t s, uint32_t* pos_of_row,
+                unsigned long long* key_out, unsigned long long key_xor) {
   constexpr int P = int(sizeof(K));
   const MsdWork w = msd_work(m, ITEMS, device_alloc);
   {
@@ -580,7 +582,7 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
   {
     ProfScope ps("sort_msd_lists");
     hipLaunchKernelGGL(msd_init_lists<P>, dim3(1), dim3(BLOCK), 0, s, ctl, w.mc, epoch, w.large[0], w.local,
-                       w.local_cap, w.lc);
+                       w.local_cap, w.lc, key_out, key_xor);
     AM_HIP(hipGetLastError());
   }
   return msd_levels<K, ITEMS>(w, m, P - 1, mask, false, LOCAL_MAX, status, k0, r0, k1, r1, out_idx, s);
@@ -588,11 +590,13 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
 
 // With pos_of_row (the ORDER BY of whole tables): only the hybrid sort runs, with positions for rows
 // (msd_part's POS_ROWS form); when it cannot (too few rows, a constant top digit, a full bucket list)
-// the call returns false and out_idx holds nothing to use. *value_rows receives m.
+// the call returns false and out_idx holds nothing to use. *value_rows receives m. With key_out (an
+// integer key) the bucket sorts also write the value block of the sorted key column there.
 template <typename K, int W, int ITEMS>
 bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
               int placement, int forder, uint32_t* out_idx, hipStream_t s, uint32_t* pos_of_row = nullptr,
-              uint32_t* value_rows = nullptr, uint32_t* bucket_base = nullptr) {
+              uint32_t* value_rows = nullptr, uint32_t* bucket_base = nullptr, unsigned long long* key_out = nullptr,
+              unsigned long long key_xor = 0) {
   constexpr int P = int(sizeof(K));
   if (pos_of_row && !(P == 8 && n >= (1u << 18) && !capturing(s))) return false;
   constexpr int TILE = BLOCK * ITEMS;
@@ -675,7 +679,8 @@ bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     if (hybrid_ok && analyse && ((plan.mask >> (P - 1)) & 1u) &&
         run_hybrid<K, W, AM_MSD_FITEMS, ITEMS>(
             vals, type, valid, voff, n, dsc, forder, ctl, status, epoch, plan.m, plan.mask, kp0, rp0, kp1, rp1,
-            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, unit_shift, s, pos_of_row))
+            out_idx, unit_hist, unit_hist + size_t(nunits) * NBINS_FIRST, nunits, unit_shift, s, pos_of_row,
+            key_out, key_xor))
       return true;
   }
   if (pos_of_row) return false;
@@ -939,10 +944,24 @@ void ob_move_column(const Array& c, bool nulls, const uint32_t* pos_of_row, cons
 
 template <int ITEMS>
 bool ob_sort(const Array& key, bool desc, int placement, int forder, uint32_t* out_idx, uint32_t* pos_of_row,
-             uint32_t* m, uint32_t* bucket_base, hipStream_t s) {
+             uint32_t* m, uint32_t* bucket_base, hipStream_t s, unsigned long long* key_out,
+             unsigned long long key_xor) {
   return run_sort<unsigned long long, 8, ITEMS>(key.data(), key.type, key.has_nulls() ? key.valid_bits() : nullptr,
                                                 key.offset, uint32_t(key.length), desc, placement, forder, out_idx,
-                                                s, pos_of_row, m, bucket_base);
+                                                s, pos_of_row, m, bucket_base, key_out, key_xor);
+}
+
+// The rest of a sorted integer key column whose value block the bucket sorts wrote: zeros outside the
+// value block [vbase, vbase + m), and the validity bitmap (the value block valid, the rest null).
+__global__ void ob_key_rest(unsigned long long* out, uint32_t n, uint32_t vbase, uint32_t m,
+                            unsigned long long* out_valid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in_val = i - vbase < m;
+  if (i < n && !in_val) out[i] = 0ull;
+  if (out_valid) {
+    const unsigned long long bits = __ballot(i < n && in_val);
+    if ((threadIdx.x & 63) == 0 && i < n) out_valid[i / 64u] = bits;
+  }
 }
 
 }  // namespace
@@ -1017,13 +1036,24 @@ bool order_by_columns(const Array& key, bool descending, int null_placement, int
   auto* bbase = static_cast<uint32_t*>(bb->ptr);
   uint32_t m = 0;
   const int place = null_placement == 1 ? 1 : 0;
+  // An int64 / uint64 key column among the columns comes out of the bucket sorts themselves: its
+  // mapped keys are a bijection of its values (x ^ 2^63 for int64, then ~ when descending).
+  DevBuf key_vals;
+  unsigned long long key_xor = 0;
+  const bool key_direct = (key.type == T_I64 || key.type == T_U64) &&
+                          std::find(cols.begin(), cols.end(), &key) != cols.end();
+  if (key_direct) {
+    key_vals = device_alloc(size_t(n) * 8);
+    key_xor = (key.type == T_I64 ? 0x8000000000000000ull : 0ull) ^ (descending ? ~0ull : 0ull);
+  }
+  unsigned long long* kout = key_direct ? static_cast<unsigned long long*>(key_vals->ptr) : nullptr;
   bool ok;
   switch (sort_tune(n, 8).items) {
-    case 4: ok = ob_sort<4>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
-    case 8: ok = ob_sort<8>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
-    case 12: ok = ob_sort<12>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
-    case 20: ok = ob_sort<20>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
-    default: ok = ob_sort<16>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s); break;
+    case 4: ok = ob_sort<4>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s, kout, key_xor); break;
+    case 8: ok = ob_sort<8>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s, kout, key_xor); break;
+    case 12: ok = ob_sort<12>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s, kout, key_xor); break;
+    case 20: ok = ob_sort<20>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s, kout, key_xor); break;
+    default: ok = ob_sort<16>(key, descending, place, float_order, out_idx, pos_of_row, &m, bbase, s, kout, key_xor); break;
   }
   if (!ok) return false;
   // The value block: [0, m) with nulls last, [n - m, n) with nulls first.
@@ -1032,11 +1062,19 @@ bool order_by_columns(const Array& key, bool descending, int null_placement, int
     for (const Array* c : cols) {
       Array* o = new_array(c->format, c->type, n);
       out.push_back(o);
-      o->values = device_alloc(size_t(n) * c->width());
       int64_t nulls = c->validity ? c->null_count : 0;
       if (nulls < 0) nulls = count_nulls(*c);
       o->null_count = nulls;   // a permutation keeps the null count
       if (nulls > 0) o->validity = device_alloc(size_t((n + 63) / 64) * 8);
+      if (c == &key && key_direct) {
+        o->values = key_vals;
+        hipLaunchKernelGGL(ob_key_rest, dim3((uint32_t(n) + 255u) / 256u), dim3(256), 0, s,
+                           static_cast<unsigned long long*>(key_vals->ptr), uint32_t(n), vbase, m,
+                           nulls > 0 ? static_cast<unsigned long long*>(o->validity->ptr) : nullptr);
+        AM_HIP(hipGetLastError());
+        continue;
+      }
+      o->values = device_alloc(size_t(n) * c->width());
       auto* ctrp = static_cast<uint32_t*>(ctr->ptr);
       switch (c->width()) {
         case 8: ob_move_column<8>(*c, nulls > 0, pos_of_row, bbase, out_idx, uint32_t(n), vbase, m, o, ctrp, s); break;
