@@ -780,7 +780,8 @@ struct CountP {
   static constexpr bool USES_VALUE = false;
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
-  static constexpr bool PART = true;   // may run through the partitioned passes (rows not needed)
+  static constexpr bool PART = true;        // may run through the partitioned passes
+  static constexpr bool NEEDS_ROW = false;  // its contribution depends on the row number
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long, unsigned long long (&c)[1]) {
     c[0] = 1;
     return true;
@@ -796,6 +797,7 @@ struct SumCountP {
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
   static constexpr bool PART = true;
+  static constexpr bool NEEDS_ROW = false;
   __device__ static bool contrib(const AggArgs&, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     c[0] = x;
     c[1] = 1;
@@ -813,6 +815,7 @@ struct MinMaxP {
   static constexpr int FIXED_VW = 0;
   static constexpr bool AUX = false;
   static constexpr bool PART = true;
+  static constexpr bool NEEDS_ROW = false;
   __device__ static bool contrib(const AggArgs& a, int64_t, long long, unsigned long long x, unsigned long long (&c)[2]) {
     unsigned long long b = x, key;
     switch (a.v.type) {
@@ -862,7 +865,8 @@ struct FsScanP {
   static constexpr bool USES_VALUE = true;
   static constexpr int FIXED_VW = VW;
   static constexpr bool AUX = false;
-  static constexpr bool PART = false;   // word 3 is a row number
+  static constexpr bool PART = true;
+  static constexpr bool NEEDS_ROW = true;   // word 3 is the lowest NaN row
   __device__ static bool contrib(const AggArgs&, int64_t row, long long, unsigned long long raw,
                                  unsigned long long (&c)[4]) {
     const unsigned long long v = fs_bits<VW>(raw);
@@ -898,7 +902,8 @@ struct FsFixP {
   static constexpr int FIXED_VW = VW;
   // The group's grid word from fs_window, loaded for all rows in flight before the contributions.
   static constexpr bool AUX = true;
-  static constexpr bool PART = false;
+  static constexpr bool PART = true;
+  static constexpr bool NEEDS_ROW = false;
   using AuxT = uint32_t;
   __device__ static AuxT aux_load(const AggArgs& a, long long g) {
     return reinterpret_cast<const uint32_t*>(a.aux)[g];
@@ -1557,13 +1562,14 @@ __global__ __launch_bounds__(1024) void gb_part_bases(const uint32_t* total, uin
 // families that read no value). Within a tile the records are ranked per bucket in LDS (their order
 // does not matter to any family here) and staged in bucket order, then written out in order: the run
 // of bucket b starts at base[b] + off[b][tile].
-template <class P, int IDW, int VW>
+template <class P, int IDW, int VW, bool ROWS>
 __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, uint32_t ntiles, const uint32_t* off,
                                                        const uint32_t* base, uint32_t* rec_gid,
-                                                       unsigned long long* rec_val) {
+                                                       unsigned long long* rec_val, uint32_t* rec_row) {
   __shared__ uint32_t s_cnt[PART_MAX_NB], s_off[PART_MAX_NB], s_dst[PART_MAX_NB];
   __shared__ uint32_t s_gid[PART_TILE];
   __shared__ unsigned long long s_val[VW ? PART_TILE : 1];
+  __shared__ uint32_t s_row[ROWS ? PART_TILE : 1];
   __shared__ uint32_t s_wsum[4];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     s_cnt[b] = 0u;
@@ -1617,6 +1623,7 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
     const uint32_t p = s_off[uint32_t(gs[k]) >> PART_LOG_G] + rank[k];
     s_gid[p] = uint32_t(gs[k]);
     if constexpr (VW != 0) s_val[p] = vr[k];
+    if constexpr (ROWS) s_row[p] = uint32_t(t0 + int64_t(k) * blockDim.x + threadIdx.x);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < tile_count; p += blockDim.x) {
@@ -1625,6 +1632,7 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
     const uint32_t dst = s_dst[b] + (p - s_off[b]);
     rec_gid[dst] = g;
     if constexpr (VW != 0) rec_val[dst] = s_val[p];
+    if constexpr (ROWS) rec_row[dst] = s_row[p];
   }
 }
 
@@ -1657,61 +1665,96 @@ __device__ __forceinline__ unsigned long long wave_combine64(unsigned long long 
   return x;
 }
 
-template <int VW>
-struct PartRecs {
+// The records of one reduce step (PART_RED_ROWS per thread): group ids, raw values, and row numbers for
+// the families that need them.
+template <int VW, bool ROWS>
+struct PartStep {
   uint32_t g[PART_RED_ROWS];
   unsigned long long raw[PART_RED_ROWS];
+  uint32_t row[PART_RED_ROWS];
+};
+
+// The partitioned records of one set of rows.
+struct PartIn {
+  const uint32_t* gid;
+  const unsigned long long* val;
+  const uint32_t* row;
 };
 
 // Loads the records of the step at r (records past r1 read the last one; r1 > 0).
-template <int VW>
-__device__ __forceinline__ void part_load(const uint32_t* rec_gid, const unsigned long long* rec_val, int64_t r,
-                                          int64_t r1, PartRecs<VW>& x) {
+template <int VW, bool ROWS>
+__device__ __forceinline__ void part_load(const PartIn& in, int64_t r, int64_t r1, PartStep<VW, ROWS>& x) {
 #pragma unroll
   for (int k = 0; k < PART_RED_ROWS; k++) {
     const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
     const int64_t ii = i < r1 ? i : r1 - 1;
-    x.g[k] = load_global(rec_gid + ii);
-    x.raw[k] = VW ? load_global(rec_val + ii) : 0ull;
+    x.g[k] = load_global(in.gid + ii);
+    x.raw[k] = VW ? load_global(in.val + ii) : 0ull;
+    x.row[k] = ROWS ? load_global(in.row + ii) : 0u;
   }
+}
+
+// Whether every word of family P combines by ADD / MIN / MAX / OR (a 128-bit sum's two words do not
+// combine word by word across a wave).
+template <class P>
+__host__ __device__ constexpr bool wordwise() {
+  for (int w = 0; w < P::W; w++)
+    if (P::op(w) == W_ADD128 || P::op(w) == W_HI) return false;
+  return true;
 }
 
 // Applies the step's records at r to the LDS table. A hot group (many lanes of the wave on the first
 // lane's group) is combined across the wave first and applied once, instead of one LDS atomic per
 // lane on one address.
-template <class P, int VW>
-__device__ __forceinline__ void part_apply(const AggArgs& a, int64_t r, int64_t r1, const PartRecs<VW>& x,
+template <class P, int VW, bool ROWS>
+__device__ __forceinline__ void part_apply(const AggArgs& a, int64_t r, int64_t r1, const PartStep<VW, ROWS>& x,
                                            unsigned long long* lds) {
 #pragma unroll
-  for (int k = 0; k < PART_RED_ROWS; k++) asm volatile("" ::"v"(x.g[k]), "v"(x.raw[k]));   // every loaded register taken here
+  for (int k = 0; k < PART_RED_ROWS; k++) asm volatile("" ::"v"(x.g[k]), "v"(x.raw[k]), "v"(x.row[k]));   // every loaded register taken here
+  unsigned long long cs[PART_RED_ROWS][P::W];
+  bool act[PART_RED_ROWS];
+  if constexpr (P::AUX) {
+    typename P::AuxT ax[PART_RED_ROWS];
+#pragma unroll
+    for (int k = 0; k < PART_RED_ROWS; k++) ax[k] = P::aux_load(a, (long long)x.g[k]);   // all loads first
+#pragma unroll
+    for (int k = 0; k < PART_RED_ROWS; k++) {
+      const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
+      act[k] = P::contrib(a, int64_t(x.row[k]), (long long)x.g[k], val_in(a.v, x.raw[k]), ax[k], cs[k]) && i < r1;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PART_RED_ROWS; k++) {
+      const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
+      act[k] = P::contrib(a, int64_t(x.row[k]), (long long)x.g[k], val_in(a.v, x.raw[k]), cs[k]) && i < r1;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < PART_RED_ROWS; k++) {
-    const int64_t i = r + int64_t(k) * blockDim.x + threadIdx.x;
     const uint32_t g = x.g[k];
-    unsigned long long cs[P::W];
-    bool act = P::contrib(a, 0, (long long)g, val_in(a.v, x.raw[k]), cs) && i < r1;
-    const uint32_t lead = __builtin_amdgcn_readfirstlane(act ? g : 0xFFFFFFFFu);
-    const unsigned long long m = __ballot(act && g == lead);
-    if (__popcll(m) >= PART_PEEL) {   // uniform
-      unsigned long long v[P::W];
+    if constexpr (wordwise<P>()) {
+      const uint32_t lead = __builtin_amdgcn_readfirstlane(act[k] ? g : 0xFFFFFFFFu);
+      const unsigned long long m = __ballot(act[k] && g == lead);
+      if (__popcll(m) >= PART_PEEL) {   // uniform
+        unsigned long long v[P::W];
 #pragma unroll
-      for (int w = 0; w < P::W; w++) {
-        const int op = P::op(w);
-        v[w] = wave_combine64((act && g == lead) ? cs[w] : identity(op), op);
+        for (int w = 0; w < P::W; w++) {
+          const int op = P::op(w);
+          v[w] = wave_combine64((act[k] && g == lead) ? cs[k][w] : identity(op), op);
+        }
+        if ((threadIdx.x & 63) == 63) apply<P>(as_lds(&lds[size_t(lead & (PART_G - 1)) * P::W]), v);
+        act[k] = act[k] && g != lead;
       }
-      if ((threadIdx.x & 63) == 63) apply<P>(as_lds(&lds[size_t(lead & (PART_G - 1)) * P::W]), v);
-      act = act && g != lead;
     }
-    if (act) apply<P>(as_lds(&lds[size_t(g & (PART_G - 1)) * P::W]), cs);
+    if (act[k]) apply<P>(as_lds(&lds[size_t(g & (PART_G - 1)) * P::W]), cs[k]);
   }
 }
 
 // Workgroup c reduces chunk c - first[b] of bucket b into part[c] (PART_G x W words, identities where
 // no record fell). Workgroups past the last chunk do nothing.
-template <class P, int VW>
+template <class P, int VW, bool ROWS>
 __global__ __launch_bounds__(512) void gb_part_reduce(AggArgs a, uint32_t nb, const uint32_t* base, const uint32_t* first,
-                                                      const uint32_t* rec_gid, const unsigned long long* rec_val,
-                                                      unsigned long long* part) {
+                                                      PartIn in, unsigned long long* part) {
   extern __shared__ unsigned long long lds[];
   const uint32_t c = blockIdx.x;
   if (c >= first[nb]) return;   // uniform
@@ -1731,19 +1774,40 @@ __global__ __launch_bounds__(512) void gb_part_reduce(AggArgs a, uint32_t nb, co
   // Two record buffers take turns, as in gb_agg: the next step's loads are in flight while this step's
   // records are applied (LDS only, so nothing here waits on a global write).
   const int steps = __builtin_amdgcn_readfirstlane(int((r1 - r0 + step - 1) / step));
-  PartRecs<VW> ra, rb;
-  part_load<VW>(rec_gid, rec_val, r0, r1, ra);
+  PartStep<VW, ROWS> ra, rb;
+  part_load<VW, ROWS>(in, r0, r1, ra);
   for (int t = 0; t < steps; t += 2) {
     const int64_t r = r0 + int64_t(t) * step;
-    part_load<VW>(rec_gid, rec_val, r + step, r1, rb);
-    part_apply<P, VW>(a, r, r1, ra, lds);
+    part_load<VW, ROWS>(in, r + step, r1, rb);
+    part_apply<P, VW, ROWS>(a, r, r1, ra, lds);
     if (t + 1 >= steps) break;
-    part_load<VW>(rec_gid, rec_val, r + 2 * step, r1, ra);
-    part_apply<P, VW>(a, r + step, r1, rb, lds);
+    part_load<VW, ROWS>(in, r + 2 * step, r1, ra);
+    part_apply<P, VW, ROWS>(a, r + step, r1, rb, lds);
   }
   __syncthreads();
   unsigned long long* dst = part + size_t(c) * PART_G * P::W;
   for (uint32_t j = threadIdx.x; j < PART_G * P::W; j += blockDim.x) dst[j] = lds[j];
+}
+
+// acc <- acc combined with v, word by word under family P's ops; a 128-bit sum (lo word W_ADD128, hi
+// word W_HI) adds as one number, the lo words' carry into the hi word.
+template <class P>
+__device__ __forceinline__ void combine_words(unsigned long long (&acc)[P::W], const unsigned long long (&v)[P::W]) {
+#pragma unroll
+  for (int w = 0; w < P::W; w++) {
+    const int op = P::op(w);
+    if (op == W_ADD) acc[w] += v[w];
+    else if (op == W_MIN) acc[w] = v[w] < acc[w] ? v[w] : acc[w];
+    else if (op == W_MAX) acc[w] = v[w] > acc[w] ? v[w] : acc[w];
+    else if (op == W_OR) acc[w] |= v[w];
+    else if (op == W_ADD128) {
+      const unsigned long long lo = acc[w] + v[w];
+      const unsigned long long carry = lo < acc[w] ? 1ull : 0ull;
+      acc[w] = lo;
+      if (w + 1 < P::W) acc[w + 1] += v[w + 1] + carry;
+    }
+    // W_HI: added with its W_ADD128 word
+  }
 }
 
 // out[g] = the combination of part[c][g mod PART_G] over the chunks c of g's bucket. A hot bucket has
@@ -1776,14 +1840,7 @@ __global__ __launch_bounds__(1024) void gb_part_merge(const unsigned long long* 
 #pragma unroll
     for (int u = 0; u < PART_MERGE_LOADS; u++) {
       if (c + uint32_t(u * PART_MERGE_WAVES) >= c1) break;
-#pragma unroll
-      for (int w = 0; w < P::W; w++) {
-        const int op = P::op(w);
-        if (op == W_ADD) acc[w] += v[u][w];
-        else if (op == W_MIN) acc[w] = v[u][w] < acc[w] ? v[u][w] : acc[w];
-        else if (op == W_MAX) acc[w] = v[u][w] > acc[w] ? v[u][w] : acc[w];
-        else if (op == W_OR) acc[w] |= v[u][w];
-      }
+      combine_words<P>(acc, v[u]);
     }
   }
 #pragma unroll
@@ -1791,99 +1848,134 @@ __global__ __launch_bounds__(1024) void gb_part_merge(const unsigned long long* 
   __syncthreads();
   if (wave == 0 && g < K) {
     for (int x = 1; x < PART_MERGE_WAVES; x++) {
+      unsigned long long y[P::W];
 #pragma unroll
-      for (int w = 0; w < P::W; w++) {
-        const int op = P::op(w);
-        const unsigned long long y = s_acc[x][lane][w];
-        if (op == W_ADD) acc[w] += y;
-        else if (op == W_MIN) acc[w] = y < acc[w] ? y : acc[w];
-        else if (op == W_MAX) acc[w] = y > acc[w] ? y : acc[w];
-        else if (op == W_OR) acc[w] |= y;
-      }
+      for (int w = 0; w < P::W; w++) y[w] = s_acc[x][lane][w];
+      combine_words<P>(acc, y);
     }
 #pragma unroll
     for (int w = 0; w < P::W; w++) out[size_t(g) * P::W + w] = acc[w];
   }
 }
 
+// The partitioned records of one set of rows (the rows with a valid id, and a non-null value for the
+// families that skip nulls): kept by a GroupBy for its last value column, so that the aggregates of
+// one column (min and max, then the exact sum's two passes) share one count and scatter.
+struct PartSet {
+  uint64_t col = 0;          // the value column's unique id (0: none)
+  bool skip_null = false;    // records of rows with a non-null value only
+  bool rows = false;         // row numbers kept (rec_row)
+  int vw = 0;                // 0: no values kept
+  uint32_t nb = 0, ntiles = 0, chunk_bound = 0;
+  DevBuf meta, gid, val, row;   // meta: base[nb + 1], first[nb + 1]
+};
+
 // Family P over many groups through the partitioned passes into `o` (K x W words); false, with
-// nothing launched, where they do not apply (a family that needs its rows or its own grid words, too
-// many groups, too few rows).
+// nothing launched, where they do not apply (too many groups, too few rows). `cache` (or null) holds
+// the records of an earlier family over the same column, used again when they fit; `keep_rows` keeps
+// row numbers in new records even if P does not need them (a float column, whose exact sum will).
 template <class P>
-bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned long long* o) {
+bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned long long* o, PartSet* cache,
+                  uint64_t col, bool keep_rows) {
   if constexpr (!P::PART) {
     return false;
   } else {
     const long long K = a.K;
     const uint32_t nb = uint32_t((K + PART_G - 1) / PART_G);
     if (a.n < int64_t(PART_TILE) * 16 || nb == 0 || nb > PART_MAX_NB || a.n >= (int64_t(1) << 32)) return false;
-    const uint32_t ntiles = uint32_t((a.n + PART_TILE - 1) / PART_TILE);
-    const int idw = a.ids.type == T_I64 ? 8 : 4;
     const int vw = !P::USES_VALUE ? 0 : type_width(a.v.type);
-    DevBuf hist = device_alloc(size_t(nb) * ntiles * 4);
-    DevBuf meta = device_alloc(size_t(3) * (nb + 1) * 4);
-    uint32_t* base = static_cast<uint32_t*>(meta->ptr);
-    uint32_t* first = base + nb + 1;
-    uint32_t* total = first + nb + 1;
-    DevBuf gid = device_alloc(size_t(a.n) * 4);
-    DevBuf val = device_alloc(vw ? size_t(a.n) * 8 : 8);
-    const uint32_t chunk_bound = uint32_t((a.n + PART_CHUNK - 1) / PART_CHUNK) + nb;
-    DevBuf part = device_alloc(size_t(chunk_bound) * PART_G * P::W * 8);
-    uint32_t* h = static_cast<uint32_t*>(hist->ptr);
-    uint32_t* rg = static_cast<uint32_t*>(gid->ptr);
-    unsigned long long* rv = static_cast<unsigned long long*>(val->ptr);
-    {
-      ProfScope ps("gb_part_hist");
-      if (idw == 8) hipLaunchKernelGGL((gb_part_hist<P, 8>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
-      else hipLaunchKernelGGL((gb_part_hist<P, 4>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
-      AM_HIP(hipGetLastError());
-    }
-    {
-      ProfScope ps("gb_part_scan");
-      hipLaunchKernelGGL(gb_part_scan, dim3(nb), dim3(1024), 0, s, h, ntiles, total);
-      AM_HIP(hipGetLastError());
-      hipLaunchKernelGGL(gb_part_bases, dim3(1), dim3(1024), 0, s, total, nb, base, first);
-      AM_HIP(hipGetLastError());
-    }
-    {
-      ProfScope ps("gb_part_scatter");
-#define AM_PART_SCATTER(IW, VV) \
-  hipLaunchKernelGGL((gb_part_scatter<P, IW, VV>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h, base, rg, rv)
-      if (idw == 8) {
-        switch (vw) {
-          case 0: AM_PART_SCATTER(8, 0); break;
-          case 8: AM_PART_SCATTER(8, 8); break;
-          case 4: AM_PART_SCATTER(8, 4); break;
-          case 2: AM_PART_SCATTER(8, 2); break;
-          default: AM_PART_SCATTER(8, 1); break;
-        }
-      } else {
-        switch (vw) {
-          case 0: AM_PART_SCATTER(4, 0); break;
-          case 8: AM_PART_SCATTER(4, 8); break;
-          case 4: AM_PART_SCATTER(4, 4); break;
-          case 2: AM_PART_SCATTER(4, 2); break;
-          default: AM_PART_SCATTER(4, 1); break;
-        }
+    const bool reuse = cache && col && cache->col == col && cache->skip_null == P::SKIP_NULL && cache->nb == nb &&
+                       (cache->vw != 0 || vw == 0) && (cache->rows || !P::NEEDS_ROW) && a.v.xop == VX_NONE;
+    PartSet fresh;
+    PartSet& ps_ = reuse ? *cache : fresh;
+    if (!reuse) {
+      ps_.col = col;
+      ps_.skip_null = P::SKIP_NULL;
+      ps_.rows = P::NEEDS_ROW || keep_rows;
+      ps_.vw = vw;
+      ps_.nb = nb;
+      ps_.ntiles = uint32_t((a.n + PART_TILE - 1) / PART_TILE);
+      ps_.chunk_bound = uint32_t((a.n + PART_CHUNK - 1) / PART_CHUNK) + nb;
+      const uint32_t ntiles = ps_.ntiles;
+      const int idw = a.ids.type == T_I64 ? 8 : 4;
+      DevBuf hist = device_alloc(size_t(nb) * ntiles * 4);
+      ps_.meta = device_alloc(size_t(3) * (nb + 1) * 4);
+      uint32_t* base = static_cast<uint32_t*>(ps_.meta->ptr);
+      uint32_t* first = base + nb + 1;
+      uint32_t* total = first + nb + 1;
+      ps_.gid = device_alloc(size_t(a.n) * 4);
+      ps_.val = device_alloc(vw ? size_t(a.n) * 8 : 8);
+      ps_.row = device_alloc(ps_.rows ? size_t(a.n) * 4 : 4);
+      uint32_t* h = static_cast<uint32_t*>(hist->ptr);
+      uint32_t* rg = static_cast<uint32_t*>(ps_.gid->ptr);
+      unsigned long long* rv = static_cast<unsigned long long*>(ps_.val->ptr);
+      uint32_t* rr = static_cast<uint32_t*>(ps_.row->ptr);
+      {
+        ProfScope ps("gb_part_hist");
+        if (idw == 8) hipLaunchKernelGGL((gb_part_hist<P, 8>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
+        else hipLaunchKernelGGL((gb_part_hist<P, 4>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
+        AM_HIP(hipGetLastError());
       }
+      {
+        ProfScope ps("gb_part_scan");
+        hipLaunchKernelGGL(gb_part_scan, dim3(nb), dim3(1024), 0, s, h, ntiles, total);
+        AM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gb_part_bases, dim3(1), dim3(1024), 0, s, total, nb, base, first);
+        AM_HIP(hipGetLastError());
+      }
+      {
+        ProfScope ps("gb_part_scatter");
+#define AM_PART_SCATTER(IW, VV)                                                                                  \
+  do {                                                                                                           \
+    if (ps_.rows)                                                                                                \
+      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, true>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h,    \
+                         base, rg, rv, rr);                                                                      \
+    else                                                                                                         \
+      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, false>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h,   \
+                         base, rg, rv, rr);                                                                      \
+  } while (0)
+        if (idw == 8) {
+          switch (vw) {
+            case 0: AM_PART_SCATTER(8, 0); break;
+            case 8: AM_PART_SCATTER(8, 8); break;
+            case 4: AM_PART_SCATTER(8, 4); break;
+            case 2: AM_PART_SCATTER(8, 2); break;
+            default: AM_PART_SCATTER(8, 1); break;
+          }
+        } else {
+          switch (vw) {
+            case 0: AM_PART_SCATTER(4, 0); break;
+            case 8: AM_PART_SCATTER(4, 8); break;
+            case 4: AM_PART_SCATTER(4, 4); break;
+            case 2: AM_PART_SCATTER(4, 2); break;
+            default: AM_PART_SCATTER(4, 1); break;
+          }
+        }
 #undef AM_PART_SCATTER
-      AM_HIP(hipGetLastError());
+        AM_HIP(hipGetLastError());
+      }
+      if (cache && col) *cache = ps_;
     }
+    uint32_t* base = static_cast<uint32_t*>(ps_.meta->ptr);
+    uint32_t* first = base + nb + 1;
+    const PartIn in{static_cast<const uint32_t*>(ps_.gid->ptr), static_cast<const unsigned long long*>(ps_.val->ptr),
+                    static_cast<const uint32_t*>(ps_.row->ptr)};
+    DevBuf part = device_alloc(size_t(ps_.chunk_bound) * PART_G * P::W * 8);
     {
       ProfScope ps(name);
       const size_t shmem = size_t(PART_G) * P::W * 8;
+      auto launch = [&](auto kern) {
+        if (shmem > 65536)
+          AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(shmem)));
+        hipLaunchKernelGGL(kern, dim3(ps_.chunk_bound), dim3(512), shmem, s, a, nb, base, first, in,
+                           static_cast<unsigned long long*>(part->ptr));
+      };
       if (vw) {
-        if (shmem > 65536)
-          AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_part_reduce<P, 8>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-        hipLaunchKernelGGL((gb_part_reduce<P, 8>), dim3(chunk_bound), dim3(512), shmem, s, a, nb, base, first, rg, rv,
-                           static_cast<unsigned long long*>(part->ptr));
+        if constexpr (P::NEEDS_ROW) launch(gb_part_reduce<P, 8, true>);
+        else launch(gb_part_reduce<P, 8, false>);
       } else {
-        if (shmem > 65536)
-          AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(gb_part_reduce<P, 0>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)));
-        hipLaunchKernelGGL((gb_part_reduce<P, 0>), dim3(chunk_bound), dim3(512), shmem, s, a, nb, base, first, rg, rv,
-                           static_cast<unsigned long long*>(part->ptr));
+        launch(gb_part_reduce<P, 0, false>);
       }
       AM_HIP(hipGetLastError());
     }
@@ -1899,10 +1991,12 @@ bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
 
 // Runs family P over the rows into a fresh global table of K x W words.
 template <class P>
-DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name) {
+DevBuf run_agg(const AggArgs& a, hipStream_t s, const char* name, PartSet* cache = nullptr, uint64_t col = 0,
+               bool keep_rows = false) {
   const long long K = std::max<long long>(a.K, 1);
   DevBuf out = device_alloc(size_t(K) * P::W * 8);
-  if (a.K > (long long)agg_dense_cap<P>() && run_agg_part<P>(a, s, name, static_cast<unsigned long long*>(out->ptr)))
+  if (a.K > (long long)agg_dense_cap<P>() &&
+      run_agg_part<P>(a, s, name, static_cast<unsigned long long*>(out->ptr), cache, col, keep_rows))
     return out;
   DevBuf tabs;
   if (a.n > 0) tabs = device_alloc(size_t(NXCD) * size_t(K) * P::W * 8);
@@ -2167,6 +2261,7 @@ struct GroupBy {
   DevBuf cnt_tab;          // K x 2 integer sum and count of non-null values of column cnt_col
   bool cnt_all_rows = false;   // column cnt_col had no nulls: its counts are the rows per group
   DevBuf count_all_tab;    // K x 1 rows per group
+  PartSet part;            // the partitioned records of the last value column (many groups)
   DevBuf hot;              // HOT_GROUPS sampled frequent ids (sums and counts in registers)
   ~GroupBy() {
     for (Array* a : columns) release_array(a);
@@ -2891,18 +2986,18 @@ AggArgs make_args(const IdSource& ids, const Array* v, int64_t n, long long K) {
 // binary64 bits (prec 53), or as binary32 results widened to binary64 bits (prec 24), with validity
 // bytes.
 template <int P, int EMIN, int VW>
-void exact_fsum_t(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
+void exact_fsum_t(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb, PartSet* recs, uint64_t col) {
   hipStream_t s = stream();
   const long long K = base.K;
   const size_t k1 = size_t(std::max<long long>(K, 1));
   AggArgs a = base;
-  DevBuf scan = run_agg<FsScanP<VW>>(a, s, "fs_scan");
+  DevBuf scan = run_agg<FsScanP<VW>>(a, s, "fs_scan", recs, col, true);
   DevBuf grid = device_alloc(k1 * 4);
   hipLaunchKernelGGL(fs_window, dim3(uint32_t((K + 255) / 256)), dim3(256), 0, s,
                      static_cast<const unsigned long long*>(scan->ptr), K, static_cast<uint32_t*>(grid->ptr));
   AM_HIP(hipGetLastError());
   a.aux = grid->ptr;
-  DevBuf fix = run_agg<FsFixP<VW>>(a, s, "fs_fixed");
+  DevBuf fix = run_agg<FsFixP<VW>>(a, s, "fs_fixed", recs, col, true);
   sum = device_alloc(k1 * 8);
   mean = device_alloc(k1 * 8);
   vb = device_alloc(k1);
@@ -2938,10 +3033,12 @@ void exact_fsum_t(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
 
 // Float64 values to binary64 results; float32 values to binary64 results (am_group_agg_ex) or to
 // binary32 results (the dense am_group_by sum).
-void exact_fsum(const AggArgs& base, bool f32_result, DevBuf& sum, DevBuf& mean, DevBuf& vb) {
-  if (base.v.type == T_F64) exact_fsum_t<53, -1022, 8>(base, sum, mean, vb);
-  else if (f32_result) exact_fsum_t<24, -126, 4>(base, sum, mean, vb);
-  else exact_fsum_t<53, -1022, 4>(base, sum, mean, vb);
+// recs / col (or null / 0): the group-by's shared partitioned records and the value column's id.
+void exact_fsum(const AggArgs& base, bool f32_result, DevBuf& sum, DevBuf& mean, DevBuf& vb, PartSet* recs = nullptr,
+                uint64_t col = 0) {
+  if (base.v.type == T_F64) exact_fsum_t<53, -1022, 8>(base, sum, mean, vb, recs, col);
+  else if (f32_result) exact_fsum_t<24, -126, 4>(base, sum, mean, vb, recs, col);
+  else exact_fsum_t<53, -1022, 4>(base, sum, mean, vb, recs, col);
 }
 
 Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::string& fmt, int type) {
@@ -3021,7 +3118,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     if (cache && cache->count_all_tab && !v->has_nulls()) return finalize_simple(cache->count_all_tab, 1, K, 3, T_I64, "l", T_I64);
     AggArgs a = make_args(ids, v, n, K);
     a.hot = hot_groups(ids, n, K, cache, hot_holder);
-    DevBuf t = run_agg<CountP<true>>(a, s, "gb_count");
+    DevBuf t = run_agg<CountP<true>>(a, s, "gb_count", cache ? &cache->part : nullptr, v->uid);
     return finalize_simple(t, 1, K, 3, T_I64, "l", T_I64);
   }
   if (!type_is_primitive(v->type))
@@ -3039,7 +3136,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
       if (cache && cache->fx_col == v->uid) {
         sum = cache->fx_sum; mean = cache->fx_mean; vb = cache->fx_valid;
       } else {
-        exact_fsum(a, false, sum, mean, vb);
+        exact_fsum(a, false, sum, mean, vb, cache ? &cache->part : nullptr, v->uid);
         if (cache) { cache->fx_col = v->uid; cache->fx_sum = sum; cache->fx_mean = mean; cache->fx_valid = vb; }
       }
       return bits_array(op == 0 ? sum : mean, vb, K, "g", T_F64);
@@ -3049,7 +3146,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
       t = cache->cnt_tab;
     } else {
       a.hot = hot_groups(ids, n, K, cache, hot_holder);
-      t = run_agg<SumCountP>(a, s, "gb_sum");
+      t = run_agg<SumCountP>(a, s, "gb_sum", cache ? &cache->part : nullptr, v->uid);
       if (cache) { cache->cnt_col = v->uid; cache->cnt_tab = t; cache->cnt_all_rows = !v->has_nulls(); }
     }
     if (op == 0) {
@@ -3063,7 +3160,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
     if (cache && cache->mm_col == v->uid) {
       t = cache->mm_tab;
     } else {
-      t = run_agg<MinMaxP>(a, s, "gb_minmax");
+      t = run_agg<MinMaxP>(a, s, "gb_minmax", cache ? &cache->part : nullptr, v->uid, type_is_float(v->type));
       if (cache) { cache->mm_col = v->uid; cache->mm_tab = t; }
     }
     return finalize_simple(t, 2, K, op, v->type, v->format, v->type);
