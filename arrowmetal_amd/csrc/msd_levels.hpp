@@ -92,19 +92,19 @@ __device__ __forceinline__ KeyOut key_out_of(const MsdCtl* mc) {
 // Beside the rows a bucket sort writes (dest[j] != ~0u), their keys, when the key column is asked for:
 // load_keys reads keys_in[src[j]] (src[j] is always a position of the bucket, so every load stays in
 // bounds; issued with the row loads, before any store), write_keys stores them.
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __device__ __forceinline__ void load_keys(const KeyOut& ko, const K* keys_in, const uint32_t (&src)[LI],
                                           unsigned long long (&v)[LI]) {
-  if constexpr (sizeof(K) == 8) {
+  if constexpr (KEYS && sizeof(K) == 8) {
     if (!ko.p) return;   // uniform
 #pragma unroll
     for (int j = 0; j < LI; j++) v[j] = keys_in[src[j]];
   }
 }
 
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __device__ __forceinline__ void write_keys(const KeyOut& ko, const uint32_t (&dest)[LI], const unsigned long long (&v)[LI]) {
-  if constexpr (sizeof(K) == 8) {
+  if constexpr (KEYS && sizeof(K) == 8) {
     if (!ko.p) return;   // uniform
 #pragma unroll
     for (int j = 0; j < LI; j++)
@@ -537,7 +537,7 @@ __device__ __forceinline__ void local_split(const K (&keys)[LI], const uint32_t 
 // first split stably by it; when no digit run is longer than run_max rows, a row's place inside its
 // run is the number of rows there with a smaller key, or an equal key earlier (LDS order is input
 // order inside a run). Otherwise stable LSD passes run over every varying digit.
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
                                           uint32_t low_mask, uint32_t run_max, uint32_t* out_idx, const KeyOut& ko,
                                           uint32_t (*wc)[WC_STRIDE], uint32_t* tile_total, uint32_t* bin_start,
@@ -577,11 +577,11 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
       uint32_t r[LI];
       unsigned long long kv[LI];
       load_rows<LI>(rows_in, off, src, r);
-      load_keys<K, LI>(ko, keys_in, src, kv);
+      load_keys<K, LI, KEYS>(ko, keys_in, src, kv);
 #pragma unroll
       for (int j = 0; j < LI; j++)
         if (dest[j] != ~0u) out_idx[dest[j]] = r[j];
-      write_keys<K, LI>(ko, dest, kv);
+      write_keys<K, LI, KEYS>(ko, dest, kv);
       return;
     }
   }
@@ -602,14 +602,14 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
 #pragma unroll
   for (int j = 0; j < LI; j++) src[j] = off + min(pos[j], size - 1u);
   load_rows<LI>(rows_in, off, src, r);
-  load_keys<K, LI>(ko, keys_in, src, kv);
+  load_keys<K, LI, KEYS>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++) {
     const uint32_t e = base_e + uint32_t(j) * 64u;
     dest[j] = e < size ? off + e : ~0u;
     if (e < size) out_idx[off + e] = r[j];
   }
-  write_keys<K, LI>(ko, dest, kv);
+  write_keys<K, LI, KEYS>(ko, dest, kv);
 }
 
 // A group of small sibling buckets (at most GROUP_ROWS rows in all; their digits `top` in one aligned
@@ -619,7 +619,7 @@ __device__ __forceinline__ void local_lsd(const K* keys_in, const uint32_t* rows
 // words there below its own. Returns false, having written nothing, when a bin holds more than SUB_MAX
 // rows. g: RADIX << span_log words of LDS (counts, then starts, then, after the scatter, each bin's
 // end).
-template <typename K, int LI, uint32_t SPAN_LOG>
+template <typename K, int LI, uint32_t SPAN_LOG, bool KEYS>
 __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* rows_in, uint32_t off, uint32_t size,
                                             int top, int sub, uint32_t* out, const KeyOut& ko, uint32_t* tmp, uint32_t* g,
                                             K* s_words, uint16_t* s_bin) {
@@ -687,11 +687,11 @@ __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* ro
   uint32_t r[LI];
   unsigned long long kv[LI];
   load_rows<LI>(rows_in, off, src, r);   // every row and key load before the first store
-  load_keys<K, LI>(ko, keys_in, src, kv);
+  load_keys<K, LI, KEYS>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
-  write_keys<K, LI>(ko, dest, kv);
+  write_keys<K, LI, KEYS>(ko, dest, kv);
   return true;
 }
 
@@ -704,7 +704,7 @@ __device__ __forceinline__ bool local_group(const K* keys_in, const uint32_t* ro
 // number of words below its own. A bucket whose words would not fit 64 bits (top = 7), or with a run
 // longer than SUB_MAX rows, is queued for hyb_local_lsd instead. A bucket of equal keys (no varying
 // digit left), of any size, is copied.
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, uint32_t idx, const K* keys0,
                                           const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
                                           uint32_t* out_idx, uint32_t* queue, uint32_t* lc, LocalLds& L,
@@ -718,7 +718,7 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
   const KeyOut ko = key_out_of(mc);
   if (low_mask == 0u) {   // equal keys: the bucket is in order already
     for (uint32_t p = threadIdx.x; p < size; p += BLOCK) out[off + p] = rows_in ? rows_in[off + p] : p;
-    if constexpr (sizeof(K) == 8) {
+    if constexpr (KEYS && sizeof(K) == 8) {
       if (ko.p)
         for (uint32_t p = threadIdx.x; p < size; p += BLOCK) ko.p[off + p] = keys_in[off + p] ^ ko.x;
     }
@@ -781,16 +781,16 @@ __device__ __forceinline__ void local_one(const MsdCtl* mc, const Bucket* list, 
   uint32_t r[LI];
   unsigned long long kv[LI];
   load_rows<LI>(rows_in, off, src, r);   // every row and key load before the first store
-  load_keys<K, LI>(ko, keys_in, src, kv);
+  load_keys<K, LI, KEYS>(ko, keys_in, src, kv);
 #pragma unroll
   for (int j = 0; j < LI; j++)
     if (dest[j] != ~0u) out[dest[j]] = r[j];
-  write_keys<K, LI>(ko, dest, kv);
+  write_keys<K, LI, KEYS>(ko, dest, kv);
 }
 
 // The bucket sort over its list, an entry per workgroup in turn (nothing when the list ran out of room).
 // LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 1) bytes.
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __global__ __launch_bounds__(BLOCK) void hyb_local(const MsdCtl* mc, const Bucket* list, uint32_t local_cap,
                                                    const K* keys0, const uint32_t* rows0, const K* keys1,
                                                    const uint32_t* rows1, uint32_t* out_idx, uint32_t* queue,
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local(const MsdCtl* mc, const Bucke
   if (list_overflow(lc, local_cap)) return;   // uniform
   const uint32_t nlocal = lc[LC_LOCAL];
   for (uint32_t i = blockIdx.x; i < nlocal; i += gridDim.x) {
-    local_one<K, LI>(mc, list, i, keys0, rows0, keys1, rows1, out_idx, queue, lc, L, s_words, s_dig);
+    local_one<K, LI, KEYS>(mc, list, i, keys0, rows0, keys1, rows1, out_idx, queue, lc, L, s_words, s_dig);
     __syncthreads();
   }
 }
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local(const MsdCtl* mc, const Bucke
 // The groups of small buckets (the far end of the list), a group per workgroup in turn; a group with a
 // bin longer than SUB_MAX rows is queued for hyb_local_lsd. LDS for the group is dynamic:
 // GROUP_ROWS * (sizeof(K) + 2) + GROUP_BINS * 4 bytes.
-template <typename K>
+template <typename K, bool KEYS>
 __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucket* list, uint32_t local_cap,
                                                    const K* keys0, const uint32_t* rows0, const K* keys1,
                                                    const uint32_t* rows1, uint32_t* out_idx, uint32_t* queue,
@@ -838,11 +838,11 @@ __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucke
     bool done = bk.size > GROUP_ROWS;   // never
     if (!done) {
       switch (span_log) {
-        case 0: done = local_group<K, LI, 0>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
-        case 1: done = local_group<K, LI, 1>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
-        case 2: done = local_group<K, LI, 2>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
-        case 3: done = local_group<K, LI, 3>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
-        default: done = local_group<K, LI, 4>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 0: done = local_group<K, LI, 0, KEYS>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 1: done = local_group<K, LI, 1, KEYS>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 2: done = local_group<K, LI, 2, KEYS>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        case 3: done = local_group<K, LI, 3, KEYS>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
+        default: done = local_group<K, LI, 4, KEYS>(keys_in, rows_in, bk.start, bk.size, top, sub, out, ko, tmp, g_bins, s_words, s_bin); break;
       }
     }
     if (!done && threadIdx.x == 0) queue[atomicAdd(&lc[LC_QLEN], 1u)] = idx;
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_group(const MsdCtl* mc, const Bucke
 
 // The queued buckets, each by local_lsd: persistent workgroups take them from a counter until none is
 // left. LDS for the bucket is dynamic: LI * BLOCK * (sizeof(K) + 2) bytes.
-template <typename K, int LI>
+template <typename K, int LI, bool KEYS>
 __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const Bucket* list, const K* keys0,
                                                        const uint32_t* rows0, const K* keys1, const uint32_t* rows1,
                                                        uint32_t* out_idx, const uint32_t* queue, uint32_t* lc) {
@@ -874,7 +874,7 @@ __global__ __launch_bounds__(BLOCK) void hyb_local_lsd(const MsdCtl* mc, const B
     const Bucket bk = list[queue[qi]];
     const uint32_t low_mask = mc->active_mask & ((1u << (bk.level & LEVEL_BITS)) - 1u);
     if (bk.size <= uint32_t(LI * BLOCK))   // always: the host sizes LI for every queued bucket
-      local_lsd<K, LI>(bucket_keys(bk, keys0, keys1), bucket_rows(bk, rows0, rows1), bk.start, bk.size, low_mask, SUB_MAX,
+      local_lsd<K, LI, KEYS>(bucket_keys(bk, keys0, keys1), bucket_rows(bk, rows0, rows1), bk.start, bk.size, low_mask, SUB_MAX,
                        out_idx + mc->value_base, key_out_of(mc), wc, tile_total, bin_start, tmp, s_keys, s_pos);
     __syncthreads();
   }
@@ -948,7 +948,7 @@ inline MsdWork msd_work(uint32_t m, int items, DevBuf (*alloc)(size_t), uint32_t
 template <typename K, int ITEMS>
 bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind, uint32_t local_max,
                 unsigned long long* status, K* k0, uint32_t* r0, K* k1, uint32_t* r1, uint32_t* out_idx,
-                hipStream_t s) {
+                hipStream_t s, bool keys = false) {
   constexpr int P = int(sizeof(K));
   constexpr uint32_t TILE = BLOCK * ITEMS;
   const uint32_t cus = uint32_t(device().cu_count);
@@ -1025,38 +1025,49 @@ bool msd_levels(const MsdWork& w, uint32_t m, int top, uint32_t mask, bool blind
   // was).
   const dim3 lsd_grid(cus * 4u);
   const size_t lds = size_t(li) * BLOCK * (sizeof(K) + 1), lsd_lds = size_t(li_lsd) * BLOCK * (sizeof(K) + 2);
-  if (grid != 0u) {
-    ProfScope ps("sort_msd_local");
-    if (li == 4)
-      hipLaunchKernelGGL((hyb_local<K, 4>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
-                         r1, out_idx, w.queue, w.lc);
-    else if (li == 8)
-      hipLaunchKernelGGL((hyb_local<K, 8>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
-                         r1, out_idx, w.queue, w.lc);
-    else
-      hipLaunchKernelGGL((hyb_local<K, 16>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0, k1,
-                         r1, out_idx, w.queue, w.lc);
-    AM_HIP(hipGetLastError());
-  }
-  if (group_grid != 0u) {
-    ProfScope ps("sort_msd_group");
-    hipLaunchKernelGGL((hyb_group<K>), dim3(group_grid), dim3(BLOCK),
-                       GROUP_ROWS * (sizeof(K) + 2) + GROUP_BINS * 4, s, w.mc, w.local, w.local_cap, k0, r0, k1, r1,
-                       out_idx, w.queue, w.lc);
-    AM_HIP(hipGetLastError());
-  }
-  {
-    ProfScope ps("sort_msd_local_lsd");
-    if (li_lsd == 4)
-      hipLaunchKernelGGL((hyb_local_lsd<K, 4>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
+  // The bucket sorts in the form that also writes the sorted key column (keys), or not: a separate
+  // instantiation, so that the plain sort keeps its register budget.
+  auto bucket_sorts = [&](auto keys_tag) {
+    constexpr bool KEYS = decltype(keys_tag)::value;
+    if (grid != 0u) {
+      ProfScope ps("sort_msd_local");
+      if (li == 4)
+        hipLaunchKernelGGL((hyb_local<K, 4, KEYS>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0,
+                           k1, r1, out_idx, w.queue, w.lc);
+      else if (li == 8)
+        hipLaunchKernelGGL((hyb_local<K, 8, KEYS>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0, r0,
+                           k1, r1, out_idx, w.queue, w.lc);
+      else
+        hipLaunchKernelGGL((hyb_local<K, 16, KEYS>), dim3(grid), dim3(BLOCK), lds, s, w.mc, w.local, w.local_cap, k0,
+                           r0, k1, r1, out_idx, w.queue, w.lc);
+      AM_HIP(hipGetLastError());
+    }
+    if (group_grid != 0u) {
+      ProfScope ps("sort_msd_group");
+      hipLaunchKernelGGL((hyb_group<K, KEYS>), dim3(group_grid), dim3(BLOCK),
+                         GROUP_ROWS * (sizeof(K) + 2) + GROUP_BINS * 4, s, w.mc, w.local, w.local_cap, k0, r0, k1, r1,
                          out_idx, w.queue, w.lc);
-    else if (li_lsd == 8)
-      hipLaunchKernelGGL((hyb_local_lsd<K, 8>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
-                         out_idx, w.queue, w.lc);
-    else
-      hipLaunchKernelGGL((hyb_local_lsd<K, 16>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1, r1,
-                         out_idx, w.queue, w.lc);
-    AM_HIP(hipGetLastError());
+      AM_HIP(hipGetLastError());
+    }
+    {
+      ProfScope ps("sort_msd_local_lsd");
+      if (li_lsd == 4)
+        hipLaunchKernelGGL((hyb_local_lsd<K, 4, KEYS>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1,
+                           r1, out_idx, w.queue, w.lc);
+      else if (li_lsd == 8)
+        hipLaunchKernelGGL((hyb_local_lsd<K, 8, KEYS>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1,
+                           r1, out_idx, w.queue, w.lc);
+      else
+        hipLaunchKernelGGL((hyb_local_lsd<K, 16, KEYS>), lsd_grid, dim3(BLOCK), lsd_lds, s, w.mc, w.local, k0, r0, k1,
+                           r1, out_idx, w.queue, w.lc);
+      AM_HIP(hipGetLastError());
+    }
+  };
+  if constexpr (sizeof(K) == 8) {
+    if (keys) bucket_sorts(std::true_type{});
+    else bucket_sorts(std::false_type{});
+  } else {
+    bucket_sorts(std::false_type{});
   }
   return true;
 }

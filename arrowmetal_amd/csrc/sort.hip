@@ -585,7 +585,8 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
                        w.local_cap, w.lc, key_out, key_xor);
     AM_HIP(hipGetLastError());
   }
-  return msd_levels<K, ITEMS>(w, m, P - 1, mask, false, LOCAL_MAX, status, k0, r0, k1, r1, out_idx, s);
+  return msd_levels<K, ITEMS>(w, m, P - 1, mask, false, LOCAL_MAX, status, k0, r0, k1, r1, out_idx, s,
+                              key_out != nullptr);
 }
 
 // With pos_of_row (the ORDER BY of whole tables): only the hybrid sort runs, with positions for rows

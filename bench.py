@@ -147,7 +147,7 @@ ROCPROF_NAME = {
     "sort_msd_first": "msd_part<u64, 8, 16, false>",
     "sort_msd_hist": "lvl_hist<u64>",
     "sort_msd_scatter": "lvl_pass<u64, 20>",
-    "sort_msd_local": "hyb_local<u64, 8>",
+    "sort_msd_local": "hyb_local<u64, 8, false>",
     "sort_first": "sort_first<u64, 8, 16>",
     "sort_scatter": "sort_mid<u64, 20, 0>",
     "sort_scatter_last": "sort_mid<u64, 20, 1>",
@@ -155,8 +155,8 @@ ROCPROF_NAME = {
     "gb_build": "gb_build<false, 8>",
     "gb_ids": "gb_ids",
     "gb_part_hist": "gb_part_hist<SumCountP, 4>",
-    "gb_part_scatter": "gb_part_scatter<SumCountP, 4, 8>",
-    "gb_sum": "gb_part_reduce<SumCountP, 8>",
+    "gb_part_scatter": "gb_part_scatter<SumCountP, 4, 8, false>",
+    "gb_sum": "gb_part_reduce<SumCountP, 8, false>",
     "order_by_partition": "ob_partition<8>",
     "order_by_gather": "ob_gather<8>",
 }
