@@ -138,6 +138,24 @@ void retain(const DevBuf& b);
 // Runs `f` after the stream has drained at the next sync point (deferred GPU-side error flags).
 void after_sync(std::function<void()> f);
 
+// Small read-backs (status words, counts: at most PINNED_SLOT bytes, a multiple of 4, 4-byte aligned).
+// A one-wave kernel copies the words into pinned host memory, which the host reads once the stream has
+// drained: a copy into pageable memory would be staged and costs ~10 us more per read-back.
+constexpr size_t PINNED_SLOT = 256;
+// Copies `bytes` of device memory at `dev` to `host` and waits for the thread's stream.
+void read_back(void* host, const void* dev, size_t bytes);
+template <class T>
+T read_back(const T* dev) {
+  T v;
+  read_back(&v, dev, sizeof(T));
+  return v;
+}
+// A pinned slot of PINNED_SLOT bytes (returned to a shared free list when released), for a read-back
+// whose check is deferred to the next sync point.
+std::shared_ptr<void> pinned_slot();
+// Queues the copy of `bytes` of device memory at `dev` into the pinned slot `dst` on the thread's stream.
+void copy_to_pinned(void* dst, const void* dev, size_t bytes);
+
 // Kernel-time profiling of the library's own launches (extension API am_prof_*): when enabled,
 // each named launch is bracketed with HIP events on the launching stream.
 struct ProfScope {

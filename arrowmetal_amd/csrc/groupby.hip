@@ -208,47 +208,231 @@ __device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsig
   }
 }
 
-// Distinct-key estimate (HyperLogLog, 2^12 registers) that sizes the hash table before the build, so
-// the table is about twice the distinct keys rather than twice the rows, and stays in L2.
+// Statistics of a key column, read back once before the column is mapped:
+//  - a HyperLogLog estimate of its distinct keys (2^12 registers), which sizes the hash table at about
+//    2.5 slots per distinct key rather than two per row, so that the table stays in L2;
+//  - for integer keys, the smallest and largest order key, the nulls and the lowest null row, which
+//    decide whether the range path maps the column.
 constexpr int HLL_BITS = 12;
 constexpr uint32_t HLL_M = 1u << HLL_BITS;
 
-template <int W>
-__global__ __launch_bounds__(256) void gb_hll(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                              uint32_t n, uint32_t* regs) {
+struct RangeStat {
+  unsigned long long kmin, kmax, nulls;
+  uint32_t null_first, pad;
+};
+
+struct KeyStat {
+  RangeStat r;
+  unsigned long long est;
+};
+
+__global__ __launch_bounds__(256) void gb_key_init(uint32_t* regs, RangeStat* st) {
+  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) regs[j] = 0u;
+  if (threadIdx.x == 0) {
+    st->kmin = ~0ull;
+    st->kmax = 0ull;
+    st->nulls = 0ull;
+    st->null_first = 0xFFFFFFFFu;
+    st->pad = 0u;
+  }
+}
+
+// One pass over the keys: each workgroup keeps its own registers in LDS, then raises the global ones.
+// A global register is read first and raised only when the workgroup's is larger: registers only
+// grow, so a stale read can only send a value to the atomic, which decides. Integer keys also give
+// the smallest and largest order key, the nulls and the lowest null row.
+//
+// The key type is a template argument (no type switch per row) and each thread has two steps of
+// SCAN_ROWS rows in flight: the next step's keys and validity bytes are loaded before this step's are
+// hashed, and every loaded register is taken at the top of a step, so that no load is moved into a
+// branch where it would be waited for alone.
+constexpr int SCAN_ROWS = 8;
+#ifndef AM_SCAN_BLOCKS_PER_CU
+#define AM_SCAN_BLOCKS_PER_CU 4
+#endif
+#ifndef AM_OCC_BLOCKS_PER_CU
+#define AM_OCC_BLOCKS_PER_CU 2
+#endif
+
+struct ScanRows {
+  unsigned long long raw[SCAN_ROWS];
+  uint32_t vb[SCAN_ROWS];
+};
+
+template <int W, bool HAS_VALID>
+__device__ __forceinline__ void scan_load(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+                                          uint64_t row0, uint32_t gsize, ScanRows& x) {
+#pragma unroll
+  for (int r = 0; r < SCAN_ROWS; r++) {
+    const uint64_t i = row0 + uint64_t(r) * gsize;
+    const int64_t ii = int64_t(i < n ? i : uint64_t(n) - 1u);
+    x.raw[r] = ld_raw<W>(vals, ii);
+    if (HAS_VALID) x.vb[r] = load_global(valid + ((voff + ii) >> 3));
+  }
+}
+
+// Runs rows(row0, step) over every step of the grid-stride row loop, one step's rows loaded while the
+// step before is processed (the step count is the same for every thread).
+template <int W, bool HAS_VALID, class F>
+__device__ __forceinline__ void scan_steps(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n, F&& rows) {
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  const uint64_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = uint64_t(gsize) * SCAN_ROWS;
+  const int steps = int((uint64_t(n) + stride - 1) / stride);
+  if (steps == 0) return;
+  ScanRows xa, xb;
+  scan_load<W, HAS_VALID>(vals, valid, voff, n, tid, gsize, xa);
+  for (int t = 0; t < steps; t += 2) {
+    const uint64_t base = uint64_t(t) * stride + tid;
+    scan_load<W, HAS_VALID>(vals, valid, voff, n, base + stride, gsize, xb);
+    rows(base, xa);
+    if (t + 1 >= steps) break;
+    scan_load<W, HAS_VALID>(vals, valid, voff, n, base + 2 * stride, gsize, xa);
+    rows(base + stride, xb);
+  }
+}
+
+// Takes every loaded register of a step at once (before any branch uses one).
+template <bool HAS_VALID>
+__device__ __forceinline__ void scan_take(const ScanRows& x) {
+#pragma unroll
+  for (int r = 0; r < SCAN_ROWS; r++) {
+    asm volatile("" ::"v"(x.raw[r]));
+    if (HAS_VALID) asm volatile("" ::"v"(x.vb[r]));
+  }
+}
+
+constexpr int width_of(int t) {
+  return (t == T_I8 || t == T_U8) ? 1 : (t == T_I16 || t == T_U16) ? 2 : (t == T_I64 || t == T_U64 || t == T_F64) ? 8 : 4;
+}
+
+// Calls f(std::integral_constant<int, TYPE>) for a primitive key type.
+template <class F>
+void with_key_type(int type, F&& f) {
+  switch (type) {
+    case T_I8: f(std::integral_constant<int, T_I8>{}); break;
+    case T_U8: f(std::integral_constant<int, T_U8>{}); break;
+    case T_I16: f(std::integral_constant<int, T_I16>{}); break;
+    case T_U16: f(std::integral_constant<int, T_U16>{}); break;
+    case T_I32: f(std::integral_constant<int, T_I32>{}); break;
+    case T_U32: f(std::integral_constant<int, T_U32>{}); break;
+    case T_F32: f(std::integral_constant<int, T_F32>{}); break;
+    case T_I64: f(std::integral_constant<int, T_I64>{}); break;
+    case T_U64: f(std::integral_constant<int, T_U64>{}); break;
+    case T_F64: f(std::integral_constant<int, T_F64>{}); break;
+    default: fail("group-by: key type " + std::to_string(type) + " is not a primitive type");
+  }
+}
+
+struct ScanAcc {
+  unsigned long long lo = ~0ull, hi = 0ull, nz = 0ull;
+  uint32_t nf = 0xFFFFFFFFu;
+};
+
+template <int TYPE, bool HAS_VALID>
+__device__ __forceinline__ void scan_rows(int64_t voff, uint32_t n, uint64_t row0, uint32_t gsize, const ScanRows& x,
+                                          uint32_t* r_lds, ScanAcc& acc) {
+  constexpr bool RANGE = TYPE >= T_I8 && TYPE <= T_U64;
+  scan_take<HAS_VALID>(x);
+  uint32_t idx[SCAN_ROWS], rank[SCAN_ROWS];
+#pragma unroll
+  for (int r = 0; r < SCAN_ROWS; r++) {
+    const uint64_t i = row0 + uint64_t(r) * gsize;
+    const bool in = i < n;
+    const int64_t ii = int64_t(in ? i : uint64_t(n) - 1u);
+    const bool v = HAS_VALID ? vbit(voff, ii, x.vb[r]) != 0u : true;
+    const bool ok = in && v;
+    const unsigned long long gk = key_of_raw(x.raw[r], TYPE);
+    if (RANGE) {
+      const unsigned long long k = order_key(gk, TYPE);
+      acc.lo = ok && k < acc.lo ? k : acc.lo;
+      acc.hi = ok && k > acc.hi ? k : acc.hi;
+      if (HAS_VALID) {
+        const bool null_row = in && !v;
+        acc.nz += null_row ? 1u : 0u;
+        acc.nf = null_row ? min(acc.nf, uint32_t(i)) : acc.nf;
+      }
+    }
+    const unsigned long long h = mix64(gk ^ 0x2545F4914F6CDD1Dull);
+    idx[r] = uint32_t(h >> (64 - HLL_BITS));
+    const unsigned long long w = h << HLL_BITS;
+    rank[r] = !ok ? 0u : w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
+  }
+  // The step's register reads are issued together; a register is raised only when the row's rank is
+  // larger (a null row or a row past the end has rank 0).
+  uint32_t cur[SCAN_ROWS];
+#pragma unroll
+  for (int r = 0; r < SCAN_ROWS; r++) cur[r] = r_lds[idx[r]];
+#pragma unroll
+  for (int r = 0; r < SCAN_ROWS; r++)
+    if (rank[r] > cur[r]) atomicMax(&r_lds[idx[r]], rank[r]);
+}
+
+template <int W, int TYPE, bool HAS_VALID>
+__global__ __launch_bounds__(256) void gb_key_scan(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+                                                   uint32_t* regs, RangeStat* st) {
+  constexpr bool RANGE = TYPE >= T_I8 && TYPE <= T_U64;
   __shared__ uint32_t r[HLL_M];
+  __shared__ unsigned long long s_lo[4], s_hi[4], s_nz[4];
+  __shared__ uint32_t s_nf[4];
   for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) r[j] = 0u;
   __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
-  constexpr int U = 8;   // rows per thread per step, loaded together
-  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * U) {
-    unsigned long long raw[U];
-    bool ok[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint64_t iu = base + uint64_t(u) * gsize;
-      const int64_t ii = int64_t(iu < n ? iu : uint64_t(n) - 1u);
-      raw[u] = ld_raw<W>(vals, ii);
-      const bool v = ld_valid(valid, voff, ii);
-      ok[u] = v && iu < n;
+  ScanAcc acc;
+  scan_steps<W, HAS_VALID>(vals, valid, voff, n, [&](uint64_t row0, const ScanRows& x) {
+    scan_rows<TYPE, HAS_VALID>(voff, n, row0, gsize, x, r, acc);
+  });
+  if (RANGE) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long l2 = __shfl_xor(acc.lo, off, 64), h2 = __shfl_xor(acc.hi, off, 64);
+      acc.lo = l2 < acc.lo ? l2 : acc.lo;
+      acc.hi = h2 > acc.hi ? h2 : acc.hi;
+      acc.nz += __shfl_xor(acc.nz, off, 64);
+      acc.nf = min(acc.nf, uint32_t(__shfl_xor(int(acc.nf), off, 64)));
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (!ok[u]) continue;
-      const unsigned long long h = mix64(key_of_raw(raw[u], type) ^ 0x2545F4914F6CDD1Dull);
-      const uint32_t idx = uint32_t(h >> (64 - HLL_BITS));
-      const unsigned long long w = h << HLL_BITS;
-      const uint32_t rank = w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
-      if (rank > r[idx]) atomicMax(&r[idx], rank);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      s_lo[wv] = acc.lo;
+      s_hi[wv] = acc.hi;
+      s_nz[wv] = acc.nz;
+      s_nf[wv] = acc.nf;
     }
   }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x)
-    if (r[j]) atomicMax(&regs[j], r[j]);
+  if (RANGE && threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      acc.lo = s_lo[k] < acc.lo ? s_lo[k] : acc.lo;
+      acc.hi = s_hi[k] > acc.hi ? s_hi[k] : acc.hi;
+      acc.nz += s_nz[k];
+      acc.nf = min(acc.nf, s_nf[k]);
+    }
+    if (acc.lo != ~0ull) atomicMin(&st->kmin, acc.lo);
+    if (acc.hi != 0ull) atomicMax(&st->kmax, acc.hi);
+    if (acc.nz) {
+      atomicAdd(&st->nulls, acc.nz);
+      atomicMin(&st->null_first, acc.nf);
+    }
+  }
+  for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) {
+    const uint32_t mine = r[j];
+    if (mine > load_global(&regs[j])) atomicMax(&regs[j], mine);
+  }
 }
 
-// One block: the HyperLogLog estimate (with the linear-counting correction for small counts).
-__global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, unsigned long long* est) {
+void key_scan(int type, dim3 grid, hipStream_t s, const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+              uint32_t* regs, RangeStat* st) {
+  with_key_type(type, [&](auto tc) {
+    constexpr int TT = decltype(tc)::value;
+    if (valid)
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, true>), grid, dim3(256), 0, s, vals, valid, voff, n, regs, st);
+    else
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, false>), grid, dim3(256), 0, s, vals, valid, voff, n, regs, st);
+  });
+}
+
+// One block: the HyperLogLog estimate (with the linear-counting correction for small counts), written
+// with the range statistics straight into pinned host memory.
+__global__ __launch_bounds__(256) void gb_key_estimate(const uint32_t* regs, const RangeStat* st, KeyStat* out) {
   __shared__ float s_sum[256];
   __shared__ uint32_t s_zero[256];
   float sum = 0.f;
@@ -271,7 +455,8 @@ __global__ __launch_bounds__(256) void gb_hll_estimate(const uint32_t* regs, uns
     const double m = double(HLL_M);
     double e = (0.7213 / (1.0 + 1.079 / m)) * m * m / double(s_sum[0]);
     if (e <= 2.5 * m && s_zero[0] > 0) e = m * log(m / double(s_zero[0]));
-    *est = (unsigned long long)(e + 0.5);
+    out->r = *st;
+    out->est = (unsigned long long)(e + 0.5);
   }
 }
 
@@ -2263,11 +2448,25 @@ struct GroupBy {
   DevBuf count_all_tab;    // K x 1 rows per group
   PartSet part;            // the partitioned records of the last value column (many groups)
   DevBuf hot;              // HOT_GROUPS sampled frequent ids (sums and counts in registers)
+  // The mapping is left running when am_group_by_keys returns: `ready` is recorded on `owner` after
+  // its last kernel. Calls on another thread's stream wait for it, and the handle's release waits
+  // before the key columns can be freed.
+  hipEvent_t ready = nullptr;
+  hipStream_t owner = nullptr;
   ~GroupBy() {
+    if (ready) {
+      (void)hipEventSynchronize(ready);
+      (void)hipEventDestroy(ready);
+    }
     for (Array* a : columns) release_array(a);
     for (Array* a : key_results) release_array(a);
   }
 };
+
+// Orders the calling thread's stream after the handle's mapping.
+void gb_use(const GroupBy* g) {
+  if (g->ready && g->owner != stream()) AM_HIP(hipStreamWaitEvent(stream(), g->ready, 0));
+}
 
 namespace {
 
@@ -2292,112 +2491,84 @@ bool range_path_enabled() {
 // marks every present key in an occupancy array over the span (and its lowest row when representative
 // rows are tracked), a scan of the occupancy turns each present key into its rank, which is its id
 // in ascending key order, and one pass writes every row's id (the null group, last, is D).
-struct RangeStat {
-  unsigned long long kmin, kmax, nulls;
-  uint32_t null_first, pad;
-};
-
-__global__ void gb_range_init(RangeStat* st) {
-  st->kmin = ~0ull;
-  st->kmax = 0ull;
-  st->nulls = 0ull;
-  st->null_first = 0xFFFFFFFFu;
-}
-
-constexpr int RANGE_ROWS = 8;   // rows per thread per step, loaded together
-
-// Over rows 0, stride, 2 * stride, ... (n of them): the smallest and largest order key, the nulls and
-// the lowest null row. A stride above 1 is the sample that rules out a wide span before the full pass.
-template <int W>
-__global__ __launch_bounds__(256) void gb_range_minmax(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                                       uint32_t n, uint32_t stride, RangeStat* st) {
-  __shared__ unsigned long long s_lo[4], s_hi[4], s_nz[4];
-  __shared__ uint32_t s_nf[4];
-  const uint32_t gsize = gridDim.x * blockDim.x;
-  unsigned long long lo = ~0ull, hi = 0ull, nz = 0ull;
-  uint32_t nf = 0xFFFFFFFFu;
-  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
-    unsigned long long raw[RANGE_ROWS];
-    bool ok[RANGE_ROWS];
-#pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      const uint64_t ii = (i < n ? i : n - 1) * stride;
-      raw[u] = ld_raw<W>(vals, int64_t(ii));
-      ok[u] = ld_valid(valid, voff, int64_t(ii));
-    }
-#pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      if (i >= n) continue;
-      if (!ok[u]) {
-        nz++;
-        nf = min(nf, uint32_t(i * stride));
-        continue;
-      }
-      const unsigned long long k = order_key(key_of_raw(raw[u], type), type);
-      lo = k < lo ? k : lo;
-      hi = k > hi ? k : hi;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
-    lo = l2 < lo ? l2 : lo;
-    hi = h2 > hi ? h2 : hi;
-    nz += __shfl_xor(nz, off, 64);
-    nf = min(nf, uint32_t(__shfl_xor(int(nf), off, 64)));
-  }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    s_lo[w] = lo;
-    s_hi[w] = hi;
-    s_nz[w] = nz;
-    s_nf[w] = nf;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < 4; k++) {
-      lo = s_lo[k] < lo ? s_lo[k] : lo;
-      hi = s_hi[k] > hi ? s_hi[k] : hi;
-      nz += s_nz[k];
-      nf = min(nf, s_nf[k]);
-    }
-    if (lo != ~0ull) atomicMin(&st->kmin, lo);
-    if (hi != 0ull) atomicMax(&st->kmax, hi);
-    if (nz) {
-      atomicAdd(&st->nulls, nz);
-      atomicMin(&st->null_first, nf);
-    }
-  }
-}
 
 // occ[k - kmin] = 1 for every present key; first[k - kmin] = its lowest row when tracked. Both only
-// ever move one way, so a read that already shows the value skips the write.
-template <int W, bool TRACK>
-__global__ __launch_bounds__(256) void gb_range_occ(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                                    uint32_t n, unsigned long long kmin, uint32_t* occ, uint32_t* first) {
+// ever move one way, so a read that already shows the value skips the write. A step's reads are
+// issued together (a null row reads entry 0, which exists, and writes nothing).
+template <int W, int TYPE, bool HAS_VALID, bool TRACK>
+__global__ __launch_bounds__(256) void gb_range_occ(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+                                                    unsigned long long kmin, uint32_t* occ, uint32_t* first) {
   const uint32_t gsize = gridDim.x * blockDim.x;
-  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
-    unsigned long long raw[RANGE_ROWS];
-    bool ok[RANGE_ROWS];
+  scan_steps<W, HAS_VALID>(vals, valid, voff, n, [&](uint64_t row0, const ScanRows& x) {
+    scan_take<HAS_VALID>(x);
+    uint32_t xs[SCAN_ROWS];
+    bool ok[SCAN_ROWS];
 #pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      const uint64_t ii = i < n ? i : n - 1;
-      raw[u] = ld_raw<W>(vals, int64_t(ii));
-      ok[u] = ld_valid(valid, voff, int64_t(ii));
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      const uint64_t i = row0 + uint64_t(r) * gsize;
+      const int64_t ii = int64_t(i < n ? i : uint64_t(n) - 1u);
+      ok[r] = i < n && (HAS_VALID ? vbit(voff, ii, x.vb[r]) != 0u : true);
+      xs[r] = ok[r] ? uint32_t(order_key(key_of_raw(x.raw[r], TYPE), TYPE) - kmin) : 0u;
+    }
+    uint32_t cur[SCAN_ROWS], low[SCAN_ROWS];
+#pragma unroll
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      cur[r] = occ[xs[r]];
+      if (TRACK) low[r] = first[xs[r]];
+    }
+    // Plain accesses: a stale 0 costs one more store of the same 1; a stale row is larger than the
+    // current lowest one, so it never skips a needed atomicMin.
+#pragma unroll
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      if (ok[r] && cur[r] == 0u) occ[xs[r]] = 1u;
+      const uint32_t i = uint32_t(row0 + uint64_t(r) * gsize);
+      if (TRACK && ok[r] && i < low[r]) atomicMin(&first[xs[r]], i);
+    }
+  });
+}
+
+// The occupancy of a span of at most RANGE_BITS_MAX keys without representative rows: each workgroup
+// marks its rows' keys in a bitmap of the span in LDS, then ORs its words into the global bitmap (a word
+// already holding the bits is only read). gb_range_unpack turns the bitmap into occ[].
+constexpr uint32_t RANGE_BITS_MAX = 1u << 19;   // 64 KiB of LDS
+
+template <int W, int TYPE, bool HAS_VALID>
+__global__ __launch_bounds__(256) void gb_range_occ_bits(const void* vals, const uint8_t* valid, int64_t voff,
+                                                         uint32_t n, unsigned long long kmin, uint32_t words,
+                                                         uint32_t* bits) {
+  extern __shared__ uint32_t s_bits[];
+  for (uint32_t j = threadIdx.x; j < words; j += blockDim.x) s_bits[j] = 0u;
+  __syncthreads();
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  scan_steps<W, HAS_VALID>(vals, valid, voff, n, [&](uint64_t row0, const ScanRows& x) {
+    scan_take<HAS_VALID>(x);
+    // The step's words are read together; a bit already set (a frequent key) needs no atomic.
+    uint32_t wd[SCAN_ROWS], bit[SCAN_ROWS], cur[SCAN_ROWS];
+#pragma unroll
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      const uint64_t i = row0 + uint64_t(r) * gsize;
+      const int64_t ii = int64_t(i < n ? i : uint64_t(n) - 1u);
+      const bool ok = i < n && (HAS_VALID ? vbit(voff, ii, x.vb[r]) != 0u : true);
+      const uint32_t xk = ok ? uint32_t(order_key(key_of_raw(x.raw[r], TYPE), TYPE) - kmin) : 0u;
+      wd[r] = xk >> 5;
+      bit[r] = ok ? 1u << (xk & 31u) : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      if (i >= n || !ok[u]) continue;
-      const uint32_t x = uint32_t(order_key(key_of_raw(raw[u], type), type) - kmin);
-      // Plain accesses: a stale 0 costs one more store of the same 1; a stale row is larger than the
-      // current lowest one, so it never skips a needed atomicMin.
-      if (occ[x] == 0u) occ[x] = 1u;
-      if (TRACK && uint32_t(i) < first[x]) atomicMin(&first[x], uint32_t(i));
-    }
+    for (int r = 0; r < SCAN_ROWS; r++) cur[r] = s_bits[wd[r]];
+#pragma unroll
+    for (int r = 0; r < SCAN_ROWS; r++)
+      if ((cur[r] & bit[r]) != bit[r]) atomicOr(&s_bits[wd[r]], bit[r]);
+  });
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < words; j += blockDim.x) {
+    const uint32_t v = s_bits[j];
+    if (v && (load_global(&bits[j]) & v) != v) atomicOr(&bits[j], v);
   }
+}
+
+__global__ void gb_range_unpack(const uint32_t* bits, uint32_t span, uint32_t* occ) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < span) occ[x] = (bits[x >> 5] >> (x & 31u)) & 1u;
 }
 
 constexpr uint32_t RANGE_TILE = 4096;   // occupancy entries per workgroup of the scan (16 per thread)
@@ -2472,31 +2643,33 @@ __global__ __launch_bounds__(256) void gb_range_rank(uint32_t* occ, uint32_t spa
   }
 }
 
-template <int W>
-__global__ __launch_bounds__(256) void gb_range_ids(const void* vals, int type, const uint8_t* valid, int64_t voff,
-                                                    uint32_t n, unsigned long long kmin, const uint32_t* rank,
-                                                    uint32_t null_gid, uint32_t* ids) {
+// Every row's id: the rank of its key's occupancy entry, or null_gid for a null row. With DENSE every
+// key of the span is present, so a key's rank is its offset in the span and nothing is looked up.
+template <int W, int TYPE, bool HAS_VALID, bool DENSE>
+__global__ __launch_bounds__(256) void gb_range_ids(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+                                                    unsigned long long kmin, const uint32_t* rank, uint32_t null_gid,
+                                                    uint32_t* ids) {
   const uint32_t gsize = gridDim.x * blockDim.x;
-  for (uint64_t base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += uint64_t(gsize) * RANGE_ROWS) {
-    unsigned long long raw[RANGE_ROWS];
-    bool ok[RANGE_ROWS];
+  scan_steps<W, HAS_VALID>(vals, valid, voff, n, [&](uint64_t row0, const ScanRows& x) {
+    scan_take<HAS_VALID>(x);
+    uint32_t xs[SCAN_ROWS];
+    bool ok[SCAN_ROWS];
 #pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      const uint64_t ii = i < n ? i : n - 1;
-      raw[u] = ld_raw<W>(vals, int64_t(ii));
-      ok[u] = ld_valid(valid, voff, int64_t(ii));
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      const uint64_t i = row0 + uint64_t(r) * gsize;
+      const int64_t ii = int64_t(i < n ? i : uint64_t(n) - 1u);
+      ok[r] = HAS_VALID ? vbit(voff, ii, x.vb[r]) != 0u : true;
+      xs[r] = ok[r] ? uint32_t(order_key(key_of_raw(x.raw[r], TYPE), TYPE) - kmin) : 0u;
     }
-    uint32_t g[RANGE_ROWS];
+    uint32_t g[SCAN_ROWS];
 #pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++)
-      g[u] = ok[u] ? rank[uint32_t(order_key(key_of_raw(raw[u], type), type) - kmin)] : null_gid;
+    for (int r = 0; r < SCAN_ROWS; r++) g[r] = DENSE ? xs[r] : rank[xs[r]];
 #pragma unroll
-    for (int u = 0; u < RANGE_ROWS; u++) {
-      const uint64_t i = base + uint64_t(u) * gsize;
-      if (i < n) __builtin_nontemporal_store(g[u], ids + i);
+    for (int r = 0; r < SCAN_ROWS; r++) {
+      const uint64_t i = row0 + uint64_t(r) * gsize;
+      if (i < n) __builtin_nontemporal_store(ok[r] ? g[r] : null_gid, ids + i);
     }
-  }
+  });
 }
 
 __global__ void gb_iota(uint32_t* p, uint32_t n) {
@@ -2510,50 +2683,22 @@ __global__ void gb_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-template <int W>
-void range_launch_minmax(int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
-                         uint32_t n, uint32_t stride, RangeStat* st) {
-  hipLaunchKernelGGL(gb_range_minmax<W>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, stride, st);
-}
+__global__ void gb_put_u32(uint32_t* p, uint32_t v) { *p = v; }
 
-constexpr uint32_t RANGE_SAMPLE = 4096;   // rows read to rule out a wide span
-
-// The range path's mapping, or false (after one read-back) when the keys' span is too wide for it.
-bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track, Mapped& out) {
+// The range path's mapping from the keys' statistics, or false when their span is too wide for it.
+bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, int64_t n, bool track,
+               const RangeStat& host, Mapped& out) {
   hipStream_t s = stream();
   const uint32_t un = uint32_t(n);
-  const int w = type_width(type);
-  DevBuf stb = device_alloc(sizeof(RangeStat));
-  auto* st = static_cast<RangeStat*>(stb->ptr);
   const unsigned long long limit = std::min<unsigned long long>(1ull << 24, std::max<unsigned long long>(1ull << 16, 4ull * uint64_t(n)));
-  // Pass 1 (sample, then all rows): the keys' smallest and largest values. A sample whose own span is
-  // already too wide sends the keys to the hash path after reading RANGE_SAMPLE rows.
-  RangeStat host{};
-  for (int full = 0; full < 2; full++) {
-    const uint32_t count = full ? un : std::min<uint32_t>(un, RANGE_SAMPLE);
-    const uint32_t stride = full ? 1u : un / count;
-    const int grid = grid_for((int64_t(count) + RANGE_ROWS - 1) / RANGE_ROWS, 8);
-    hipLaunchKernelGGL(gb_range_init, dim3(1), dim3(1), 0, s, st);
-    {
-      ProfScope ps(full ? "gb_range_minmax" : "gb_range_sample");
-      switch (w) {
-        case 8: range_launch_minmax<8>(grid, s, vals, type, valid, voff, count, stride, st); break;
-        case 4: range_launch_minmax<4>(grid, s, vals, type, valid, voff, count, stride, st); break;
-        case 2: range_launch_minmax<2>(grid, s, vals, type, valid, voff, count, stride, st); break;
-        default: range_launch_minmax<1>(grid, s, vals, type, valid, voff, count, stride, st); break;
-      }
-      AM_HIP(hipGetLastError());
-    }
-    AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
+  {
     const bool any = host.kmin <= host.kmax;
     const unsigned long long span64 = any ? host.kmax - host.kmin + 1ull : 0ull;
     if (any && (span64 == 0ull || span64 > limit)) return false;   // span64 == 0: the full 2^64 range
-    if (count == un) break;   // the sample was every row
   }
   const bool any = host.kmin <= host.kmax;
   const unsigned long long span64 = any ? host.kmax - host.kmin + 1ull : 0ull;
-  const int grid = grid_for((n + RANGE_ROWS - 1) / RANGE_ROWS, 8);
+  const int grid = grid_for((n + SCAN_ROWS - 1) / SCAN_ROWS, 8);
   const uint32_t span = uint32_t(std::max<unsigned long long>(span64, 1ull));
   DevBuf occ = device_alloc(size_t(span) * 4);
   DevBuf first;
@@ -2567,16 +2712,29 @@ bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, i
   auto* f = track ? static_cast<uint32_t*>(first->ptr) : nullptr;
   if (any) {
     ProfScope ps("gb_range_occ");
-#define AM_RANGE_OCC(WW)                                                                                       \
-  if (track) hipLaunchKernelGGL((gb_range_occ<WW, true>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, f); \
-  else hipLaunchKernelGGL((gb_range_occ<WW, false>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, f)
-    switch (w) {
-      case 8: AM_RANGE_OCC(8); break;
-      case 4: AM_RANGE_OCC(4); break;
-      case 2: AM_RANGE_OCC(2); break;
-      default: AM_RANGE_OCC(1); break;
-    }
-#undef AM_RANGE_OCC
+    with_key_type(type, [&](auto tc) {
+      constexpr int TT = decltype(tc)::value;
+      if constexpr (TT != T_F32 && TT != T_F64) {
+        constexpr int WW = width_of(TT);
+        if (!track && span <= RANGE_BITS_MAX) {
+          const uint32_t words = (span + 31u) / 32u;
+          DevBuf bits = device_alloc(size_t(words) * 4);
+          auto* b = static_cast<uint32_t*>(bits->ptr);
+          hipLaunchKernelGGL(gb_fill_u32, dim3(grid_for(words, 4)), dim3(256), 0, s, b, size_t(words), 0u);
+          const dim3 bg(grid_for((n + SCAN_ROWS - 1) / SCAN_ROWS, AM_OCC_BLOCKS_PER_CU));
+          const size_t lds = size_t(words) * 4;
+          if (valid) hipLaunchKernelGGL((gb_range_occ_bits<WW, TT, true>), bg, dim3(256), lds, s, vals, valid, voff, un, host.kmin, words, b);
+          else hipLaunchKernelGGL((gb_range_occ_bits<WW, TT, false>), bg, dim3(256), lds, s, vals, valid, voff, un, host.kmin, words, b);
+          hipLaunchKernelGGL(gb_range_unpack, dim3((span + 255) / 256), dim3(256), 0, s, static_cast<const uint32_t*>(b), span, o);
+        } else if (valid) {
+          if (track) hipLaunchKernelGGL((gb_range_occ<WW, TT, true, true>), dim3(grid), dim3(256), 0, s, vals, valid, voff, un, host.kmin, o, f);
+          else hipLaunchKernelGGL((gb_range_occ<WW, TT, true, false>), dim3(grid), dim3(256), 0, s, vals, valid, voff, un, host.kmin, o, f);
+        } else {
+          if (track) hipLaunchKernelGGL((gb_range_occ<WW, TT, false, true>), dim3(grid), dim3(256), 0, s, vals, valid, voff, un, host.kmin, o, f);
+          else hipLaunchKernelGGL((gb_range_occ<WW, TT, false, false>), dim3(grid), dim3(256), 0, s, vals, valid, voff, un, host.kmin, o, f);
+        }
+      }
+    });
     AM_HIP(hipGetLastError());
   }
   const uint32_t tiles = (span + RANGE_TILE - 1) / RANGE_TILE;
@@ -2593,29 +2751,36 @@ bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, i
                        track ? static_cast<uint32_t*>(out.rep->ptr) : nullptr);
     AM_HIP(hipGetLastError());
   }
-  uint32_t D = 0;
-  AM_HIP(hipMemcpyAsync(&D, ts + tiles, 4, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
+  const uint32_t D = read_back(ts + tiles);
   out.D = any ? D : 0u;
   out.has_null = host.nulls > 0;
   out.K = int64_t(out.D) + (out.has_null ? 1 : 0);
-  if (track && out.has_null) AM_HIP(hipMemcpyAsync(static_cast<uint32_t*>(out.rep->ptr) + out.D, &host.null_first, 4,
-                                                   hipMemcpyHostToDevice, s));
+  if (track && out.has_null)
+    hipLaunchKernelGGL(gb_put_u32, dim3(1), dim3(1), 0, s, static_cast<uint32_t*>(out.rep->ptr) + out.D, host.null_first);
   out.perm = device_alloc(size_t(std::max<uint32_t>(out.D, 1)) * 4);
   hipLaunchKernelGGL(gb_iota, dim3((out.D + 255) / 256 + 1), dim3(256), 0, s, static_cast<uint32_t*>(out.perm->ptr), out.D);
   out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
   {
     ProfScope ps("gb_ids");
     auto* ids = static_cast<uint32_t*>(out.ids->ptr);
-    switch (w) {
-      case 8: hipLaunchKernelGGL(gb_range_ids<8>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
-      case 4: hipLaunchKernelGGL(gb_range_ids<4>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
-      case 2: hipLaunchKernelGGL(gb_range_ids<2>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
-      default: hipLaunchKernelGGL(gb_range_ids<1>, dim3(grid), dim3(256), 0, s, vals, type, valid, voff, un, host.kmin, o, out.D, ids); break;
-    }
+    with_key_type(type, [&](auto tc) {
+      constexpr int TT = decltype(tc)::value;
+      if constexpr (TT != T_F32 && TT != T_F64) {
+        constexpr int WW = width_of(TT);
+        const bool dense = out.D == span;
+#define AM_RANGE_IDS(HV, DN) hipLaunchKernelGGL((gb_range_ids<WW, TT, HV, DN>), dim3(grid), dim3(256), 0, s, vals, valid, voff, un, host.kmin, o, out.D, ids)
+        if (valid) {
+          if (dense) AM_RANGE_IDS(true, true);
+          else AM_RANGE_IDS(true, false);
+        } else {
+          if (dense) AM_RANGE_IDS(false, true);
+          else AM_RANGE_IDS(false, false);
+        }
+#undef AM_RANGE_IDS
+      }
+    });
     AM_HIP(hipGetLastError());
   }
-  AM_HIP(hipStreamSynchronize(s));   // host.null_first is read by the copy above
   return true;
 }
 
@@ -2638,47 +2803,45 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   hipStream_t s = stream();
   uint32_t un = uint32_t(n);
   Mapped out;
-  if (type_is_int(type) && n >= (int64_t(1) << 16) && range_path_enabled() && map_range(vals, type, valid, voff, n, track, out))
-    return out;
-  out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
-  DevBuf stb = device_alloc(sizeof(BuildStatus));
-  auto* st = static_cast<BuildStatus*>(stb->ptr);
   // Capacity: about 2.5 slots per distinct key, from a HyperLogLog estimate on inputs large enough
   // for the table size to matter; otherwise two slots per row. Never more than two slots per row, and
-  // grown (eightfold) when a build runs out of probes.
+  // grown (eightfold) when a build runs out of probes. The same pass gives an integer column's span,
+  // which sends a small one to the range path.
   const uint64_t want = std::max<uint64_t>(1024, 2 * uint64_t(n));
   uint64_t full = 1024;
   while (full < want) full <<= 1;
   uint64_t cap = 1024;
   if (n >= (int64_t(1) << 16)) {
+    const bool range = type_is_int(type) && range_path_enabled();
     DevBuf regs = device_alloc(HLL_M * 4);
-    DevBuf est = device_alloc(8);
-    AM_HIP(hipMemsetAsync(regs->ptr, 0, HLL_M * 4, s));
+    DevBuf rsb = device_alloc(sizeof(RangeStat));
+    std::shared_ptr<void> landing = pinned_slot();
+    auto* ks = static_cast<KeyStat*>(landing.get());
     {
-      ProfScope ps("gb_hll");
+      ProfScope ps("gb_key_scan");
       auto* rg = static_cast<uint32_t*>(regs->ptr);
-      const dim3 hg(grid_for(n, 4));
-      switch (type_width(type)) {
-        case 8: hipLaunchKernelGGL(gb_hll<8>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
-        case 4: hipLaunchKernelGGL(gb_hll<4>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
-        case 2: hipLaunchKernelGGL(gb_hll<2>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
-        default: hipLaunchKernelGGL(gb_hll<1>, hg, dim3(256), 0, s, vals, type, valid, voff, un, rg); break;
-      }
-      hipLaunchKernelGGL(gb_hll_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(regs->ptr),
-                         static_cast<unsigned long long*>(est->ptr));
+      auto* rs = static_cast<RangeStat*>(rsb->ptr);
+      hipLaunchKernelGGL(gb_key_init, dim3(1), dim3(256), 0, s, rg, rs);
+      key_scan(type, dim3(grid_for(n, AM_SCAN_BLOCKS_PER_CU)), s, vals, valid, voff, un, rg, rs);
+      hipLaunchKernelGGL(gb_key_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(rg),
+                         static_cast<const RangeStat*>(rs), ks);
       AM_HIP(hipGetLastError());
     }
-    unsigned long long d = 0;
-    AM_HIP(hipMemcpyAsync(&d, est->ptr, 8, hipMemcpyDeviceToHost, s));
     AM_HIP(hipStreamSynchronize(s));
+    const KeyStat stat = *ks;
+    if (range && map_range(vals, type, valid, voff, n, track, stat.r, out)) return out;
+    const unsigned long long d = stat.est;
     const uint64_t target = std::max<uint64_t>(1024, d * 2 + d / 2);
     while (cap < target) cap <<= 1;
   } else {
     while (cap < want && cap < (1ull << 21)) cap <<= 1;
   }
   cap = std::min(cap, full);
+  out.ids = device_alloc(size_t(std::max<int64_t>(n, 1)) * 4);
+  DevBuf stb = device_alloc(sizeof(BuildStatus));
+  auto* st = static_cast<BuildStatus*>(stb->ptr);
   BuildStatus host{};
-  DevBuf tkeys, tfirst, gkeys, gfirst, xmap;
+  DevBuf tkeys, tfirst, gkeys, gfirst, xmap, slots;
   Table T{};          // the table the rows were inserted into (eight private copies, or one)
   Table G{};          // the merged table of distinct keys
   bool priv = false;
@@ -2722,25 +2885,23 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
                          uint32_t(cap), st, G, static_cast<uint32_t*>(xmap->ptr));
       AM_HIP(hipGetLastError());
     }
-    AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
-    AM_HIP(hipStreamSynchronize(s));
+    // Compact the occupied slots before the status is read: a build that ran out of probes left at
+    // most cap + 1 of them, which the compacted arrays hold, and is built again.
+    out.okeys = device_alloc(size_t(cap + 1) * 8);
+    slots = device_alloc(size_t(cap + 1) * 4);
+    {
+      ProfScope ps("gb_compact");
+      uint32_t blocks = uint32_t(std::min<uint64_t>((cap + 1 + 4095) / 4096, uint64_t(device().cu_count) * 4));
+      uint32_t per_block = uint32_t((cap + 1 + blocks - 1) / blocks);
+      hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, G.key, uint32_t(cap), type, st,
+                         static_cast<unsigned long long*>(out.okeys->ptr), static_cast<uint32_t*>(slots->ptr), per_block);
+      AM_HIP(hipGetLastError());
+    }
+    host = read_back(st);
     if (!host.overflow) break;
     if (attempt > 8) fail("group-by: the key table could not be sized");
     cap = std::min<uint64_t>(cap * 8, full);
   }
-  // Compact the occupied slots and sort their keys.
-  out.okeys = device_alloc(size_t(cap + 1) * 8);
-  DevBuf slots = device_alloc(size_t(cap + 1) * 4);
-  {
-    ProfScope ps("gb_compact");
-    uint32_t blocks = uint32_t(std::min<uint64_t>((cap + 1 + 4095) / 4096, uint64_t(device().cu_count) * 4));
-    uint32_t per_block = uint32_t((cap + 1 + blocks - 1) / blocks);
-    hipLaunchKernelGGL(gb_compact, dim3(blocks), dim3(256), 0, s, G.key, uint32_t(cap), type, st,
-                       static_cast<unsigned long long*>(out.okeys->ptr), static_cast<uint32_t*>(slots->ptr), per_block);
-    AM_HIP(hipGetLastError());
-  }
-  AM_HIP(hipMemcpyAsync(&host, st, sizeof host, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
   out.D = host.distinct;
   out.has_null = host.nulls > 0;
   out.K = int64_t(out.D) + (out.has_null ? 1 : 0);
@@ -2774,7 +2935,6 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
                        out.D);
     AM_HIP(hipGetLastError());
   }
-  AM_HIP(hipStreamSynchronize(s));
   return out;
 }
 
@@ -2848,7 +3008,13 @@ GroupBy* group_by_keys_impl(Array* const* columns, int64_t count, bool need_rep)
     }
     if (K > int64_t(INT32_MAX)) fail("group-by: more than 2^31 groups");
     g->K = K;
-    sync();
+    // No wait here: the ids are only read by later calls, which are ordered after the mapping (the
+    // same stream, or `ready`), so the caller's next call is prepared while the last kernels run.
+    if (!in_batch()) {
+      AM_HIP(hipEventCreateWithFlags(&g->ready, hipEventDisableTiming));
+      AM_HIP(hipEventRecord(g->ready, s));
+      g->owner = s;
+    }
   } catch (...) {
     delete g;
     throw;
@@ -2863,6 +3029,8 @@ GroupBy* group_by_keys(Array* const* columns, int64_t count) { return group_by_k
 int64_t group_by_group_count(const GroupBy* g) { return g->K; }
 
 Array* group_by_ids(GroupBy* g) {
+  gb_use(g);
+  sync();   // the returned array is complete, like every other result
   Array* out = new_array("i", T_I32, g->n);
   out->values = g->ids;
   out->null_count = 0;
@@ -2870,6 +3038,7 @@ Array* group_by_ids(GroupBy* g) {
 }
 
 Array* group_by_keys_result(GroupBy* g, int64_t i) {
+  gb_use(g);
   if (i < 0 || i >= int64_t(g->columns.size()))
     fail_invalid("key column " + std::to_string(i) + " of " + std::to_string(g->columns.size()));
   if (g->key_results.empty()) g->key_results.assign(g->columns.size(), nullptr);
@@ -2952,9 +3121,7 @@ Array* finalize_simple(const DevBuf& gtab, int W, long long K, int kind, int vty
       hipLaunchKernelGGL(pack_validity, dim3(grid_for(K, 4)), dim3(256), 0, s, static_cast<const uint8_t*>(vb->ptr), K,
                          static_cast<unsigned long long*>(out->validity->ptr), static_cast<unsigned long long*>(nz->ptr));
       AM_HIP(hipGetLastError());
-      unsigned long long nulls = 0;
-      AM_HIP(hipMemcpyAsync(&nulls, nz->ptr, 8, hipMemcpyDeviceToHost, s));
-      AM_HIP(hipStreamSynchronize(s));
+      const unsigned long long nulls = read_back(static_cast<const unsigned long long*>(nz->ptr));
       out->null_count = int64_t(nulls);
       if (nulls == 0) out->validity.reset();
     } else {
@@ -3012,9 +3179,7 @@ void exact_fsum_t(const AggArgs& base, DevBuf& sum, DevBuf& mean, DevBuf& vb, Pa
                        static_cast<uint32_t*>(redo->ptr), static_cast<uint32_t*>(rc->ptr));
     AM_HIP(hipGetLastError());
   }
-  uint32_t nredo = 0;
-  AM_HIP(hipMemcpyAsync(&nredo, rc->ptr, 4, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
+  const uint32_t nredo = read_back(static_cast<const uint32_t*>(rc->ptr));
   if (nredo == 0) return;
   if (base.n >= (int64_t(1) << 31)) fail("exact float sum: more than 2^31 rows in a group the grid cannot settle");
   DevBuf digits = device_alloc(size_t(nredo) * XD * 8);
@@ -3054,9 +3219,7 @@ Array* bits_array(const DevBuf& vals, const DevBuf& vb, long long K, const std::
       hipLaunchKernelGGL(pack_validity, dim3(grid_for(K, 4)), dim3(256), 0, s, static_cast<const uint8_t*>(vb->ptr), K,
                          static_cast<unsigned long long*>(out->validity->ptr), static_cast<unsigned long long*>(nz->ptr));
       AM_HIP(hipGetLastError());
-      unsigned long long nulls = 0;
-      AM_HIP(hipMemcpyAsync(&nulls, nz->ptr, 8, hipMemcpyDeviceToHost, s));
-      AM_HIP(hipStreamSynchronize(s));
+      const unsigned long long nulls = read_back(static_cast<const unsigned long long*>(nz->ptr));
       out->null_count = int64_t(nulls);
       if (nulls == 0) out->validity.reset();
     } catch (...) {
@@ -3187,6 +3350,7 @@ Array* aggregate(const IdSource& ids, int64_t n, long long K, const Array* v, in
 }  // namespace
 
 Array* group_agg(GroupBy* g, const Array* values, int op, double /*p1*/) {
+  gb_use(g);
   IdSource ids{g->ids->ptr, T_I32, nullptr, 0};
   if (op != 1 && values && values->length != g->n)
     fail_length_mismatch(g->n, values->length);

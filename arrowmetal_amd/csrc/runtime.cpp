@@ -5,7 +5,9 @@
 // CInterop.swift) wraps Apple unified memory, where an imported array is visible to the GPU without a
 // copy. Here host memory and HBM are separate: am_import copies a CPU array into HBM once, and
 // am_import_device takes ROCm device buffers without a copy. Work runs on one HIP stream per calling
-// thread; every call is synchronous unless the thread has a batch open (MetalContext.swift:245-312).
+// thread; every call that returns arrays is synchronous unless the thread has a batch open
+// (MetalContext.swift:245-312). am_group_by_keys returns a handle while its last kernels still run:
+// the calls that use the handle are ordered after them.
 #include "am_internal.hpp"
 
 #include <algorithm>
@@ -325,6 +327,67 @@ void sync() {
   run_deferred();
 }
 
+// ---------------------------------------------------------------- pinned read-backs
+
+namespace {
+
+__global__ void copy_words(const uint32_t* src, uint32_t* dst, uint32_t words) {
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
+// Pinned slots are carved from chunks of 256 and never freed (a few KiB per thread that reads back).
+struct PinnedSlots {
+  std::mutex mu;
+  std::vector<void*> free;
+};
+PinnedSlots& pinned_slots() {
+  static PinnedSlots* p = new PinnedSlots();   // outlives every thread's slots
+  return *p;
+}
+
+void check_read_back(const void* dev, size_t bytes) {
+  if (bytes > PINNED_SLOT || bytes % 4 != 0 || reinterpret_cast<uintptr_t>(dev) % 4 != 0)
+    fail("internal: read-back of " + std::to_string(bytes) + " bytes is not a pinned-slot copy");
+}
+
+}  // namespace
+
+std::shared_ptr<void> pinned_slot() {
+  PinnedSlots& P = pinned_slots();
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.free.empty()) {
+      constexpr size_t CHUNK = 256;
+      char* c = nullptr;
+      AM_HIP(hipHostMalloc(reinterpret_cast<void**>(&c), CHUNK * PINNED_SLOT, hipHostMallocDefault));
+      for (size_t i = 0; i < CHUNK; i++) P.free.push_back(c + i * PINNED_SLOT);
+    }
+    p = P.free.back();
+    P.free.pop_back();
+  }
+  return std::shared_ptr<void>(p, [](void* q) {
+    PinnedSlots& S = pinned_slots();
+    std::lock_guard<std::mutex> g(S.mu);
+    S.free.push_back(q);
+  });
+}
+
+void copy_to_pinned(void* dst, const void* dev, size_t bytes) {
+  check_read_back(dev, bytes);
+  hipLaunchKernelGGL(copy_words, dim3(1), dim3(64), 0, stream(), static_cast<const uint32_t*>(dev),
+                     static_cast<uint32_t*>(dst), uint32_t(bytes / 4));
+  AM_HIP(hipGetLastError());
+}
+
+void read_back(void* host, const void* dev, size_t bytes) {
+  thread_local std::shared_ptr<void> landing;   // one per thread: each read-back waits before returning
+  if (!landing) landing = pinned_slot();
+  copy_to_pinned(landing.get(), dev, bytes);
+  AM_HIP(hipStreamSynchronize(stream()));
+  std::memcpy(host, landing.get(), bytes);
+}
+
 void batch_begin() { t_state.batch_depth++; }
 
 void batch_end() {
@@ -458,10 +521,7 @@ int64_t count_nulls(const Array& a) {
                        static_cast<unsigned long long*>(cnt->ptr));
     AM_HIP(hipGetLastError());
   }
-  unsigned long long z = 0;
-  AM_HIP(hipMemcpyAsync(&z, cnt->ptr, 8, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
-  return int64_t(z);
+  return int64_t(read_back(static_cast<const unsigned long long*>(cnt->ptr)));
 }
 
 // ---------------------------------------------------------------- import / export

@@ -86,9 +86,7 @@ void set_validity(Array* a, const uint8_t* bytes, int64_t n) {
   hipLaunchKernelGGL(sc_pack, dim3(blocks_for(n)), dim3(256), 0, s, bytes, n,
                      static_cast<unsigned long long*>(a->validity->ptr), static_cast<unsigned long long*>(nz->ptr));
   AM_HIP(hipGetLastError());
-  unsigned long long nulls = 0;
-  AM_HIP(hipMemcpyAsync(&nulls, nz->ptr, 8, hipMemcpyDeviceToHost, s));
-  AM_HIP(hipStreamSynchronize(s));
+  const unsigned long long nulls = read_back(static_cast<const unsigned long long*>(nz->ptr));
   a->null_count = int64_t(nulls);
   if (nulls == 0) a->validity.reset();
 }

@@ -127,16 +127,17 @@ Array* take(const Array& a, const Array& ix) {
         default: launch_take<unsigned long long>(a, ix, out, st, s); break;
       }
     }
-    auto host = std::make_shared<TakeStatus>();
-    AM_HIP(hipMemcpyAsync(host.get(), st, sizeof(TakeStatus), hipMemcpyDeviceToHost, s));
+    std::shared_ptr<void> slot = pinned_slot();   // the status lands in pinned memory
+    copy_to_pinned(slot.get(), st, sizeof(TakeStatus));
     int64_t len = a.length;
     Array* res = out;
     retain_array(res);   // the check below may run at the end of a batch
-    after_sync([host, len, res, need_validity] {
+    after_sync([slot, len, res, need_validity] {
       struct Drop {
         Array* a;
         ~Drop() { release_array(a); }
       } drop{res};
+      const TakeStatus* host = static_cast<const TakeStatus*>(slot.get());
       if (host->out_of_range) fail_invalid("take: index out of range (array length " + std::to_string(len) + ")");
       res->null_count = need_validity ? int64_t(host->nulls) : 0;
       if (res->null_count == 0) res->validity.reset();

@@ -151,7 +151,7 @@ ROCPROF_NAME = {
     "sort_first": "sort_first<u64, 8, 16>",
     "sort_scatter": "sort_mid<u64, 20, 0>",
     "sort_scatter_last": "sort_mid<u64, 20, 1>",
-    "gb_hll": "gb_hll<8>",
+    "gb_key_scan": "gb_key_scan<8, true>",
     "gb_build": "gb_build<false, 8>",
     "gb_ids": "gb_ids",
     "gb_part_hist": "gb_part_hist<SumCountP, 4>",
@@ -204,7 +204,7 @@ def orderby_alg_bytes(n: int, m: int):
 def group_by_alg_bytes(n: int):
     """Algorithmic bytes per launch of the group-by kernels over n rows of 8-byte keys and values."""
     return {
-        "gb_hll": (8 * n, "distinct-key estimate: read key; 8 B/row"),
+        "gb_key_scan": (8 * n, "key statistics (distinct-key estimate, span): read key; 8 B/row"),
         "gb_build": (12 * n, "hash insert: read key, write slot; 12 B/row"),
         "gb_ids": (8 * n, "slot to ascending id: read slot, write id; 8 B/row"),
         "gb_part_hist": (4 * n, "partitioned aggregate, bucket counts per tile: read id; 4 B/row"),
@@ -634,7 +634,6 @@ def group_by_leg(args, log):
     msr = 1e3 * float(np.median(tr))
     rawr = profiled(stepr, 3)
     algr = dict(alg)
-    algr["gb_range_minmax"] = (8 * n, "range path: smallest and largest key; read key; 8 B/row")
     algr["gb_range_occ"] = (8 * n, "range path: occupancy of the span; read key; 8 B/row")
     algr["gb_ids"] = (12 * n, "row to ascending id: read key, write id; 12 B/row")
     roofr, rowsr = roofline_of(rawr, algr)

@@ -412,7 +412,7 @@ def test_count_reuse_in_any_op_order(with_nulls):
 
 
 @pytest.mark.parametrize("case", ["small_span", "negative", "nulls", "all_null", "u64_top", "int8", "two_columns",
-                                  "unique_rows"])
+                                  "unique_rows", "gaps", "bits_edge", "bits_over"])
 def test_range_path_equals_hash_path(case, monkeypatch):
     """Integer keys over a small span take the range path (occupancy, scan, rank; GroupByKeys.swift:268-337);
     the same keys through the hash path (ARROWMETAL_GROUPBY_RANGE=0) give the same ids, keys, counts and
@@ -430,6 +430,14 @@ def test_range_path_equals_hash_path(case, monkeypatch):
     elif case == "all_null":
         cols = [rng.integers(0, 9, n).astype(np.int64)]
         valid = [np.zeros(n, bool)]
+    elif case == "gaps":   # a third of the span present, with nulls: ranks are looked up
+        cols = [rng.integers(0, 5000, n).astype(np.int64) * 3 - 7000]
+        valid = [rng.random(n) > 0.1]
+    elif case in ("bits_edge", "bits_over"):   # the span at, and one past, the LDS bitmap's limit
+        span = (1 << 19) + (case == "bits_over")
+        k = rng.integers(0, span, n).astype(np.int64)
+        k[:2] = [0, span - 1]
+        cols = [k + 123]
     elif case == "u64_top":
         cols = [(np.uint64(2**64 - 1000) + rng.integers(0, 1000, n).astype(np.uint64))]
     elif case == "int8":
@@ -451,7 +459,7 @@ def test_range_path_equals_hash_path(case, monkeypatch):
         ids = g.ids().to_numpy()[0]
         am.prof_enable(False)
         names = set(am.prof_report())
-        assert ("gb_range_minmax" in names) == (mode == "1"), names
+        assert ("gb_range_scan" in names) == (mode == "1"), names
         assert g.group_count == K
         np.testing.assert_array_equal(ids, want_ids)
         keys = [k.to_numpy() for k in g.key_arrays()]
