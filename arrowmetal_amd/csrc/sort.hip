@@ -15,7 +15,7 @@
 //   sort_first     the first active pass reads the raw column again, maps keys on the fly, and
 //                  scatters values by digit while NaN and null rows go straight to their final slots
 //   sort_mid       one launch per further active digit; the last active pass writes only the row
-//                  numbers, into the output. Inputs of 2^18 rows and more read the plan back once and
+//                  numbers, into the output. Inputs of 2^16 rows and more read the plan back once and
 //                  launch only the active passes; smaller ones (and graph captures) launch one kernel
 //                  per digit that exits at once when its digit is inactive
 //
@@ -593,13 +593,19 @@ bool run_hybrid(const void* vals, int type, const uint8_t* valid, int64_t voff, 
 // (msd_part's POS_ROWS form); when it cannot (too few rows, a constant top digit, a full bucket list)
 // the call returns false and out_idx holds nothing to use. *value_rows receives m. With key_out (an
 // integer key) the bucket sorts also write the value block of the sorted key column there.
+#ifndef AM_HYBRID_MIN_LOG
+#define AM_HYBRID_MIN_LOG 16
+#endif
+// Rows from which a 64-bit key takes the hybrid sort (and the plan is read back).
+constexpr uint32_t HYBRID_MIN = 1u << AM_HYBRID_MIN_LOG;
+
 template <typename K, int W, int ITEMS>
 bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n, bool desc,
               int placement, int forder, uint32_t* out_idx, hipStream_t s, uint32_t* pos_of_row = nullptr,
               uint32_t* value_rows = nullptr, uint32_t* bucket_base = nullptr, unsigned long long* key_out = nullptr,
               unsigned long long key_xor = 0) {
   constexpr int P = int(sizeof(K));
-  if (pos_of_row && !(P == 8 && n >= (1u << 18) && !capturing(s))) return false;
+  if (pos_of_row && !(P == 8 && n >= HYBRID_MIN && !capturing(s))) return false;
   constexpr int TILE = BLOCK * ITEMS;
   // The first pass keeps at most 16 keys per thread: its raw-column loads and 258-bin classification
   // need more registers, and 16 keeps it at three workgroups per CU.
@@ -624,7 +630,7 @@ bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     AM_HIP(hipGetLastError());
   }
   // Large 64-bit inputs may take the hybrid sort: the upsweep then also counts its units.
-  const bool hybrid_ok = P == 8 && n >= (1u << 18) && !capturing(s);
+  const bool hybrid_ok = P == 8 && n >= HYBRID_MIN && !capturing(s);
   const uint32_t unit_shift = n >= (1u << 22) ? UNIT_SHIFT_LARGE : UNIT_SHIFT_SMALL;
   const uint32_t unit_rows = 1u << unit_shift;
   const uint32_t nunits = hybrid_ok ? (n + unit_rows - 1) / unit_rows : 0u;
@@ -661,7 +667,7 @@ bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
   // Large inputs read the plan back once (one sync, as the reference's analysis readback does), so
   // that only active passes are launched, over the value rows only.
   struct Plan { uint32_t mask, first, last, m; } plan{0xFFu, 0, uint32_t(P - 1), n};
-  const bool analyse = n >= (1u << 18) && !capturing(s);
+  const bool analyse = n >= HYBRID_MIN && !capturing(s);
   if (analyse) {
     Plan* hp = reinterpret_cast<Plan*>(host_plan_words());
     hipLaunchKernelGGL(sort_read_plan, dim3(1), dim3(1), 0, s, ctl, reinterpret_cast<uint32_t*>(hp));
