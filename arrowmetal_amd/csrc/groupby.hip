@@ -471,9 +471,21 @@ __global__ __launch_bounds__(256) void gb_key_estimate(const uint32_t* regs, con
 // large share of all table reads, on one L2 channel each.
 constexpr int BUILD_ROWS = 8;
 #ifndef AM_BUILD_CACHE
-#define AM_BUILD_CACHE 2048
+#define AM_BUILD_CACHE 8192
 #endif
 constexpr uint32_t BUILD_CACHE = AM_BUILD_CACHE;   // a power of two
+// Threads per workgroup of the build and of gb_ids (the same geometry, so that a row's id lookup runs
+// on the XCD whose table the row went into), and workgroups per CU. One workgroup of 16 waves per CU
+// shares one 96 KiB cache: a key is missed once per CU rather than once per 256 threads, and the 8192
+// entries hold more of a skewed key set (50M rows: 0.45 -> 0.41 ms with 100k Zipf keys, 0.29 -> 0.20
+// ms with 200 keys).
+#ifndef AM_BUILD_THREADS
+#define AM_BUILD_THREADS 1024
+#endif
+#ifndef AM_BUILD_BLOCKS_PER_CU
+#define AM_BUILD_BLOCKS_PER_CU 1
+#endif
+constexpr uint32_t BUILD_THREADS = AM_BUILD_THREADS;
 
 // Caches key -> slot in the workgroup's LDS if the key's entry is still free. The slot is written after
 // the key; a reader that finds the key with no slot yet reads the table instead.
@@ -599,7 +611,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
 }
 
 template <bool TRACK_FIRST, int W>
-__global__ __launch_bounds__(256) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
+__global__ __launch_bounds__(BUILD_THREADS) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st, int private_tables) {
   const uint32_t gsize = gridDim.x * blockDim.x;
@@ -753,7 +765,7 @@ __global__ void gb_relabel(const uint32_t* perm, const uint32_t* slots, uint32_t
 }
 
 #ifndef AM_IDS_CACHE_BITS
-#define AM_IDS_CACHE_BITS 11
+#define AM_IDS_CACHE_BITS 13
 #endif
 constexpr int IDS_CACHE_BITS = AM_IDS_CACHE_BITS;
 constexpr uint32_t IDS_CACHE = 1u << IDS_CACHE_BITS;   // (slot * 2654435761) >> (32 - bits) indexes it
@@ -806,7 +818,7 @@ __device__ __forceinline__ void ids_step(uint32_t* slot_of, uint32_t n, const ui
 // of hot groups do not all read the same few lines of the id table. Two steps' slots take turns: the
 // next step's are loaded before this step's ids are looked up and stored. A step only reads slots that
 // no earlier step rewrites (each row is its own thread's), so the prefetch never sees an id.
-__global__ __launch_bounds__(256) void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot,
+__global__ __launch_bounds__(BUILD_THREADS) void gb_ids(uint32_t* slot_of, uint32_t n, const uint32_t* gid_of_slot,
                                               uint32_t null_gid) {
   __shared__ unsigned long long cache[IDS_CACHE];
   for (uint32_t j = threadIdx.x; j < IDS_CACHE; j += blockDim.x) cache[j] = 0ull;
@@ -2790,10 +2802,10 @@ void launch_build(int w, int grid, hipStream_t s, const void* vals, int type, co
                   uint32_t n, Table T, uint32_t mask, uint32_t budget, uint32_t* slot_of, BuildStatus* st, bool priv) {
   const int p = priv ? 1 : 0;
   switch (w) {
-    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(256), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
   }
 }
 
@@ -2845,7 +2857,8 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   Table T{};          // the table the rows were inserted into (eight private copies, or one)
   Table G{};          // the merged table of distinct keys
   bool priv = false;
-  const int build_grid = grid_for((n + BUILD_ROWS - 1) / BUILD_ROWS, 8);
+  const int build_grid = int(std::max<int64_t>(1, std::min<int64_t>((n + BUILD_ROWS * BUILD_THREADS - 1) / (BUILD_ROWS * BUILD_THREADS),
+                                                                     int64_t(device().cu_count) * AM_BUILD_BLOCKS_PER_CU)));
   for (int attempt = 0;; attempt++) {
     bool last_try = cap >= full;
     // Tables up to 2^20 slots are private to each XCD (eight copies): a key line is then only changed
@@ -2931,7 +2944,7 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
   }
   if (n > 0) {
     ProfScope ps("gb_ids");
-    hipLaunchKernelGGL(gb_ids, dim3(build_grid), dim3(256), 0, s, static_cast<uint32_t*>(out.ids->ptr), un, row_gid,
+    hipLaunchKernelGGL(gb_ids, dim3(build_grid), dim3(BUILD_THREADS), 0, s, static_cast<uint32_t*>(out.ids->ptr), un, row_gid,
                        out.D);
     AM_HIP(hipGetLastError());
   }
