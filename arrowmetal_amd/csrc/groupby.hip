@@ -248,8 +248,13 @@ __global__ __launch_bounds__(256) void gb_key_init(uint32_t* regs, RangeStat* st
 // branch where it would be waited for alone.
 constexpr int SCAN_ROWS = 8;
 #ifndef AM_SCAN_BLOCKS_PER_CU
-#define AM_SCAN_BLOCKS_PER_CU 4
+#define AM_SCAN_BLOCKS_PER_CU 1
 #endif
+#ifndef AM_SCAN_THREADS
+#define AM_SCAN_THREADS 1024
+#endif
+constexpr uint32_t SCAN_THREADS = AM_SCAN_THREADS;   // threads per workgroup of the key scan
+constexpr uint32_t SCAN_WAVES = SCAN_THREADS / 64;
 #ifndef AM_OCC_BLOCKS_PER_CU
 #define AM_OCC_BLOCKS_PER_CU 2
 #endif
@@ -369,12 +374,12 @@ __device__ __forceinline__ void scan_rows(int64_t voff, uint32_t n, uint64_t row
 }
 
 template <int W, int TYPE, bool HAS_VALID>
-__global__ __launch_bounds__(256) void gb_key_scan(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+__global__ __launch_bounds__(SCAN_THREADS) void gb_key_scan(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
                                                    uint32_t* regs, RangeStat* st) {
   constexpr bool RANGE = TYPE >= T_I8 && TYPE <= T_U64;
   __shared__ uint32_t r[HLL_M];
-  __shared__ unsigned long long s_lo[4], s_hi[4], s_nz[4];
-  __shared__ uint32_t s_nf[4];
+  __shared__ unsigned long long s_lo[SCAN_WAVES], s_hi[SCAN_WAVES], s_nz[SCAN_WAVES];
+  __shared__ uint32_t s_nf[SCAN_WAVES];
   for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) r[j] = 0u;
   __syncthreads();
   const uint32_t gsize = gridDim.x * blockDim.x;
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(256) void gb_key_scan(const void* vals, const uint8
   }
   __syncthreads();
   if (RANGE && threadIdx.x == 0) {
-    for (int k = 1; k < 4; k++) {
+    for (uint32_t k = 1; k < SCAN_WAVES; k++) {
       acc.lo = s_lo[k] < acc.lo ? s_lo[k] : acc.lo;
       acc.hi = s_hi[k] > acc.hi ? s_hi[k] : acc.hi;
       acc.nz += s_nz[k];
@@ -424,9 +429,9 @@ void key_scan(int type, dim3 grid, hipStream_t s, const void* vals, const uint8_
   with_key_type(type, [&](auto tc) {
     constexpr int TT = decltype(tc)::value;
     if (valid)
-      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, true>), grid, dim3(256), 0, s, vals, valid, voff, n, regs, st);
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, true>), grid, dim3(SCAN_THREADS), 0, s, vals, valid, voff, n, regs, st);
     else
-      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, false>), grid, dim3(256), 0, s, vals, valid, voff, n, regs, st);
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, false>), grid, dim3(SCAN_THREADS), 0, s, vals, valid, voff, n, regs, st);
   });
 }
 
@@ -2834,7 +2839,9 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
       auto* rg = static_cast<uint32_t*>(regs->ptr);
       auto* rs = static_cast<RangeStat*>(rsb->ptr);
       hipLaunchKernelGGL(gb_key_init, dim3(1), dim3(256), 0, s, rg, rs);
-      key_scan(type, dim3(grid_for(n, AM_SCAN_BLOCKS_PER_CU)), s, vals, valid, voff, un, rg, rs);
+      const int64_t scan_blocks = (n + int64_t(SCAN_THREADS) * SCAN_ROWS - 1) / (int64_t(SCAN_THREADS) * SCAN_ROWS);
+      key_scan(type, dim3(uint32_t(std::max<int64_t>(1, std::min<int64_t>(scan_blocks, int64_t(device().cu_count) * AM_SCAN_BLOCKS_PER_CU)))),
+               s, vals, valid, voff, un, rg, rs);
       hipLaunchKernelGGL(gb_key_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(rg),
                          static_cast<const RangeStat*>(rs), ks);
       AM_HIP(hipGetLastError());

@@ -95,40 +95,51 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
   uint32_t end = min(n, begin + rows_per_block);
   K o = 0, a = K(~K(0));
   uint32_t cm = 0, cq = 0, cz = 0;
-  // UP_ITEMS rows per thread per step, all loaded before any is counted, so that each wave keeps
-  // several loads in flight.
+  // UP_ITEMS rows per thread per step, loaded before any is counted; two steps' rows take turns, so that
+  // the next step's loads are in flight while this step is counted, and every loaded register is taken
+  // at the top of a step (a load the compiler moved into one of the classification's branches would be
+  // waited for there alone). The step count is the same for every thread of the block.
   constexpr int UP_ITEMS = 8;
-  for (uint32_t base = begin; base < end; base += BLOCK * UP_ITEMS) {
+  constexpr uint32_t UP_STEP = BLOCK * UP_ITEMS;
+  const int steps = end > begin ? int((end - begin + UP_STEP - 1) / UP_STEP) : 0;
+  uint64_t raw_a[UP_ITEMS], raw_b[UP_ITEMS];
+  uint32_t vb_a[UP_ITEMS], vb_b[UP_ITEMS];
+  auto load_step = [&](uint64_t base, uint64_t (&raw)[UP_ITEMS], uint32_t (&vb)[UP_ITEMS]) {
+#pragma unroll
+    for (int u = 0; u < UP_ITEMS; u++) {   // rows past the end read the last row
+      const uint32_t i = uint32_t(std::min<uint64_t>(base + uint64_t(u) * BLOCK + threadIdx.x, uint64_t(end) - 1u));
+      uint32_t bit;
+      raw[u] = load_raw<W>(vals, i);
+      vb[u] = load_valid_byte(valid, voff, i, bit);
+    }
+  };
+  auto count_step = [&](uint64_t base, const uint64_t (&raw)[UP_ITEMS], const uint32_t (&vb)[UP_ITEMS]) {
+#pragma unroll
+    for (int u = 0; u < UP_ITEMS; u++) asm volatile("" ::"v"(raw[u]), "v"(vb[u]));
     K kk[UP_ITEMS];
     int cc[UP_ITEMS];
-    uint64_t raw[UP_ITEMS];
-    uint32_t vb[UP_ITEMS], bit[UP_ITEMS];
-#pragma unroll
-    for (int u = 0; u < UP_ITEMS; u++) {   // every load first (rows past the end read the last row)
-      const uint32_t i = min(base + uint32_t(u) * BLOCK + threadIdx.x, end - 1u);
-      raw[u] = load_raw<W>(vals, i);
-      vb[u] = load_valid_byte(valid, voff, i, bit[u]);
-    }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
-      const uint32_t i = base + uint32_t(u) * BLOCK + threadIdx.x;
+      const uint64_t i = base + uint64_t(u) * BLOCK + threadIdx.x;
+      const uint32_t ic = uint32_t(std::min<uint64_t>(i, uint64_t(end) - 1u));
+      const uint32_t bit = valid ? uint32_t((voff + int64_t(ic)) & 7) : 0u;
       kk[u] = 0;
-      cc[u] = i < end ? classify_raw<K>(raw[u], (vb[u] >> bit[u]) & 1u, type, forder, desc, kk[u]) : 3;
+      cc[u] = i < end ? classify_raw<K>(raw[u], (vb[u] >> bit) & 1u, type, forder, desc, kk[u]) : 3;
     }
 #pragma unroll
     for (int u = 0; u < UP_ITEMS; u++) {
-      const uint32_t ui = (base + uint32_t(u) * BLOCK + threadIdx.x - begin) >> unit_shift;
+      const uint32_t ui = uint32_t(base + uint64_t(u) * BLOCK + threadIdx.x - begin) >> unit_shift;
       if (cc[u] == 0) {
         K k = kk[u];
         cm++;
         o |= k;
         a &= k;
-        // all_digits == 0 (the hybrid sort's analysis): only the top two digits' histograms.
+        // all_digits == 0 (the hybrid sort's analysis): only the top digit's histogram, per unit (the
+        // levels count the lower digits per bucket; the LSD passes, if they run instead, analyse again).
         if (all_digits) {
 #pragma unroll
-          for (int p = 0; p < P - 2; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
+          for (int p = 0; p < P - 1; p++) atomicAdd(&h[p][uint32_t(k >> (8 * p)) & 0xFFu], 1u);
         }
-        if (P >= 2) atomicAdd(&h[P - 2][uint32_t(k >> (8 * (P - 2))) & 0xFFu], 1u);
         const uint32_t top = uint32_t(k >> (8 * (P - 1))) & 0xFFu;
         atomicAdd(units ? &hu[ui & (UNITS_PER_BLOCK - 1)][top] : &h[P - 1][top], 1u);
       } else if (cc[u] == 1) {
@@ -138,6 +149,17 @@ __global__ __launch_bounds__(BLOCK) void sort_upsweep(const void* vals, const ui
         cz++;
         if (units) atomicAdd(&hu[ui & (UNITS_PER_BLOCK - 1)][BIN_NULL], 1u);
       }
+    }
+  };
+  if (steps > 0) {
+    load_step(begin, raw_a, vb_a);
+    for (int t = 0; t < steps; t += 2) {
+      const uint64_t base = uint64_t(begin) + uint64_t(t) * UP_STEP;
+      load_step(base + UP_STEP, raw_b, vb_b);
+      count_step(base, raw_a, vb_a);
+      if (t + 1 >= steps) break;
+      load_step(base + 2 * UP_STEP, raw_a, vb_a);
+      count_step(base + UP_STEP, raw_b, vb_b);
     }
   }
   // Wave reductions of the counters and the OR / AND, then one LDS atomic per wave.
@@ -641,7 +663,7 @@ bool run_sort(const void* vals, int type, const uint8_t* valid, int64_t voff, ui
     unit_hist = static_cast<uint32_t*>(units_buf->ptr);
   }
   // The upsweep and the plan. With units (the hybrid sort's analysis) the upsweep counts only the top
-  // two digits; if the LSD passes run after all, the analysis is redone with every digit.
+  // digit; if the LSD passes run after all, the analysis is redone with every digit.
   auto analyse_column = [&](bool units) {
     uint32_t max_blocks = uint32_t(device().cu_count) * 4u;
     uint32_t blocks = std::max(1u, std::min(max_blocks, (n + 8191u) / 8192u));
