@@ -151,7 +151,7 @@ ROCPROF_NAME = {
     "sort_first": "sort_first<u64, 8, 16>",
     "sort_scatter": "sort_mid<u64, 20, 0>",
     "sort_scatter_last": "sort_mid<u64, 20, 1>",
-    "gb_key_scan": "gb_key_scan<8, true>",
+    "gb_key_scan": "gb_key_scan<8, 3, false>",
     "gb_build": "gb_build<false, 8>",
     "gb_ids": "gb_ids",
     "gb_part_hist": "gb_part_hist<SumCountP, 4>",
