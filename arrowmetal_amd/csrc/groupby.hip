@@ -1768,7 +1768,9 @@ template <class P, int IDW, int VW, bool ROWS>
 __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, uint32_t ntiles, const uint32_t* off,
                                                        const uint32_t* base, uint32_t* rec_gid,
                                                        unsigned long long* rec_val, uint32_t* rec_row) {
-  __shared__ uint32_t s_cnt[PART_MAX_NB], s_off[PART_MAX_NB], s_dst[PART_MAX_NB];
+  // s_cnt holds the tile's count per bucket, then (scanned in place) its offset: with 52 KiB of LDS in
+  // all, three workgroups fit on a CU.
+  __shared__ uint32_t s_cnt[PART_MAX_NB], s_dst[PART_MAX_NB];
   __shared__ uint32_t s_gid[PART_TILE];
   __shared__ unsigned long long s_val[VW ? PART_TILE : 1];
   __shared__ uint32_t s_row[ROWS ? PART_TILE : 1];
@@ -1815,14 +1817,14 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
   uint32_t wbase = 0;
   for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) wbase += s_wsum[w];
   const uint32_t excl = wbase + incl - x;
-  if (b0 < nb) s_off[b0] = excl;
-  if (b1 < nb) s_off[b1] = excl + c0;
+  if (b0 < nb) s_cnt[b0] = excl;   // this thread's own two counts were read before the barrier above
+  if (b1 < nb) s_cnt[b1] = excl + c0;
   const uint32_t tile_count = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PART_ROWS; k++) {
     if (gs[k] < 0) continue;
-    const uint32_t p = s_off[uint32_t(gs[k]) >> PART_LOG_G] + rank[k];
+    const uint32_t p = s_cnt[uint32_t(gs[k]) >> PART_LOG_G] + rank[k];
     s_gid[p] = uint32_t(gs[k]);
     if constexpr (VW != 0) s_val[p] = vr[k];
     if constexpr (ROWS) s_row[p] = uint32_t(t0 + int64_t(k) * blockDim.x + threadIdx.x);
@@ -1831,7 +1833,7 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
   for (uint32_t p = threadIdx.x; p < tile_count; p += blockDim.x) {
     const uint32_t g = s_gid[p];
     const uint32_t b = g >> PART_LOG_G;
-    const uint32_t dst = s_dst[b] + (p - s_off[b]);
+    const uint32_t dst = s_dst[b] + (p - s_cnt[b]);
     rec_gid[dst] = g;
     if constexpr (VW != 0) rec_val[dst] = s_val[p];
     if constexpr (ROWS) rec_row[dst] = s_row[p];
