@@ -1841,6 +1841,10 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
 }
 
 constexpr int PART_RED_ROWS = 4;   // records per thread per step of the reduce
+#ifndef AM_RED_THREADS
+#define AM_RED_THREADS 1024
+#endif
+constexpr uint32_t RED_THREADS = AM_RED_THREADS;   // threads per workgroup of the reduce
 #ifndef AM_PART_PEEL
 #define AM_PART_PEEL 16
 #endif
@@ -1957,7 +1961,7 @@ __device__ __forceinline__ void part_apply(const AggArgs& a, int64_t r, int64_t 
 // Workgroup c reduces chunk c - first[b] of bucket b into part[c] (PART_G x W words, identities where
 // no record fell). Workgroups past the last chunk do nothing.
 template <class P, int VW, bool ROWS>
-__global__ __launch_bounds__(512) void gb_part_reduce(AggArgs a, uint32_t nb, const uint32_t* base, const uint32_t* first,
+__global__ __launch_bounds__(RED_THREADS) void gb_part_reduce(AggArgs a, uint32_t nb, const uint32_t* base, const uint32_t* first,
                                                       PartIn in, unsigned long long* part) {
   extern __shared__ unsigned long long lds[];
   const uint32_t c = blockIdx.x;
@@ -2172,7 +2176,7 @@ bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
         if (shmem > 65536)
           AM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      int(shmem)));
-        hipLaunchKernelGGL(kern, dim3(ps_.chunk_bound), dim3(512), shmem, s, a, nb, base, first, in,
+        hipLaunchKernelGGL(kern, dim3(ps_.chunk_bound), dim3(RED_THREADS), shmem, s, a, nb, base, first, in,
                            static_cast<unsigned long long*>(part->ptr));
       };
       if (vw) {
