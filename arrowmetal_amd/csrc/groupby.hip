@@ -1667,16 +1667,22 @@ __device__ __forceinline__ uint32_t part_rank(uint32_t* s_cnt, bool valid, uint3
   return valid ? atomicAdd(&s_cnt[b], 1u) : 0u;
 }
 
+#ifndef AM_HIST_THREADS
+#define AM_HIST_THREADS 256
+#endif
+constexpr uint32_t HIST_THREADS = AM_HIST_THREADS;          // threads per workgroup of the histogram
+constexpr int HIST_ROWS = int(PART_TILE / HIST_THREADS);    // its rows per thread (one tile per workgroup)
+
 template <class P, int IDW>
-__global__ __launch_bounds__(256) void gb_part_hist(AggArgs a, uint32_t nb, uint32_t ntiles, uint32_t* hist) {
+__global__ __launch_bounds__(HIST_THREADS) void gb_part_hist(AggArgs a, uint32_t nb, uint32_t ntiles, uint32_t* hist) {
   __shared__ uint32_t s_cnt[PART_MAX_NB];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) s_cnt[b] = 0u;
   __syncthreads();
   const int64_t t0 = int64_t(blockIdx.x) * PART_TILE;
-  unsigned long long idr[PART_ROWS];
-  uint32_t ib[PART_ROWS], vb[PART_ROWS];
+  unsigned long long idr[HIST_ROWS];
+  uint32_t ib[HIST_ROWS], vb[HIST_ROWS];
 #pragma unroll
-  for (int k = 0; k < PART_ROWS; k++) {   // every load first; rows past the end read the last row
+  for (int k = 0; k < HIST_ROWS; k++) {   // every load first; rows past the end read the last row
     const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
     const int64_t ii = i < a.n ? i : a.n - 1;
     idr[k] = ld_raw<IDW>(a.ids.ids, ii);
@@ -1684,7 +1690,7 @@ __global__ __launch_bounds__(256) void gb_part_hist(AggArgs a, uint32_t nb, uint
     vb[k] = P::SKIP_NULL ? ld_vbyte(a.v.valid, a.v.voff, ii) : 0xFFu;
   }
 #pragma unroll
-  for (int k = 0; k < PART_ROWS; k++) {
+  for (int k = 0; k < HIST_ROWS; k++) {
     const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
     const long long g = part_gid<P, IDW>(a, i, a.n, idr[k], ib[k], vb[k]);
     (void)part_rank(s_cnt, g >= 0, uint32_t(g) >> PART_LOG_G);
@@ -1764,27 +1770,34 @@ __global__ __launch_bounds__(1024) void gb_part_bases(const uint32_t* total, uin
 // families that read no value). Within a tile the records are ranked per bucket in LDS (their order
 // does not matter to any family here) and staged in bucket order, then written out in order: the run
 // of bucket b starts at base[b] + off[b][tile].
+#ifndef AM_SCAT_THREADS
+#define AM_SCAT_THREADS 1024
+#endif
+constexpr uint32_t SCAT_THREADS = AM_SCAT_THREADS;              // threads per workgroup of the scatter
+constexpr int SCAT_ROWS = int(PART_TILE / SCAT_THREADS);        // its rows per thread (one tile per workgroup)
+constexpr uint32_t SCAT_PER = (PART_MAX_NB + SCAT_THREADS - 1) / SCAT_THREADS;   // buckets per thread in its scan
+
 template <class P, int IDW, int VW, bool ROWS>
-__global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, uint32_t ntiles, const uint32_t* off,
+__global__ __launch_bounds__(SCAT_THREADS) void gb_part_scatter(AggArgs a, uint32_t nb, uint32_t ntiles, const uint32_t* off,
                                                        const uint32_t* base, uint32_t* rec_gid,
                                                        unsigned long long* rec_val, uint32_t* rec_row) {
-  // s_cnt holds the tile's count per bucket, then (scanned in place) its offset: with 52 KiB of LDS in
-  // all, three workgroups fit on a CU.
+  // s_cnt holds the tile's count per bucket, then (scanned in place) its offset: 52 KiB of LDS in all,
+  // so that two 1024-thread workgroups (the CU's 32 waves) fit on a CU.
   __shared__ uint32_t s_cnt[PART_MAX_NB], s_dst[PART_MAX_NB];
   __shared__ uint32_t s_gid[PART_TILE];
   __shared__ unsigned long long s_val[VW ? PART_TILE : 1];
   __shared__ uint32_t s_row[ROWS ? PART_TILE : 1];
-  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_wsum[SCAT_THREADS / 64];
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     s_cnt[b] = 0u;
     s_dst[b] = base[b] + off[size_t(b) * ntiles + blockIdx.x];
   }
   __syncthreads();
   const int64_t t0 = int64_t(blockIdx.x) * PART_TILE;
-  unsigned long long idr[PART_ROWS], vr[PART_ROWS];
-  uint32_t ib[PART_ROWS], vb[PART_ROWS];
+  unsigned long long idr[SCAT_ROWS], vr[SCAT_ROWS];
+  uint32_t ib[SCAT_ROWS], vb[SCAT_ROWS];
 #pragma unroll
-  for (int k = 0; k < PART_ROWS; k++) {
+  for (int k = 0; k < SCAT_ROWS; k++) {
     const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
     const int64_t ii = i < a.n ? i : a.n - 1;
     idr[k] = ld_raw<IDW>(a.ids.ids, ii);
@@ -1793,19 +1806,26 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
     if constexpr (VW != 0) vr[k] = ld_raw<VW>(a.v.vals, ii);
     vb[k] = P::SKIP_NULL ? ld_vbyte(a.v.valid, a.v.voff, ii) : 0xFFu;
   }
-  long long gs[PART_ROWS];
-  uint32_t rank[PART_ROWS];
+  long long gs[SCAT_ROWS];
+  uint32_t rank[SCAT_ROWS];
 #pragma unroll
-  for (int k = 0; k < PART_ROWS; k++) {
+  for (int k = 0; k < SCAT_ROWS; k++) {
     const int64_t i = t0 + int64_t(k) * blockDim.x + threadIdx.x;
     gs[k] = part_gid<P, IDW>(a, i, a.n, idr[k], ib[k], vb[k]);
     rank[k] = part_rank(s_cnt, gs[k] >= 0, uint32_t(gs[k]) >> PART_LOG_G);
   }
   __syncthreads();
-  // Exclusive scan of the tile's bucket counts (nb <= 512: two per thread, then the four waves).
-  const uint32_t b0 = 2u * threadIdx.x, b1 = b0 + 1u;
-  const uint32_t c0 = b0 < nb ? s_cnt[b0] : 0u, c1 = b1 < nb ? s_cnt[b1] : 0u;
-  uint32_t x = c0 + c1, incl = x;
+  // Exclusive scan of the tile's bucket counts (nb <= 512: SCAT_PER consecutive buckets per thread,
+  // then the waves).
+  uint32_t c[SCAT_PER];
+  uint32_t x = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < SCAT_PER; q++) {
+    const uint32_t bq = SCAT_PER * threadIdx.x + q;
+    c[q] = bq < nb ? s_cnt[bq] : 0u;
+    x += c[q];
+  }
+  uint32_t incl = x;
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -1814,15 +1834,21 @@ __global__ __launch_bounds__(256) void gb_part_scatter(AggArgs a, uint32_t nb, u
   }
   if (lane == 63) s_wsum[threadIdx.x >> 6] = incl;
   __syncthreads();
-  uint32_t wbase = 0;
-  for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) wbase += s_wsum[w];
-  const uint32_t excl = wbase + incl - x;
-  if (b0 < nb) s_cnt[b0] = excl;   // this thread's own two counts were read before the barrier above
-  if (b1 < nb) s_cnt[b1] = excl + c0;
-  const uint32_t tile_count = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+  uint32_t wbase = 0, tile_count = 0;
+  for (uint32_t w = 0; w < SCAT_THREADS / 64; w++) {
+    wbase += w < (threadIdx.x >> 6) ? s_wsum[w] : 0u;
+    tile_count += s_wsum[w];
+  }
+  uint32_t run = wbase + incl - x;   // this thread's own counts were read before the barrier above
+#pragma unroll
+  for (uint32_t q = 0; q < SCAT_PER; q++) {
+    const uint32_t bq = SCAT_PER * threadIdx.x + q;
+    if (bq < nb) s_cnt[bq] = run;
+    run += c[q];
+  }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < PART_ROWS; k++) {
+  for (int k = 0; k < SCAT_ROWS; k++) {
     if (gs[k] < 0) continue;
     const uint32_t p = s_cnt[uint32_t(gs[k]) >> PART_LOG_G] + rank[k];
     s_gid[p] = uint32_t(gs[k]);
@@ -2120,8 +2146,8 @@ bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
       uint32_t* rr = static_cast<uint32_t*>(ps_.row->ptr);
       {
         ProfScope ps("gb_part_hist");
-        if (idw == 8) hipLaunchKernelGGL((gb_part_hist<P, 8>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
-        else hipLaunchKernelGGL((gb_part_hist<P, 4>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h);
+        if (idw == 8) hipLaunchKernelGGL((gb_part_hist<P, 8>), dim3(ntiles), dim3(HIST_THREADS), 0, s, a, nb, ntiles, h);
+        else hipLaunchKernelGGL((gb_part_hist<P, 4>), dim3(ntiles), dim3(HIST_THREADS), 0, s, a, nb, ntiles, h);
         AM_HIP(hipGetLastError());
       }
       {
@@ -2136,10 +2162,10 @@ bool run_agg_part(const AggArgs& a, hipStream_t s, const char* name, unsigned lo
 #define AM_PART_SCATTER(IW, VV)                                                                                  \
   do {                                                                                                           \
     if (ps_.rows)                                                                                                \
-      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, true>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h,    \
+      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, true>), dim3(ntiles), dim3(SCAT_THREADS), 0, s, a, nb, ntiles, h, \
                          base, rg, rv, rr);                                                                      \
     else                                                                                                         \
-      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, false>), dim3(ntiles), dim3(256), 0, s, a, nb, ntiles, h,   \
+      hipLaunchKernelGGL((gb_part_scatter<P, IW, VV, false>), dim3(ntiles), dim3(SCAT_THREADS), 0, s, a, nb, ntiles, h, \
                          base, rg, rv, rr);                                                                      \
   } while (0)
         if (idw == 8) {
