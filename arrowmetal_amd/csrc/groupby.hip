@@ -506,7 +506,9 @@ struct BuildRows {
   uint32_t vb[BUILD_ROWS];
 };
 
-template <int W>
+// KT: the key type when fixed at compile time (the common 64-bit integer key), or -1 for the
+// column's type at run time; NV: the column has no validity bitmap (no byte is loaded).
+template <int W, bool NV>
 __device__ __forceinline__ void build_load(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
                                            uint64_t base, uint32_t gsize, uint32_t tid, BuildRows& x) {
 #pragma unroll
@@ -514,7 +516,7 @@ __device__ __forceinline__ void build_load(const void* vals, const uint8_t* vali
     const uint64_t iu = base + uint64_t(r) * gsize + tid;
     const int64_t ii = int64_t(iu < uint64_t(n) ? iu : uint64_t(n) - 1u);
     x.k[r] = ld_raw<W>(vals, ii);
-    x.vb[r] = ld_vbyte(valid, voff, ii);
+    x.vb[r] = NV ? 0xFFu : ld_vbyte(valid, voff, ii);
   }
 }
 
@@ -534,7 +536,7 @@ struct BuildState {
 // next step's rows are loaded (into `nxt`, at `nbase`), so that those loads are in flight while this
 // step waits for its slots, and waiting for the slots never waits for them (memory operations
 // complete in issue order).
-template <bool TRACK_FIRST, int W>
+template <bool TRACK_FIRST, int W, int KT, bool NV>
 __device__ __forceinline__ void build_step(const void* vals, int type, const uint8_t* valid, int64_t voff, uint32_t n,
                                            uint64_t base, uint64_t nbase, uint32_t gsize, uint32_t tid,
                                            const BuildRows& cur, BuildRows& nxt, uint32_t* slot_of, BuildStatus* st,
@@ -547,14 +549,14 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
 #pragma unroll
   for (int r = 0; r < BUILD_ROWS; r++) {
     const uint64_t i = base + uint64_t(r) * gsize + tid;
-    const bool vok = vbit(voff, int64_t(i < n ? i : uint64_t(n) - 1u), cur.vb[r]) != 0u;
+    const bool vok = NV || vbit(voff, int64_t(i < n ? i : uint64_t(n) - 1u), cur.vb[r]) != 0u;
     cat[r] = 3;
     k[r] = 0;
     if (i < n) {
       if (!vok) {
         cat[r] = 1;
       } else {
-        k[r] = key_of_raw(cur.k[r], type);
+        k[r] = key_of_raw(cur.k[r], KT >= 0 ? KT : type);
         cat[r] = k[r] == EMPTY ? 2 : 0;
       }
     }
@@ -572,7 +574,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
   }
 #pragma unroll
   for (int r = 0; r < BUILD_ROWS; r++) cv[r] = cat[r] == 0 ? probe_load(&b.tkey[s[r]]) : 0ull;
-  build_load<W>(vals, valid, voff, n, nbase, gsize, tid, nxt);
+  build_load<W, NV>(vals, valid, voff, n, nbase, gsize, tid, nxt);
   // Rows whose key already sits in its home slot (most rows of a repeated key) finish here; the
   // others (a first insert or a collision) go through the probe loop one at a time below, with the
   // key read again, so that the loop is not inlined per row.
@@ -605,7 +607,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
     const int r = __builtin_ctz(slow);
     slow &= slow - 1u;
     const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
-    const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), type);
+    const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), KT >= 0 ? KT : type);
     const uint32_t ss = uint32_t(mix64(kk)) & b.cap_mask;
     const uint32_t slot = probe_insert(b.tkey, kk, ss, probe_load(&b.tkey[ss]), b.cap_mask, b.budget);
     if (slot == NULL_SLOT) b.overflow = true;
@@ -615,7 +617,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
   }
 }
 
-template <bool TRACK_FIRST, int W>
+template <bool TRACK_FIRST, int W, int KT, bool NV>
 __global__ __launch_bounds__(BUILD_THREADS) void gb_build(const void* vals, int type, const uint8_t* valid, int64_t voff,
                                                 uint32_t n, Table table, uint32_t cap_mask, uint32_t budget,
                                                 uint32_t* slot_of, BuildStatus* st, int private_tables) {
@@ -649,12 +651,12 @@ __global__ __launch_bounds__(BUILD_THREADS) void gb_build(const void* vals, int 
   if (steps > 0) {
     // Two row buffers take turns, so that no loaded register is copied (a copy would wait for it).
     BuildRows xa, xb;
-    build_load<W>(vals, valid, voff, n, 0, gsize, tid, xa);
+    build_load<W, NV>(vals, valid, voff, n, 0, gsize, tid, xa);
     for (int t = 0; t < steps; t += 2) {
       const uint64_t base = uint64_t(t) * stride;
-      build_step<TRACK_FIRST, W>(vals, type, valid, voff, n, base, base + stride, gsize, tid, xa, xb, slot_of, st, b);
+      build_step<TRACK_FIRST, W, KT, NV>(vals, type, valid, voff, n, base, base + stride, gsize, tid, xa, xb, slot_of, st, b);
       if (t + 1 >= steps) break;
-      build_step<TRACK_FIRST, W>(vals, type, valid, voff, n, base + stride, base + 2 * stride, gsize, tid, xb, xa,
+      build_step<TRACK_FIRST, W, KT, NV>(vals, type, valid, voff, n, base + stride, base + 2 * stride, gsize, tid, xb, xa,
                                  slot_of, st, b);
     }
   }
@@ -2833,16 +2835,21 @@ bool map_range(const void* vals, int type, const uint8_t* valid, int64_t voff, i
   return true;
 }
 
-// gb_build for the key column's element width.
+// gb_build for the key column's element width: 64-bit integer keys without a validity bitmap (the
+// common case) take an instantiation with the key mapping and the validity fixed.
 template <bool TRACK>
 void launch_build(int w, int grid, hipStream_t s, const void* vals, int type, const uint8_t* valid, int64_t voff,
                   uint32_t n, Table T, uint32_t mask, uint32_t budget, uint32_t* slot_of, BuildStatus* st, bool priv) {
   const int p = priv ? 1 : 0;
+  if (!valid && (type == T_I64 || type == T_U64)) {
+    hipLaunchKernelGGL((gb_build<TRACK, 8, T_U64, true>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p);
+    return;
+  }
   switch (w) {
-    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
-    default: hipLaunchKernelGGL((gb_build<TRACK, 1>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 8: hipLaunchKernelGGL((gb_build<TRACK, 8, -1, false>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 4: hipLaunchKernelGGL((gb_build<TRACK, 4, -1, false>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    case 2: hipLaunchKernelGGL((gb_build<TRACK, 2, -1, false>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
+    default: hipLaunchKernelGGL((gb_build<TRACK, 1, -1, false>), dim3(grid), dim3(BUILD_THREADS), 0, s, vals, type, valid, voff, n, T, mask, budget, slot_of, st, p); break;
   }
 }
 
