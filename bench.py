@@ -152,7 +152,7 @@ ROCPROF_NAME = {
     "sort_scatter": "sort_mid<u64, 20, 0>",
     "sort_scatter_last": "sort_mid<u64, 20, 1>",
     "gb_key_scan": "gb_key_scan<8, 3, false>",
-    "gb_build": "gb_build<false, 8>",
+    "gb_build": "gb_build<false, 8, 7, true>",
     "gb_ids": "gb_ids",
     "gb_part_hist": "gb_part_hist<SumCountP, 4>",
     "gb_part_scatter": "gb_part_scatter<SumCountP, 4, 8, false>",

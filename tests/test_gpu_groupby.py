@@ -518,3 +518,41 @@ def test_partitioned_aggregates_many_groups(case):
             assert_agg(got, O.group_agg(ids_w, K, vals, op, vv), f"{case} {name}")
     am.prof_enable(False)
     assert "gb_part_scatter" in am.prof_report(), "the partitioned passes did not run"
+
+
+def test_handle_used_from_another_thread():
+    """am_group_by_keys returns while its last kernels may still run on the calling thread's stream;
+    aggregates, ids and keys asked for from another thread (its own stream) are ordered after them,
+    and the handle can be released there. Results equal the oracle's."""
+    import threading
+    n, K = 3_000_000, 50_000
+    rng = np.random.default_rng(0x7AD)
+    table = O.splitmix64(0x7AD, K).view(np.int64)
+    keys = table[rng.integers(0, K, n)]
+    vals = rng.integers(-10**9, 10**9 + 1, n).astype(np.int64)
+    mk, mv = am.MetalArray.from_numpy(keys), am.MetalArray.from_numpy(vals)
+    ids_w, K2, _ = O.group_keys([keys])
+    for rep in range(3):
+        g = am.group_by([mk])   # no wait for the mapping here
+        out = {}
+
+        def worker():
+            try:
+                out["sum"] = g.sum(mv)
+                out["ids"] = g.ids().to_numpy()[0]
+                out["keys"] = g.key_arrays()[0].to_numpy()[0]
+                out["count"] = g.count_all()
+            except Exception as e:   # reported below, in the test's thread
+                out["error"] = e
+
+        t = threading.Thread(target=worker)
+        t.start()
+        t.join()
+        assert "error" not in out, out.get("error")
+        assert g.group_count == K2
+        np.testing.assert_array_equal(out["ids"], ids_w)
+        assert_agg(out["sum"], O.group_agg(ids_w, K2, vals, 0), f"cross-thread sum, rep {rep}")
+        assert_agg(out["count"], O.group_agg(ids_w, K2, vals, 1), f"cross-thread count_all, rep {rep}")
+        want_keys = np.sort(np.unique(keys))
+        np.testing.assert_array_equal(out["keys"], want_keys)
+        del g
