@@ -216,6 +216,18 @@ __device__ __forceinline__ uint32_t probe_insert(unsigned long long* tkey, unsig
 constexpr int HLL_BITS = 12;
 constexpr uint32_t HLL_M = 1u << HLL_BITS;
 
+// The register hash: the high and low halves of a Fibonacci multiplicative hash of the key, folded
+// and mixed by the first steps of the MurmurHash3 32-bit finaliser (published functions). 32 bits are
+// enough: the estimate only sizes a table of at most 2^32 slots.
+__device__ __forceinline__ uint32_t hll_hash(unsigned long long k) {
+  const unsigned long long h = k * 0x9E3779B97F4A7C15ull;
+  uint32_t x = uint32_t(h >> 32) ^ uint32_t(h);
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  return x;
+}
+
 struct RangeStat {
   unsigned long long kmin, kmax, nulls;
   uint32_t null_first, pad;
@@ -358,10 +370,10 @@ __device__ __forceinline__ void scan_rows(int64_t voff, uint32_t n, uint64_t row
         acc.nf = null_row ? min(acc.nf, uint32_t(i)) : acc.nf;
       }
     }
-    const unsigned long long h = mix64(gk ^ 0x2545F4914F6CDD1Dull);
-    idx[r] = uint32_t(h >> (64 - HLL_BITS));
-    const unsigned long long w = h << HLL_BITS;
-    rank[r] = !ok ? 0u : w ? uint32_t(__clzll((long long)w)) + 1u : uint32_t(64 - HLL_BITS + 1);
+    const uint32_t h = hll_hash(gk);
+    idx[r] = h >> (32 - HLL_BITS);
+    const uint32_t w = h << HLL_BITS;
+    rank[r] = !ok ? 0u : w ? uint32_t(__clz(int(w))) + 1u : uint32_t(32 - HLL_BITS + 1);
   }
   // The step's register reads are issued together; a register is raised only when the row's rank is
   // larger (a null row or a row past the end has rank 0).
