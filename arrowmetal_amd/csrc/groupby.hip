@@ -259,14 +259,13 @@ __global__ __launch_bounds__(256) void gb_key_init(uint32_t* regs, RangeStat* st
 // hashed, and every loaded register is taken at the top of a step, so that no load is moved into a
 // branch where it would be waited for alone.
 constexpr int SCAN_ROWS = 8;
-#ifndef AM_SCAN_BLOCKS_PER_CU
-#define AM_SCAN_BLOCKS_PER_CU 1
-#endif
-#ifndef AM_SCAN_THREADS
-#define AM_SCAN_THREADS 1024
-#endif
-constexpr uint32_t SCAN_THREADS = AM_SCAN_THREADS;   // threads per workgroup of the key scan
-constexpr uint32_t SCAN_WAVES = SCAN_THREADS / 64;
+// Threads per workgroup of the key scan, 1024 threads per CU: one workgroup of 1024, or, for a column
+// with a validity bitmap (whose bytes need more registers than the 128 a 1024-thread workgroup allows),
+// two of 512.
+template <bool HAS_VALID>
+constexpr uint32_t scan_threads() {
+  return HAS_VALID ? 512u : 1024u;
+}
 #ifndef AM_OCC_BLOCKS_PER_CU
 #define AM_OCC_BLOCKS_PER_CU 2
 #endif
@@ -386,10 +385,11 @@ __device__ __forceinline__ void scan_rows(int64_t voff, uint32_t n, uint64_t row
 }
 
 template <int W, int TYPE, bool HAS_VALID>
-__global__ __launch_bounds__(SCAN_THREADS) void gb_key_scan(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+__global__ __launch_bounds__(scan_threads<HAS_VALID>()) void gb_key_scan(const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
                                                    uint32_t* regs, RangeStat* st) {
   constexpr bool RANGE = TYPE >= T_I8 && TYPE <= T_U64;
   __shared__ uint32_t r[HLL_M];
+  constexpr uint32_t SCAN_WAVES = scan_threads<HAS_VALID>() / 64;
   __shared__ unsigned long long s_lo[SCAN_WAVES], s_hi[SCAN_WAVES], s_nz[SCAN_WAVES];
   __shared__ uint32_t s_nf[SCAN_WAVES];
   for (uint32_t j = threadIdx.x; j < HLL_M; j += blockDim.x) r[j] = 0u;
@@ -436,14 +436,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void gb_key_scan(const void* vals, co
   }
 }
 
-void key_scan(int type, dim3 grid, hipStream_t s, const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
+void key_scan(int type, hipStream_t s, const void* vals, const uint8_t* valid, int64_t voff, uint32_t n,
               uint32_t* regs, RangeStat* st) {
+  const int64_t threads = valid ? scan_threads<true>() : scan_threads<false>();
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t(n) + threads * SCAN_ROWS - 1) / (threads * SCAN_ROWS),
+                                                                int64_t(device().cu_count) * (1024 / threads)));
+  const dim3 grid{uint32_t(blocks)}, block{uint32_t(threads)};
   with_key_type(type, [&](auto tc) {
     constexpr int TT = decltype(tc)::value;
     if (valid)
-      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, true>), grid, dim3(SCAN_THREADS), 0, s, vals, valid, voff, n, regs, st);
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, true>), grid, block, 0, s, vals, valid, voff, n, regs, st);
     else
-      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, false>), grid, dim3(SCAN_THREADS), 0, s, vals, valid, voff, n, regs, st);
+      hipLaunchKernelGGL((gb_key_scan<width_of(TT), TT, false>), grid, block, 0, s, vals, valid, voff, n, regs, st);
   });
 }
 
@@ -2438,7 +2442,7 @@ __global__ void fs_redo_rows(AggArgs a, const uint32_t* redo, long long* digits)
 }
 
 template <int P, int EMIN>
-__global__ void fs_redo_settle(const uint32_t* redo, long long K, const long long* digits,
+__global__ __launch_bounds__(256) void fs_redo_settle(const uint32_t* redo, long long K, const long long* digits,
                                const unsigned long long* scan, unsigned long long* out_sum,
                                unsigned long long* out_mean) {
   long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2890,9 +2894,7 @@ Mapped map_column(const void* vals, int type, const uint8_t* valid, int64_t voff
       auto* rg = static_cast<uint32_t*>(regs->ptr);
       auto* rs = static_cast<RangeStat*>(rsb->ptr);
       hipLaunchKernelGGL(gb_key_init, dim3(1), dim3(256), 0, s, rg, rs);
-      const int64_t scan_blocks = (n + int64_t(SCAN_THREADS) * SCAN_ROWS - 1) / (int64_t(SCAN_THREADS) * SCAN_ROWS);
-      key_scan(type, dim3(uint32_t(std::max<int64_t>(1, std::min<int64_t>(scan_blocks, int64_t(device().cu_count) * AM_SCAN_BLOCKS_PER_CU)))),
-               s, vals, valid, voff, un, rg, rs);
+      key_scan(type, s, vals, valid, voff, un, rg, rs);
       hipLaunchKernelGGL(gb_key_estimate, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(rg),
                          static_cast<const RangeStat*>(rs), ks);
       AM_HIP(hipGetLastError());
