@@ -508,11 +508,14 @@ constexpr uint32_t BUILD_CACHE = AM_BUILD_CACHE;   // a power of two
 #endif
 constexpr uint32_t BUILD_THREADS = AM_BUILD_THREADS;
 
-// Caches key -> slot in the workgroup's LDS if the key's entry is still free. The slot is written after
-// the key; a reader that finds the key with no slot yet reads the table instead.
-__device__ __forceinline__ void cache_slot(unsigned long long* ckey, uint32_t* cslot, unsigned long long k,
+// The cache entry of a key from its hash (bits above the table slot's).
+__device__ __forceinline__ uint32_t cache_entry(unsigned long long h) { return uint32_t(h >> 40) & (BUILD_CACHE - 1u); }
+
+// Caches key -> slot in the workgroup's LDS (entry c, the key's cache_entry) if the entry is still
+// free. The slot is written after the key; a reader that finds the key with no slot yet reads the
+// table instead.
+__device__ __forceinline__ void cache_slot(unsigned long long* ckey, uint32_t* cslot, uint32_t c, unsigned long long k,
                                            uint32_t slot) {
-  const uint32_t c = uint32_t(mix64(k) >> 40) & (BUILD_CACHE - 1u);
   if (ckey[c] == EMPTY && atomicCAS(&ckey[c], EMPTY, k) == EMPTY) cslot[c] = slot;
 }
 
@@ -560,7 +563,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
 #pragma unroll
   for (int r = 0; r < BUILD_ROWS; r++) asm volatile("" ::"v"(cur.k[r]), "v"(cur.vb[r]));   // every loaded register taken here
   unsigned long long k[BUILD_ROWS], cv[BUILD_ROWS];
-  uint32_t s[BUILD_ROWS];
+  uint32_t s[BUILD_ROWS], ce[BUILD_ROWS];
   int cat[BUILD_ROWS];   // 0 key, 1 null, 2 the EMPTY key, 3 past the end, 4 cached
 #pragma unroll
   for (int r = 0; r < BUILD_ROWS; r++) {
@@ -578,9 +581,10 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
     }
     const unsigned long long h = mix64(k[r]);
     s[r] = uint32_t(h) & b.cap_mask;
+    ce[r] = cache_entry(h);
     // A cached key takes its slot from LDS (cat 4).
     if (cat[r] == 0) {
-      const uint32_t c = uint32_t(h >> 40) & (BUILD_CACHE - 1u);
+      const uint32_t c = ce[r];
       const uint32_t cs = b.cslot[c];
       if (b.ckey[c] == k[r] && cs != NULL_SLOT) {
         cat[r] = 4;
@@ -611,7 +615,7 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
       slot = s[r];
     } else if (cv[r] == k[r]) {
       slot = s[r];
-      cache_slot(b.ckey, b.cslot, k[r], slot);
+      cache_slot(b.ckey, b.cslot, ce[r], k[r], slot);
     } else {
       slow |= 1u << r;
       continue;
@@ -624,10 +628,11 @@ __device__ __forceinline__ void build_step(const void* vals, int type, const uin
     slow &= slow - 1u;
     const uint32_t i = uint32_t(base + uint64_t(r) * gsize + tid);
     const unsigned long long kk = key_of_raw(ld_raw<W>(vals, int64_t(i)), KT >= 0 ? KT : type);
-    const uint32_t ss = uint32_t(mix64(kk)) & b.cap_mask;
+    const unsigned long long hh = mix64(kk);
+    const uint32_t ss = uint32_t(hh) & b.cap_mask;
     const uint32_t slot = probe_insert(b.tkey, kk, ss, probe_load(&b.tkey[ss]), b.cap_mask, b.budget);
     if (slot == NULL_SLOT) b.overflow = true;
-    else cache_slot(b.ckey, b.cslot, kk, slot);
+    else cache_slot(b.ckey, b.cslot, cache_entry(hh), kk, slot);
     __builtin_nontemporal_store(slot == NULL_SLOT ? NULL_SLOT : b.tbase + slot, slot_of + i);
     if (TRACK_FIRST && slot != NULL_SLOT) lower_first(&b.tfirst[slot], i);
   }
