@@ -40,7 +40,8 @@ struct SegTile {
 };
 
 struct SegChunk {
-  uint32_t seg, start, end, pad;
+  uint32_t seg, start, end;
+  uint32_t whole;   // a group-by chunk that holds its whole batch (1), else 0
 };
 
 // A value key of a batched sort: NaN (ieee order) becomes the largest key after the descending
@@ -426,6 +427,75 @@ __global__ __launch_bounds__(256) void bg_agg_global(const void* keys, int ktype
   }
 }
 
+// The rows of one step of bg_agg: ROWS per thread, blockDim.x apart.
+template <int ROWS>
+struct BgRows {
+  unsigned long long k[ROWS], v[ROWS];
+};
+
+// Loads the 64-bit keys and values of the step's rows from i0; rows past `end` read its last row.
+template <int ROWS>
+__device__ __forceinline__ void bg_load_wide(const unsigned long long* keys, const unsigned long long* vals, uint32_t i0,
+                                             uint32_t end, BgRows<ROWS>& x) {
+#pragma unroll
+  for (int r = 0; r < ROWS; r++) {
+    const uint32_t i = min(i0 + uint32_t(r) * blockDim.x, end - 1u);
+    x.k[r] = keys[i];
+    x.v[r] = vals[i];
+  }
+}
+
+// Adds the step's rows before `end` to the LDS table (cap slots, linear probing; slot cap holds the
+// EMPTY key). The rows probe in rounds: each round reads the current slot of every row not yet placed
+// (those reads in flight together), then claims empty slots and moves the others one slot on, so a
+// step waits on about one LDS read per round rather than one per row. Sets `over` and applies nothing
+// when a key finds no slot.
+template <int ROWS>
+__device__ __forceinline__ void bg_apply(const BgRows<ROWS>& x, uint32_t i0, uint32_t end, unsigned long long* lkey,
+                                         unsigned long long* lsum, uint32_t* lcnt, uint32_t cap, bool& over) {
+#pragma unroll
+  for (int r = 0; r < ROWS; r++) asm volatile("" ::"v"(x.k[r]), "v"(x.v[r]));   // every loaded register taken here
+  if (over) return;
+  const uint32_t mask = cap - 1u;
+  uint32_t s[ROWS];
+  uint32_t live = 0, todo = 0;   // rows of the chunk; rows whose slot is not known yet
+#pragma unroll
+  for (int r = 0; r < ROWS; r++) {
+    const bool in = i0 + uint32_t(r) * blockDim.x < end;
+    live |= uint32_t(in) << r;
+    const bool key = in && x.k[r] != BG_EMPTY;
+    todo |= uint32_t(key) << r;
+    s[r] = key ? bg_hash(x.k[r], mask) : cap;
+  }
+  for (uint32_t round = 0; todo; round++) {
+    if (round >= cap) {
+      over = true;
+      return;
+    }
+    unsigned long long cur[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) cur[r] = (todo >> r) & 1u ? lkey[s[r]] : 0ull;
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      if (!((todo >> r) & 1u)) continue;
+      const unsigned long long k = x.k[r];
+      unsigned long long c = cur[r];
+      if (c == BG_EMPTY) {
+        c = atomicCAS(&lkey[s[r]], BG_EMPTY, k);
+        if (c == BG_EMPTY) c = k;
+      }
+      if (c == k) todo &= ~(1u << r);
+      else s[r] = (s[r] + 1u) & mask;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; r++) {
+    if (!((live >> r) & 1u)) continue;
+    atomicAdd(&lsum[s[r]], x.v[r]);
+    atomicAdd(&lcnt[s[r]], 1u);
+  }
+}
+
 // Group-by strategy 0 (LDS). Persistent: workgroups take chunks from the queue until it is empty (an
 // exit every wave reaches). Each chunk is aggregated in an LDS table of `cap` slots (+1 for the
 // EMPTY key; counts in 32 bits, a chunk holding at most CHUNK_ROWS rows) and flushed into its batch's
@@ -461,43 +531,34 @@ __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const
     if (ci >= nchunks) break;
     const SegChunk c = chunks[ci];
     bool over = false;
-    for (uint32_t i0 = c.start + threadIdx.x; i0 < c.end && !over; i0 += blockDim.x * ROWS) {
-      unsigned long long kk[ROWS], vv[ROWS];
-#pragma unroll
-      for (int r = 0; r < ROWS; r++) {
-        const uint32_t i = min(i0 + uint32_t(r) * blockDim.x, c.end - 1u);
-        if (WIDE) {
-          kk[r] = static_cast<const unsigned long long*>(keys)[i];
-          vv[r] = static_cast<const unsigned long long*>(vals)[i];
-        } else {
-          kk[r] = bg_key(keys, i, ktype);
-          vv[r] = (unsigned long long)bg_value(vals, i, vtype);
-        }
+    if constexpr (WIDE) {
+      // Two row buffers take turns: the next step's rows are loaded before this step's are applied,
+      // so that the loads are in flight while this step works in LDS. The step count is the same for
+      // every thread; rows past the chunk's end read its last row and are not applied.
+      const auto* k64 = static_cast<const unsigned long long*>(keys);
+      const auto* v64 = static_cast<const unsigned long long*>(vals);
+      const uint32_t stride = blockDim.x * ROWS;
+      const int steps = int((c.end - c.start + stride - 1u) / stride);
+      BgRows<ROWS> xa, xb;
+      bg_load_wide(k64, v64, c.start + threadIdx.x, c.end, xa);
+      for (int t = 0; t < steps; t += 2) {
+        const uint32_t i0 = c.start + threadIdx.x + uint32_t(t) * stride;
+        if (t + 1 < steps) bg_load_wide(k64, v64, i0 + stride, c.end, xb);
+        bg_apply(xa, i0, c.end, lkey, lsum, lcnt, cap, over);
+        if (t + 1 >= steps) break;
+        if (t + 2 < steps) bg_load_wide(k64, v64, i0 + 2u * stride, c.end, xa);
+        bg_apply(xb, i0 + stride, c.end, lkey, lsum, lcnt, cap, over);
       }
+    } else {
+      for (uint32_t i0 = c.start + threadIdx.x; i0 < c.end && !over; i0 += blockDim.x * ROWS) {
+        BgRows<ROWS> x;
 #pragma unroll
-      for (int r = 0; r < ROWS; r++) {
-        if (over || i0 + uint32_t(r) * blockDim.x >= c.end) break;
-        const unsigned long long k = kk[r];
-        uint32_t s;
-        if (k == BG_EMPTY) {
-          s = cap;
-        } else {
-          s = bg_hash(k, mask);
-          uint32_t probes = 0;
-          while (true) {
-            unsigned long long cur = lkey[s];
-            if (cur == k) break;
-            if (cur == BG_EMPTY) {
-              unsigned long long prev = atomicCAS(&lkey[s], BG_EMPTY, k);
-              if (prev == BG_EMPTY || prev == k) break;
-            }
-            if (++probes >= cap) { over = true; break; }
-            s = (s + 1u) & mask;
-          }
-          if (over) break;
+        for (int r = 0; r < ROWS; r++) {
+          const uint32_t i = min(i0 + uint32_t(r) * blockDim.x, c.end - 1u);
+          x.k[r] = bg_key(keys, i, ktype);
+          x.v[r] = (unsigned long long)bg_value(vals, i, vtype);
         }
-        atomicAdd(&lsum[s], vv[r]);
-        atomicAdd(&lcnt[s], 1u);
+        bg_apply(x, i0, c.end, lkey, lsum, lcnt, cap, over);
       }
     }
     if (over) atomicOr(&s_over, 1u);
@@ -510,6 +571,18 @@ __global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const
     unsigned long long* gk = tkey + size_t(c.seg) * (cap + 1);
     unsigned long long* gs = tsum + size_t(c.seg) * (cap + 1);
     unsigned long long* gc = tcnt + size_t(c.seg) * (cap + 1);
+    if (c.whole) {
+      // The chunk is its batch's only one: its table, of the batch table's shape and hash, is the
+      // batch's table as it stands (slot for slot, with plain stores; the rest stay empty).
+      for (uint32_t j = threadIdx.x; j <= cap; j += blockDim.x) {
+        const uint32_t cnt = lcnt[j];
+        if (!cnt) continue;
+        gk[j] = lkey[j];
+        gs[j] = lsum[j];
+        gc[j] = cnt;
+      }
+      continue;
+    }
     bool gover = false;
     for (uint32_t j = threadIdx.x; j <= cap; j += blockDim.x) {
       const uint32_t cnt = lcnt[j];
@@ -990,7 +1063,8 @@ Batched* batched_create(const int64_t* offsets, int64_t nbatches, int hash_capac
     for (uint32_t sgi = 0; sgi < b->nseg; sgi++) {
       uint32_t a = b->offsets[sgi], e = b->offsets[sgi + 1];
       for (uint32_t x = a; x < e; x += 65536u) chunks.push_back({sgi, x, std::min(e, x + 65536u), 0});
-      for (uint32_t x = a; x < e; x += CHUNK_ROWS) gchunks.push_back({sgi, x, std::min(e, x + CHUNK_ROWS), 0});
+      for (uint32_t x = a; x < e; x += CHUNK_ROWS)
+        gchunks.push_back({sgi, x, std::min(e, x + CHUNK_ROWS), uint32_t(x == a && e - a <= CHUNK_ROWS)});
     }
     b->nchunks = uint32_t(chunks.size());
     b->ngb_chunks = uint32_t(gchunks.size());
