@@ -502,12 +502,24 @@ __device__ __forceinline__ void bg_apply(const BgRows<ROWS>& x, uint32_t i0, uin
 // table of the same shape. A batch with more distinct keys than `cap` is flagged. WIDE: 64-bit keys
 // and values, loaded directly with ROWS rows in flight per thread; otherwise one row at a time, of any
 // integer types.
+// Threads per workgroup of bg_agg and 64-bit rows per thread per step. The LDS table (20 B per slot)
+// bounds the workgroups per CU (three at 2,048 slots), so 512 threads give twice the waves of 256 in the
+// same LDS; 4 rows per step keep a wave within the ~85 VGPRs that six waves per SIMD leave (191.4M
+// rows in 2,048 batches: 1.74 ms at 256 x 8, 1.20 ms at 512 x 4, 1.21 ms at 512 x 6).
+#ifndef AM_BG_THREADS
+#define AM_BG_THREADS 512
+#endif
+#ifndef AM_BG_ROWS
+#define AM_BG_ROWS 4
+#endif
+constexpr uint32_t BG_THREADS = AM_BG_THREADS;
+
 template <bool WIDE>
-__global__ __launch_bounds__(256) void bg_agg(const void* keys, int ktype, const void* vals, int vtype,
+__global__ __launch_bounds__(BG_THREADS) void bg_agg(const void* keys, int ktype, const void* vals, int vtype,
                                               const SegChunk* chunks, uint32_t nchunks, SegCtl* ctl,
                                               unsigned long long* tkey, unsigned long long* tsum,
                                               unsigned long long* tcnt, uint32_t cap, uint32_t* overflow) {
-  constexpr int ROWS = WIDE ? 8 : 1;
+  constexpr int ROWS = WIDE ? AM_BG_ROWS : 1;
   static_assert(CHUNK_ROWS <= 0xFFFFFFFFu, "chunk counts fit 32 bits");
   extern __shared__ unsigned long long lds[];
   unsigned long long* lkey = lds;                                    // [cap + 1]
@@ -871,7 +883,7 @@ void group_launch(Batched& b, const void* keys, int ktype, const void* vals, int
                          tk, ts, tc, b.cap, static_cast<uint32_t*>(b.overflow->ptr));
     } else {
       const bool wide = type_width(ktype) == 8 && type_width(vtype) == 8;
-      hipLaunchKernelGGL(wide ? bg_agg<true> : bg_agg<false>, dim3(grid), dim3(256), agg_lds(b.cap), s, keys, ktype,
+      hipLaunchKernelGGL(wide ? bg_agg<true> : bg_agg<false>, dim3(grid), dim3(BG_THREADS), agg_lds(b.cap), s, keys, ktype,
                          vals, vtype, static_cast<const SegChunk*>(b.gb_chunks->ptr), b.ngb_chunks, ctl, tk, ts, tc,
                          b.cap, static_cast<uint32_t*>(b.overflow->ptr));
     }
