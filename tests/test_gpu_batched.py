@@ -147,6 +147,29 @@ def test_batched_msd_list_overflow_falls_back(monkeypatch):
         check_sort(idx, sk, offs, False, 0, f"run {rep}")
 
 
+def test_batched_group_chunk_boundaries():
+    """Batches at the group-by's 32,768-row chunk boundary: a batch of at most one chunk stores its
+    LDS table as the batch's table, a larger one merges its chunks' tables into it."""
+    lens = [32_767, 32_768, 32_769, 65_536, 65_537, 1, 0, 98_304, 5]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(offs[-1])
+    print(f"batched chunk boundaries n={n}", flush=True)
+    sk = O.splitmix64(0xB0B0, n).view(np.int64)
+    rng = np.random.default_rng(11)
+    gk = rng.integers(0, 900, n).astype(np.int64) * 7_919 - 3_000_000
+    # The smallest int64 (the LDS tables' empty-slot word, kept in a slot of its own) in a batch of one
+    # chunk and in one of two.
+    gk[offs[0]:offs[1]:5] = np.iinfo(np.int64).min
+    gk[offs[2]:offs[3]:3] = np.iinfo(np.int64).min
+    gv = rng.integers(-10**9, 10**9 + 1, n).astype(np.int64)
+    plan = am.Batched(offs, hash_capacity=2048)
+    msk, mgk, mgv = (am.MetalArray.from_numpy(x) for x in (sk, gk, gv))
+    for rep in range(2):
+        plan.run(msk, mgk, mgv, descending=False, use_graph=False)
+        check_groups(plan, gk, gv, offs, f"chunk boundaries run {rep}")
+    assert plan.config["regrouped_batches"] == 0
+
+
 def test_batched_overflow_regroups():
     """Batches with more distinct keys than the hash capacity are regrouped by the general path."""
     offs = np.array([0, 100, 5100, 5100, 9000, 40_000, 40_050], dtype=np.int64)
